@@ -1,0 +1,106 @@
+"""Write the golden fixtures under tests/golden/ by running the reference fastLPR.
+
+Run in the development container only (the reference checkout is not present
+where the GPU tests run):
+
+    FASTLPR_QUIET=1 python tests/golden/make_golden.py /root/reference/fastLPR_py/src
+
+Each fixture stores the inputs (or, for the large case, the seeded recipe and a
+hash of the inputs), the options, and the reference outputs of
+``cv_fastlpr`` / ``cv_fastkde``: yhat (or a seeded subsample of it plus its
+hash), gcv_m, gcv_sd, dof, dof_stderr, idmin, id1se, the selected bandwidth,
+the grid values at the selected bandwidth, s0 and the variance factor d.
+These end-to-end compositions (probe stream x interpolation x float32
+rounding x output-grid offset) are what only the reference itself can pin.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+import warnings
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _hash(a) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def recipe_inputs(name: str):
+    """Seeded inputs; every case is regenerated identically by tests/golden/recipes.py."""
+    sys.path.insert(0, HERE)
+    import recipes
+    return recipes.make(name)
+
+
+def run_lpr(fl, name, store_inputs=True, n_sub=None):
+    x, y, h, opt = recipe_inputs(name)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        r = fl.cv_fastlpr(x, y, h, dict(opt))
+    g = r.gcv_yhat
+    out = dict(
+        kind="lpr", name=name, options=json.dumps(opt), hlist=np.asarray(h, float),
+        gcv_m=np.asarray(g["gcv_m"]), gcv_sd=np.asarray(g["gcv_sd"]),
+        dof=np.asarray(g["dof"]), dof_stderr=np.asarray(g["dof_stderr"]),
+        idmin=int(g["idmin"]), id1se=int(g["id1se"]), h1se=np.asarray(g["h1se"]),
+        grid_values=np.asarray(r.fpp_yhat.values if hasattr(r.fpp_yhat, "values") else np.nan),
+        s0=np.asarray(r.s0), d=np.nan if r.d is None else float(r.d),
+        x_hash=_hash(x), y_hash=_hash(y), yhat_hash=_hash(np.asarray(r.yhat)),
+    )
+    if store_inputs:
+        out.update(x=x, y=y, yhat=np.asarray(r.yhat))
+    else:
+        idx = np.random.default_rng(1).choice(len(x), size=n_sub, replace=False)
+        idx.sort()
+        out.update(yhat_idx=idx, yhat_sub=np.asarray(r.yhat)[idx])
+    return out
+
+
+def run_kde(fl, name):
+    x, _, h, opt = recipe_inputs(name)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        r = fl.cv_fastkde(x, h, dict(opt))
+    lc = r.lcv
+    return dict(kind="kde", name=name, options=json.dumps(opt), hlist=np.asarray(h, float),
+                x=x, fhat=np.asarray(r.fhat), lcv_m=np.asarray(lc["lcv_m"]),
+                lcv_sd=np.asarray(lc["lcv_sd"]), idmax=int(lc["idmax"]),
+                id1se=int(lc["id1se"]), h1se=np.asarray(lc["h1se"]),
+                grid=np.stack([np.asarray(g) for g in r.grid]) if len({len(g) for g in r.grid}) == 1
+                else np.asarray(np.concatenate(r.grid)))
+
+
+def main(ref_src: str):
+    sys.path.insert(0, ref_src)
+    os.environ.setdefault("FASTLPR_QUIET", "1")
+    import fastlpr as fl
+    import scipy
+    from fastlpr.backend_selection import select_accumulation_backend
+    meta = dict(numpy=np.__version__, scipy=scipy.__version__,
+                backend=select_accumulation_backend())
+    sys.path.insert(0, HERE)
+    import recipes
+    for name in recipes.LPR_SMALL:
+        out = run_lpr(fl, name)
+        out["meta"] = json.dumps(meta)
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+        print("wrote", name)
+    for name in recipes.LPR_LARGE:
+        out = run_lpr(fl, name, store_inputs=False, n_sub=4096)
+        out["meta"] = json.dumps(meta)
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+        print("wrote", name)
+    for name in recipes.KDE:
+        out = run_kde(fl, name)
+        out["meta"] = json.dumps(meta)
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+        print("wrote", name)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference/fastLPR_py/src")
