@@ -1,0 +1,190 @@
+"""Benchmark: cv_fastlpr fits/sec on the 2-D order-2 heteroscedastic workload.
+
+Workload (one "step"): the heteroscedastic pair on N = 1e6 points in [0,1]^2,
+y = sin(2 pi x1) cos(2 pi x2) + (0.05 + 0.2 x1) N(0,1) (float32 inputs),
+grid 256 x 256, 30 log-spaced bandwidths in [0.005, 0.5], order 2, GCV with
+10 Hutchinson probes:
+    pass 1  cv_fastlpr(x, y)                          (mean)
+    pass 2  cv_fastlpr(x, (y - yhat)^2, variance)     (variance)
+so one step is two fits; value = fits per second = 2 K / time.  Inputs are
+resident in HBM before the timed region; outputs stay on the device.
+
+The JSON line carries the chosen form (float32 MFMA convolution, float64 spread
+and solves), the plain form (float64 end to end) for comparison, the maximum
+relative yhat difference between them, a roofline entry for the dominant
+kernel (flpr_conv, measured with HIP events on the launch stream during the
+timed steps) and a CPU baseline (the oracle restatement, single thread, on a
+bounded sample: the mean pass with 1 and with 2 of the 30 bandwidths,
+extrapolated linearly to 30).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOAD = {"workload": "cv_fastlpr 2-D order-2 heteroscedastic pair (mean + variance pass)",
+            "n_points": 1_000_000, "grid": [256, 256], "bandwidths": 30,
+            "h_range": [0.005, 0.5], "order": 2, "dof_probes": 10, "inputs": "float32",
+            "seed": 0}
+PEAK_MFMA_FP32 = 157.3e12
+PEAK_MFMA_FP64 = 78.6e12
+
+
+def make_inputs(n: int, seed: int = 0):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0, 1, (n, 2)).astype(np.float32)
+    s = 0.05 + 0.2 * x[:, 0]
+    y = (np.sin(2 * np.pi * x[:, 0]) * np.cos(2 * np.pi * x[:, 1])
+         + s * rng.standard_normal(n)).astype(np.float32)
+    return x, y
+
+
+def hlist():
+    from fastlpr_amd import get_hlist
+    h = get_hlist(30, [0.005, 0.5])[:, 0]
+    return np.column_stack([h, h])
+
+
+def step(cv, xd, yd, hl, prec):
+    opt = {"order": 2, "N": [256, 256], "precision": prec}
+    r1 = cv(xd, yd, hl, opt)
+    r2y = (yd - r1.yhat) ** 2
+    r2 = cv(xd, r2y, hl, dict(opt, y_type_out="variance"))
+    return r1, r2
+
+
+def cpu_baseline(x, y, hl):
+    """Oracle (NumPy restatement of the reference dataflow, float32 'ref' mode),
+    one thread, mean pass with 1 and 2 bandwidths; fits/s extrapolated to 30."""
+    from threadpoolctl import threadpool_limits
+    from oracle import restate as R
+    xs, ys = x.astype(np.float64), y.astype(np.float64)
+    times = {}
+    with threadpool_limits(1):
+        for k in (1, 2):
+            sub = hl[14:14 + k]
+            t = time.perf_counter()
+            R.cv_fastlpr(xs, ys, sub, {"order": 2, "N": [256, 256]}, mode="ref")
+            times[k] = time.perf_counter() - t
+    slope = times[2] - times[1]
+    per_fit = times[1] + (hl.shape[0] - 1) * slope
+    return {"value": 1.0 / per_fit, "unit": "fits/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle restatement of the mean pass on the same 1e6 points, 256x256 "
+                       f"grid, with 1 and 2 of the 30 bandwidths ({times[1]:.1f} s, "
+                       f"{times[2]:.1f} s), extrapolated linearly to 30 bandwidths: "
+                       f"{per_fit:.1f} s per fit")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the float64 plain form")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the hot path shards by bandwidth, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import fastlpr_amd
+    from fastlpr_amd import engine
+    cv = fastlpr_amd.cv_fastlpr
+    n = WORKLOAD["n_points"]
+    x, y = make_inputs(n, WORKLOAD["seed"])
+    hl = hlist()
+    dev = torch.device("cuda", 0)
+    xd = torch.as_tensor(x, device=dev).to(torch.float64)
+    yd = torch.as_tensor(y, device=dev).to(torch.float64)
+    print(f"[bench] {torch.cuda.get_device_name(0)}; warmup {args.warmup}", file=sys.stderr,
+          flush=True)
+
+    for _ in range(max(args.warmup, 1)):
+        step(cv, xd, yd, hl, "fp32")
+    torch.cuda.synchronize()
+    engine.TIMER.start()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r1, r2 = step(cv, xd, yd, hl, "fp32")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    engine.TIMER.stop()
+    kt = engine.TIMER.collect()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    fits_s = 2.0 * args.steps / (t1 - t0)
+    print(f"[bench] chosen: {ms_step:.2f} ms/step, {fits_s:.1f} fits/s", file=sys.stderr,
+          flush=True)
+    conv = kt.get("flpr_conv", [])
+    conv_ms = float(np.mean([c[0] for c in conv])) if conv else float("nan")
+    conv_flop = float(np.mean([c[1] for c in conv])) if conv else float("nan")
+    achieved = conv_flop / (conv_ms * 1e-3) / 1e12 if conv else float("nan")
+    conv_share = sum(c[0] for c in conv) / (args.steps * ms_step) if conv else float("nan")
+
+    extra = {}
+    if not args.no_plain:
+        for _ in range(1):
+            step(cv, xd, yd, hl, "fp64")
+        torch.cuda.synchronize()
+        tp0 = time.perf_counter()
+        ksteps = max(2, args.steps // 3)
+        for _ in range(ksteps):
+            p1, p2 = step(cv, xd, yd, hl, "fp64")
+        torch.cuda.synchronize()
+        tp1 = time.perf_counter()
+        plain_fits = 2.0 * ksteps / (tp1 - tp0)
+        e1 = float((r1.yhat - p1.yhat).abs().max() / p1.yhat.abs().max())
+        e2 = float((r2.yhat - p2.yhat).abs().max() / p2.yhat.abs().max())
+        extra = {"plain_fp64_fits_per_s": plain_fits,
+                 "chosen_vs_plain_yhat_max_rel_err": {"mean_pass": e1, "variance_pass": e2},
+                 "selected_h_index": {"mean_pass": int(r1.gcv_yhat["id1se"]),
+                                      "variance_pass": int(r2.gcv_yhat["id1se"]),
+                                      "plain_mean_pass": int(p1.gcv_yhat["id1se"])}}
+        print(f"[bench] plain fp64: {plain_fits:.1f} fits/s; chosen-vs-plain yhat err "
+              f"{e1:.2e} / {e2:.2e}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] cpu baseline (oracle, 1 thread, bounded sample) ...", file=sys.stderr,
+              flush=True)
+        cpu = cpu_baseline(x, y, hl)
+
+    line = {
+        "metric": "cv_fastlpr fits/sec, 2D N=1e6 order2 30 h",
+        "value": fits_s,
+        "unit": "fits/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": dict(WORKLOAD, form="chosen: float32 MFMA convolution, float64 spread/solve",
+                       fits_per_step=2),
+        "roofline": {"bound": "mfma", "kernel": "flpr_conv",
+                     "achieved": achieved, "peak": PEAK_MFMA_FP32 / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / (PEAK_MFMA_FP32 / 1e12) if conv else None,
+                     "traffic": None, "avg_launch_ms": conv_ms,
+                     "flop_per_launch": conv_flop, "launches_timed": len(conv),
+                     "share_of_step_time": conv_share},
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

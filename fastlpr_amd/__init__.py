@@ -1,0 +1,21 @@
+"""fastlpr_amd: MI355X engine for the fastLPR cv_fastlpr / cv_fastkde hot path.
+
+Public API (same names and semantics as the reference package ``fastlpr``):
+``cv_fastlpr``, ``cv_fastkde``, ``get_hlist``, ``RegressionOutput``, ``KDEOutput``.
+"""
+from .bandwidth import get_hlist
+from .structures import KDEOutput, RegressionOutput
+
+__version__ = "0.1.0"
+
+
+def __getattr__(name):
+    # the API pulls in torch and the HIP library lazily
+    if name in ("cv_fastlpr", "cv_fastkde", "clear_caches", "GridInterpolant"):
+        from . import api
+        return getattr(api, name)
+    raise AttributeError(name)
+
+
+__all__ = ["cv_fastlpr", "cv_fastkde", "get_hlist", "RegressionOutput", "KDEOutput",
+           "clear_caches"]

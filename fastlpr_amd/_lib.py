@@ -1,0 +1,134 @@
+"""ctypes binding of the C-ABI library ``libfastlpr_amd.so`` (include/fastlpr_amd.h).
+
+The library is built in-tree by ``fastlpr_amd._build.build()``.  Loading fails
+loudly when it is missing: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+LIB_NAME = "libfastlpr_amd.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+MAX_DIM = 3
+
+EXPORTS = (
+    "flpr_last_hip_error",
+    "flpr_xstats",
+    "flpr_points_prepare",
+    "flpr_points_bin",
+    "flpr_spread",
+    "flpr_taps",
+    "flpr_conv",
+    "flpr_max_moment",
+    "flpr_key_to_double",
+    "flpr_solve",
+    "flpr_adjoint_interp",
+    "flpr_interp_rss",
+    "flpr_interp_eval",
+    "flpr_f64_to_f32",
+)
+
+
+class FlprGeom(C.Structure):
+    """Mirror of ``FlprGeom`` in include/fastlpr_amd.h (field order matters)."""
+
+    _fields_ = [
+        ("d", C.c_int32),
+        ("msp", C.c_int32),
+        ("n_grid", C.c_int32 * MAX_DIM),
+        ("L", C.c_int32 * MAX_DIM),
+        ("E", C.c_int32 * MAX_DIM),
+        ("G", C.c_int32 * MAX_DIM),
+        ("tile", C.c_int32 * MAX_DIM),
+        ("ntile", C.c_int32 * MAX_DIM),
+        ("hx", C.c_double * MAX_DIM),
+        ("inv4tau", C.c_double * MAX_DIM),
+        ("mean", C.c_double * MAX_DIM),
+        ("std_", C.c_double * MAX_DIM),
+        ("centre", C.c_double * MAX_DIM),
+        ("xmin_s", C.c_double * MAX_DIM),
+        ("scale_s", C.c_double * MAX_DIM),
+        ("xmax_s", C.c_double * MAX_DIM),
+        ("ks", C.c_double * MAX_DIM),
+        ("idx0", C.c_double * MAX_DIM),
+        ("shift", C.c_double * MAX_DIM),
+        ("graw0", C.c_double * MAX_DIM),
+        ("graw1", C.c_double * MAX_DIM),
+    ]
+
+
+class FlprError(RuntimeError):
+    pass
+
+
+_P = C.c_void_p
+_I32 = C.c_int32
+_I64 = C.c_int64
+_D = C.c_double
+_GP = C.POINTER(FlprGeom)
+
+_SIGS = {
+    "flpr_last_hip_error": [C.c_char_p, _I32],
+    "flpr_xstats": [_P, _I64, _I32, _P, _P, _P],
+    "flpr_points_prepare": [_P, _I64, _GP, _P, _P, _P, _P, _P],
+    "flpr_points_bin": [_P, _I64, _I32, _P, _P, _P, _P, _P],
+    "flpr_spread": [_GP, _P, _P, _P, _P, _I32, _P, _I32, _P, _I32, _P, _P],
+    "flpr_taps": [_GP, _P, _I32, _I32, _I32, _D, _P, _P, _P],
+    "flpr_conv": [_I32, _I32, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P],
+    "flpr_max_moment": [_I32, _P, _I64, _I32, C.POINTER(_I64), _I32, _I64, _I32, _P, _P],
+    "flpr_solve": [_I32, _P, _I64, _I32, _I64, _I32, _I32, C.POINTER(_I64), C.POINTER(_I64), _P,
+                   _D, _D, _I32, _I32, _P, _I32, _I32, _P, _P, _P],
+    "flpr_adjoint_interp": [_GP, _P, _I64, _P, _P, _P, _I32, _P, _P],
+    "flpr_interp_rss": [_GP, _P, _I64, _P, _P, _I32, _I32, _P, _I32, _P, _P],
+    "flpr_interp_eval": [_GP, _P, _I64, _P, _I32, _P, _P, _P, _P],
+    "flpr_f64_to_f32": [_P, _P, _I64, _P],
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the library (once) and declare every exported signature."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    # Bind to the HIP runtime torch already carries: torch loads its own
+    # libamdhip64 (same soname); loading this library first would pull a second
+    # runtime from /opt/rocm into the process.
+    import torch  # noqa: F401
+    if not os.path.exists(path):
+        raise FlprError(
+            f"{LIB_NAME} not found at {path}; build it with fastlpr_amd._build.build() "
+            "(hipcc --offload-arch=gfx950).  There is no CPU fallback.")
+    lib = C.CDLL(path)
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    lib.flpr_key_to_double.argtypes = [C.c_uint64]
+    lib.flpr_key_to_double.restype = C.c_double
+    _lib = lib
+    return lib
+
+
+def call(name: str, *args) -> None:
+    """Call a status-returning entry point; raise FlprError on a nonzero status."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        kinds = {-1: "invalid argument", -2: "HIP launch/runtime error", -3: "table too small"}
+        detail = ""
+        if rc == -2:
+            buf = C.create_string_buffer(256)
+            code = lib.flpr_last_hip_error(buf, 256)
+            detail = f": hipError {code} {buf.value.decode(errors='replace')}"
+        raise FlprError(f"{name} failed with status {rc} ({kinds.get(rc, 'unknown')}){detail}")
+
+
+def i64_array(values):
+    arr = (C.c_int64 * max(1, len(values)))()
+    for i, v in enumerate(values):
+        arr[i] = int(v)
+    return arr

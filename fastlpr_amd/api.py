@@ -1,0 +1,488 @@
+"""Drop-in ``cv_fastlpr`` / ``cv_fastkde`` on the MI355X engine.
+
+Signatures, option keys, defaults, validation errors and warnings follow the
+reference API (api.py:32-164, 167-832, 835-1184).  All numeric work on the
+data runs in the HIP kernels of libfastlpr_amd.so (see ``engine``); the host
+does validation, option handling, probe generation (NumPy MT19937, so the
+Hutchinson probes are the reference's), and the O(dh) GCV / LCV selection.
+
+Inputs may be NumPy arrays (results come back as NumPy) or torch tensors on a
+GPU (results stay on the device as torch tensors, nothing crosses PCIe).
+
+Extra option (no counterpart in the reference):
+  ``precision``: "fp32" (default; float32 MFMA convolution with float64
+  spread and float64 solves) or "fp64" (float64 end to end).
+"""
+from __future__ import annotations
+
+import warnings
+from typing import Dict, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import engine
+from .bandwidth import get_hlist
+from .structures import KDEOutput, RegressionOutput
+
+__all__ = ["cv_fastlpr", "cv_fastkde", "get_hlist", "GridInterpolant"]
+
+
+# ---------------------------------------------------------------------------
+# option and input helpers
+# ---------------------------------------------------------------------------
+
+def set_defaults(options: Optional[dict], **defaults) -> dict:
+    """Shallow copy, then setdefault (utils/__init__.py:18-60)."""
+    out = dict(options) if options is not None else {}
+    for k, v in defaults.items():
+        out.setdefault(k, v)
+    return out
+
+
+def _device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("fastlpr_amd needs a ROCm GPU (torch.cuda.is_available() is False); "
+                           "there is no CPU fallback")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _is_dev(a) -> bool:
+    return isinstance(a, torch.Tensor) and a.is_cuda
+
+
+def _as_column_matrix(x):
+    """1-D -> column, complex columns split into [real, imag] (utils/helpers.py:31-71)."""
+    if isinstance(x, torch.Tensor):
+        t = x
+        if t.ndim == 1:
+            t = t[:, None]
+        if t.ndim != 2:
+            raise ValueError("Input must be a 1D or 2D array.")
+        if t.is_complex():
+            cplx = torch.any(t.imag != 0, dim=0)
+            t = torch.cat([t.real, t.imag[:, cplx]], dim=1) if bool(cplx.any()) else t.real
+        return t.to(torch.float64)
+    arr = np.asarray(x)
+    if arr.ndim == 1:
+        arr = arr[:, None]
+    if arr.ndim != 2:
+        raise ValueError("Input must be a 1D or 2D array.")
+    if np.iscomplexobj(arr):
+        cplx = np.any(arr.imag != 0, axis=0)
+        arr = np.column_stack([arr.real, arr.imag[:, cplx]]) if np.any(cplx) else arr.real
+    return arr.astype(float)
+
+
+def _std_ddof1(x) -> np.ndarray:
+    if isinstance(x, torch.Tensor):
+        s = torch.std(x, dim=0, correction=1).cpu().numpy() if x.shape[0] > 1 \
+            else np.full(x.shape[1], np.nan)
+    else:
+        s = np.std(x, axis=0, ddof=1)
+    return np.asarray(s, dtype=float)
+
+
+def _default_hrange(x) -> np.ndarray:
+    """[0.1, 1] x std(ddof=1) per dimension (utils/helpers.py:74-92)."""
+    s = _std_ddof1(x)
+    s = np.where(s == 0, 1.0, s)
+    return np.column_stack((0.1 * s, 1.0 * s))
+
+
+def _prepare_bandwidths(h, dims: int, options: dict, default_range) -> np.ndarray:
+    """Shape and validate bandwidths (api.py:32-70)."""
+    if h is None:
+        dh = options.get("dh", None)
+        if dh is None:
+            dh = 20 if dims == 1 else [20] * dims
+        bw = get_hlist(dh, default_range)
+    else:
+        a = np.asarray(h.cpu() if isinstance(h, torch.Tensor) else h, dtype=float)
+        if a.ndim == 0:
+            bw = np.full((1, dims), a.item())
+        elif a.ndim == 1:
+            if a.size != dims:
+                raise ValueError("Scalar bandwidth must match the number of dimensions.")
+            bw = a.reshape(1, -1)
+        else:
+            if a.shape[1] != dims:
+                raise ValueError(
+                    "Bandwidth array must have the same dimensionality as the input data.")
+            bw = a.reshape(-1, dims)
+    if not np.all(np.isfinite(bw)):
+        raise ValueError("Bandwidth values must be finite (no NaN or Inf).")
+    if np.any(bw <= 0):
+        raise ValueError(
+            "Bandwidth values must be strictly positive. "
+            f"Found non-positive values: {bw[bw <= 0]}")
+    return bw
+
+
+def _one_se_rule(scores, errs, bandwidths, prefer_smaller: bool, dstd: float = 1.0) -> int:
+    """1-SE selection over finite scores; ties on sum(h) broken by the smaller
+    score (api.py:73-164)."""
+    scores = np.asarray(scores, dtype=float)
+    errs = np.asarray(errs, dtype=float)
+    bandwidths = np.asarray(bandwidths, dtype=float)
+    valid = np.isfinite(scores)
+    if not np.any(valid):
+        raise ValueError(
+            "All GCV scores are NaN or Inf. Regression failed for all bandwidths. "
+            "Try: (1) Use larger bandwidth h, (2) Increase grid points, "
+            "(3) Check for data quality issues")
+    vidx = np.where(valid)[0]
+    vs, ve = scores[valid], errs[valid]
+    if prefer_smaller:
+        best = int(np.argmin(vs))
+        cand = vs <= vs[best] + dstd * ve[best]
+        cidx = vidx[cand]
+        if cidx.size == 0:
+            return int(vidx[best])
+        sums = bandwidths[cidx].sum(axis=1)
+        ties = np.where(sums == sums.max())[0]
+        return int(cidx[ties[int(np.argmin(vs[cand][ties]))]])
+    best = int(np.argmax(vs))
+    cand = vs >= vs[best] - dstd * ve[best]
+    cidx = vidx[cand]
+    if cidx.size == 0:
+        return int(vidx[best])
+    return int(cidx[np.argmax(bandwidths[cidx].sum(axis=1))])
+
+
+def _check_finite(a, what: str, fn: str):
+    if isinstance(a, torch.Tensor):
+        t = torch.view_as_real(a) if a.is_complex() else a
+        nan = bool(torch.isnan(t).any())
+        inf = bool(torch.isinf(t).any())
+    else:
+        nan = bool(np.any(np.isnan(a)))
+        inf = bool(np.any(np.isinf(a)))
+    if nan:
+        raise ValueError(f"{what} contains NaN values. Please remove or impute NaN values "
+                         f"before calling {fn}.")
+    if inf:
+        raise ValueError(f"{what} contains Inf values. Please remove or handle Inf values "
+                         f"before calling {fn}.")
+
+
+def _grid_shape(opts: dict, n: int, dims: int) -> Tuple[int, ...]:
+    """Grid nodes per dimension: options['N'] or ceil(n^(1/d)) (model.py:78-83, core.py:76-82)."""
+    N = opts.get("N", None)
+    if N is None:
+        dn = int(np.ceil(n ** (1.0 / dims)))
+        N = [dn] * dims
+    elif np.isscalar(N):
+        N = [int(N)] * dims
+    else:
+        N = [int(v) for v in np.asarray(N).ravel()]
+    if len(N) != dims:
+        raise ValueError("options['N'] must give one grid size per dimension.")
+    if min(N) < 2:
+        raise ValueError("The GPU engine needs at least 2 grid nodes per dimension.")
+    return tuple(N)
+
+
+# ---------------------------------------------------------------------------
+# Hutchinson probes: NumPy MT19937 stream, cached on the device
+# ---------------------------------------------------------------------------
+
+_PROBES: Dict[tuple, Tuple[torch.Tensor, np.ndarray]] = {}
+
+
+def _probes(n: int, K: int, seed, random_matrix, dev) -> Tuple[torch.Tensor, np.ndarray]:
+    """Probe matrix p (n, K) = Generator(MT19937(seed)).standard_normal((n, K))
+    (regression.py:1184-1194) and p'p per column.  The matrix depends only on
+    (n, K, seed), not on the data, so it is generated once per key and kept on
+    the device; a caller-supplied random_matrix is used as is."""
+    if random_matrix is not None:
+        p = np.asarray(random_matrix.cpu() if isinstance(random_matrix, torch.Tensor)
+                       else random_matrix, dtype=float)
+        if p.shape[0] != n:
+            raise ValueError(f"random_matrix has wrong shape: expected ({n}, *), got {p.shape}")
+        return torch.as_tensor(p, device=dev).contiguous(), np.sum(p * p, axis=0)
+    key = (n, K, int(seed), str(dev))
+    hit = _PROBES.get(key)
+    if hit is None:
+        p = np.random.Generator(np.random.MT19937(seed)).standard_normal((n, K))
+        hit = (torch.as_tensor(p, device=dev).contiguous(), np.sum(p * p, axis=0))
+        if len(_PROBES) > 8:
+            _PROBES.clear()
+        _PROBES[key] = hit
+    return hit
+
+
+def clear_caches():
+    """Drop the cached probe matrices and conv plans."""
+    _PROBES.clear()
+    engine._PLANS.clear()
+
+
+# ---------------------------------------------------------------------------
+# interpolant returned as fpp_yhat / fpp
+# ---------------------------------------------------------------------------
+
+class GridInterpolant:
+    """Multilinear interpolant of grid values (RegularGridInterpolator 'linear'
+    semantics, linear extrapolation outside the grid) evaluated by the
+    flpr_interp_eval kernel.  Grid values stay on the device."""
+
+    def __init__(self, geo, axes, values: torch.Tensor, grid_shape, cplx: bool, ncol: int,
+                 as_numpy: bool, squeeze: bool):
+        self._geo = geo
+        self.grid_axes = tuple(axes)
+        self._vals = values  # (ncol_real, nodes) float64 on device
+        self._shape = tuple(grid_shape)
+        self._cplx = cplx
+        self._ncol = ncol
+        self._as_numpy = as_numpy
+        self._squeeze = squeeze
+
+    @property
+    def values(self) -> np.ndarray:
+        v = self._vals.cpu().numpy()
+        if self._cplx:
+            v = v[0::2] + 1j * v[1::2]
+        v = np.moveaxis(v.reshape((v.shape[0],) + self._shape), 0, -1)
+        return v[..., 0] if self._squeeze else v
+
+    def __call__(self, points):
+        dev = self._vals.device
+        if isinstance(points, torch.Tensor):
+            pts = points.to(device=dev, dtype=torch.float64)
+        else:
+            pts = torch.as_tensor(np.atleast_2d(np.asarray(points, dtype=float)), device=dev)
+        if pts.ndim == 1:
+            pts = pts[:, None] if len(self._shape) == 1 else pts[None, :]
+        pts = pts.contiguous()
+        out, _ = engine.interp_eval(self._geo, pts, self._vals, self._vals.shape[0])
+        if self._cplx:
+            out = torch.complex(out[:, 0::2], out[:, 1::2])
+        if self._squeeze:
+            out = out[:, 0]
+        return out.cpu().numpy() if self._as_numpy else out
+
+
+# ---------------------------------------------------------------------------
+# cv_fastlpr
+# ---------------------------------------------------------------------------
+
+def _hutchinson(dot: np.ndarray, pp: np.ndarray, n: int, y_is_complex: bool):
+    """tr(H) ~ n (p'phat)/(p'p) per probe and bandwidth, NaN -> n, clip [0, n];
+    penalty 1/max(pdof, .01)^2 mean and std (regression.py:1276-1354)."""
+    K = dot.shape[1]
+    tr = n * (dot.T / pp[:, None])          # (K, dh)
+    tr = np.where(np.isnan(tr), n, tr)
+    tr = np.clip(tr, 0.0, n)
+    dof = np.mean(tr, axis=0)
+    dh = tr.shape[1]
+    if K > 1:
+        stderr = np.std(tr, axis=0, ddof=1) / np.sqrt(K)
+        pd = 1 - tr / n
+        if y_is_complex:
+            pd = 2 * pd - 1
+        pc = np.clip(pd, 0.0, 1.0)
+        pen = 1.0 / np.maximum(pc, 0.01) ** 2
+        pen_std = np.std(pen, axis=0, ddof=1)
+        pen_mean = np.mean(pen, axis=0)
+        if y_is_complex:
+            dof = n * (1 - np.mean(pc, axis=0))
+    else:
+        stderr = np.zeros(dh)
+        pen_std = np.zeros(dh)
+        pen_mean = None
+    return dof, stderr, pen_std, pen_mean
+
+
+def _gcv(rss: np.ndarray, n: int, dof: float, pen_std: float, pen_mean, aggregation: str):
+    """GCV = mse * penalty, stderr = mse * penalty_std (regression.py:378-421)."""
+    mse = rss / n
+    pd = min(max(1.0 - dof / n, 0.0), 1.0)
+    pen = float(pen_mean) if pen_mean is not None else 1.0 / max(pd, 0.01) ** 2
+    g = mse * pen
+    if aggregation == "first":
+        score, m = float(g[0]), float(mse[0])
+    elif aggregation == "average":
+        score, m = float(np.mean(g)), float(np.mean(mse))
+    elif aggregation == "sum":
+        score, m = float(np.sum(g)), float(np.sum(mse))
+    else:
+        raise ValueError(f"Unknown aggregation method: {aggregation}. "
+                         "Use 'first', 'average', or 'sum'.")
+    se = m * pen_std if pen_std > 0 else 0.0
+    return score, se
+
+
+def _y_channels(y_arr, dev):
+    """(n, dy) real/complex -> (n, ny) float64 device, complex as interleaved (re, im)."""
+    if isinstance(y_arr, torch.Tensor):
+        t = y_arr.to(dev)
+        if t.is_complex():
+            t = torch.view_as_real(t.to(torch.complex128)).reshape(t.shape[0], -1)
+        return t.to(torch.float64).contiguous()
+    if np.iscomplexobj(y_arr):
+        yc = np.asarray(y_arr, dtype=np.complex128)
+        return torch.as_tensor(yc.view(np.float64).reshape(yc.shape[0], -1), device=dev).contiguous()
+    return torch.as_tensor(np.asarray(y_arr, dtype=np.float64), device=dev).contiguous()
+
+
+def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
+               options: Optional[dict] = None) -> RegressionOutput:
+    """Local polynomial regression (order 0/1/2) with GCV bandwidth selection
+    (api.py:167-832), computed on the GPU.
+
+    x (N, d), y (N,) / (N, dy) / (1, N), real or complex; h scalar, (d,) or (k, d);
+    options as the reference: order, regularization, num_dof_sample, dof_seed,
+    random_matrix, y_type_out ('mean' | 'variance'), calc_dof, gcv_selection,
+    dstd, accuracy, N, xrange, flag_power2, dof_interp_method,
+    gcv_interp_method, interp_method; plus ``precision`` ('fp32' | 'fp64').
+    """
+    as_numpy = not (_is_dev(x) or _is_dev(y))
+    x_arr = _as_column_matrix(x)
+    if isinstance(y, torch.Tensor):
+        y_arr = y
+        if y_arr.ndim == 1:
+            y_arr = y_arr[:, None]
+        elif y_arr.ndim == 2:
+            if y_arr.shape[0] == 1 and y_arr.shape[1] > 1:
+                y_arr = y_arr.T
+        else:
+            raise ValueError(f"Response y must be 1D or 2D array. Got {y_arr.ndim}D array.")
+    else:
+        y_arr = np.asarray(y)
+        if y_arr.ndim == 1:
+            y_arr = y_arr[:, None]
+        elif y_arr.ndim == 2:
+            if y_arr.shape[0] == 1 and y_arr.shape[1] > 1:
+                y_arr = y_arr.T
+        else:
+            raise ValueError(f"Response y must be 1D or 2D array. Got {y_arr.ndim}D array.")
+    dy = y_arr.shape[1]
+    if y_arr.shape[0] != x_arr.shape[0]:
+        raise ValueError(
+            f"Response y must have same number of samples as x. "
+            f"x has {x_arr.shape[0]} samples, y has {y_arr.shape[0]} samples.")
+    if x_arr.shape[0] == 0:
+        raise ValueError("Input x must have at least one sample.")
+    _check_finite(x_arr, "Input x", "cv_fastlpr")
+    _check_finite(y_arr, "Response y", "cv_fastlpr")
+
+    opts = set_defaults(options, order=0, regularization=1e-8, num_dof_sample=10,
+                        kernel_type="gaussian", y_type_out="mean", calc_dof=True,
+                        gcv_selection="first")
+    n, dims = x_arr.shape
+    if opts.get("accuracy") is None:
+        opts["accuracy"] = max(6 - int(np.ceil(np.log10(n))), 4)
+    order = int(opts["order"])
+    if order not in (0, 1, 2):
+        raise ValueError("Polynomial order must be 0, 1, or 2.")
+    if dims > 10:
+        raise ValueError(f"dx={dims} exceeds maximum supported dimension (10). "
+                         "High-dimensional regression suffers from the curse of dimensionality.")
+    elif dims > 6:
+        warnings.warn(f"dx={dims} is high-dimensional and may cause memory issues. "
+                      "Consider dimensionality reduction techniques.", UserWarning)
+    if dims > 3:
+        raise NotImplementedError("the GPU engine supports 1-3 dimensions")
+    if str(opts.get("kernel_type", "gaussian")).lower() not in ("gauss", "gaussian", "normal"):
+        raise NotImplementedError("the GPU engine implements the Gaussian kernel")
+    bandwidths = _prepare_bandwidths(h, dims, opts, _default_hrange(x_arr))
+    variance = str(opts.get("y_type_out", "mean")) == "variance"
+    fp64 = str(opts.get("precision", "fp32")).lower() == "fp64"
+    N = _grid_shape(opts, n, dims)
+    dev = x.device if _is_dev(x) else (y.device if _is_dev(y) else _device())
+    if not torch.cuda.is_available():
+        _device()
+
+    xd = (x_arr.to(dev) if isinstance(x_arr, torch.Tensor)
+          else torch.as_tensor(x_arr, device=dev)).to(torch.float64).contiguous()
+    y_is_complex = bool(y_arr.is_complex()) if isinstance(y_arr, torch.Tensor) \
+        else bool(np.iscomplexobj(y_arr))
+    yd = _y_channels(y_arr, dev)
+    y_raw_orig = yd[:, 0].clone() if variance else None
+    if variance:
+        yd = torch.log(yd + 1.0 / n)  # api.py:390-396
+    ny = yd.shape[1]
+    reg = float(opts["regularization"])
+    calc_dof = bool(opts.get("calc_dof", True))
+    xrange = opts.get("xrange", None)
+    flag_power2 = bool(opts.get("flag_power2", True))
+    lin_default = "linear" if (dims == 1 or n >= 5000) else "cubic"
+
+    if not calc_dof:
+        method = opts.get("interp_method", "linear")
+        if method != "linear":
+            raise NotImplementedError("only linear interpolation is implemented on the GPU")
+        fit = engine.run_grid(xd, yd, ny, None, 0, bandwidths, order, N, int(opts["accuracy"]),
+                              fp64, reg, xrange, flag_power2, with_ones_eps=False)
+        sel = 0
+        gcv_result = None
+        dof_sel = dof_err_sel = np.nan
+    else:
+        for key in ("dof_interp_method", "gcv_interp_method"):
+            if opts.get(key, lin_default) != "linear":
+                raise NotImplementedError(
+                    f"{key}='cubic' (the reference default for 2-D/3-D problems with fewer than "
+                    "5000 points) is not implemented on the GPU yet; pass "
+                    f"options['{key}']='linear'")
+        K = int(opts.get("num_dof_sample", 10))
+        rm = opts.get("random_matrix", None)
+        probes, pp = _probes(n, K, opts.get("dof_seed", 42), rm, dev)
+        K = probes.shape[1]
+        fit = engine.run_grid(xd, yd, ny, probes, K, bandwidths, order, N, int(opts["accuracy"]),
+                              fp64, reg, xrange, flag_power2, rss_y=yd, rss_cplx=y_is_complex)
+        dof_all, dof_se, pen_std, pen_mean = _hutchinson(fit.dot, pp, n, y_is_complex)
+        agg = str(opts.get("gcv_selection", "first"))
+        scores, errs = [], []
+        for ih in range(fit.h.shape[0]):
+            s_, e_ = _gcv(fit.rss[ih], n, dof_all[ih], pen_std[ih],
+                          None if pen_mean is None else pen_mean[ih], agg)
+            scores.append(s_)
+            errs.append(e_)
+        scores, errs = np.asarray(scores), np.asarray(errs)
+        sel = _one_se_rule(scores, errs, fit.h, True, float(opts.get("dstd", 0.0)))
+        idmin = int(np.argmin(scores))
+        gcv_result = {"bandwidths": fit.h, "gcv_m": scores, "gcv_sd": errs, "idmin": idmin,
+                      "id1se": sel, "h1se": fit.h[sel], "hmin": fit.h[idmin],
+                      "dof": dof_all, "dof_stderr": dof_se}
+        dof_sel, dof_err_sel = float(dof_all[sel]), float(dof_se[sel])
+    if fit.warn:
+        warnings.warn(fit.warn, stacklevel=2)
+
+    geo = fit.geo
+    grid_vals = fit.m_y[sel]                          # (ny, nodes)
+    out, expsum = engine.interp_eval(geo, xd, grid_vals, ny,
+                                     yvar=y_raw_orig if variance else None)
+    d_norm = None
+    if variance:
+        d_norm = 1.0 / ((1.0 / n) * float(expsum.item()))  # api.py:771-778
+        out = torch.exp(out) / d_norm
+        grid_vals = torch.exp(grid_vals) / d_norm
+    if y_is_complex:
+        yhat = torch.complex(out[:, 0::2], out[:, 1::2])
+    else:
+        yhat = out
+    if dy == 1:
+        yhat = yhat[:, 0]
+    s0 = fit.s0_all[sel].reshape(N)
+    interp = GridInterpolant(geo, geo.axes_raw, grid_vals.contiguous(), N, y_is_complex, dy,
+                             as_numpy, dy == 1)
+    if gcv_result is None:
+        gcv_result = {"bandwidths": fit.h, "gcv_m": np.array([np.nan]),
+                      "gcv_sd": np.array([np.nan]), "idmin": 0, "id1se": 0,
+                      "h1se": fit.h[0], "hmin": fit.h[0], "dof": np.array([np.nan]),
+                      "dof_stderr": np.array([np.nan])}
+    yref = y_raw_orig if variance else (torch.complex(yd[:, 0::2], yd[:, 1::2]) if y_is_complex
+                                        else yd)
+    resid = yref.reshape(-1) - yhat.reshape(-1) if dy == 1 or variance else \
+        (yref - yhat).reshape(-1)
+
+    def conv(t):
+        return t.cpu().numpy() if as_numpy else t
+
+    metadata = {"options": opts, "order": order, "bandwidths_tested": fit.h,
+                "fitted_at_samples": conv(yhat), "residuals": conv(resid),
+                "mean": geo.mean, "std": geo.std, "dof": dof_sel, "dof_stderr": dof_err_sel}
+    return RegressionOutput(yhat=conv(yhat), grid=tuple(geo.axes_raw), h=fit.h[sel],
+                            gcv_yhat=gcv_result, fpp_yhat=interp, metadata=metadata,
+                            dof=dof_sel, dof_stderr=dof_err_sel, d=d_norm, s0=conv(s0))

@@ -1,0 +1,522 @@
+// Grid-side kernels: kernel-moment taps, the separable banded convolution on
+// MFMA, the global-regularisation reduction and the per-node solves.
+//
+// The reference convolves every spread grid with every kernel-moment grid
+// K_h(u) u1^a u2^b ... through zero-padded FFTs (convolution.py:747-794,
+// 904-951).  Because the padded circular convolution never wraps on the data
+// support (L >= 2N) and the Gaussian kernel is a product of 1-D factors, the
+// same numbers are a sequence of 1-D banded Toeplitz products along each axis
+// of the unwrapped spread grid.  Each pass is a batched GEMM in which one
+// operand is the Toeplitz band, generated in LDS from a 1-D tap strip; the
+// GEMM runs on v_mfma_f32_16x16x4f32 (float32 path) or v_mfma_f64_16x16x4f64
+// (float64 path).
+#include "flpr_common.h"
+
+namespace {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+// Operands (both forms): lane l holds A[l & 15][l >> 4] and B[l >> 4][l & 15].
+// Results: register r of lane l is D[row(l, r)][l & 15] with
+//   f32 16x16x4: row = 4 (l >> 4) + r
+//   f64 16x16x4: row = (l >> 4) + 4 r
+// (both checked on gfx950 with a probe of known operands).
+template <>
+struct Mfma<float> {
+  typedef v4f acc;
+  __device__ static inline acc run(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+template <>
+struct Mfma<double> {
+  typedef v4d acc;
+  __device__ static inline acc run(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+
+// ---------------------------------------------------------------------------
+// taps
+// ---------------------------------------------------------------------------
+
+constexpr int TAP_THREADS = 256;
+
+__global__ __launch_bounds__(TAP_THREADS) void k_taps(FlprGeom g, const double* __restrict__ h,
+                                                      int dh, int npow, int nmax, double norm,
+                                                      double* __restrict__ tap64,
+                                                      float* __restrict__ tap32) {
+#pragma clang fp contract(off)
+  const int64_t total = (int64_t)dh * g.d * nmax;
+  const double inv_sqrt_2pi = 1.0 / sqrt(2.0 * M_PI);
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int t = (int)(idx % nmax);
+    const int64_t r = idx / nmax;
+    const int k = (int)(r % g.d);
+    const int b = (int)(r / g.d);
+    const int N = g.n_grid[k];
+    const int64_t o = ((int64_t)(b * g.d + k) * npow) * nmax + t;
+    if (t >= N) {
+      for (int a = 0; a < npow; ++a) {
+        tap64[o + (int64_t)a * nmax] = 0.0;
+        if (tap32) tap32[o + (int64_t)a * nmax] = 0.0f;
+      }
+      continue;
+    }
+    // numpy linspace(lo, hi, N): t*step + lo, last element exactly hi
+    const double lo = g.xmin_s[k], hi = g.xmax_s[k];
+    double xg = lo;
+    if (N > 1) {
+      const double step = (hi - lo) / (double)(N - 1);
+      xg = (double)t * step;
+      xg = xg + lo;
+      if (t == N - 1) xg = hi;
+    }
+    const double u = xg / h[b * g.d + k];  // convolution.py:443-447 (x / h)
+    double e = exp(-0.5 * (u * u));        // kernel.py:351-356
+    if (k == 0) {
+      double ph = 1.0;
+      for (int kk = 0; kk < g.d; ++kk) ph *= h[b * g.d + kk];
+      e = e * inv_sqrt_2pi;
+      e = e / ph;    // kernel.py:301-304 (divide by det h)
+      e = e / norm;  // nufftn_type1.py:709-711 (divide by the number of points)
+    }
+    double pw = 1.0;
+    for (int a = 0; a < npow; ++a) {
+      const double v = e * pw;
+      tap64[o + (int64_t)a * nmax] = v;
+      if (tap32) tap32[o + (int64_t)a * nmax] = (float)v;
+      pw = pw * u;  // design_matrix_optimized.py:268-274 (cumulative powers)
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// banded Toeplitz GEMM on MFMA
+// ---------------------------------------------------------------------------
+
+constexpr int CT = 256;  // 4 waves
+constexpr int TM = 64;   // block tile rows
+constexpr int TN = 64;   // block tile cols
+constexpr int KC = 32;   // K chunk staged in LDS
+constexpr int STRIP = TM + KC - 1;
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(CT) void k_conv(const T* __restrict__ in, T* __restrict__ out,
+                                             const T* __restrict__ tap,
+                                             const int64_t* __restrict__ jobs,
+                                             const int32_t* __restrict__ bands, int K, int nout,
+                                             int inner, int base, int ntn, int ntm) {
+  __shared__ T dense[(MODE == 0) ? TM * (KC + 1) : KC * (TN + 1)];
+  __shared__ T strip[STRIP + 1];
+  const int64_t* jb = jobs + 8 * (int64_t)blockIdx.y;
+  const int64_t in_off = jb[0], out_off = jb[1], tap_off = jb[2];
+  const int band = (int)jb[3], ro = (int)jb[4];
+  const int t_lo = bands[2 * band], t_hi = bands[2 * band + 1];
+  int bx = blockIdx.x;
+  const int tn = bx % ntn;
+  bx /= ntn;
+  int tm, o, M, N;
+  if (MODE == 0) {
+    tm = bx;
+    o = 0;
+    M = ro;
+    N = nout;
+    if (tm * TM >= M) return;
+  } else {
+    tm = bx % ntm;
+    o = bx / ntm;
+    M = nout;
+    N = inner;
+    if (o >= ro) return;
+  }
+  const int m0 = tm * TM, n0 = tn * TN;
+  if (n0 >= N) return;
+  const int i0 = (MODE == 0) ? n0 : m0;  // first output index along the convolved axis
+  int s_begin = i0 + base - t_hi;
+  int s_end = i0 + TM - 1 + base - t_lo;
+  s_begin = max(s_begin, 0);
+  s_end = min(s_end, K - 1);
+  const T* src = in + in_off + (MODE == 0 ? 0 : (int64_t)o * K * inner);
+  T* dst = out + out_off + (MODE == 0 ? 0 : (int64_t)o * nout * inner);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  typename Mfma<T>::acc acc[2][2];
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[rb][cb][j] = T(0);
+
+  for (int s0 = s_begin; s0 <= s_end; s0 += KC) {
+    if (MODE == 0) {
+      for (int e = threadIdx.x; e < TM * KC; e += CT) {
+        const int r = e / KC, kk = e % KC;
+        const int gr = m0 + r, gs = s0 + kk;
+        dense[r * (KC + 1) + kk] = (gr < M && gs <= s_end) ? src[(int64_t)gr * K + gs] : T(0);
+      }
+    } else {
+      for (int e = threadIdx.x; e < KC * TN; e += CT) {
+        const int kk = e / TN, c = e % TN;
+        const int gs = s0 + kk, gc = n0 + c;
+        dense[kk * (TN + 1) + c] = (gs <= s_end && gc < N) ? src[(int64_t)gs * inner + gc] : T(0);
+      }
+    }
+    const int tf = i0 + base - s0 - (KC - 1);
+    for (int q = threadIdx.x; q < STRIP; q += CT) {
+      const int t = tf + q;
+      strip[q] = (t >= t_lo && t <= t_hi) ? tap[tap_off + t] : T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KC; kk += 4) {
+      const int kl = kk + (lane >> 4);
+      T a[2], b[2];
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) {
+        const int r = wr + rb * 16 + (lane & 15);
+        a[rb] = (MODE == 0) ? dense[r * (KC + 1) + kl] : strip[r - kl + KC - 1];
+      }
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) {
+        const int c = wc + cb * 16 + (lane & 15);
+        b[cb] = (MODE == 0) ? strip[c - kl + KC - 1] : dense[kl * (TN + 1) + c];
+      }
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) acc[rb][cb] = Mfma<T>::run(a[rb], b[cb], acc[rb][cb]);
+    }
+    __syncthreads();
+  }
+  const int ld = (MODE == 0) ? nout : inner;
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int row = m0 + wr + rb * 16 + Mfma<T>::row(lane, j);
+        const int col = n0 + wc + cb * 16 + (lane & 15);
+        if (row < M && col < N) dst[(int64_t)row * ld + col] = acc[rb][cb][j];
+      }
+}
+
+// ---------------------------------------------------------------------------
+// largest (|) moment (|) over nodes and bandwidths
+// ---------------------------------------------------------------------------
+
+struct OffTab {
+  int64_t v[FLPR_MAX_MOMENTS + 1];
+};
+
+constexpr int RED_THREADS = 256;
+
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_maxmom(const T* __restrict__ src, int64_t per_h,
+                                                        int dh, OffTab offs, int nslab,
+                                                        int64_t nodes, int absval,
+                                                        uint64_t* __restrict__ key) {
+  __shared__ double sh[RED_THREADS / 64];
+  double mx = -INFINITY;
+  const int64_t total = (int64_t)dh * nslab * nodes;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t h = idx / ((int64_t)nslab * nodes);
+    const int64_t r = idx % ((int64_t)nslab * nodes);
+    const int k = (int)(r / nodes);
+    const int64_t node = r % nodes;
+    double v = (double)src[h * per_h + offs.v[k] + node];
+    if (absval) v = fabs(v);
+    mx = fmax(mx, v);
+  }
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = sh[0];
+    for (int w = 1; w < RED_THREADS / 64; ++w) r = fmax(r, sh[w]);
+    atomicMax((unsigned long long*)key, (unsigned long long)flpr_encode(r));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// per-node solves
+// ---------------------------------------------------------------------------
+
+struct SolveTab {
+  int64_t s_off[FLPR_MAX_TERMS * (FLPR_MAX_TERMS + 1) / 2];
+  int64_t t_off[FLPR_MAX_TERMS];
+};
+
+constexpr int SOLVE_THREADS = 256;
+
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// One thread per (node, bandwidth).  Order >= 1: Cholesky of S + lam I in
+// float64 (S + lam I is symmetric positive definite for lam > 0), then
+// z = (S + lam I)^-1 e1 and m_c = z . T_c for every channel (the intercept
+// e1' S^-1 T that the reference's Cramer formulas evaluate).  Order 0:
+// m_c = T_c / max(S + eps, thr).  Probe channels are reduced on the fly
+// against the adjoint-interpolated probes q: dot[h][k] += q_k(node) m_k(node).
+template <typename T, int NT>
+__global__ __launch_bounds__(SOLVE_THREADS) void k_solve(
+    const T* __restrict__ mom, int64_t per_h, int dh, int64_t nodes, SolveTab tab,
+    const uint64_t* __restrict__ key, double lam_scale, double eps_add, int nout, int cf,
+    double* __restrict__ m_out, int pf, int np, const double* __restrict__ q,
+    double* __restrict__ dot) {
+  const double kmax = flpr_decode(*key);
+  const int64_t total = (int64_t)dh * nodes;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t start = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // iterate so that every lane of a wave runs the same number of trips
+  const int64_t trips = (total + stride - 1) / stride;
+  for (int64_t it = 0; it < trips; ++it) {
+    const int64_t idx = start + it * stride;
+    const bool active = idx < total;
+    const int64_t h = active ? idx / nodes : -1;
+    const int64_t node = active ? idx % nodes : 0;
+    double z[NT];
+    double den = 1.0;
+    const T* b = mom + (active ? h : 0) * per_h;
+    if (active) {
+      if (NT == 1) {
+        const double s = (double)b[tab.s_off[0] + node] + eps_add;  // regression.py:127
+        const double thr = (kmax + eps_add) * lam_scale;             // regression.py:217
+        den = fmax(s, thr);
+        z[0] = 1.0;
+      } else {
+        const double lam = kmax * lam_scale;  // regression.py:243-267
+        double A[NT][NT];
+        int p = 0;
+#pragma unroll
+        for (int i = 0; i < NT; ++i)
+#pragma unroll
+          for (int j = i; j < NT; ++j) {
+            const double v = (double)b[tab.s_off[p] + node];
+            A[i][j] = v;
+            A[j][i] = v;
+            ++p;
+          }
+#pragma unroll
+        for (int i = 0; i < NT; ++i) A[i][i] += lam;
+        // Cholesky: lower triangle of A becomes L
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          double dsum = A[j][j];
+#pragma unroll
+          for (int k = 0; k < j; ++k) dsum -= A[j][k] * A[j][k];
+          const double ljj = sqrt(dsum);
+          A[j][j] = ljj;
+          const double inv = 1.0 / ljj;
+#pragma unroll
+          for (int i = j + 1; i < NT; ++i) {
+            double s = A[i][j];
+#pragma unroll
+            for (int k = 0; k < j; ++k) s -= A[i][k] * A[j][k];
+            A[i][j] = s * inv;
+          }
+        }
+        // L y = e1
+        double y[NT];
+#pragma unroll
+        for (int i = 0; i < NT; ++i) {
+          double s = (i == 0) ? 1.0 : 0.0;
+#pragma unroll
+          for (int k = 0; k < i; ++k) s -= A[i][k] * y[k];
+          y[i] = s / A[i][i];
+        }
+        // L' z = y
+#pragma unroll
+        for (int i = NT - 1; i >= 0; --i) {
+          double s = y[i];
+#pragma unroll
+          for (int k = i + 1; k < NT; ++k) s -= A[k][i] * z[k];
+          z[i] = s / A[i][i];
+        }
+      }
+      for (int c = cf; c < cf + nout; ++c) {
+        double m = 0.0;
+        if (NT == 1) {
+          m = (double)b[tab.t_off[0] + (int64_t)c * nodes + node] / den;
+        } else {
+#pragma unroll
+          for (int i = 0; i < NT; ++i) m += z[i] * (double)b[tab.t_off[i] + (int64_t)c * nodes + node];
+        }
+        m_out[((int64_t)h * nout + (c - cf)) * nodes + node] = m;
+      }
+    }
+    if (np > 0 && q) {
+      // the wave shares one bandwidth when its first and last lanes do
+      const int64_t h_first = __shfl(h, 0);
+      const int64_t h_last = __shfl(h, 63);
+      const bool uniform = (h_first == h_last) && (h_first >= 0);
+      for (int kp = 0; kp < np; ++kp) {
+        double contrib = 0.0;
+        if (active) {
+          const int c = pf + kp;
+          double m = 0.0;
+          if (NT == 1) {
+            m = (double)b[tab.t_off[0] + (int64_t)c * nodes + node] / den;
+          } else {
+#pragma unroll
+            for (int i = 0; i < NT; ++i)
+              m += z[i] * (double)b[tab.t_off[i] + (int64_t)c * nodes + node];
+          }
+          contrib = q[(int64_t)kp * nodes + node] * m;
+        }
+        if (uniform) {
+          const double s = wave_sum(contrib);
+          if ((threadIdx.x & 63) == 0) atomicAdd(&dot[h_first * np + kp], s);
+        } else if (active) {
+          atomicAdd(&dot[h * np + kp], contrib);
+        }
+      }
+    }
+  }
+}
+
+template <typename T, int NT>
+void launch_solve(hipStream_t st, int blocks, const T* mom, int64_t per_h, int dh, int64_t nodes,
+                  const SolveTab& tab, const uint64_t* key, double lam_scale, double eps_add,
+                  int nout, int cf, double* m_out, int pf, int np, const double* q, double* dot) {
+  hipLaunchKernelGGL((k_solve<T, NT>), dim3(blocks), dim3(SOLVE_THREADS), 0, st, mom, per_h, dh,
+                     nodes, tab, key, lam_scale, eps_add, nout, cf, m_out, pf, np, q, dot);
+}
+
+template <typename T>
+int dispatch_solve(int nt, hipStream_t st, int blocks, const T* mom, int64_t per_h, int dh,
+                   int64_t nodes, const SolveTab& tab, const uint64_t* key, double lam_scale,
+                   double eps_add, int nout, int cf, double* m_out, int pf, int np,
+                   const double* q, double* dot) {
+#define FLPR_SOLVE_CASE(N)                                                                  \
+  case N:                                                                                   \
+    launch_solve<T, N>(st, blocks, mom, per_h, dh, nodes, tab, key, lam_scale, eps_add, nout, \
+                       cf, m_out, pf, np, q, dot);                                          \
+    return 0;
+  switch (nt) {
+    FLPR_SOLVE_CASE(1)
+    FLPR_SOLVE_CASE(2)
+    FLPR_SOLVE_CASE(3)
+    FLPR_SOLVE_CASE(4)
+    FLPR_SOLVE_CASE(6)
+    FLPR_SOLVE_CASE(10)
+    default:
+      return -1;
+  }
+#undef FLPR_SOLVE_CASE
+}
+
+}  // namespace
+
+extern "C" {
+
+int flpr_taps(const FlprGeom* g, const double* h, int32_t dh, int32_t npow, int32_t nmax,
+              double norm, double* tap64, float* tap32, void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!flpr_geom_ok(g) || !h || !tap64 || dh < 1 || npow < 1 || nmax < 1 || norm <= 0) return -1;
+  for (int k = 0; k < g->d; ++k)
+    if (g->n_grid[k] > nmax) return -3;
+  const int64_t total = (int64_t)dh * g->d * nmax;
+  hipLaunchKernelGGL(k_taps, dim3(flpr_grid_for(total, TAP_THREADS, 4096)), dim3(TAP_THREADS), 0,
+                     as_stream(stream), *g, h, dh, npow, nmax, norm, tap64, tap32);
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+int flpr_conv(int32_t mode, int32_t fp64, const void* in, void* out, const void* tap,
+              const int64_t* jobs, const int32_t* bands, int32_t nj, int32_t max_rows_or_outer,
+              int32_t K, int32_t nout, int32_t inner, int32_t base, void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!in || !out || !tap || !jobs || !bands || nj < 1 || nj > 65535 || max_rows_or_outer < 1 || K < 1 ||
+      nout < 1 || (mode != 0 && mode != 1) || (mode == 1 && inner < 1))
+    return -1;
+  const int ntn = (mode == 0) ? (nout + TN - 1) / TN : (inner + TN - 1) / TN;
+  const int ntm = (mode == 0) ? (max_rows_or_outer + TM - 1) / TM : (nout + TM - 1) / TM;
+  const int64_t gx = (mode == 0) ? (int64_t)ntn * ntm : (int64_t)ntn * ntm * max_rows_or_outer;
+  if (gx > INT32_MAX) return -1;
+  dim3 grd((unsigned)gx, (unsigned)nj);
+  hipStream_t st = as_stream(stream);
+  if (fp64) {
+    if (mode == 0)
+      hipLaunchKernelGGL((k_conv<double, 0>), grd, dim3(CT), 0, st, (const double*)in,
+                         (double*)out, (const double*)tap, jobs, bands, K, nout, inner, base, ntn, ntm);
+    else
+      hipLaunchKernelGGL((k_conv<double, 1>), grd, dim3(CT), 0, st, (const double*)in,
+                         (double*)out, (const double*)tap, jobs, bands, K, nout, inner, base, ntn, ntm);
+  } else {
+    if (mode == 0)
+      hipLaunchKernelGGL((k_conv<float, 0>), grd, dim3(CT), 0, st, (const float*)in, (float*)out,
+                         (const float*)tap, jobs, bands, K, nout, inner, base, ntn, ntm);
+    else
+      hipLaunchKernelGGL((k_conv<float, 1>), grd, dim3(CT), 0, st, (const float*)in, (float*)out,
+                         (const float*)tap, jobs, bands, K, nout, inner, base, ntn, ntm);
+  }
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+int flpr_max_moment(int32_t fp64, const void* src, int64_t per_h, int32_t dh, const int64_t* offs,
+                    int32_t nslab, int64_t nodes, int32_t absval, uint64_t* key, void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!src || !offs || !key || dh < 1 || nslab < 1 || nslab > FLPR_MAX_MOMENTS || nodes < 1)
+    return -1;
+  OffTab t;
+  for (int k = 0; k < nslab; ++k) t.v[k] = offs[k];
+  const int64_t total = (int64_t)dh * nslab * nodes;
+  const int blocks = flpr_grid_for(total, RED_THREADS, 2048);
+  if (fp64)
+    hipLaunchKernelGGL((k_maxmom<double>), dim3(blocks), dim3(RED_THREADS), 0, as_stream(stream),
+                       (const double*)src, per_h, dh, t, nslab, nodes, absval, key);
+  else
+    hipLaunchKernelGGL((k_maxmom<float>), dim3(blocks), dim3(RED_THREADS), 0, as_stream(stream),
+                       (const float*)src, per_h, dh, t, nslab, nodes, absval, key);
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+double flpr_key_to_double(uint64_t key) { return flpr_decode(key); }
+
+int flpr_solve(int32_t fp64, const void* mom, int64_t per_h, int32_t dh, int64_t nodes,
+               int32_t order, int32_t nt, const int64_t* s_off, const int64_t* t_off,
+               const uint64_t* key, double lam_scale, double eps_add, int32_t n_out_chan,
+               int32_t chan_first, double* m_out, int32_t probe_first, int32_t nprobe,
+               const double* q, double* dot, void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!mom || !s_off || !t_off || !key || dh < 1 || nodes < 1 || nt < 1 || nt > FLPR_MAX_TERMS)
+    return -1;
+  if ((order == 0) != (nt == 1)) return -1;
+  if (n_out_chan > 0 && !m_out) return -1;
+  if (nprobe > 0 && (!q || !dot)) return -1;
+  SolveTab tab;
+  const int ns = nt * (nt + 1) / 2;
+  for (int i = 0; i < ns; ++i) tab.s_off[i] = s_off[i];
+  for (int i = 0; i < nt; ++i) tab.t_off[i] = t_off[i];
+  const int64_t total = (int64_t)dh * nodes;
+  const int blocks = flpr_grid_for(total, SOLVE_THREADS, 8192);
+  hipStream_t st = as_stream(stream);
+  int rc;
+  if (fp64)
+    rc = dispatch_solve<double>(nt, st, blocks, (const double*)mom, per_h, dh, nodes, tab, key,
+                                lam_scale, eps_add, n_out_chan, chan_first, m_out, probe_first,
+                                nprobe, q, dot);
+  else
+    rc = dispatch_solve<float>(nt, st, blocks, (const float*)mom, per_h, dh, nodes, tab, key,
+                               lam_scale, eps_add, n_out_chan, chan_first, m_out, probe_first,
+                               nprobe, q, dot);
+  if (rc) return rc;
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,600 @@
+"""Device pipeline of one fit: the cv_fastlpr / cv_fastkde hot path on the GPU.
+
+Stages (each a C-ABI call into libfastlpr_amd.so, all on the current HIP stream):
+
+  1. flpr_xstats            min / max / mean / M2 of x          (model.py:58-68)
+  2. flpr_points_prepare    knot positions, spread cells, tiles (convolution.py:169-199,
+                                                                 nufftn_type1.py:140-176)
+     flpr_points_bin        counting sort of points by tile
+  3. flpr_spread            Gaussian gridding of 1, y, probes   (nufftn_type1.py:520-636)
+  4. flpr_taps              kernel x monomial taps per h, axis  (convolution.py:412-512)
+     flpr_conv (x d)        separable banded convolution passes (convolution.py:904-951)
+  5. flpr_max_moment        global regularisation key           (regression.py:217, 243-267)
+  6. flpr_adjoint_interp    probe adjoint on the raw grid       (regression.py:1256-1280)
+     flpr_solve             per-node solve + probe dot products (regression.py:162-277)
+  7. flpr_interp_rss        residual sums per bandwidth         (api.py:686-723)
+  8. flpr_interp_eval       final yhat at the data points       (api.py:760-786)
+
+The host computes only O(d + dh) scalars: the geometry from the x
+statistics, the bandwidth filter and band limits, and, after one
+device-to-host copy of dh x (ncol + nprobe) sums, the Hutchinson DOF, GCV and
+the 1-SE selection.
+
+Layout on the device
+--------------------
+* spread grid: (C, G_0, ..., G_{d-1}) float64, C = [1] + y columns + probes,
+  G_k = 2 E_k + 1 cells covering signed cell positions [-E_k, E_k] (the
+  reference's circular FFT grid unwrapped; it never wraps on the data support).
+* stage-k moment slabs (after convolving axes k..d-1): for every bandwidth and
+  every partial power tuple beta, shape (C_beta, G_0..G_{k-1}, N_k..N_{d-1}),
+  where C_beta = C when sum(beta) <= order (the T channels need it) and 1
+  (only the ones channel, for S) otherwise.
+* stage-0 slabs: (C_alpha, nodes) per bandwidth and monomial alpha, nodes
+  row-major over (N_0, ..., N_{d-1}).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import FlprGeom, call, i64_array
+
+TILE = {1: 256, 2: 16, 3: 8}
+# |u| beyond which the Gaussian taps are dropped: exp(-u^2/2) u^4 relative to the
+# peak is 1.7e-14 at u = 9 (float32 path) and 1e-27 at u = 12 (float64 path).
+BAND_CUT = {False: 9.0, True: 12.0}
+# intermediate (stage >= 1) memory budget per bandwidth chunk, bytes
+STAGE_BUDGET = 6 << 30
+
+
+def stream_handle() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+class KernelTimer:
+    """Optional HIP-event timing of named C-ABI launches on the current stream.
+
+    Disabled by default.  When enabled, every timed call records an event pair
+    around the launch; ``collect()`` synchronises and returns per-name lists of
+    milliseconds together with the algorithmic work attached to each launch."""
+
+    def __init__(self):
+        self.enabled = False
+        self._pending = []
+
+    def start(self):
+        self.enabled = True
+        self._pending = []
+
+    def stop(self):
+        self.enabled = False
+
+    def timed(self, name: str, work: float, fn, *args):
+        if not self.enabled:
+            return fn(*args)
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        out = fn(*args)
+        b.record()
+        self._pending.append((name, work, a, b))
+        return out
+
+    def collect(self):
+        torch.cuda.synchronize()
+        res: Dict[str, list] = {}
+        for name, work, a, b in self._pending:
+            res.setdefault(name, []).append((a.elapsed_time(b), work))
+        self._pending = []
+        return res
+
+
+TIMER = KernelTimer()
+
+
+def ptr(t: Optional[torch.Tensor]):
+    return None if t is None else t.data_ptr()
+
+
+# ---------------------------------------------------------------------------
+# polynomial bookkeeping (design_matrix_optimized.py:30-82, convolution.py:488-497)
+# ---------------------------------------------------------------------------
+
+def poly_powers(d: int, order: int) -> List[Tuple[int, ...]]:
+    """Monomial exponents in the reference's term order."""
+    if order == 0:
+        return [(0,) * d]
+    out: List[Tuple[int, ...]] = []
+    for tot in range(order + 1):
+        if d == 1:
+            out.append((tot,))
+        elif d == 2:
+            for px in range(tot, -1, -1):
+                out.append((px, tot - px))
+        else:
+            for px in range(tot, -1, -1):
+                for py in range(tot - px, -1, -1):
+                    out.append((px, py, tot - px - py))
+    return out
+
+
+def pair_list(nt: int) -> List[Tuple[int, int]]:
+    return [(i, j) for i in range(nt) for j in range(i, nt)]
+
+
+# ---------------------------------------------------------------------------
+# static plan: pass structure, slab offsets, conv job tables
+# ---------------------------------------------------------------------------
+
+@dataclass
+class Pass:
+    axis: int
+    mode: int            # 0 ROW (contiguous axis), 1 COL (strided axis)
+    K: int               # input axis length (G_axis)
+    nout: int            # output axis length (N_axis)
+    inner: int           # contiguous extent after the axis (mode 1)
+    base: int            # tap index = i + base - s
+    max_ro: int
+    jobs: torch.Tensor   # (nj, 8) int64 on device
+    in_stage: int
+    out_stage: int
+    jobs_host: np.ndarray = None
+
+
+def conv_flops(ps: "Pass", bands: np.ndarray) -> float:
+    """Algorithmic flops of one convolution pass: 2 per (output element, tap
+    inside the band whose input cell lies on the grid)."""
+    jh = ps.jobs_host
+    i = np.arange(ps.nout)
+    total = 0.0
+    cache: Dict[int, float] = {}
+    for rec in jh:
+        band, ro = int(rec[3]), int(rec[4])
+        per = cache.get(band)
+        if per is None:
+            lo, hi = int(bands[2 * band]), int(bands[2 * band + 1])
+            t0 = np.maximum(lo, i + ps.base - (ps.K - 1))
+            t1 = np.minimum(hi, i + ps.base)
+            per = float(np.sum(np.maximum(t1 - t0 + 1, 0)))
+            cache[band] = per
+        total += 2.0 * per * ro * (ps.inner if ps.mode == 1 else 1)
+    return total
+
+
+class Plan:
+    """Everything that depends only on (d, order, N, channels, dh chunk, precision)."""
+
+    def __init__(self, d: int, order: int, N: Tuple[int, ...], L: Tuple[int, ...], msp: int,
+                 C: int, dh: int, fp64: bool, device):
+        self.d, self.order, self.N, self.L, self.msp = d, order, tuple(N), tuple(L), msp
+        self.C, self.dh, self.fp64, self.device = C, dh, fp64, device
+        self.E = tuple(n // 2 + 1 + msp for n in N)
+        self.G = tuple(2 * e + 1 for e in self.E)
+        self.nodes = int(np.prod(N))
+        self.npow = 2 * order + 1
+        self.nmax = int(max(N))
+        self.powers = poly_powers(d, order)
+        self.nt = len(self.powers)
+        # stage slabs: stage k holds partial tuples over axes k..d-1
+        self.stage_slabs: List[Dict[Tuple[int, ...], Tuple[int, int]]] = [None] * (d + 1)
+        self.stage_per_h: List[int] = [0] * (d + 1)
+        for k in range(d, -1, -1):
+            if k == d:
+                tuples = [()]
+            else:
+                tuples = []
+                for beta in self.stage_slabs[k + 1]:
+                    for a in range(2 * order - sum(beta) + 1):
+                        tuples.append((a,) + beta)
+            off = 0
+            slabs = {}
+            for beta in tuples:
+                cb = C if sum(beta) <= order else 1
+                size = cb * self._slab_cells(k)
+                slabs[beta] = (off, cb)
+                off += size
+            self.stage_slabs[k] = slabs
+            self.stage_per_h[k] = off
+        # chunking of bandwidths for the intermediate stages
+        inter = max([self.stage_per_h[k] for k in range(1, d)] + [0])
+        esz = 8 if fp64 else 4
+        self.chunk = dh if inter == 0 else max(1, min(dh, STAGE_BUDGET // max(1, 2 * inter * esz)))
+        self.passes: List[List[Pass]] = []  # one list per chunk start
+        self.chunks = [(h0, min(dh, h0 + self.chunk)) for h0 in range(0, dh, self.chunk)]
+        self.pass_tables = [self._build_passes(h0, h1) for (h0, h1) in self.chunks]
+        # stage-0 offsets for the solve
+        s0 = self.stage_slabs[0]
+        self.s_off = []
+        for i, j in pair_list(self.nt):
+            alpha = tuple(self.powers[i][k] + self.powers[j][k] for k in range(d))
+            self.s_off.append(s0[alpha][0])
+        self.t_off = [s0[p][0] for p in self.powers]
+        self.diag_off = []
+        for p in self.powers:
+            self.diag_off.append(s0[tuple(2 * v for v in p)][0])
+
+    def _slab_cells(self, k: int) -> int:
+        cells = 1
+        for j in range(self.d):
+            cells *= self.G[j] if j < k else self.N[j]
+        return cells
+
+    def _build_passes(self, h0: int, h1: int) -> List[Pass]:
+        d = self.d
+        out: List[Pass] = []
+        for k in range(d, 0, -1):
+            axis = k - 1
+            mode = 0 if axis == d - 1 else 1
+            K = self.G[axis]
+            nout = self.N[axis]
+            inner = 1
+            for j in range(axis + 1, d):
+                inner *= self.N[j]
+            prefix = 1
+            for j in range(axis):
+                prefix *= self.G[j]
+            L, N = self.L[axis], self.N[axis]
+            qout0 = -(-(L - N) // 2)
+            pad_before = (L - N) // 2 + 1
+            base = (qout0 - pad_before) + self.E[axis]
+            jobs = []
+            max_ro = 1
+            in_per_h = self.stage_per_h[k]
+            out_per_h = self.stage_per_h[k - 1]
+            for hl in range(h1 - h0):
+                hg = h0 + hl
+                for beta, (ioff, _icb) in self.stage_slabs[k].items():
+                    for a in range(2 * self.order - sum(beta) + 1):
+                        nb = (a,) + beta
+                        ooff, ocb = self.stage_slabs[k - 1][nb]
+                        ro = ocb * prefix
+                        max_ro = max(max_ro, ro)
+                        # stage d (spread grid) has no bandwidth axis; stage 0 is
+                        # indexed by the global bandwidth; other stages by chunk-local h
+                        in_h = 0 if k == d else hl
+                        out_h = hg if k - 1 == 0 else hl
+                        tap_off = ((hg * d + axis) * self.npow + a) * self.nmax
+                        jobs.append([in_h * in_per_h + ioff, out_h * out_per_h + ooff, tap_off,
+                                     hg * d + axis, ro, 0, 0, 0])
+            jh = np.asarray(jobs, dtype=np.int64)
+            jt = torch.tensor(jh, device=self.device)
+            out.append(Pass(axis, mode, K, nout, inner, base, max_ro, jt, k, k - 1, jh))
+        return out
+
+
+_PLANS: Dict[tuple, Plan] = {}
+
+
+def get_plan(d, order, N, L, msp, C, dh, fp64, device) -> Plan:
+    key = (d, order, tuple(N), tuple(L), msp, C, dh, fp64, str(device))
+    p = _PLANS.get(key)
+    if p is None:
+        p = Plan(d, order, N, L, msp, C, dh, fp64, device)
+        _PLANS[key] = p
+    return p
+
+
+# ---------------------------------------------------------------------------
+# geometry from the x statistics
+# ---------------------------------------------------------------------------
+
+@dataclass
+class Stats:
+    xmin: np.ndarray
+    xmax: np.ndarray
+    mean: np.ndarray
+    std: np.ndarray
+
+
+def x_stats(x: torch.Tensor) -> Stats:
+    """Per-dimension min, max, mean and std(ddof=1) of x (n, d) float64 on the device."""
+    n, d = x.shape
+    dev = x.device
+    part = torch.empty(5 * d * 512, dtype=torch.float64, device=dev)
+    out = torch.empty(5 * d, dtype=torch.float64, device=dev)
+    call("flpr_xstats", ptr(x), n, d, ptr(part), ptr(out), stream_handle())
+    o = out.cpu().numpy().reshape(d, 5)
+    std = np.sqrt(o[:, 3] / (n - 1)) if n > 1 else np.zeros(d)
+    std = np.where(std == 0, 1.0, std)
+    return Stats(o[:, 0].copy(), o[:, 1].copy(), o[:, 2].copy(), std)
+
+
+def padded_sizes(N: Tuple[int, ...], flag_power2: bool = True) -> Tuple[int, ...]:
+    """L = 2^ceil(log2(2N-1)) (convolution.py:390-395)."""
+    if not flag_power2:
+        raise NotImplementedError("flag_power2=False (L = N + 2) is not supported on the GPU path")
+    return tuple(int(2 ** math.ceil(math.log2(2 * n - 1))) for n in N)
+
+
+@dataclass
+class Geometry:
+    geom: FlprGeom
+    N: Tuple[int, ...]
+    L: Tuple[int, ...]
+    msp: int
+    xmin_s: np.ndarray
+    xmax_s: np.ndarray
+    axes_raw: List[np.ndarray]
+    axes_scaled: List[np.ndarray]
+    mean: np.ndarray
+    std: np.ndarray
+    centre: np.ndarray
+
+
+def build_geometry(st: Stats, N: Tuple[int, ...], accuracy: int, xrange=None,
+                   flag_power2: bool = True) -> Geometry:
+    """Knot-mapping constants exactly as the reference computes them in float64
+    (model.py:58-72, convolution.py:169-199, nufftn_type1.py:140-176,
+    kernel.py:97-156); the extreme points give the min / max of every
+    monotone float64 step, so no extra pass over the data is needed."""
+    d = len(N)
+    if accuracy > 6:
+        raise NotImplementedError("accuracy > 6 (oversampled NUFFT with deconvolution) is not "
+                                  "supported on the GPU path")
+    msp = int(accuracy)
+    if msp < 4:
+        raise NotImplementedError("accuracy < 4 is not supported on the GPU path")
+    L = padded_sizes(N, flag_power2)
+    g = FlprGeom()
+    g.d = d
+    g.msp = msp
+    centre = np.zeros(d)
+    xmin_s = np.zeros(d)
+    xmax_s = np.zeros(d)
+    axes_raw, axes_scaled = [], []
+    for k in range(d):
+        n_k, L_k = int(N[k]), int(L[k])
+        mean, std = float(st.mean[k]), float(st.std[k])
+        tmin = (float(st.xmin[k]) - mean) / std
+        tmax = (float(st.xmax[k]) - mean) / std
+        c = (tmax + tmin) / 2
+        lo, hi = tmin - c, tmax - c
+        scale = hi - lo
+        if scale == 0:
+            scale = 1.0
+        idx = np.linspace(-0.5, 0.5 - 1 / L_k, L_k)
+        q0 = (L_k - n_k) // 2
+        q1 = q0 + n_k - 1
+        idx0 = float(idx[q0])
+        ks = float(idx[q1] - idx[q0])
+        kmin_pt = (idx0 + 0.5) - 0.5
+        kmax_pt = ((ks + idx0) + 0.5) - 0.5
+        kmin = (kmin_pt + kmax_pt) * 0.5 - 0.5
+        shift = -0.5 - kmin
+        tau = (math.pi * msp) / (L_k * L_k * 1 * (1 - 0.5))
+        E = n_k // 2 + 1 + msp
+        G = 2 * E + 1
+        tile = TILE[d]
+        g.n_grid[k] = n_k
+        g.L[k] = L_k
+        g.E[k] = E
+        g.G[k] = G
+        g.tile[k] = tile
+        g.ntile[k] = -(-G // tile)
+        g.hx[k] = 2 * np.pi / L_k
+        g.inv4tau[k] = 1.0 / (4 * tau)
+        g.mean[k] = mean
+        g.std_[k] = std
+        g.centre[k] = c
+        g.xmin_s[k] = lo
+        g.scale_s[k] = scale
+        g.xmax_s[k] = hi
+        g.ks[k] = ks
+        g.idx0[k] = idx0
+        g.shift[k] = shift
+        if xrange is None:
+            g0, g1 = float(st.xmin[k]), float(st.xmax[k])
+        else:
+            g0, g1 = float(xrange[k][0]), float(xrange[k][1])
+        g.graw0[k] = g0
+        g.graw1[k] = g1
+        centre[k], xmin_s[k], xmax_s[k] = c, lo, hi
+        axes_raw.append(np.linspace(g0, g1, n_k))
+        axes_scaled.append(np.linspace(lo, hi, n_k))
+    return Geometry(g, tuple(N), L, msp, xmin_s, xmax_s, axes_raw, axes_scaled,
+                    st.mean.copy(), st.std.copy(), centre)
+
+
+def filter_bandwidths(geo: Geometry, h: np.ndarray, n: int):
+    """Too-small bandwidths (kernel sum < 10 eps) are dropped; all bad -> Silverman
+    (convolution.py:542-605).  The d-dimensional kernel sum is the product of
+    1-D sums because the Gaussian factorises."""
+    d = len(geo.N)
+    sums = np.ones(h.shape[0])
+    for k in range(d):
+        u = geo.axes_scaled[k][None, :] / h[:, k:k + 1]
+        sums *= np.sum(np.exp(-0.5 * u * u), axis=1)
+    sums = sums / math.sqrt(2 * math.pi) / np.prod(h, axis=1)
+    ihbad = np.abs(sums) < np.finfo(np.float64).eps * 10
+    warn = None
+    if np.any(ihbad) and not np.all(ihbad):
+        warn = f"{int(np.sum(ihbad))} bandwidth(s) are too small for this grid and will be removed"
+        h = h[~ihbad]
+    elif np.all(ihbad):
+        hs = (4.0 / (d + 2) / n) ** (1.0 / (d + 4))
+        warn = f"All bandwidths too small. Using Silverman's rule: h = {hs:.4f}"
+        h = np.array([[hs] * d])
+    return h, ihbad, warn
+
+
+def tap_bands(geo: Geometry, h: np.ndarray, fp64: bool) -> np.ndarray:
+    """[t_lo, t_hi] per (bandwidth, axis): taps with |u| <= cut."""
+    d = len(geo.N)
+    cut = BAND_CUT[fp64]
+    out = np.zeros((h.shape[0], d, 2), dtype=np.int32)
+    for b in range(h.shape[0]):
+        for k in range(d):
+            u = np.abs(geo.axes_scaled[k] / h[b, k])
+            idx = np.nonzero(u <= cut)[0]
+            if idx.size:
+                out[b, k] = (max(0, idx[0] - 1), min(len(u) - 1, idx[-1] + 1))
+            else:
+                out[b, k] = (0, -1)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# the fit
+# ---------------------------------------------------------------------------
+
+@dataclass
+class DeviceFit:
+    h: np.ndarray                    # filtered bandwidths (dh, d)
+    m_y: torch.Tensor                # (dh, ny, nodes) float64
+    rss: Optional[np.ndarray]        # (dh, ncol_out)
+    dot: Optional[np.ndarray]        # (dh, K)
+    s0_all: torch.Tensor             # (dh, nodes) S_00 (float64)
+    geo: Geometry
+    plan: Plan
+    warn: Optional[str]
+
+
+class Workspace:
+    """Per-fit device buffers (allocated through torch's caching allocator)."""
+
+    def __init__(self, device):
+        self.device = device
+
+    def empty(self, shape, dtype):
+        return torch.empty(shape, dtype=dtype, device=self.device)
+
+    def zeros(self, shape, dtype):
+        return torch.zeros(shape, dtype=dtype, device=self.device)
+
+
+def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
+             probes: Optional[torch.Tensor], K: int, h: np.ndarray, order: int,
+             N: Tuple[int, ...], accuracy: int, fp64: bool, regularization: float,
+             xrange=None, flag_power2: bool = True, rss_y: Optional[torch.Tensor] = None,
+             rss_cplx: bool = False, stats: Optional[Stats] = None,
+             with_ones_eps: bool = True, debug: Optional[dict] = None) -> DeviceFit:
+    """Spread, convolve and solve for all bandwidths, plus the probe dot products and
+    the residual sums.
+
+    x: (n, d) float64 on the device; vals_y: (n, ny) float64 or None (complex y is
+    passed as interleaved (re, im) columns); probes: (n, K) float64 or None;
+    rss_y: the (n, ny) float64 responses against which residual sums are formed
+    (None to skip).  with_ones_eps adds float64 eps to S for order 0
+    (regression.py:127; the calc_dof=False path of the reference omits it).
+    debug: when a dict is given, the intermediate device buffers are stored in it.
+    """
+    n, d = x.shape
+    dev = x.device
+    s = stream_handle()
+    st = stats if stats is not None else x_stats(x)
+    geo = build_geometry(st, N, accuracy, xrange, flag_power2)
+    h, _ihbad, warn = filter_bandwidths(geo, np.asarray(h, dtype=float), n)
+    dh = h.shape[0]
+    C = 1 + ny + K
+    plan = get_plan(d, order, geo.N, geo.L, geo.msp, C, dh, fp64, dev)
+    g = geo.geom
+    gp = _lib.C.byref(g)
+    f64, i32 = torch.float64, torch.int32
+
+    # --- points: knots, tiles, binning
+    ntiles = int(np.prod([g.ntile[k] for k in range(d)]))
+    xmod = torch.empty((n, d), dtype=f64, device=dev)
+    cell = torch.empty((n, d), dtype=i32, device=dev)
+    tile = torch.empty((n,), dtype=i32, device=dev)
+    cnt = torch.zeros((2 * ntiles,), dtype=i32, device=dev)
+    tile_count, tile_cursor = cnt[:ntiles], cnt[ntiles:]
+    tile_start = torch.empty((ntiles + 1,), dtype=i32, device=dev)
+    order_idx = torch.empty((n,), dtype=i32, device=dev)
+    call("flpr_points_prepare", ptr(x), n, gp, ptr(xmod), ptr(cell), ptr(tile), ptr(tile_count), s)
+    call("flpr_points_bin", ptr(tile), n, ntiles, ptr(tile_count), ptr(tile_start),
+         ptr(tile_cursor), ptr(order_idx), s)
+
+    # --- spread (float64 accumulation)
+    cells = int(np.prod(plan.G))
+    grid64 = torch.zeros((C * cells,), dtype=f64, device=dev)
+    call("flpr_spread", gp, ptr(xmod), ptr(cell), ptr(order_idx), ptr(tile_start), 1,
+         ptr(vals_y), ny, ptr(probes), K, ptr(grid64), s)
+    fdt = f64 if fp64 else torch.float32
+    if fp64:
+        grid = grid64
+    else:
+        grid = torch.empty((C * cells,), dtype=fdt, device=dev)
+        call("flpr_f64_to_f32", ptr(grid64), ptr(grid), C * cells, s)
+
+    # --- taps and bands
+    h_dev = torch.tensor(h, dtype=f64, device=dev)
+    ntap = dh * d * plan.npow * plan.nmax
+    tap64 = torch.empty((ntap,), dtype=f64, device=dev)
+    tap32 = None if fp64 else torch.empty((ntap,), dtype=torch.float32, device=dev)
+    call("flpr_taps", gp, ptr(h_dev), dh, plan.npow, plan.nmax, float(n), ptr(tap64), ptr(tap32), s)
+    tap = tap64 if fp64 else tap32
+    band_np = tap_bands(geo, h, fp64).reshape(-1)
+    bands = torch.tensor(band_np, dtype=i32, device=dev)
+
+    # --- convolution passes (bandwidth chunks share the intermediate buffers)
+    stage0 = torch.empty((dh * plan.stage_per_h[0],), dtype=fdt, device=dev)
+    inter = {k: torch.empty((plan.chunk * plan.stage_per_h[k],), dtype=fdt, device=dev)
+             for k in range(1, d)}
+    for ci in range(len(plan.chunks)):
+        for ps in plan.pass_tables[ci]:
+            src = grid if ps.in_stage == d else inter[ps.in_stage]
+            dst = stage0 if ps.out_stage == 0 else inter[ps.out_stage]
+            work = conv_flops(ps, band_np) if TIMER.enabled else 0.0
+            TIMER.timed("flpr_conv", work, call, "flpr_conv", ps.mode, int(fp64), ptr(src),
+                        ptr(dst), ptr(tap), ptr(ps.jobs), ptr(bands), int(ps.jobs.shape[0]),
+                        ps.max_ro, ps.K, ps.nout, ps.inner, ps.base, s)
+    del inter
+
+    # --- global regularisation key (regression.py:217, 243-267)
+    key = torch.zeros((1,), dtype=torch.int64, device=dev)
+    per_h0 = plan.stage_per_h[0]
+    if order == 0:
+        offs, absval = [plan.s_off[0]], 0
+        lam_scale = float(regularization)
+        eps_add = float(np.finfo(float).eps) if with_ones_eps else 0.0
+    else:
+        offs, absval = plan.diag_off, 1
+        lam_scale, eps_add = 1e-6, 0.0
+    call("flpr_max_moment", int(fp64), ptr(stage0), per_h0, dh, i64_array(offs), len(offs),
+         plan.nodes, absval, ptr(key), s)
+
+    # --- probe adjoint, solve (+ probe dot products), residual sums
+    nodes = plan.nodes
+    q = dot = None
+    if K > 0:
+        q = torch.zeros((K * nodes,), dtype=f64, device=dev)
+        call("flpr_adjoint_interp", gp, ptr(x), n, ptr(order_idx), ptr(tile_start), ptr(probes),
+             K, ptr(q), s)
+        dot = torch.zeros((dh * K,), dtype=f64, device=dev)
+    m_y = torch.empty((dh * max(ny, 1) * nodes,), dtype=f64, device=dev)
+    call("flpr_solve", int(fp64), ptr(stage0), per_h0, dh, nodes, order, plan.nt,
+         i64_array(plan.s_off), i64_array(plan.t_off), ptr(key), lam_scale, eps_add,
+         ny, 1, ptr(m_y) if ny > 0 else None, 1 + ny, K, ptr(q), ptr(dot), s)
+    rss = None
+    if rss_y is not None and ny > 0:
+        ncol_out = ny // 2 if rss_cplx else ny
+        rss = torch.zeros((dh * ncol_out,), dtype=f64, device=dev)
+        call("flpr_interp_rss", gp, ptr(x), n, ptr(order_idx), ptr(rss_y), ny, int(rss_cplx),
+             ptr(m_y), dh, ptr(rss), s)
+    off0 = plan.s_off[0]
+    s0_all = stage0.view(dh, per_h0)[:, off0:off0 + nodes].to(f64)
+    if order == 0:
+        s0_all = s0_all + eps_add
+    if debug is not None:
+        debug.update(grid64=grid64, stage0=stage0, key=key, q=q, dot=dot, m_y=m_y, plan=plan,
+                     geo=geo, h=h, order_idx=order_idx, tile_start=tile_start, xmod=xmod,
+                     cell=cell, lam_scale=lam_scale, eps_add=eps_add)
+    rss_h = None if rss is None else rss.cpu().numpy().reshape(dh, -1)
+    dot_h = None if dot is None else dot.cpu().numpy().reshape(dh, K)
+    return DeviceFit(h, m_y.view(dh, max(ny, 1), nodes), rss_h, dot_h, s0_all, geo, plan, warn)
+
+
+def interp_eval(geo: Geometry, x: torch.Tensor, m: torch.Tensor, ncol: int,
+                yvar: Optional[torch.Tensor] = None):
+    """Multilinear interpolation of (ncol, nodes) float64 grid values at x (n, d)."""
+    n = x.shape[0]
+    out = torch.empty((n, ncol), dtype=torch.float64, device=x.device)
+    es = torch.zeros((1,), dtype=torch.float64, device=x.device) if yvar is not None else None
+    call("flpr_interp_eval", _lib.C.byref(geo.geom), ptr(x), n, ptr(m.contiguous()), ncol,
+         ptr(out), ptr(yvar), ptr(es), stream_handle())
+    return out, es

@@ -1,0 +1,100 @@
+"""Host-side API behaviour that needs no GPU: validation errors, bandwidth
+lists, the 1-SE rule and the Hutchinson/GCV arithmetic, against the
+reference's documented semantics (api.py:32-164, 280-471; bandwidth.py:22-137)."""
+import numpy as np
+import pytest
+
+import fastlpr_amd
+from fastlpr_amd import api
+from oracle import restate as R
+
+
+def test_get_hlist_matches_oracle_and_closed_form():
+    h = fastlpr_amd.get_hlist(5, [0.1, 1.0])
+    assert h.shape == (5, 1)
+    np.testing.assert_allclose(h[:, 0], np.logspace(-1, 0, 5))
+    h2 = fastlpr_amd.get_hlist([3, 2], [[0.1, 1.0], [0.2, 2.0]])
+    # first dimension varies fastest (MATLAB ndgrid order)
+    np.testing.assert_allclose(h2[:3, 1], 0.2)
+    np.testing.assert_allclose(h2[3:, 1], 2.0)
+    np.testing.assert_array_equal(h2, R.get_hlist([3, 2], [[0.1, 1.0], [0.2, 2.0]]))
+    assert fastlpr_amd.get_hlist(1, [0.5, 0.5]).tolist() == [[0.5]]
+
+
+def test_prepare_bandwidths_shapes_and_errors():
+    rng = np.array([[0.1, 1.0], [0.1, 1.0]])
+    assert api._prepare_bandwidths(0.3, 2, {}, rng).shape == (1, 2)
+    assert api._prepare_bandwidths([0.3, 0.4], 2, {}, rng).shape == (1, 2)
+    assert api._prepare_bandwidths(None, 2, {}, rng).shape == (400, 2)
+    with pytest.raises(ValueError):
+        api._prepare_bandwidths([0.3], 2, {}, rng)
+    with pytest.raises(ValueError):
+        api._prepare_bandwidths([[0.1, -0.2]], 2, {}, rng)
+    with pytest.raises(ValueError):
+        api._prepare_bandwidths([[0.1, np.nan]], 2, {}, rng)
+
+
+def test_one_se_rule_matches_oracle_and_rules():
+    rs = np.random.default_rng(0)
+    for _ in range(50):
+        k = 12
+        s = rs.uniform(0.5, 1.5, k)
+        e = rs.uniform(0, 0.2, k)
+        s[rs.integers(0, k)] = np.nan
+        bw = np.sort(rs.uniform(0.01, 1, (k, 1)), axis=0)
+        for ps in (True, False):
+            for dstd in (0.0, 1.0, 3.0):
+                assert api._one_se_rule(s, e, bw, ps, dstd) == R.one_se_rule(s, e, bw, ps, dstd)
+    # dstd = 0 picks the minimum; all-NaN raises
+    assert api._one_se_rule(np.array([3.0, 1.0, 2.0]), np.zeros(3), np.ones((3, 1)), True, 0.0) == 1
+    with pytest.raises(ValueError):
+        api._one_se_rule(np.array([np.nan, np.inf]), np.zeros(2), np.ones((2, 1)), True)
+
+
+def test_hutchinson_matches_oracle():
+    rs = np.random.default_rng(1)
+    n, K, dh = 500, 10, 7
+    p = rs.standard_normal((n, K))
+    phat = 0.3 * p[:, :, None] + 0.01 * rs.standard_normal((n, K, dh))
+    for cplx in (False, True):
+        dof, se, ps, pm, pph, pp = R.hutchinson(p, phat, n, cplx)
+        d2, se2, ps2, pm2 = api._hutchinson(pph.T, pp, n, cplx)
+        np.testing.assert_allclose(d2, dof)
+        np.testing.assert_allclose(se2, se)
+        np.testing.assert_allclose(ps2, ps)
+        np.testing.assert_allclose(pm2, pm)
+
+
+def test_gcv_arithmetic_matches_oracle():
+    rs = np.random.default_rng(2)
+    y = rs.standard_normal((300, 2))
+    yh = y + 0.1 * rs.standard_normal((300, 2))
+    rss = np.sum((y - yh) ** 2, axis=0)
+    for agg in ("first", "average", "sum"):
+        for pm in (None, 1.7):
+            a = api._gcv(rss, 300, 12.0, 0.3, pm, agg)
+            b = R.gcv_score(y, yh, 12.0, 0.3, pm, agg)
+            np.testing.assert_allclose(a, b[:2])
+    with pytest.raises(ValueError):
+        api._gcv(rss, 300, 12.0, 0.3, None, "median")
+
+
+@pytest.mark.parametrize("bad", ["nan_x", "inf_y", "order", "shape", "dims"])
+def test_cv_fastlpr_validation_errors(bad):
+    x = np.random.default_rng(0).uniform(size=(50, 1))
+    y = np.sin(x[:, 0])
+    opts = {"order": 1}
+    if bad == "nan_x":
+        x = x.copy()
+        x[3, 0] = np.nan
+    elif bad == "inf_y":
+        y = y.copy()
+        y[2] = np.inf
+    elif bad == "order":
+        opts = {"order": 3}
+    elif bad == "shape":
+        y = y[:40]
+    elif bad == "dims":
+        x = np.random.default_rng(0).uniform(size=(50, 11))
+    with pytest.raises(ValueError):
+        fastlpr_amd.cv_fastlpr(x, y, [[0.2]] if bad != "dims" else None, opts)

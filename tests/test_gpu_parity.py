@@ -1,0 +1,131 @@
+"""GPU parity: the HIP path (through the C-ABI) against the oracle and the reference.
+
+* precision="fp64": float64 end to end; must equal the float64 oracle
+  (same discrete estimator, FFT convolution there, banded MFMA here) to 1e-8
+  relative on yhat / grid values / GCV, with identical selected index.
+* precision="fp32" (the default): float32 MFMA convolution, float64 spread and
+  solve; the gate is max|yhat - yhat*| / max|yhat*| <= 1e-5 against the float64
+  oracle.
+* golden fixtures: the reference's own outputs (float32 S/T/solve inside the
+  reference), compared at float32-level tolerance (5e-5) with identical 1-SE index.
+"""
+import json
+import os
+import warnings
+
+import numpy as np
+import pytest
+
+import recipes
+from oracle import restate as R
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.dirname(recipes.__file__)
+
+
+def _rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _cases():
+    rng = np.random.default_rng(7)
+    out = {}
+    n = 3000
+    x1 = rng.uniform(0, 1, (n, 1))
+    y1 = np.sin(2 * np.pi * x1[:, 0]) + 0.2 * rng.standard_normal(n)
+    h1 = np.logspace(np.log10(0.03), 0, 10)[:, None]
+    for o in (0, 1, 2):
+        out[f"1d_o{o}_m128"] = (x1, y1, h1, {"order": o, "N": 128})
+        out[f"1d_o{o}_m101"] = (x1, y1, h1, {"order": o, "N": 101})
+    out["1d_cplx_o1"] = (x1, (0.5 + 2j) * y1, h1, {"order": 1, "N": 64})
+    r2 = (0.05 + 0.3 * x1[:, 0]) ** 2 * rng.standard_normal(n) ** 2
+    out["1d_var_o1"] = (x1, r2, h1, {"order": 1, "N": 64, "y_type_out": "variance"})
+    out["1d_multi_o1"] = (x1, np.column_stack([y1, np.cos(3 * x1[:, 0])]), h1,
+                          {"order": 1, "N": 64, "gcv_selection": "average"})
+    n2 = 8000
+    x2 = rng.uniform(0, 1, (n2, 2))
+    y2 = np.sin(2 * np.pi * x2[:, 0]) * np.cos(2 * np.pi * x2[:, 1]) + 0.1 * rng.standard_normal(n2)
+    hh = np.logspace(np.log10(0.02), np.log10(0.4), 7)
+    h2 = np.column_stack([hh, hh])
+    for o in (0, 1, 2):
+        out[f"2d_o{o}_m40"] = (x2, y2, h2, {"order": o, "N": [40, 40]})
+    out["2d_o2_m33x40"] = (x2, y2, h2, {"order": 2, "N": [33, 40]})
+    out["2d_o1_nodof"] = (x2, y2, h2, {"order": 1, "N": [40, 40], "calc_dof": False})
+    n3 = 6000
+    x3 = rng.uniform(0, 1, (n3, 3))
+    y3 = np.sum((x3 - 0.5) ** 2, axis=1) + 0.1 * rng.standard_normal(n3)
+    h3 = np.column_stack([np.logspace(-1, np.log10(0.5), 4)] * 3)
+    out["3d_o1_m14"] = (x3, y3, h3, {"order": 1, "N": [14, 14, 14]})
+    return out
+
+
+CASES = _cases()
+
+
+@pytest.fixture(scope="module")
+def fl():
+    import fastlpr_amd
+    from fastlpr_amd import _lib
+    _lib.load()
+    return fastlpr_amd
+
+
+def _oracle(x, y, h, opt):
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        return R.cv_fastlpr(x, y, h, dict(opt), mode="f64")
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_fp64_path_equals_fp64_oracle(fl, name):
+    x, y, h, opt = CASES[name]
+    o = _oracle(x, y, h, opt)
+    r = fl.cv_fastlpr(x, y, h, dict(opt, precision="fp64"))
+    assert r.gcv_yhat["id1se"] == o.gcv["id1se"]
+    assert _rel(r.yhat, o.yhat) < 1e-8
+    assert _rel(r.fpp_yhat.values, o.grid_values) < 1e-8
+    assert _rel(r.s0, o.s0) < 1e-8
+    if opt.get("calc_dof", True):
+        assert _rel(r.gcv_yhat["gcv_m"], o.gcv["gcv_m"]) < 1e-8
+        assert _rel(r.gcv_yhat["dof"], o.gcv["dof"]) < 1e-8
+        assert _rel(r.gcv_yhat["gcv_sd"], o.gcv["gcv_sd"]) < 1e-6
+    if o.d is not None:
+        assert abs(r.d - o.d) / abs(o.d) < 1e-8
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_fp32_path_within_1e5_of_fp64_oracle(fl, name):
+    x, y, h, opt = CASES[name]
+    o = _oracle(x, y, h, opt)
+    r = fl.cv_fastlpr(x, y, h, dict(opt))
+    assert r.gcv_yhat["id1se"] == o.gcv["id1se"]
+    assert _rel(r.yhat, o.yhat) <= 1e-5
+    if opt.get("calc_dof", True):
+        assert _rel(r.gcv_yhat["gcv_m"], o.gcv["gcv_m"]) < 1e-5
+
+
+@pytest.mark.parametrize("name", recipes.LPR_SMALL)
+def test_gpu_matches_reference_golden(fl, name):
+    g = np.load(os.path.join(GOLDEN, f"{name}.npz"), allow_pickle=False)
+    x, y, h, opt = recipes.make(name)
+    assert json.loads(str(g["options"])) == opt
+    r = fl.cv_fastlpr(x, y, h, dict(opt))
+    assert r.gcv_yhat["id1se"] == int(g["id1se"])
+    assert r.gcv_yhat["idmin"] == int(g["idmin"])
+    tol = 5e-5
+    assert _rel(r.yhat, g["yhat"]) < tol
+    assert _rel(r.fpp_yhat.values, g["grid_values"]) < tol
+    assert _rel(r.gcv_yhat["gcv_m"], g["gcv_m"]) < 1e-5
+    assert _rel(r.s0, g["s0"]) < 1e-5
+
+
+def test_gpu_matches_reference_config1_full_size(fl):
+    g = np.load(os.path.join(GOLDEN, "cfg1_1d_o1_n1e5.npz"), allow_pickle=False)
+    x, y, h, opt = recipes.make("cfg1_1d_o1_n1e5")
+    r = fl.cv_fastlpr(x, y, h, dict(opt))
+    assert r.gcv_yhat["id1se"] == int(g["id1se"])
+    idx = g["yhat_idx"]
+    assert _rel(r.yhat[idx], g["yhat_sub"]) < 5e-5
+    assert _rel(r.gcv_yhat["gcv_m"], g["gcv_m"]) < 1e-5

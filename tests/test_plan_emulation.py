@@ -1,0 +1,125 @@
+"""Host-side plan of the separable banded convolution, checked on the CPU.
+
+The engine's job tables (slab offsets, Toeplitz base, tap bands, channel
+prefixes) are executed here by a NumPy emulation of the flpr_conv semantics
+(out[r][i] = sum_s in[r][s] tap[i + base - s] / out[o][i][c] = sum_s tap[...] in[o][s][c])
+on a NumPy spread grid built with the same knot mapping, and the resulting
+S and T moments are compared with the oracle's FFT convolution in float64.
+A plausible indexing mistake (wrong qout/pad offset = one-cell shift, wrong
+slab offset, a dropped channel prefix) changes the moments by O(1).
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from fastlpr_amd import engine as E
+from oracle import restate as R
+
+
+def _emulate(x, y, h, order, N):
+    n, d = x.shape
+    acc = R.accuracy_for(n)
+    sd = x.std(0, ddof=1)
+    st = E.Stats(x.min(0), x.max(0), x.mean(0), np.where(sd == 0, 1, sd))
+    geo = E.build_geometry(st, tuple(N), acc)
+    g = geo.geom
+    h, _, _ = E.filter_bandwidths(geo, h, n)
+    dh = h.shape[0]
+    C = 2
+    plan = E.Plan(d, order, geo.N, geo.L, geo.msp, C, dh, True, "cpu")
+    xmod = np.zeros((n, d))
+    cell = np.zeros((n, d), int)
+    for k in range(d):
+        t = (x[:, k] - g.mean[k]) / g.std_[k]
+        xs = t - g.centre[k]
+        xn = (xs - g.xmin_s[k]) / g.scale_s[k] * g.ks[k] + g.idx0[k]
+        xn = np.mod(xn + 0.5, 1.0)
+        kk = (xn - 0.5) + g.shift[k]
+        xmod[:, k] = np.mod(kk * 2 * np.pi, 2 * np.pi)
+        cell[:, k] = np.round(xmod[:, k] / g.hx[k]).astype(int)
+    grid = np.zeros((C,) + plan.G)
+    msp = geo.msp
+    vals = np.column_stack([np.ones(n), y])
+    j = np.arange(-msp, msp + 1)
+    for i in range(n):
+        ws, idx = [], []
+        for k in range(d):
+            m = cell[i, k]
+            pm = m - g.L[k] if m > g.L[k] // 2 else m
+            diff = xmod[i, k] - g.hx[k] * (m + j)
+            ws.append(np.exp(-(diff * diff * g.inv4tau[k])))
+            idx.append(pm + j + g.E[k])
+        if d == 1:
+            grid[:, idx[0]] += vals[i][:, None] * ws[0][None]
+        else:
+            grid[:, idx[0][:, None], idx[1][None, :]] += vals[i][:, None, None] * np.outer(ws[0], ws[1])[None]
+    tap = np.zeros(dh * d * plan.npow * plan.nmax)
+    for b in range(dh):
+        for k in range(d):
+            xg = np.linspace(g.xmin_s[k], g.xmax_s[k], geo.N[k])
+            u = xg / h[b, k]
+            e = np.exp(-0.5 * u * u)
+            if k == 0:
+                e = e / math.sqrt(2 * math.pi) / np.prod(h[b]) / n
+            for a in range(plan.npow):
+                o = ((b * d + k) * plan.npow + a) * plan.nmax
+                tap[o:o + geo.N[k]] = e * u ** a
+    bands = E.tap_bands(geo, h, True).reshape(-1)
+    stage0 = np.zeros(dh * plan.stage_per_h[0])
+    for ci in range(len(plan.chunks)):
+        inter = {k: np.zeros(plan.chunk * plan.stage_per_h[k]) for k in range(1, d)}
+        for ps in plan.pass_tables[ci]:
+            src = grid.reshape(-1) if ps.in_stage == d else inter[ps.in_stage]
+            dst = stage0 if ps.out_stage == 0 else inter[ps.out_stage]
+            for rec in ps.jobs_host:
+                ioff, ooff, toff, band, ro = (int(v) for v in rec[:5])
+                lo, hi = bands[2 * band], bands[2 * band + 1]
+                t = np.arange(ps.nout)[:, None] + ps.base - np.arange(ps.K)[None, :]
+                A = np.where((t >= lo) & (t <= hi), tap[np.clip(toff + t, 0, len(tap) - 1)], 0.0)
+                if ps.mode == 0:
+                    In = src[ioff:ioff + ro * ps.K].reshape(ro, ps.K)
+                    dst[ooff:ooff + ro * ps.nout] = (In @ A.T).reshape(-1)
+                else:
+                    In = src[ioff:ioff + ro * ps.K * ps.inner].reshape(ro, ps.K, ps.inner)
+                    dst[ooff:ooff + ro * ps.nout * ps.inner] = \
+                        np.einsum("is,osc->oic", A, In).reshape(-1)
+    return plan, stage0, h, acc
+
+
+CASES = [(1, o, [40]) for o in (0, 1, 2)] + [(1, 1, [33])] + \
+        [(2, o, [12, 10]) for o in (0, 1, 2)] + [(2, 2, [11, 9])]
+
+
+@pytest.mark.parametrize("d,order,N", CASES)
+def test_banded_plan_reproduces_fft_moments(d, order, N):
+    rng = np.random.default_rng(3 + d + order)
+    n = 300
+    x = rng.uniform(0, 1, (n, d))
+    y = np.sin(6 * x[:, 0]) + 0.1 * rng.standard_normal(n)
+    h = np.array([[0.05] * d, [0.2] * d, [0.6] * d]) if d == 1 else np.array([[0.1, 0.1], [0.3, 0.5]])
+    plan, st0, h, acc = _emulate(x, y, h, order, N)
+    P = R.Problem(x, h, {"order": order, "accuracy": acc, "N": N}, "f64")
+    S = P.design(add_eps=False)
+    T = P.weighted(y, True)
+    per_h, nodes, dh = plan.stage_per_h[0], plan.nodes, h.shape[0]
+    for p, off in enumerate(plan.s_off):
+        em = np.stack([st0[b * per_h + off:b * per_h + off + nodes] for b in range(dh)], -1)
+        ref = S[p].reshape(nodes, -1)
+        assert np.max(np.abs(em - ref)) <= 1e-12 * np.max(np.abs(ref))
+    for i, off in enumerate(plan.t_off):
+        em = np.stack([st0[b * per_h + off + nodes:b * per_h + off + 2 * nodes] for b in range(dh)], -1)
+        ref = T[i][..., 0].reshape(nodes, -1)
+        assert np.max(np.abs(em - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_plan_stage_layout_counts():
+    """2-D order 2 with 12 channels: 15 final moments, 6 of them carry all channels."""
+    p = E.Plan(2, 2, (256, 256), (512, 512), 4, 12, 30, False, "cpu")
+    s0 = p.stage_slabs[0]
+    assert len(s0) == 15
+    assert sum(1 for _, cb in s0.values() if cb == 12) == 6
+    assert p.stage_per_h[0] == (6 * 12 + 9) * 256 * 256
+    assert len(p.s_off) == 21 and len(p.t_off) == 6
+    assert [len(t.jobs_host) for t in p.pass_tables[0]] == [5 * 30, 15 * 30]
