@@ -18,16 +18,17 @@ EXPORTS = (
     "flpr_xstats",
     "flpr_points_prepare",
     "flpr_points_bin",
+    "flpr_points_pack",
     "flpr_spread",
+    "flpr_grid_center",
     "flpr_taps",
     "flpr_conv",
     "flpr_max_moment",
     "flpr_key_to_double",
     "flpr_solve",
-    "flpr_adjoint_interp",
-    "flpr_interp_rss",
+    "flpr_adjoint",
+    "flpr_quadform",
     "flpr_interp_eval",
-    "flpr_f64_to_f32",
 )
 
 
@@ -71,19 +72,20 @@ _GP = C.POINTER(FlprGeom)
 
 _SIGS = {
     "flpr_last_hip_error": [C.c_char_p, _I32],
-    "flpr_xstats": [_P, _I64, _I32, _P, _P, _P],
-    "flpr_points_prepare": [_P, _I64, _GP, _P, _P, _P, _P, _P],
+    "flpr_xstats": [_P, _I64, _I32, _P, _I32, _P, _P, _P],
+    "flpr_points_prepare": [_P, _I64, _GP, _P, _P, _P],
     "flpr_points_bin": [_P, _I64, _I32, _P, _P, _P, _P, _P],
-    "flpr_spread": [_GP, _P, _P, _P, _P, _I32, _P, _I32, _P, _I32, _P, _P],
+    "flpr_points_pack": [_GP, _P, _P, _I64, _P, _I32, _P, _I32, _P, _P, _P, _P, _P],
+    "flpr_spread": [_GP, _P, _P, _P, _I64, _I32, _I32, _P, _P, _P, _P],
+    "flpr_grid_center": [_P, _I32, _I64, _P, _P, _P, _P],
     "flpr_taps": [_GP, _P, _I32, _I32, _I32, _D, _P, _P, _P],
     "flpr_conv": [_I32, _I32, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P],
     "flpr_max_moment": [_I32, _P, _I64, _I32, C.POINTER(_I64), _I32, _I64, _I32, _P, _P],
     "flpr_solve": [_I32, _P, _I64, _I32, _I64, _I32, _I32, C.POINTER(_I64), C.POINTER(_I64), _P,
-                   _D, _D, _I32, _I32, _P, _I32, _I32, _P, _P, _P],
-    "flpr_adjoint_interp": [_GP, _P, _I64, _P, _P, _P, _I32, _P, _P],
-    "flpr_interp_rss": [_GP, _P, _I64, _P, _P, _I32, _I32, _P, _I32, _P, _P],
+                   _D, _D, _P, _I32, _I32, _P, _P, _P, _P],
+    "flpr_adjoint": [_GP, _P, _P, _P, _P, _I64, _I32, _P, _P, _P, _P],
+    "flpr_quadform": [_GP, _P, _I32, _I32, _P, _P, _P],
     "flpr_interp_eval": [_GP, _P, _I64, _P, _I32, _P, _P, _P, _P],
-    "flpr_f64_to_f32": [_P, _P, _I64, _P],
 }
 
 _lib = None

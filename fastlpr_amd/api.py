@@ -341,22 +341,15 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
     x_arr = _as_column_matrix(x)
     if isinstance(y, torch.Tensor):
         y_arr = y
-        if y_arr.ndim == 1:
-            y_arr = y_arr[:, None]
-        elif y_arr.ndim == 2:
-            if y_arr.shape[0] == 1 and y_arr.shape[1] > 1:
-                y_arr = y_arr.T
-        else:
-            raise ValueError(f"Response y must be 1D or 2D array. Got {y_arr.ndim}D array.")
     else:
         y_arr = np.asarray(y)
-        if y_arr.ndim == 1:
-            y_arr = y_arr[:, None]
-        elif y_arr.ndim == 2:
-            if y_arr.shape[0] == 1 and y_arr.shape[1] > 1:
-                y_arr = y_arr.T
-        else:
-            raise ValueError(f"Response y must be 1D or 2D array. Got {y_arr.ndim}D array.")
+    if y_arr.ndim == 1:
+        y_arr = y_arr[:, None]
+    elif y_arr.ndim == 2:
+        if y_arr.shape[0] == 1 and y_arr.shape[1] > 1:
+            y_arr = y_arr.T
+    else:
+        raise ValueError(f"Response y must be 1D or 2D array. Got {y_arr.ndim}D array.")
     dy = y_arr.shape[1]
     if y_arr.shape[0] != x_arr.shape[0]:
         raise ValueError(
@@ -364,8 +357,12 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
             f"x has {x_arr.shape[0]} samples, y has {y_arr.shape[0]} samples.")
     if x_arr.shape[0] == 0:
         raise ValueError("Input x must have at least one sample.")
-    _check_finite(x_arr, "Input x", "cv_fastlpr")
-    _check_finite(y_arr, "Response y", "cv_fastlpr")
+    if y_arr.shape[0] == 0:
+        raise ValueError("Response y must have at least one sample.")
+    if not isinstance(x_arr, torch.Tensor):
+        _check_finite(x_arr, "Input x", "cv_fastlpr")
+    if not isinstance(y_arr, torch.Tensor):
+        _check_finite(y_arr, "Response y", "cv_fastlpr")
 
     opts = set_defaults(options, order=0, regularization=1e-8, num_dof_sample=10,
                         kernel_type="gaussian", y_type_out="mean", calc_dof=True,
@@ -400,9 +397,17 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
         else bool(np.iscomplexobj(y_arr))
     yd = _y_channels(y_arr, dev)
     y_raw_orig = yd[:, 0].clone() if variance else None
+    ny = yd.shape[1]
+    if ny > 32:
+        raise NotImplementedError("the GPU engine takes at most 32 real response columns")
     if variance:
         yd = torch.log(yd + 1.0 / n)  # api.py:390-396
-    ny = yd.shape[1]
+    # one device pass: x statistics, non-finite checks of device inputs, sum y^2
+    stats = engine.x_stats(xd, yd)
+    if stats.bad_x:
+        _check_finite(x_arr, "Input x", "cv_fastlpr")
+    if stats.bad_y and isinstance(y_arr, torch.Tensor):
+        _check_finite(y_arr, "Response y", "cv_fastlpr")
     reg = float(opts["regularization"])
     calc_dof = bool(opts.get("calc_dof", True))
     xrange = opts.get("xrange", None)
@@ -414,7 +419,7 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
         if method != "linear":
             raise NotImplementedError("only linear interpolation is implemented on the GPU")
         fit = engine.run_grid(xd, yd, ny, None, 0, bandwidths, order, N, int(opts["accuracy"]),
-                              fp64, reg, xrange, flag_power2, with_ones_eps=False)
+                              fp64, reg, xrange, flag_power2, stats=stats, with_ones_eps=False)
         sel = 0
         gcv_result = None
         dof_sel = dof_err_sel = np.nan
@@ -430,12 +435,15 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
         probes, pp = _probes(n, K, opts.get("dof_seed", 42), rm, dev)
         K = probes.shape[1]
         fit = engine.run_grid(xd, yd, ny, probes, K, bandwidths, order, N, int(opts["accuracy"]),
-                              fp64, reg, xrange, flag_power2, rss_y=yd, rss_cplx=y_is_complex)
+                              fp64, reg, xrange, flag_power2, want_rss=True, stats=stats)
+        rss = fit.rss
+        if y_is_complex:
+            rss = rss[:, 0::2] + rss[:, 1::2]
         dof_all, dof_se, pen_std, pen_mean = _hutchinson(fit.dot, pp, n, y_is_complex)
         agg = str(opts.get("gcv_selection", "first"))
         scores, errs = [], []
         for ih in range(fit.h.shape[0]):
-            s_, e_ = _gcv(fit.rss[ih], n, dof_all[ih], pen_std[ih],
+            s_, e_ = _gcv(rss[ih], n, dof_all[ih], pen_std[ih],
                           None if pen_mean is None else pen_mean[ih], agg)
             scores.append(s_)
             errs.append(e_)

@@ -47,6 +47,12 @@ static inline int64_t flpr_ntiles(const FlprGeom* g) {
   return t;
 }
 
+static inline int64_t flpr_nkeys(const FlprGeom* g) {
+  int64_t t = flpr_ntiles(g);
+  for (int k = 0; k < g->d; ++k) t *= g->tile[k];
+  return t;
+}
+
 static inline int64_t flpr_nodes(const FlprGeom* g) {
   int64_t t = 1;
   for (int k = 0; k < g->d; ++k) t *= g->n_grid[k];

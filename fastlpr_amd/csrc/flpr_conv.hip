@@ -105,16 +105,30 @@ constexpr int CT = 256;  // 4 waves
 constexpr int TM = 64;   // block tile rows
 constexpr int TN = 64;   // block tile cols
 constexpr int KC = 32;   // K chunk staged in LDS
-constexpr int STRIP = TM + KC - 1;
+constexpr int WM = 32;   // wave tile rows (2 x 16)
+constexpr int WN = 32;   // wave tile cols (2 x 16)
+constexpr int RB = WM / 16;
+constexpr int CB = WN / 16;
 
+// Banded Toeplitz GEMM.  One block computes a TM x TN output tile; its four
+// waves own 32 x 32 sub-tiles (2 x 2 MFMA 16x16 blocks).  The dense operand of
+// chunk k+1 is fetched into registers while chunk k is multiplied from LDS
+// (double-buffered LDS, one barrier per chunk).  The Toeplitz operand is
+// generated from a 1-D strip of taps: element (i, s) is tap[i + base - s]; a
+// 16-wide block of the band whose taps miss a K step skips its MFMAs.
+// Float32 chunks are folded into float64 totals.
 template <typename T, int MODE>
 __global__ __launch_bounds__(CT) void k_conv(const T* __restrict__ in, T* __restrict__ out,
                                              const T* __restrict__ tap,
                                              const int64_t* __restrict__ jobs,
                                              const int32_t* __restrict__ bands, int K, int nout,
                                              int inner, int base, int ntn, int ntm) {
-  __shared__ T dense[(MODE == 0) ? TM * (KC + 1) : KC * (TN + 1)];
-  __shared__ T strip[STRIP + 1];
+  constexpr int DENSE = (MODE == 0) ? TM * (KC + 1) : KC * (TN + 1);
+  constexpr int NSTRIP = (MODE == 0) ? TN + KC - 1 : TM + KC - 1;
+  constexpr int PER = ((MODE == 0) ? TM * KC : KC * TN) / CT;
+  constexpr int LTOE = (MODE == 0) ? TN : TM;  // extent of the tile along the convolved axis
+  __shared__ T dense[2][DENSE];
+  __shared__ T strip[2][NSTRIP + 1];
   const int64_t* jb = jobs + 8 * (int64_t)blockIdx.y;
   const int64_t in_off = jb[0], out_off = jb[1], tap_off = jb[2];
   const int band = (int)jb[3], ro = (int)jb[4];
@@ -138,74 +152,130 @@ __global__ __launch_bounds__(CT) void k_conv(const T* __restrict__ in, T* __rest
   }
   const int m0 = tm * TM, n0 = tn * TN;
   if (n0 >= N) return;
-  const int i0 = (MODE == 0) ? n0 : m0;  // first output index along the convolved axis
-  int s_begin = i0 + base - t_hi;
-  int s_end = i0 + TM - 1 + base - t_lo;
-  s_begin = max(s_begin, 0);
-  s_end = min(s_end, K - 1);
+  const int i0 = (MODE == 0) ? n0 : m0;
+  const int s_begin = max(i0 + base - t_hi, 0);
+  const int s_end = min(i0 + LTOE - 1 + base - t_lo, K - 1);
+  const int nchunks = (s_end >= s_begin) ? (s_end - s_begin) / KC + 1 : 0;
   const T* src = in + in_off + (MODE == 0 ? 0 : (int64_t)o * K * inner);
   T* dst = out + out_off + (MODE == 0 ? 0 : (int64_t)o * nout * inner);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-  typename Mfma<T>::acc acc[2][2];
-#pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-    for (int cb = 0; cb < 2; ++cb)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[rb][cb][j] = T(0);
+  const T* tp = tap + tap_off;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * WM, wc = (wave & 1) * WN;
 
-  for (int s0 = s_begin; s0 <= s_end; s0 += KC) {
-    if (MODE == 0) {
-      for (int e = threadIdx.x; e < TM * KC; e += CT) {
+  typename Mfma<T>::acc acc[RB][CB];
+  double tot[RB][CB][4];
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        acc[rb][cb][j] = T(0);
+        tot[rb][cb][j] = 0.0;
+      }
+
+  T pre[PER];
+  T pst = T(0);
+  auto gload = [&](int s0) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = tid + i * CT;
+      if (MODE == 0) {
         const int r = e / KC, kk = e % KC;
         const int gr = m0 + r, gs = s0 + kk;
-        dense[r * (KC + 1) + kk] = (gr < M && gs <= s_end) ? src[(int64_t)gr * K + gs] : T(0);
-      }
-    } else {
-      for (int e = threadIdx.x; e < KC * TN; e += CT) {
+        pre[i] = (gr < M && gs <= s_end) ? src[(int64_t)gr * K + gs] : T(0);
+      } else {
         const int kk = e / TN, c = e % TN;
         const int gs = s0 + kk, gc = n0 + c;
-        dense[kk * (TN + 1) + c] = (gs <= s_end && gc < N) ? src[(int64_t)gs * inner + gc] : T(0);
+        pre[i] = (gs <= s_end && gc < N) ? src[(int64_t)gs * inner + gc] : T(0);
       }
     }
-    const int tf = i0 + base - s0 - (KC - 1);
-    for (int q = threadIdx.x; q < STRIP; q += CT) {
-      const int t = tf + q;
-      strip[q] = (t >= t_lo && t <= t_hi) ? tap[tap_off + t] : T(0);
+    if (tid < NSTRIP) {
+      const int t = i0 + base - s0 - (KC - 1) + tid;
+      pst = (t >= t_lo && t <= t_hi) ? tp[t] : T(0);
     }
-    __syncthreads();
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = tid + i * CT;
+      if (MODE == 0) dense[buf][(e / KC) * (KC + 1) + (e % KC)] = pre[i];
+      else dense[buf][(e / TN) * (TN + 1) + (e % TN)] = pre[i];
+    }
+    if (tid < NSTRIP) strip[buf][tid] = pst;
+  };
+
+  if (nchunks > 0) {
+    gload(s_begin);
+    sstore(0);
+  }
+  __syncthreads();
+  for (int ch = 0; ch < nchunks; ++ch) {
+    const int buf = ch & 1;
+    const bool more = ch + 1 < nchunks;
+    if (more) gload(s_begin + (ch + 1) * KC);
+    const T* dn = dense[buf];
+    const T* sp = strip[buf];
+    const int sc0 = s_begin + ch * KC;  // absolute input index of the chunk's first column
 #pragma unroll
     for (int kk = 0; kk < KC; kk += 4) {
       const int kl = kk + (lane >> 4);
-      T a[2], b[2];
+      // absolute s of this K step: [sc0 + kk, sc0 + kk + 3]; a 16-wide block of
+      // Toeplitz outputs starting at i needs s in [i + base - t_hi, i + 15 + base - t_lo]
+      bool live[2];
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb) {
+      for (int q = 0; q < 2; ++q) {
+        const int ib = (MODE == 0) ? (n0 + wc + q * 16) : (m0 + wr + q * 16);
+        live[q] = !(sc0 + kk + 3 < ib + base - t_hi || sc0 + kk > ib + 15 + base - t_lo);
+      }
+      T a[RB], b[CB];
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) {
         const int r = wr + rb * 16 + (lane & 15);
-        a[rb] = (MODE == 0) ? dense[r * (KC + 1) + kl] : strip[r - kl + KC - 1];
+        a[rb] = (MODE == 0) ? dn[r * (KC + 1) + kl] : sp[r - kl + KC - 1];
       }
 #pragma unroll
-      for (int cb = 0; cb < 2; ++cb) {
+      for (int cb = 0; cb < CB; ++cb) {
         const int c = wc + cb * 16 + (lane & 15);
-        b[cb] = (MODE == 0) ? strip[c - kl + KC - 1] : dense[kl * (TN + 1) + c];
+        b[cb] = (MODE == 0) ? sp[c - kl + KC - 1] : dn[kl * (TN + 1) + c];
       }
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb)
+      for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-        for (int cb = 0; cb < 2; ++cb) acc[rb][cb] = Mfma<T>::run(a[rb], b[cb], acc[rb][cb]);
+        for (int cb = 0; cb < CB; ++cb) {
+          const bool on = (MODE == 0) ? live[cb] : live[rb];
+          if (on) acc[rb][cb] = Mfma<T>::run(a[rb], b[cb], acc[rb][cb]);
+        }
     }
+    if constexpr (sizeof(T) == 4) {
+      // the float32 MFMA accumulator only ever holds KC terms
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            tot[rb][cb][j] += (double)acc[rb][cb][j];
+            acc[rb][cb][j] = T(0);
+          }
+    }
+    if (more) sstore(buf ^ 1);
     __syncthreads();
   }
   const int ld = (MODE == 0) ? nout : inner;
 #pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
+  for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb)
+    for (int cb = 0; cb < CB; ++cb)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int row = m0 + wr + rb * 16 + Mfma<T>::row(lane, j);
         const int col = n0 + wc + cb * 16 + (lane & 15);
-        if (row < M && col < N) dst[(int64_t)row * ld + col] = acc[rb][cb][j];
+        if (row < M && col < N) {
+          const double v = (sizeof(T) == 4) ? tot[rb][cb][j] : (double)acc[rb][cb][j];
+          dst[(int64_t)row * ld + col] = (T)v;
+        }
       }
 }
 
@@ -221,19 +291,16 @@ constexpr int RED_THREADS = 256;
 
 template <typename T>
 __global__ __launch_bounds__(RED_THREADS) void k_maxmom(const T* __restrict__ src, int64_t per_h,
-                                                        int dh, OffTab offs, int nslab,
-                                                        int64_t nodes, int absval,
-                                                        uint64_t* __restrict__ key) {
+                                                        OffTab offs, int nslab, int64_t nodes,
+                                                        int absval, uint64_t* __restrict__ key) {
   __shared__ double sh[RED_THREADS / 64];
+  const int hs = blockIdx.y;  // h * nslab + slab
+  const int h = hs / nslab, k = hs % nslab;
+  const T* p = src + (int64_t)h * per_h + offs.v[k];
   double mx = -INFINITY;
-  const int64_t total = (int64_t)dh * nslab * nodes;
-  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t h = idx / ((int64_t)nslab * nodes);
-    const int64_t r = idx % ((int64_t)nslab * nodes);
-    const int k = (int)(r / nodes);
-    const int64_t node = r % nodes;
-    double v = (double)src[h * per_h + offs.v[k] + node];
+  for (int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; node < nodes;
+       node += (int64_t)gridDim.x * blockDim.x) {
+    double v = (double)p[node];
     if (absval) v = fabs(v);
     mx = fmax(mx, v);
   }
@@ -257,152 +324,144 @@ struct SolveTab {
 };
 
 constexpr int SOLVE_THREADS = 256;
+constexpr int MAXV = 32;  // value channels reduced per solve launch
 
-__device__ inline double wave_sum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
-
-// One thread per (node, bandwidth).  Order >= 1: Cholesky of S + lam I in
-// float64 (S + lam I is symmetric positive definite for lam > 0), then
-// z = (S + lam I)^-1 e1 and m_c = z . T_c for every channel (the intercept
-// e1' S^-1 T that the reference's Cramer formulas evaluate).  Order 0:
-// m_c = T_c / max(S + eps, thr).  Probe channels are reduced on the fly
-// against the adjoint-interpolated probes q: dot[h][k] += q_k(node) m_k(node).
+// One thread per (node, bandwidth); block row blockIdx.y = bandwidth.
+// Order >= 1: Cholesky of S + lam I in float64 (symmetric positive definite
+// for lam > 0), z = (S + lam I)^-1 e1, and m_c = z . T_c for every value
+// channel (the intercept e1' S^-1 T that the reference's Cramer formulas
+// evaluate).  Order 0: m_c = T_c / max(S + eps, thr).  Channels were centred on
+// the grid by shift_c; since T_c(y) = T_c(y - shift_c) + shift_c T_0, the
+// solve adds shift_c z . T_0 (shift_c S_00 for order 0) back exactly.
+// The first nout value channels are written to m_out; for channels with an
+// adjoint row in q, dot[h][c] += sum_node q_c(node) m_c(node) is reduced in
+// registers over the block's nodes and then once per block.
 template <typename T, int NT>
 __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(
-    const T* __restrict__ mom, int64_t per_h, int dh, int64_t nodes, SolveTab tab,
-    const uint64_t* __restrict__ key, double lam_scale, double eps_add, int nout, int cf,
-    double* __restrict__ m_out, int pf, int np, const double* __restrict__ q,
-    double* __restrict__ dot) {
+    const T* __restrict__ mom, int64_t per_h, int64_t nodes, SolveTab tab,
+    const uint64_t* __restrict__ key, double lam_scale, double eps_add,
+    const double* __restrict__ shift, int nval, int nout, double* __restrict__ m_out,
+    const double* __restrict__ q, double* __restrict__ dot) {
+  __shared__ double red[SOLVE_THREADS / 64][MAXV];
   const double kmax = flpr_decode(*key);
-  const int64_t total = (int64_t)dh * nodes;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  const int64_t start = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  // iterate so that every lane of a wave runs the same number of trips
-  const int64_t trips = (total + stride - 1) / stride;
-  for (int64_t it = 0; it < trips; ++it) {
-    const int64_t idx = start + it * stride;
-    const bool active = idx < total;
-    const int64_t h = active ? idx / nodes : -1;
-    const int64_t node = active ? idx % nodes : 0;
+  const int h = blockIdx.y;
+  const T* b = mom + (int64_t)h * per_h;
+  double dacc[MAXV];
+#pragma unroll
+  for (int c = 0; c < MAXV; ++c) dacc[c] = 0.0;
+  for (int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; node < nodes;
+       node += (int64_t)gridDim.x * blockDim.x) {
     double z[NT];
-    double den = 1.0;
-    const T* b = mom + (active ? h : 0) * per_h;
-    if (active) {
-      if (NT == 1) {
-        const double s = (double)b[tab.s_off[0] + node] + eps_add;  // regression.py:127
-        const double thr = (kmax + eps_add) * lam_scale;             // regression.py:217
-        den = fmax(s, thr);
-        z[0] = 1.0;
-      } else {
-        const double lam = kmax * lam_scale;  // regression.py:243-267
-        double A[NT][NT];
-        int p = 0;
+    double den = 1.0, s00 = 0.0, zt0 = 0.0;
+    if (NT == 1) {
+      s00 = (double)b[tab.s_off[0] + node];
+      const double s = s00 + eps_add;                    // regression.py:127
+      const double thr = (kmax + eps_add) * lam_scale;  // regression.py:217
+      den = fmax(s, thr);
+      z[0] = 1.0;
+    } else {
+      const double lam = kmax * lam_scale;  // regression.py:243-267
+      double A[NT][NT];
+      int p = 0;
 #pragma unroll
-        for (int i = 0; i < NT; ++i)
+      for (int i = 0; i < NT; ++i)
 #pragma unroll
-          for (int j = i; j < NT; ++j) {
-            const double v = (double)b[tab.s_off[p] + node];
-            A[i][j] = v;
-            A[j][i] = v;
-            ++p;
-          }
-#pragma unroll
-        for (int i = 0; i < NT; ++i) A[i][i] += lam;
-        // Cholesky: lower triangle of A becomes L
-#pragma unroll
-        for (int j = 0; j < NT; ++j) {
-          double dsum = A[j][j];
-#pragma unroll
-          for (int k = 0; k < j; ++k) dsum -= A[j][k] * A[j][k];
-          const double ljj = sqrt(dsum);
-          A[j][j] = ljj;
-          const double inv = 1.0 / ljj;
-#pragma unroll
-          for (int i = j + 1; i < NT; ++i) {
-            double s = A[i][j];
-#pragma unroll
-            for (int k = 0; k < j; ++k) s -= A[i][k] * A[j][k];
-            A[i][j] = s * inv;
-          }
+        for (int j = i; j < NT; ++j) {
+          const double v = (double)b[tab.s_off[p] + node];
+          A[i][j] = v;
+          A[j][i] = v;
+          ++p;
         }
-        // L y = e1
-        double y[NT];
 #pragma unroll
-        for (int i = 0; i < NT; ++i) {
-          double s = (i == 0) ? 1.0 : 0.0;
+      for (int i = 0; i < NT; ++i) A[i][i] += lam;
 #pragma unroll
-          for (int k = 0; k < i; ++k) s -= A[i][k] * y[k];
-          y[i] = s / A[i][i];
-        }
-        // L' z = y
+      for (int j = 0; j < NT; ++j) {
+        double dsum = A[j][j];
 #pragma unroll
-        for (int i = NT - 1; i >= 0; --i) {
-          double s = y[i];
+        for (int k = 0; k < j; ++k) dsum -= A[j][k] * A[j][k];
+        const double ljj = sqrt(dsum);
+        A[j][j] = ljj;
+        const double inv = 1.0 / ljj;
 #pragma unroll
-          for (int k = i + 1; k < NT; ++k) s -= A[k][i] * z[k];
-          z[i] = s / A[i][i];
+        for (int i = j + 1; i < NT; ++i) {
+          double s = A[i][j];
+#pragma unroll
+          for (int k = 0; k < j; ++k) s -= A[i][k] * A[j][k];
+          A[i][j] = s * inv;
         }
       }
-      for (int c = cf; c < cf + nout; ++c) {
-        double m = 0.0;
-        if (NT == 1) {
-          m = (double)b[tab.t_off[0] + (int64_t)c * nodes + node] / den;
-        } else {
+      double y[NT];
 #pragma unroll
-          for (int i = 0; i < NT; ++i) m += z[i] * (double)b[tab.t_off[i] + (int64_t)c * nodes + node];
-        }
-        m_out[((int64_t)h * nout + (c - cf)) * nodes + node] = m;
+      for (int i = 0; i < NT; ++i) {
+        double s = (i == 0) ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = 0; k < i; ++k) s -= A[i][k] * y[k];
+        y[i] = s / A[i][i];
       }
+#pragma unroll
+      for (int i = NT - 1; i >= 0; --i) {
+        double s = y[i];
+#pragma unroll
+        for (int k = i + 1; k < NT; ++k) s -= A[k][i] * z[k];
+        z[i] = s / A[i][i];
+      }
+      // z . T_0 (ones channel) for the centring correction
+#pragma unroll
+      for (int i = 0; i < NT; ++i) zt0 += z[i] * (double)b[tab.t_off[i] + node];
     }
-    if (np > 0 && q) {
-      // the wave shares one bandwidth when its first and last lanes do
-      const int64_t h_first = __shfl(h, 0);
-      const int64_t h_last = __shfl(h, 63);
-      const bool uniform = (h_first == h_last) && (h_first >= 0);
-      for (int kp = 0; kp < np; ++kp) {
-        double contrib = 0.0;
-        if (active) {
-          const int c = pf + kp;
-          double m = 0.0;
-          if (NT == 1) {
-            m = (double)b[tab.t_off[0] + (int64_t)c * nodes + node] / den;
-          } else {
 #pragma unroll
-            for (int i = 0; i < NT; ++i)
-              m += z[i] * (double)b[tab.t_off[i] + (int64_t)c * nodes + node];
-          }
-          contrib = q[(int64_t)kp * nodes + node] * m;
-        }
-        if (uniform) {
-          const double s = wave_sum(contrib);
-          if ((threadIdx.x & 63) == 0) atomicAdd(&dot[h_first * np + kp], s);
-        } else if (active) {
-          atomicAdd(&dot[h * np + kp], contrib);
-        }
+    for (int cc = 0; cc < MAXV; ++cc) {
+      if (cc >= nval) continue;
+      const int c = cc + 1;  // channel 0 is the ones channel
+      const double sh = shift ? shift[c] : 0.0;
+      double m;
+      if (NT == 1) {
+        m = ((double)b[tab.t_off[0] + (int64_t)c * nodes + node] + sh * s00) / den;
+      } else {
+        m = sh * zt0;
+#pragma unroll
+        for (int i = 0; i < NT; ++i) m += z[i] * (double)b[tab.t_off[i] + (int64_t)c * nodes + node];
       }
+      if (cc < nout) m_out[((int64_t)h * nout + cc) * nodes + node] = m;
+      if (q) dacc[cc] += q[(int64_t)cc * nodes + node] * m;
+    }
+  }
+  if (q) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int cc = 0; cc < MAXV; ++cc) {
+      if (cc >= nval) continue;
+      double v = dacc[cc];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      if (lane == 0) red[wid][cc] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)nval) {
+      double s = 0.0;
+      for (int w = 0; w < SOLVE_THREADS / 64; ++w) s += red[w][threadIdx.x];
+      atomicAdd(&dot[(int64_t)h * nval + threadIdx.x], s);
     }
   }
 }
 
 template <typename T, int NT>
-void launch_solve(hipStream_t st, int blocks, const T* mom, int64_t per_h, int dh, int64_t nodes,
+void launch_solve(hipStream_t st, dim3 grd, const T* mom, int64_t per_h, int64_t nodes,
                   const SolveTab& tab, const uint64_t* key, double lam_scale, double eps_add,
-                  int nout, int cf, double* m_out, int pf, int np, const double* q, double* dot) {
-  hipLaunchKernelGGL((k_solve<T, NT>), dim3(blocks), dim3(SOLVE_THREADS), 0, st, mom, per_h, dh,
-                     nodes, tab, key, lam_scale, eps_add, nout, cf, m_out, pf, np, q, dot);
+                  const double* shift, int nval, int nout, double* m_out, const double* q,
+                  double* dot) {
+  hipLaunchKernelGGL((k_solve<T, NT>), grd, dim3(SOLVE_THREADS), 0, st, mom, per_h, nodes, tab,
+                     key, lam_scale, eps_add, shift, nval, nout, m_out, q, dot);
 }
 
 template <typename T>
-int dispatch_solve(int nt, hipStream_t st, int blocks, const T* mom, int64_t per_h, int dh,
-                   int64_t nodes, const SolveTab& tab, const uint64_t* key, double lam_scale,
-                   double eps_add, int nout, int cf, double* m_out, int pf, int np,
-                   const double* q, double* dot) {
-#define FLPR_SOLVE_CASE(N)                                                                  \
-  case N:                                                                                   \
-    launch_solve<T, N>(st, blocks, mom, per_h, dh, nodes, tab, key, lam_scale, eps_add, nout, \
-                       cf, m_out, pf, np, q, dot);                                          \
+int dispatch_solve(int nt, hipStream_t st, dim3 grd, const T* mom, int64_t per_h, int64_t nodes,
+                   const SolveTab& tab, const uint64_t* key, double lam_scale, double eps_add,
+                   const double* shift, int nval, int nout, double* m_out, const double* q,
+                   double* dot) {
+#define FLPR_SOLVE_CASE(N)                                                                   \
+  case N:                                                                                    \
+    launch_solve<T, N>(st, grd, mom, per_h, nodes, tab, key, lam_scale, eps_add, shift, nval, \
+                       nout, m_out, q, dot);                                                 \
     return 0;
   switch (nt) {
     FLPR_SOLVE_CASE(1)
@@ -473,14 +532,16 @@ int flpr_max_moment(int32_t fp64, const void* src, int64_t per_h, int32_t dh, co
     return -1;
   OffTab t;
   for (int k = 0; k < nslab; ++k) t.v[k] = offs[k];
-  const int64_t total = (int64_t)dh * nslab * nodes;
-  const int blocks = flpr_grid_for(total, RED_THREADS, 2048);
+  if ((int64_t)dh * nslab > 65535) return -1;
+  // about a thousand blocks in total: every block ends in one atomic on *key
+  const int per = 1024 / (dh * nslab) > 1 ? 1024 / (dh * nslab) : 1;
+  dim3 grd(flpr_grid_for(nodes, RED_THREADS, per), (unsigned)(dh * nslab));
   if (fp64)
-    hipLaunchKernelGGL((k_maxmom<double>), dim3(blocks), dim3(RED_THREADS), 0, as_stream(stream),
-                       (const double*)src, per_h, dh, t, nslab, nodes, absval, key);
+    hipLaunchKernelGGL((k_maxmom<double>), grd, dim3(RED_THREADS), 0, as_stream(stream),
+                       (const double*)src, per_h, t, nslab, nodes, absval, key);
   else
-    hipLaunchKernelGGL((k_maxmom<float>), dim3(blocks), dim3(RED_THREADS), 0, as_stream(stream),
-                       (const float*)src, per_h, dh, t, nslab, nodes, absval, key);
+    hipLaunchKernelGGL((k_maxmom<float>), grd, dim3(RED_THREADS), 0, as_stream(stream),
+                       (const float*)src, per_h, t, nslab, nodes, absval, key);
   FLPR_CHECK_LAUNCH();
   return 0;
 }
@@ -489,31 +550,30 @@ double flpr_key_to_double(uint64_t key) { return flpr_decode(key); }
 
 int flpr_solve(int32_t fp64, const void* mom, int64_t per_h, int32_t dh, int64_t nodes,
                int32_t order, int32_t nt, const int64_t* s_off, const int64_t* t_off,
-               const uint64_t* key, double lam_scale, double eps_add, int32_t n_out_chan,
-               int32_t chan_first, double* m_out, int32_t probe_first, int32_t nprobe,
-               const double* q, double* dot, void* stream) {
+               const uint64_t* key, double lam_scale, double eps_add, const double* shift,
+               int32_t nval, int32_t nout, double* m_out, const double* q, double* dot,
+               void* stream) {
   FLPR_CLEAR_ERROR();
-  if (!mom || !s_off || !t_off || !key || dh < 1 || nodes < 1 || nt < 1 || nt > FLPR_MAX_TERMS)
+  if (!mom || !s_off || !t_off || !key || dh < 1 || dh > 65535 || nodes < 1 || nt < 1 ||
+      nt > FLPR_MAX_TERMS || nval < 0 || nval > MAXV || nout < 0 || nout > nval)
     return -1;
   if ((order == 0) != (nt == 1)) return -1;
-  if (n_out_chan > 0 && !m_out) return -1;
-  if (nprobe > 0 && (!q || !dot)) return -1;
+  if (nout > 0 && !m_out) return -1;
+  if (q && !dot) return -1;
   SolveTab tab;
   const int ns = nt * (nt + 1) / 2;
   for (int i = 0; i < ns; ++i) tab.s_off[i] = s_off[i];
   for (int i = 0; i < nt; ++i) tab.t_off[i] = t_off[i];
-  const int64_t total = (int64_t)dh * nodes;
-  const int blocks = flpr_grid_for(total, SOLVE_THREADS, 8192);
+  const int bx = flpr_grid_for(nodes, SOLVE_THREADS, 1024 / (dh > 64 ? 64 : dh) + 1);
+  dim3 grd((unsigned)bx, (unsigned)dh);
   hipStream_t st = as_stream(stream);
   int rc;
   if (fp64)
-    rc = dispatch_solve<double>(nt, st, blocks, (const double*)mom, per_h, dh, nodes, tab, key,
-                                lam_scale, eps_add, n_out_chan, chan_first, m_out, probe_first,
-                                nprobe, q, dot);
+    rc = dispatch_solve<double>(nt, st, grd, (const double*)mom, per_h, nodes, tab, key,
+                                lam_scale, eps_add, shift, nval, nout, m_out, q, dot);
   else
-    rc = dispatch_solve<float>(nt, st, blocks, (const float*)mom, per_h, dh, nodes, tab, key,
-                               lam_scale, eps_add, n_out_chan, chan_first, m_out, probe_first,
-                               nprobe, q, dot);
+    rc = dispatch_solve<float>(nt, st, grd, (const float*)mom, per_h, nodes, tab, key,
+                               lam_scale, eps_add, shift, nval, nout, m_out, q, dot);
   if (rc) return rc;
   FLPR_CHECK_LAUNCH();
   return 0;

@@ -1,6 +1,8 @@
-// Point-side kernels: x statistics, knot mapping + spread-tile binning,
-// Gaussian-gridding spread, adjoint interpolation of the probes, residual sums
-// and final interpolation.  All point arithmetic is float64.
+// Point-side kernels: x statistics, knot mapping and cell binning, packing of
+// the sorted point data, Gaussian-gridding spread, grid centring, adjoint
+// interpolation sums for the Hutchinson dots and the residual sums, the
+// residual quadratic form, and the final interpolation.  Point arithmetic is
+// float64.
 //
 // Reference dataflow restated here (fastLPR_py/src/fastlpr/...):
 //   model.py:58-72            z-score (ddof=1) and centring
@@ -9,13 +11,29 @@
 //   nufftn_type1.py:520-636   round(xmod/hx), (2 Msp + 1)^d taps, exp(-diff^2/(4 tau))
 //   spreading_fused.py:67-94  fused per-point spread (2-D form)
 //   regression.py:1256-1280   Hutchinson p' * interp(m)
-//   api.py:686-723            RegularGridInterpolator 'linear' at the data points
+//   api.py:686-723            RegularGridInterpolator 'linear' at the data points, RSS
+//
+// Spread.  Points are counting-sorted by spread cell (cells grouped by tile).
+// One block owns a tile; one work item owns (cell, channel) and accumulates
+// the cell's points into the full (2 Msp + 1)^d footprint in float64
+// registers, then adds the footprint once into the tile's LDS window.  The
+// tap weights factor as exp(-(delta - hx k)^2 c) = A B^k G_k with
+// A = exp(-delta^2 c), B = exp(2 delta hx c), G_k = exp(-hx^2 k^2 c), so a
+// point needs two exponentials per dimension.
+//
+// Residual sums.  yhat_i(h) = sum_node w_i(node) m_h(node) is linear in m, so
+//   sum_i p_ik phat_ik(h) = sum_node Q_k(node) m_hk(node),  Q_k = sum_i p_ik w_i
+//   sum_i (y_i - yhat_i)^2 = sum_i y_i^2 - 2 sum_node Q_y m_h + sum_cell m' W m
+// with W the per-cell Gram matrix of the 2^d interpolation weights.  Q and W
+// are accumulated once per fit from the sorted points; the per-bandwidth work
+// is on the grid only.
 #include "flpr_common.h"
 
 namespace {
 
 constexpr int XS_BLOCKS = 512;
 constexpr int XS_THREADS = 256;
+constexpr int MAXY = 32;  // y columns summed by the statistics pass
 
 struct Welford {
   double n, mean, m2, mn, mx;
@@ -26,7 +44,7 @@ __device__ inline Welford wf_merge(Welford a, Welford b) {
   if (b.n == 0) return a;
   Welford r;
   r.n = a.n + b.n;
-  double delta = b.mean - a.mean;
+  const double delta = b.mean - a.mean;
   r.mean = a.mean + delta * (b.n / r.n);
   r.m2 = a.m2 + b.m2 + delta * delta * (a.n * b.n / r.n);
   r.mn = fmin(a.mn, b.mn);
@@ -34,17 +52,39 @@ __device__ inline Welford wf_merge(Welford a, Welford b) {
   return r;
 }
 
-// Chan/Welford statistics per dimension; one partial per block per dimension.
+__device__ inline double block_sum(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += sh[w];
+  return r;
+}
+
+// Per-dimension Welford statistics of x; counts of non-finite values in x and
+// y; sums of squares of the y columns.  Partials per block:
+//   [d][blocks][5] Welford, then [blocks][2] non-finite counts,
+//   then [ny][blocks] sums of squares.
 __global__ __launch_bounds__(XS_THREADS) void k_xstats(const double* __restrict__ x, int64_t n,
-                                                       int d, double* __restrict__ partials) {
+                                                       int d, const double* __restrict__ y,
+                                                       int ny, double* __restrict__ partials) {
   __shared__ Welford sh[XS_THREADS];
+  __shared__ double red[XS_THREADS / 64];
+  double bad_x = 0.0, bad_y = 0.0;
   for (int k = 0; k < d; ++k) {
     Welford w = {0.0, 0.0, 0.0, INFINITY, -INFINITY};
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-      double v = x[i * d + k];
+      const double v = x[i * d + k];
+      if (!isfinite(v)) {
+        bad_x += 1.0;
+        continue;
+      }
       w.n += 1.0;
-      double delta = v - w.mean;
+      const double delta = v - w.mean;
       w.mean += delta / w.n;
       w.m2 += delta * (v - w.mean);
       w.mn = fmin(w.mn, v);
@@ -62,17 +102,38 @@ __global__ __launch_bounds__(XS_THREADS) void k_xstats(const double* __restrict_
     }
     __syncthreads();
   }
+  double* cnt = partials + (size_t)d * gridDim.x * 5;
+  double* sq = cnt + (size_t)gridDim.x * 2;
+  for (int c = 0; c < ny; ++c) {
+    double s2 = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+      const double v = y[i * ny + c];
+      if (!isfinite(v)) bad_y += 1.0;
+      else s2 += v * v;
+    }
+    const double t = block_sum(s2, red);
+    if (threadIdx.x == 0) sq[(size_t)c * gridDim.x + blockIdx.x] = t;
+  }
+  const double bx = block_sum(bad_x, red);
+  const double by = block_sum(bad_y, red);
+  if (threadIdx.x == 0) {
+    cnt[2 * blockIdx.x] = bx;
+    cnt[2 * blockIdx.x + 1] = by;
+  }
 }
 
+// out: [d][5] = min, max, mean, M2, count; then bad_x, bad_y; then ny sums of squares.
 __global__ __launch_bounds__(XS_THREADS) void k_xstats_final(const double* __restrict__ partials,
-                                                             int nparts, int d,
+                                                             int nparts, int d, int ny,
                                                              double* __restrict__ out) {
   __shared__ Welford sh[XS_THREADS];
+  __shared__ double red[XS_THREADS / 64];
   for (int k = 0; k < d; ++k) {
     Welford w = {0.0, 0.0, 0.0, INFINITY, -INFINITY};
     for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
       const double* p = partials + ((size_t)k * nparts + b) * 5;
-      Welford o = {p[0], p[1], p[2], p[3], p[4]};
+      const Welford o = {p[0], p[1], p[2], p[3], p[4]};
       w = wf_merge(w, o);
     }
     sh[threadIdx.x] = w;
@@ -90,6 +151,25 @@ __global__ __launch_bounds__(XS_THREADS) void k_xstats_final(const double* __res
     }
     __syncthreads();
   }
+  const double* cnt = partials + (size_t)d * nparts * 5;
+  const double* sq = cnt + (size_t)nparts * 2;
+  double bx = 0.0, by = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
+    bx += cnt[2 * b];
+    by += cnt[2 * b + 1];
+  }
+  bx = block_sum(bx, red);
+  by = block_sum(by, red);
+  if (threadIdx.x == 0) {
+    out[5 * d] = bx;
+    out[5 * d + 1] = by;
+  }
+  for (int c = 0; c < ny; ++c) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) s += sq[(size_t)c * nparts + b];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) out[5 * d + 2 + c] = s;
+  }
 }
 
 // numpy's float modulo for a positive divisor (npy_divmod): fmod, then shift
@@ -104,264 +184,132 @@ __device__ inline double np_mod_pos(double a, double b) {
   return m;
 }
 
-constexpr int PREP_THREADS = 256;
-constexpr int PREP_LDS_TILES = 8192;
-
-// Knot mapping of every point and the spread-tile histogram.
-// The operation order mirrors the reference line by line so that the knot
-// positions are the same float64 values (no FMA contraction here).
-__global__ __launch_bounds__(PREP_THREADS) void k_prepare(const double* __restrict__ x, int64_t n,
-                                                          FlprGeom g, double* __restrict__ xmod,
-                                                          int32_t* __restrict__ cell,
-                                                          int32_t* __restrict__ tile,
-                                                          int32_t* __restrict__ tile_count,
-                                                          int ntiles) {
+// Knot position and spread cell of one coordinate, exactly as k_prepare
+// computes them (same float64 operation order, no contraction).
+__device__ inline double knot_of(const FlprGeom& g, int k, double v, int* m_out) {
 #pragma clang fp contract(off)
-  __shared__ int32_t hist[PREP_LDS_TILES];
-  const bool use_lds = ntiles <= PREP_LDS_TILES;
-  if (use_lds) {
-    for (int t = threadIdx.x; t < ntiles; t += blockDim.x) hist[t] = 0;
-    __syncthreads();
-  }
   const double two_pi = 2.0 * M_PI;
+  const double t = (v - g.mean[k]) / g.std_[k];
+  const double xs = t - g.centre[k];
+  double xn = (xs - g.xmin_s[k]) / g.scale_s[k];
+  xn = xn * g.ks[k];
+  xn = xn + g.idx0[k];
+  xn = xn + 0.5;
+  xn = np_mod_pos(xn, 1.0);
+  double kn = xn - 0.5;
+  kn = kn * 1.0;
+  kn = kn + g.shift[k];
+  kn = kn * two_pi;
+  const double xm = np_mod_pos(kn, two_pi);
+  *m_out = (int)rint(xm / g.hx[k]);
+  return xm;
+}
+
+constexpr int PREP_THREADS = 256;
+
+// Spread cell and sort key (tile-major cell index) of every point, from the
+// knot mapping of knot_of.
+__global__ __launch_bounds__(PREP_THREADS) void k_prepare(const double* __restrict__ x, int64_t n,
+                                                          FlprGeom g, int32_t* __restrict__ key,
+                                                          int32_t* __restrict__ key_count) {
+  int tc = 1;
+  for (int k = 0; k < g.d; ++k) tc *= g.tile[k];
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    int tid = 0;
+    int tid = 0, lid = 0;
     for (int k = 0; k < g.d; ++k) {
-      double v = x[i * g.d + k];
-      double t = (v - g.mean[k]) / g.std_[k];         // model.py:65
-      double xs = t - g.centre[k];                     // model.py:72
-      double xn = (xs - g.xmin_s[k]) / g.scale_s[k];  // convolution.py:186-188
-      xn = xn * g.ks[k];
-      xn = xn + g.idx0[k];
-      xn = xn + 0.5;                                   // convolution.py:198
-      xn = np_mod_pos(xn, 1.0);                        // convolution.py:199
-      double kn = xn - 0.5;                            // convolution.py:67
-      kn = kn * 1.0;                                   // nufftn_type1.py:159 (Fs/N = 1)
-      kn = kn + g.shift[k];                            // nufftn_type1.py:163-165
-      kn = kn * two_pi;                                // nufftn_type1.py:169
-      double xm = np_mod_pos(kn, two_pi);              // nufftn_type1.py:170
-      xmod[i * g.d + k] = xm;
-      int m = (int)rint(xm / g.hx[k]);                 // round half to even, as np.round
-      cell[i * g.d + k] = m;
-      int pm = (m > g.L[k] / 2) ? m - g.L[k] : m;
-      // the data support maps into [1 + msp, G - 2 - msp]; the clamp only
-      // keeps a malformed geometry from addressing outside the tile table
-      int s = min(max(pm + g.E[k], g.msp), g.G[k] - 1 - g.msp);
-      int tk = s / g.tile[k];
-      tid = tid * g.ntile[k] + tk;
+      int m;
+      knot_of(g, k, x[i * g.d + k], &m);
+      const int pm = (m > g.L[k] / 2) ? m - g.L[k] : m;
+      // the data support maps into [1 + msp, G - 2 - msp]; the clamp only keeps
+      // a malformed geometry from addressing outside the key table
+      const int s = min(max(pm + g.E[k], g.msp), g.G[k] - 1 - g.msp);
+      tid = tid * g.ntile[k] + s / g.tile[k];
+      lid = lid * g.tile[k] + s % g.tile[k];
     }
-    tile[i] = tid;
-    if (use_lds) atomicAdd(&hist[tid], 1);
-    else atomicAdd(&tile_count[tid], 1);
-  }
-  if (use_lds) {
-    __syncthreads();
-    for (int t = threadIdx.x; t < ntiles; t += blockDim.x)
-      if (hist[t]) atomicAdd(&tile_count[t], hist[t]);
+    const int kk = tid * tc + lid;
+    key[i] = kk;
+    atomicAdd(&key_count[kk], 1);
   }
 }
 
 constexpr int SCAN_THREADS = 1024;
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan(const int32_t* __restrict__ cnt, int nt,
-                                                       int32_t* __restrict__ start) {
-  __shared__ int32_t part[SCAN_THREADS];
-  int per = (nt + SCAN_THREADS - 1) / SCAN_THREADS;
-  int b = threadIdx.x * per;
-  int e = min(b + per, nt);
-  int s = 0;
-  for (int i = b; i < e; ++i) s += cnt[i];
-  part[threadIdx.x] = s;
+// Exclusive scan in three launches: block sums, a scan of the block sums, and a
+// block-local scan plus offset.  Every load is coalesced.
+__device__ inline int block_exclusive_scan(int v, int* sh, int* total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int incl = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(incl, off);
+    if (lane >= off) incl += t;
+  }
+  if (lane == 63) sh[wid] = incl;
   __syncthreads();
-  for (int off = 1; off < SCAN_THREADS; off <<= 1) {
-    int v = (int)threadIdx.x >= off ? part[threadIdx.x - off] : 0;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      const int t = sh[w];
+      sh[w] = run;
+      run += t;
+    }
+    sh[blockDim.x >> 6] = run;
   }
-  int run = part[threadIdx.x] - s;
-  for (int i = b; i < e; ++i) {
-    start[i] = run;
-    run += cnt[i];
-  }
-  if (threadIdx.x == SCAN_THREADS - 1) start[nt] = part[SCAN_THREADS - 1];
+  __syncthreads();
+  const int out = sh[wid] + incl - v;
+  *total = sh[blockDim.x >> 6];
+  __syncthreads();
+  return out;
 }
 
-constexpr int BIN_THREADS = 256;
-constexpr int BIN_PER_THREAD = 8;
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_blocks(const int32_t* __restrict__ cnt,
+                                                              int nt,
+                                                              int32_t* __restrict__ bsum) {
+  __shared__ int sh[SCAN_THREADS / 64 + 1];
+  const int i = blockIdx.x * SCAN_THREADS + threadIdx.x;
+  int total;
+  block_exclusive_scan(i < nt ? cnt[i] : 0, sh, &total);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
 
-// Scatter of point indices into tile order.  Each block ranks its points per
-// tile in LDS and reserves one range per (block, tile) with a single global
-// atomic, so global contention is per block, not per point.
-__global__ __launch_bounds__(BIN_THREADS) void k_bin(const int32_t* __restrict__ tile, int64_t n,
-                                                     int ntiles,
-                                                     const int32_t* __restrict__ start,
-                                                     int32_t* __restrict__ cursor,
-                                                     int32_t* __restrict__ order) {
-  __shared__ int32_t cnt[PREP_LDS_TILES];
-  __shared__ int32_t base[PREP_LDS_TILES];
-  const bool use_lds = ntiles <= PREP_LDS_TILES;
-  int64_t b0 = (int64_t)blockIdx.x * BIN_THREADS * BIN_PER_THREAD;
-  if (!use_lds) {
-    for (int j = 0; j < BIN_PER_THREAD; ++j) {
-      int64_t i = b0 + (int64_t)j * BIN_THREADS + threadIdx.x;
-      if (i < n) {
-        int t = tile[i];
-        order[start[t] + atomicAdd(&cursor[t], 1)] = (int32_t)i;
-      }
-    }
-    return;
-  }
-  for (int t = threadIdx.x; t < ntiles; t += blockDim.x) cnt[t] = 0;
-  __syncthreads();
-  int rank[BIN_PER_THREAD];
-  int tl[BIN_PER_THREAD];
-  for (int j = 0; j < BIN_PER_THREAD; ++j) {
-    int64_t i = b0 + (int64_t)j * BIN_THREADS + threadIdx.x;
-    tl[j] = -1;
-    if (i < n) {
-      tl[j] = tile[i];
-      rank[j] = atomicAdd(&cnt[tl[j]], 1);
-    }
-  }
-  __syncthreads();
-  for (int t = threadIdx.x; t < ntiles; t += blockDim.x)
-    base[t] = cnt[t] ? start[t] + atomicAdd(&cursor[t], cnt[t]) : 0;
-  __syncthreads();
-  for (int j = 0; j < BIN_PER_THREAD; ++j) {
-    int64_t i = b0 + (int64_t)j * BIN_THREADS + threadIdx.x;
-    if (tl[j] >= 0) order[base[tl[j]] + rank[j]] = (int32_t)i;
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(int32_t* __restrict__ bsum, int nb) {
+  __shared__ int sh[SCAN_THREADS / 64 + 1];
+  int run = 0;
+  for (int b0 = 0; b0 < nb; b0 += SCAN_THREADS) {
+    const int i = b0 + threadIdx.x;
+    int total;
+    const int ex = block_exclusive_scan(i < nb ? bsum[i] : 0, sh, &total);
+    if (i < nb) bsum[i] = run + ex;
+    run += total;
   }
 }
 
-constexpr int SPREAD_THREADS = 256;
-constexpr int MAX_CG = 16;  // channels per LDS group
-
-// statically unrolled channel loop: keeps the per-point channel values in registers
-#define CH_LOOP(stmt)                                    \
-  _Pragma("unroll") for (int c = 0; c < MAX_CG; ++c) {   \
-    if (c < cn) {                                        \
-      stmt;                                              \
-    }                                                    \
-  }
-
-template <int D>
-__device__ inline void tile_coords(const FlprGeom& g, int tid, int* t) {
-#pragma unroll
-  for (int k = D - 1; k >= 0; --k) {
-    t[k] = tid % g.ntile[k];
-    tid /= g.ntile[k];
-  }
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const int32_t* __restrict__ cnt,
+                                                             int nt,
+                                                             int32_t* __restrict__ bsum,
+                                                             int32_t* __restrict__ start) {
+  __shared__ int sh[SCAN_THREADS / 64 + 1];
+  const int i = blockIdx.x * SCAN_THREADS + threadIdx.x;
+  const int v = i < nt ? cnt[i] : 0;
+  const int base = bsum[blockIdx.x];
+  int total;
+  const int ex = block_exclusive_scan(v, sh, &total);  // synchronises the block
+  if (i < nt) start[i] = base + ex;
+  if (i == nt - 1) start[nt] = base + ex + v;
+  // every thread has read its block sum: clear it so the scratch is zero again
+  if (threadIdx.x == 0) bsum[blockIdx.x] = 0;
 }
 
-// Gaussian-gridding spread of one channel group for one spread tile.  The
-// tile's footprint (tile + 2 msp per dimension) is accumulated in LDS in
-// float64 and flushed to the global grid with float64 atomics (only halo cells
-// are shared with neighbouring tiles).
-template <int D, int MSP>
-__global__ __launch_bounds__(SPREAD_THREADS) void k_spread(
-    FlprGeom g, const double* __restrict__ xmod, const int32_t* __restrict__ cell,
-    const int32_t* __restrict__ order, const int32_t* __restrict__ tile_start, int with_ones,
-    const double* __restrict__ vy, int ny, const double* __restrict__ pr, int np, int cg,
-    double* __restrict__ grid, int64_t cstride) {
-  extern __shared__ double acc[];
-  constexpr int ntap = 2 * MSP + 1;
-  const int C = with_ones + ny + np;
-  const int c0 = blockIdx.y * cg;
-  const int cn = min(cg, C - c0);
-  int t[3] = {0, 0, 0};
-  tile_coords<D>(g, blockIdx.x, t);
-  int fp[3] = {1, 1, 1};
-  int org[3] = {0, 0, 0};  // storage index of the footprint origin
-  int fcells = 1;
-#pragma unroll
-  for (int k = 0; k < D; ++k) {
-    fp[k] = g.tile[k] + 2 * MSP;
-    org[k] = t[k] * g.tile[k] - MSP;
-    fcells *= fp[k];
-  }
-  for (int i = threadIdx.x; i < fcells * cn; i += blockDim.x) acc[i] = 0.0;
-  __syncthreads();
-  const int p0 = tile_start[blockIdx.x], p1 = tile_start[blockIdx.x + 1];
-  for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-    const int i = order[p];
-    double w[D][ntap];
-    int lc[3] = {0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      const double xm = xmod[(int64_t)i * D + k];
-      const int m = cell[(int64_t)i * D + k];
-#pragma unroll
-      for (int j = 0; j < ntap; ++j) {
-        const double diff = xm - g.hx[k] * (double)(m + j - MSP);
-        w[k][j] = exp(-(diff * diff * g.inv4tau[k]));
-      }
-      const int pm = (m > g.L[k] / 2) ? m - g.L[k] : m;
-      const int sk = min(max(pm + g.E[k], MSP), g.G[k] - 1 - MSP);
-      lc[k] = sk - t[k] * g.tile[k];  // in [0, tile); footprint index lc + j
-    }
-    double v[MAX_CG];
-#pragma unroll
-    for (int c = 0; c < MAX_CG; ++c) {
-      const int ch = c0 + c;
-      v[c] = 0.0;
-      if (c < cn) {
-        if (ch < with_ones) v[c] = 1.0;
-        else if (ch < with_ones + ny) v[c] = vy[(int64_t)i * ny + (ch - with_ones)];
-        else v[c] = pr[(int64_t)i * np + (ch - with_ones - ny)];
-      }
-    }
-    if constexpr (D == 1) {
-#pragma unroll
-      for (int j0 = 0; j0 < ntap; ++j0) {
-        const int idx = (lc[0] + j0) * cn;
-        CH_LOOP(atomicAdd(&acc[idx + c], w[0][j0] * v[c]));
-      }
-    } else if constexpr (D == 2) {
-#pragma unroll
-      for (int j0 = 0; j0 < ntap; ++j0) {
-#pragma unroll
-        for (int j1 = 0; j1 < ntap; ++j1) {
-          const double wv = w[0][j0] * w[1][j1];
-          const int idx = ((lc[0] + j0) * fp[1] + (lc[1] + j1)) * cn;
-          CH_LOOP(atomicAdd(&acc[idx + c], wv * v[c]));
-        }
-      }
-    } else {
-      for (int j0 = 0; j0 < ntap; ++j0) {
-#pragma unroll
-        for (int j1 = 0; j1 < ntap; ++j1) {
-          const double w01 = w[0][j0] * w[1][j1];
-#pragma unroll
-          for (int j2 = 0; j2 < ntap; ++j2) {
-            const double wv = w01 * w[2][j2];
-            const int idx = (((lc[0] + j0) * fp[1] + (lc[1] + j1)) * fp[2] + (lc[2] + j2)) * cn;
-            CH_LOOP(atomicAdd(&acc[idx + c], wv * v[c]));
-          }
-        }
-      }
-    }
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < fcells; e += blockDim.x) {
-    int rem = e;
-    int64_t lin = 0;
-    bool ok = true;
-    int sc[3] = {0, 0, 0};
-#pragma unroll
-    for (int k = D - 1; k >= 0; --k) {
-      sc[k] = org[k] + rem % fp[k];
-      rem /= fp[k];
-      if (sc[k] < 0 || sc[k] >= g.G[k]) ok = false;
-    }
-    if (!ok) continue;
-#pragma unroll
-    for (int k = 0; k < D; ++k) lin = lin * g.G[k] + sc[k];
-    for (int c = 0; c < cn; ++c) {
-      const double a = acc[e * cn + c];
-      if (a != 0.0) atomicAdd(&grid[(int64_t)(c0 + c) * cstride + lin], a);
-    }
+// Scatter of point indices into key order (one atomic per point on its cell's
+// cursor; about n / cells points share a cursor).
+__global__ __launch_bounds__(256) void k_bin(const int32_t* __restrict__ key, int64_t n,
+                                             const int32_t* __restrict__ start,
+                                             int32_t* __restrict__ cursor,
+                                             int32_t* __restrict__ order) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int k = key[i];
+    order[start[k] + atomicAdd(&cursor[k], 1)] = (int32_t)i;
   }
 }
 
@@ -388,15 +336,624 @@ __device__ inline void raw_locate(double v, double a, double b, int N, int* idx,
   *frac = (v - lo) / (hi - lo);
 }
 
+// Sorted structure-of-arrays copy of what the grid kernels need per point:
+//   wab  [2d][n]  A, B of the tap-weight factorisation per dimension
+//   vals [nv][n]  y columns then probe columns
+//   rawi [d][n]   raw-grid interval, rawf [d][n] fraction in it
 template <int D>
-struct Corners {
+__global__ __launch_bounds__(256) void k_pack(FlprGeom g, const double* __restrict__ x,
+                                              const int32_t* __restrict__ order, int64_t n,
+                                              const double* __restrict__ vy, int ny,
+                                              const double* __restrict__ pr, int np,
+                                              double* __restrict__ wab, double* __restrict__ vals,
+                                              int32_t* __restrict__ rawi,
+                                              double* __restrict__ rawf) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = order[p];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const double xv = x[i * D + k];
+      int m;
+      const double xm = knot_of(g, k, xv, &m);
+      const double delta = xm - g.hx[k] * (double)m;
+      wab[(2 * k) * n + p] = exp(-(delta * delta * g.inv4tau[k]));
+      wab[(2 * k + 1) * n + p] = exp(2.0 * delta * g.hx[k] * g.inv4tau[k]);
+      int ri;
+      double rf;
+      raw_locate(xv, g.graw0[k], g.graw1[k], g.n_grid[k], &ri, &rf);
+      rawi[k * n + p] = ri;
+      rawf[k * n + p] = rf;
+    }
+    for (int c = 0; c < ny; ++c) vals[c * n + p] = vy[i * ny + c];
+    for (int c = 0; c < np; ++c) vals[(int64_t)(ny + c) * n + p] = pr[i * np + c];
+  }
+}
+
+template <int D>
+__device__ inline void tile_coords(const FlprGeom& g, int tid, int* t) {
+#pragma unroll
+  for (int k = D - 1; k >= 0; --k) {
+    t[k] = tid % g.ntile[k];
+    tid /= g.ntile[k];
+  }
+}
+
+template <int D>
+__device__ inline void local_coords(const FlprGeom& g, int lc, int* l) {
+#pragma unroll
+  for (int k = D - 1; k >= 0; --k) {
+    l[k] = lc % g.tile[k];
+    lc /= g.tile[k];
+  }
+}
+
+// w[j] = exp(-(delta - hx (j - MSP))^2 c) from A, B and the constants G[j].
+template <int MSP>
+__device__ inline void tap_weights(double A, double B, const double* G, double* w) {
+  w[MSP] = A * G[MSP];
+  double bp = 1.0;
+#pragma unroll
+  for (int k = 1; k <= MSP; ++k) {
+    bp *= B;
+    w[MSP + k] = A * bp * G[MSP + k];
+  }
+  const double binv = 1.0 / B;
+  bp = 1.0;
+#pragma unroll
+  for (int k = 1; k <= MSP; ++k) {
+    bp *= binv;
+    w[MSP - k] = A * bp * G[MSP - k];
+  }
+}
+
+constexpr int SPREAD_THREADS = 256;
+
+// Register-accumulated spread (d = 1, 2).  Item = (cell, channel[, row j0]);
+// SPLIT items carry one footprint row (used when (2 MSP + 1)^2 float64
+// accumulators would not fit in registers).
+template <int D, int MSP, bool SPLIT>
+__global__ __launch_bounds__(SPREAD_THREADS) void k_spread_cells(
+    FlprGeom g, const int32_t* __restrict__ key_start, const double* __restrict__ wab,
+    const double* __restrict__ vals, int64_t n, int with_ones, int C, int cg,
+    double* __restrict__ part) {
+  extern __shared__ double lds[];
+  constexpr int ntap = 2 * MSP + 1;
+  const int c0 = blockIdx.y * cg;
+  const int cn = min(cg, C - c0);
+  int t[3] = {0, 0, 0};
+  tile_coords<D>(g, blockIdx.x, t);
+  int fp[3] = {1, 1, 1}, org[3] = {0, 0, 0};
+  int fcells = 1, tc = 1;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    fp[k] = g.tile[k] + 2 * MSP;
+    org[k] = t[k] * g.tile[k] - MSP;
+    fcells *= fp[k];
+    tc *= g.tile[k];
+  }
+  for (int i = threadIdx.x; i < fcells * cn; i += blockDim.x) lds[i] = 0.0;
+  double G[D][ntap];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const double c2 = g.hx[k] * g.hx[k] * g.inv4tau[k];
+#pragma unroll
+    for (int j = 0; j < ntap; ++j) G[k][j] = exp(-c2 * (double)((j - MSP) * (j - MSP)));
+  }
+  __syncthreads();
+  const int nsplit = SPLIT ? ntap : 1;
+  const int nitems = tc * cn * nsplit;
+  const int kbase = blockIdx.x * tc;
+  for (int item = threadIdx.x; item < nitems; item += blockDim.x) {
+    const int lc = item % tc;
+    const int rest = item / tc;
+    const int c = rest % cn;
+    const int js = rest / cn;  // footprint row of a SPLIT item
+    const int p0 = key_start[kbase + lc], p1 = key_start[kbase + lc + 1];
+    if (p0 == p1) continue;
+    const int ch = c0 + c;
+    const double* vch = (ch >= with_ones) ? vals + (int64_t)(ch - with_ones) * n : nullptr;
+    int l[3] = {0, 0, 0};
+    local_coords<D>(g, lc, l);
+    if constexpr (D == 1) {
+      double acc[ntap];
+#pragma unroll
+      for (int j = 0; j < ntap; ++j) acc[j] = 0.0;
+      for (int p = p0; p < p1; ++p) {
+        const double v = vch ? vch[p] : 1.0;
+        double w[ntap];
+        tap_weights<MSP>(wab[p], wab[n + p], G[0], w);
+#pragma unroll
+        for (int j = 0; j < ntap; ++j) acc[j] += v * w[j];
+      }
+#pragma unroll
+      for (int j = 0; j < ntap; ++j) atomicAdd(&lds[(l[0] + j) * cn + c], acc[j]);
+    } else if constexpr (!SPLIT) {
+      double acc[ntap][ntap];
+#pragma unroll
+      for (int a = 0; a < ntap; ++a)
+#pragma unroll
+        for (int b = 0; b < ntap; ++b) acc[a][b] = 0.0;
+      for (int p = p0; p < p1; ++p) {
+        const double v = vch ? vch[p] : 1.0;
+        double w0[ntap], w1[ntap];
+        tap_weights<MSP>(wab[p], wab[n + p], G[0], w0);
+        tap_weights<MSP>(wab[2 * n + p], wab[3 * n + p], G[1], w1);
+#pragma unroll
+        for (int a = 0; a < ntap; ++a) {
+          const double vw = v * w0[a];
+#pragma unroll
+          for (int b = 0; b < ntap; ++b) acc[a][b] += vw * w1[b];
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < ntap; ++a)
+#pragma unroll
+        for (int b = 0; b < ntap; ++b)
+          atomicAdd(&lds[((l[0] + a) * fp[1] + (l[1] + b)) * cn + c], acc[a][b]);
+    } else {
+      double acc[ntap];
+#pragma unroll
+      for (int b = 0; b < ntap; ++b) acc[b] = 0.0;
+      for (int p = p0; p < p1; ++p) {
+        const double v = vch ? vch[p] : 1.0;
+        double w0[ntap], w1[ntap];
+        tap_weights<MSP>(wab[p], wab[n + p], G[0], w0);
+        tap_weights<MSP>(wab[2 * n + p], wab[3 * n + p], G[1], w1);
+        double sel = 0.0;
+#pragma unroll
+        for (int a = 0; a < ntap; ++a) sel = (a == js) ? w0[a] : sel;
+        const double vw = v * sel;
+#pragma unroll
+        for (int b = 0; b < ntap; ++b) acc[b] += vw * w1[b];
+      }
+#pragma unroll
+      for (int b = 0; b < ntap; ++b)
+        atomicAdd(&lds[((l[0] + js) * fp[1] + (l[1] + b)) * cn + c], acc[b]);
+    }
+  }
+  __syncthreads();
+  // the tile's window goes to its own partial slot; k_spread_gather adds the
+  // overlapping windows per cell without atomics
+  double* dst = part + ((int64_t)blockIdx.x * C + c0) * fcells;
+  for (int e = threadIdx.x; e < fcells * cn; e += blockDim.x) {
+    const int c = e / fcells, cell = e % fcells;
+    dst[(int64_t)c * fcells + cell] = lds[cell * cn + c];
+  }
+}
+
+// 3-D spread: one thread per point scatters its (2 MSP + 1)^3 footprint into
+// the tile's LDS window with float64 atomics.
+template <int MSP>
+__global__ __launch_bounds__(SPREAD_THREADS) void k_spread_points3(
+    FlprGeom g, const int32_t* __restrict__ key_start, const double* __restrict__ wab,
+    const double* __restrict__ vals, int64_t n, int with_ones, int C, int cg,
+    double* __restrict__ part) {
+  extern __shared__ double lds[];
+  constexpr int D = 3;
+  constexpr int ntap = 2 * MSP + 1;
+  const int c0 = blockIdx.y * cg;
+  const int cn = min(cg, C - c0);
+  int t[3];
+  tile_coords<D>(g, blockIdx.x, t);
+  int fp[3], org[3];
+  int fcells = 1, tc = 1;
+  for (int k = 0; k < D; ++k) {
+    fp[k] = g.tile[k] + 2 * MSP;
+    org[k] = t[k] * g.tile[k] - MSP;
+    fcells *= fp[k];
+    tc *= g.tile[k];
+  }
+  for (int i = threadIdx.x; i < fcells * cn; i += blockDim.x) lds[i] = 0.0;
+  double G[D][ntap];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const double c2 = g.hx[k] * g.hx[k] * g.inv4tau[k];
+#pragma unroll
+    for (int j = 0; j < ntap; ++j) G[k][j] = exp(-c2 * (double)((j - MSP) * (j - MSP)));
+  }
+  __syncthreads();
+  const int kbase = blockIdx.x * tc;
+  const int p_begin = key_start[kbase], p_end = key_start[kbase + tc];
+  for (int p = p_begin + threadIdx.x; p < p_end; p += blockDim.x) {
+    // cell of this point: binary search of the tile's key table
+    int lo = 0, hi = tc - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (key_start[kbase + mid] <= p) lo = mid;
+      else hi = mid - 1;
+    }
+    int l[3];
+    local_coords<D>(g, lo, l);
+    double w0[ntap], w1[ntap], w2[ntap];
+    tap_weights<MSP>(wab[p], wab[n + p], G[0], w0);
+    tap_weights<MSP>(wab[2 * n + p], wab[3 * n + p], G[1], w1);
+    tap_weights<MSP>(wab[4 * n + p], wab[5 * n + p], G[2], w2);
+    for (int c = 0; c < cn; ++c) {
+      const int ch = c0 + c;
+      const double v = (ch >= with_ones) ? vals[(int64_t)(ch - with_ones) * n + p] : 1.0;
+#pragma unroll
+      for (int a = 0; a < ntap; ++a) {
+        const double va = v * w0[a];
+#pragma unroll
+        for (int b = 0; b < ntap; ++b) {
+          const double vab = va * w1[b];
+          const int base = (((l[0] + a) * fp[1] + (l[1] + b)) * fp[2] + l[2]) * cn + c;
+#pragma unroll
+          for (int e = 0; e < ntap; ++e) atomicAdd(&lds[base + e * cn], vab * w2[e]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // the tile's window goes to its own partial slot; k_spread_gather adds the
+  // overlapping windows per cell without atomics
+  double* dst = part + ((int64_t)blockIdx.x * C + c0) * fcells;
+  for (int e = threadIdx.x; e < fcells * cn; e += blockDim.x) {
+    const int c = e / fcells, cell = e % fcells;
+    dst[(int64_t)c * fcells + cell] = lds[cell * cn + c];
+  }
+}
+
+
+__device__ inline int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
+
+// Sum of the overlapping tile windows for every spread-grid cell (channel =
+// blockIdx.y); also the per-channel grid sums used by the centring step.
+template <int D>
+__global__ __launch_bounds__(256) void k_spread_gather(FlprGeom g, const double* __restrict__ part,
+                                                       int C, int64_t cells,
+                                                       double* __restrict__ grid,
+                                                       double* __restrict__ sums) {
+  __shared__ double red[4];
+  const int c = blockIdx.y;
+  const int msp = g.msp;
+  int fp[3] = {1, 1, 1};
+  int fcells = 1;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    fp[k] = g.tile[k] + 2 * msp;
+    fcells *= fp[k];
+  }
+  double csum = 0.0;
+  for (int64_t lin = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; lin < cells;
+       lin += (int64_t)gridDim.x * blockDim.x) {
+    int sc[3] = {0, 0, 0}, tlo[3] = {0, 0, 0}, thi[3] = {0, 0, 0};
+    int64_t rem = lin;
+#pragma unroll
+    for (int k = D - 1; k >= 0; --k) {
+      sc[k] = (int)(rem % g.G[k]);
+      rem /= g.G[k];
+      // windows [t*tile - msp, t*tile + tile + msp) that contain sc
+      tlo[k] = max(floordiv(sc[k] - g.tile[k] - msp, g.tile[k]) + 1, 0);
+      thi[k] = min(floordiv(sc[k] + msp, g.tile[k]), g.ntile[k] - 1);
+    }
+    double v = 0.0;
+    if constexpr (D == 1) {
+      for (int t0 = tlo[0]; t0 <= thi[0]; ++t0)
+        v += part[((int64_t)t0 * C + c) * fcells + (sc[0] - (t0 * g.tile[0] - msp))];
+    } else if constexpr (D == 2) {
+      for (int t0 = tlo[0]; t0 <= thi[0]; ++t0)
+        for (int t1 = tlo[1]; t1 <= thi[1]; ++t1) {
+          const int64_t tile = (int64_t)t0 * g.ntile[1] + t1;
+          const int l0 = sc[0] - (t0 * g.tile[0] - msp), l1 = sc[1] - (t1 * g.tile[1] - msp);
+          v += part[(tile * C + c) * fcells + l0 * fp[1] + l1];
+        }
+    } else {
+      for (int t0 = tlo[0]; t0 <= thi[0]; ++t0)
+        for (int t1 = tlo[1]; t1 <= thi[1]; ++t1)
+          for (int t2 = tlo[2]; t2 <= thi[2]; ++t2) {
+            const int64_t tile = ((int64_t)t0 * g.ntile[1] + t1) * g.ntile[2] + t2;
+            const int l0 = sc[0] - (t0 * g.tile[0] - msp), l1 = sc[1] - (t1 * g.tile[1] - msp),
+                      l2 = sc[2] - (t2 * g.tile[2] - msp);
+            v += part[(tile * C + c) * fcells + (l0 * fp[1] + l1) * fp[2] + l2];
+          }
+    }
+    grid[(int64_t)c * cells + lin] = v;
+    csum += v;
+  }
+  csum = block_sum(csum, red);
+  if (threadIdx.x == 0) atomicAdd(&sums[c], csum);
+}
+
+// Centre every channel c >= 1 on the grid: F_c -= shift_c F_0 with
+// shift_c = sum F_c / sum F_0 (the kernel-weighted mean of the channel).  The
+// per-node solve adds shift_c times the ones-channel moments back, which is
+// exact by linearity; the float32 convolution then only carries y - shift.
+__global__ __launch_bounds__(256) void k_grid_center(double* __restrict__ grid, int64_t cells,
+                                                     const double* __restrict__ sums,
+                                                     double* __restrict__ shift,
+                                                     float* __restrict__ grid32) {
+  const int c = blockIdx.y;
+  const double s0 = sums[0];
+  const double sh = (c == 0 || s0 == 0.0) ? 0.0 : sums[c] / s0;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && shift) shift[c] = sh;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cells;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double v = grid[(int64_t)c * cells + i];
+    if (c > 0) {
+      v = v - sh * grid[i];
+      grid[(int64_t)c * cells + i] = v;
+    }
+    if (grid32) grid32[(int64_t)c * cells + i] = (float)v;
+  }
+}
+
+// Corner pair (qa <= qb) of the e-th entry of a cell's interpolation Gram matrix.
+template <int D>
+__device__ inline void gram_pair(int e, int* qa, int* qb) {
+  constexpr int nq = 1 << D;
+  int a = 0;
+  while (a < nq - 1 && e >= nq - a) {
+    e -= nq - a;
+    ++a;
+  }
+  *qa = a;
+  *qb = a + e;
+}
+
+constexpr int ADJ_THREADS = 256;
+
+// Adjoint interpolation sums on the raw grid for one channel group, per
+// spread tile:
+//   mode 0: q[c][node]   += v_ic w_i(node)          (value channels)
+//   mode 1: W[e][rcell]  += w_i(qa) w_i(qb)          (Gram entries)
+// Item = (spread cell, channel); the points of a spread cell touch at most
+// three raw intervals per dimension.  Accumulated in registers, then in the
+// tile's LDS window, which is written to the tile's partial slot
+// (k_adjoint_gather adds the overlapping windows).
+template <int D>
+__global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(
+    FlprGeom g, const int32_t* __restrict__ key_start, const int32_t* __restrict__ rawi,
+    const double* __restrict__ rawf, const double* __restrict__ vals, int64_t n, int nch,
+    int mode, int cg, double* __restrict__ out, int64_t out_stride, double* __restrict__ part) {
+  extern __shared__ double lds[];
+  constexpr int NA = (D == 1) ? 4 : (D == 2 ? 16 : 64);   // 4^D node accumulators
+  constexpr int NC = (D == 1) ? 3 : (D == 2 ? 9 : 27);    // 3^D cell accumulators
+  const int c0 = blockIdx.y * cg;
+  const int cn = min(cg, nch - c0);
+  int t[3] = {0, 0, 0};
+  tile_coords<D>(g, blockIdx.x, t);
+  int ext[3] = {1, 1, 1}, lb[3] = {0, 0, 0}, lim[3] = {1, 1, 1};
+  int wcells = 1, tc = 1;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    ext[k] = g.tile[k] + 5;
+    lb[k] = t[k] * g.tile[k] - g.E[k] + (g.n_grid[k] - 1) / 2 - 2;
+    lim[k] = (mode == 0) ? g.n_grid[k] : g.n_grid[k] - 1;
+    wcells *= ext[k];
+    tc *= g.tile[k];
+  }
+  for (int i = threadIdx.x; i < wcells * cn; i += blockDim.x) lds[i] = 0.0;
+  __syncthreads();
+  const int kbase = blockIdx.x * tc;
+  const int nitems = tc * cn;
+  for (int item = threadIdx.x; item < nitems; item += blockDim.x) {
+    const int lc = item % tc;
+    const int c = item / tc;
+    const int p0 = key_start[kbase + lc], p1 = key_start[kbase + lc + 1];
+    if (p0 == p1) continue;
+    int l[3] = {0, 0, 0};
+    local_coords<D>(g, lc, l);
+    int ib[3] = {0, 0, 0};  // lowest raw interval the cell's points can fall in
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+      ib[k] = (t[k] * g.tile[k] + l[k] - g.E[k]) + (g.n_grid[k] - 1) / 2 - 1;
+    const int ch = c0 + c;
+    int qa = 0, qb = 0;
+    if (mode == 1) gram_pair<D>(ch, &qa, &qb);
+    double acc[NA];
+#pragma unroll
+    for (int a = 0; a < NA; ++a) acc[a] = 0.0;
+    for (int p = p0; p < p1; ++p) {
+      int rel[3] = {0, 0, 0};
+      double f[3] = {0, 0, 0};
+      bool inside = true;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        rel[k] = rawi[k * n + p] - ib[k];
+        f[k] = rawf[k * n + p];
+        if (rel[k] < 0 || rel[k] > 2) inside = false;
+      }
+      if (mode == 0) {
+        const double v = vals[(int64_t)ch * n + p];
+        if (!inside) {
+#pragma unroll
+          for (int q = 0; q < (1 << D); ++q) {
+            double w = v;
+            int64_t node = 0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+              const int bit = (q >> (D - 1 - k)) & 1;
+              w *= bit ? f[k] : 1.0 - f[k];
+              node = node * g.n_grid[k] + rawi[k * n + p] + bit;
+            }
+            atomicAdd(&out[(int64_t)ch * out_stride + node], w);
+          }
+          continue;
+        }
+        double wn[3][4];
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+            wn[k][a] = (a == rel[k]) ? 1.0 - f[k] : ((a == rel[k] + 1) ? f[k] : 0.0);
+        if constexpr (D == 1) {
+#pragma unroll
+          for (int a = 0; a < 4; ++a) acc[a] += v * wn[0][a];
+        } else if constexpr (D == 2) {
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            const double va = v * wn[0][a];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a * 4 + b] += va * wn[1][b];
+          }
+        } else {
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+              const double vab = v * wn[0][a] * wn[1][b];
+#pragma unroll
+              for (int e = 0; e < 4; ++e) acc[(a * 4 + b) * 4 + e] += vab * wn[2][e];
+            }
+        }
+      } else {
+        double wa = 1.0, wb = 1.0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          const int ba = (qa >> (D - 1 - k)) & 1, bb = (qb >> (D - 1 - k)) & 1;
+          wa *= ba ? f[k] : 1.0 - f[k];
+          wb *= bb ? f[k] : 1.0 - f[k];
+        }
+        const double val = wa * wb;
+        if (!inside) {
+          int64_t rc = 0;
+#pragma unroll
+          for (int k = 0; k < D; ++k) rc = rc * (g.n_grid[k] - 1) + rawi[k * n + p];
+          atomicAdd(&out[(int64_t)ch * out_stride + rc], val);
+          continue;
+        }
+        int pos = 0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) pos = pos * 3 + rel[k];
+#pragma unroll
+        for (int a = 0; a < NC; ++a) acc[a] += (a == pos) ? val : 0.0;
+      }
+    }
+    // flush into the LDS window (node window for mode 0, cell window for mode 1)
+    const int na = (mode == 0) ? NA : NC;
+    const int side = (mode == 0) ? 4 : 3;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      if (a >= na) break;
+      int rem = a, widx = 0;
+      int off[3] = {0, 0, 0};
+#pragma unroll
+      for (int k = D - 1; k >= 0; --k) {
+        off[k] = rem % side;
+        rem /= side;
+      }
+#pragma unroll
+      for (int k = 0; k < D; ++k) widx = widx * ext[k] + (ib[k] + off[k] - lb[k]);
+      if (acc[a] != 0.0) atomicAdd(&lds[widx * cn + c], acc[a]);
+    }
+  }
+  __syncthreads();
+  double* dst = part + ((int64_t)blockIdx.x * nch + c0) * wcells;
+  for (int e = threadIdx.x; e < wcells * cn; e += blockDim.x) {
+    const int c = e / wcells, cell = e % wcells;
+    dst[(int64_t)c * wcells + cell] = lds[cell * cn + c];
+  }
+}
+
+// Adds the overlapping adjoint windows into out (nodes for mode 0, raw cells
+// for mode 1), one thread per (entry, channel = blockIdx.y), no atomics.
+template <int D>
+__global__ __launch_bounds__(256) void k_adjoint_gather(FlprGeom g, const double* __restrict__ part,
+                                                        int nch, int mode, int64_t count,
+                                                        double* __restrict__ out) {
+  const int c = blockIdx.y;
+  int ext[3] = {1, 1, 1}, off[3] = {0, 0, 0}, lim[3] = {1, 1, 1};
+  int wcells = 1;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    ext[k] = g.tile[k] + 5;
+    off[k] = -g.E[k] + (g.n_grid[k] - 1) / 2 - 2;
+    lim[k] = (mode == 0) ? g.n_grid[k] : g.n_grid[k] - 1;
+    wcells *= ext[k];
+  }
+  for (int64_t lin = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; lin < count;
+       lin += (int64_t)gridDim.x * blockDim.x) {
+    int r[3] = {0, 0, 0}, tlo[3] = {0, 0, 0}, thi[3] = {0, 0, 0};
+    int64_t rem = lin;
+#pragma unroll
+    for (int k = D - 1; k >= 0; --k) {
+      r[k] = (int)(rem % lim[k]);
+      rem /= lim[k];
+      // windows [t*tile + off, t*tile + off + ext) that contain r
+      tlo[k] = max(floordiv(r[k] - off[k] - ext[k], g.tile[k]) + 1, 0);
+      thi[k] = min(floordiv(r[k] - off[k], g.tile[k]), g.ntile[k] - 1);
+    }
+    double v = 0.0;
+    if constexpr (D == 1) {
+      for (int t0 = tlo[0]; t0 <= thi[0]; ++t0)
+        v += part[((int64_t)t0 * nch + c) * wcells + (r[0] - (t0 * g.tile[0] + off[0]))];
+    } else if constexpr (D == 2) {
+      for (int t0 = tlo[0]; t0 <= thi[0]; ++t0)
+        for (int t1 = tlo[1]; t1 <= thi[1]; ++t1) {
+          const int64_t tile = (int64_t)t0 * g.ntile[1] + t1;
+          const int l0 = r[0] - (t0 * g.tile[0] + off[0]), l1 = r[1] - (t1 * g.tile[1] + off[1]);
+          v += part[(tile * nch + c) * wcells + l0 * ext[1] + l1];
+        }
+    } else {
+      for (int t0 = tlo[0]; t0 <= thi[0]; ++t0)
+        for (int t1 = tlo[1]; t1 <= thi[1]; ++t1)
+          for (int t2 = tlo[2]; t2 <= thi[2]; ++t2) {
+            const int64_t tile = ((int64_t)t0 * g.ntile[1] + t1) * g.ntile[2] + t2;
+            const int l0 = r[0] - (t0 * g.tile[0] + off[0]), l1 = r[1] - (t1 * g.tile[1] + off[1]),
+                      l2 = r[2] - (t2 * g.tile[2] + off[2]);
+            v += part[(tile * nch + c) * wcells + (l0 * ext[1] + l1) * ext[2] + l2];
+          }
+    }
+    out[(int64_t)c * count + lin] += v;  // the kernel's rare out-of-window atomics landed first
+  }
+}
+
+constexpr int QF_THREADS = 256;
+
+// quad[h][c] = sum_cell sum_{a,b} W_ab(cell) m(h,c,node_a) m(h,c,node_b)
+template <int D>
+__global__ __launch_bounds__(QF_THREADS) void k_quadform(FlprGeom g, const double* __restrict__ m,
+                                                         int64_t nodes,
+                                                         const double* __restrict__ gram,
+                                                         int64_t rcells,
+                                                         double* __restrict__ quad) {
+  __shared__ double red[QF_THREADS / 64];
+  constexpr int NQ = 1 << D;
+  const int hc = blockIdx.y;  // h * ncol + c
+  const double* mm = m + (int64_t)hc * nodes;
+  double s = 0.0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rcells;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    int id[3] = {0, 0, 0};
+    int64_t rem = r;
+#pragma unroll
+    for (int k = D - 1; k >= 0; --k) {
+      id[k] = (int)(rem % (g.n_grid[k] - 1));
+      rem /= (g.n_grid[k] - 1);
+    }
+    double mv[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      int64_t node = 0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) node = node * g.n_grid[k] + id[k] + ((q >> (D - 1 - k)) & 1);
+      mv[q] = mm[node];
+    }
+    int e = 0;
+#pragma unroll
+    for (int a = 0; a < NQ; ++a)
+#pragma unroll
+      for (int b = a; b < NQ; ++b) {
+        const double w = gram[(int64_t)e * rcells + r];
+        s += (a == b ? 1.0 : 2.0) * w * mv[a] * mv[b];
+        ++e;
+      }
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) atomicAdd(&quad[hc], s);
+}
+
+template <int D>
+struct CornersT {
   int64_t node[1 << D];
   double w[1 << D];
 };
 
 template <int D>
-__device__ inline Corners<D> corners(const FlprGeom& g, const double* __restrict__ xp) {
-  Corners<D> c;
+__device__ inline CornersT<D> corners(const FlprGeom& g, const double* __restrict__ xp) {
+  CornersT<D> c;
   int id[D];
   double fr[D];
 #pragma unroll
@@ -417,170 +974,22 @@ __device__ inline Corners<D> corners(const FlprGeom& g, const double* __restrict
   return c;
 }
 
-constexpr int ADJ_THREADS = 256;
-constexpr int ADJ_MARGIN = 5;
-
-// Adjoint of the multilinear interpolation for the probe columns, one block
-// per spread tile and channel group.  Raw nodes near the tile are accumulated
-// in LDS; a point whose nodes fall outside the LDS window goes to global atomics.
-template <int D>
-__global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(FlprGeom g, const double* __restrict__ x,
-                                                         const int32_t* __restrict__ order,
-                                                         const int32_t* __restrict__ tile_start,
-                                                         const double* __restrict__ pr, int np,
-                                                         int cg, double* __restrict__ q,
-                                                         int64_t nodes) {
-  extern __shared__ double acc[];
-  const int c0 = blockIdx.y * cg;
-  const int cn = min(cg, np - c0);
-  int t[3] = {0, 0, 0};
-  tile_coords<D>(g, blockIdx.x, t);
-  int ext[3] = {1, 1, 1}, lb[3] = {0, 0, 0};
-  int cells = 1;
-#pragma unroll
-  for (int k = 0; k < D; ++k) {
-    ext[k] = g.tile[k] + ADJ_MARGIN;
-    lb[k] = t[k] * g.tile[k] - g.E[k] + (g.n_grid[k] - 1) / 2 - 2;
-    cells *= ext[k];
-  }
-  for (int i = threadIdx.x; i < cells * cn; i += blockDim.x) acc[i] = 0.0;
-  __syncthreads();
-  const int p0 = tile_start[blockIdx.x], p1 = tile_start[blockIdx.x + 1];
-  for (int p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-    const int i = order[p];
-    const Corners<D> cr = corners<D>(g, x + (int64_t)i * D);
-    // the lowest corner decides whether the whole cell lies in the window
-    bool inside = true;
-    int base_l[3] = {0, 0, 0};
-    {
-      int64_t rem = cr.node[0];
-#pragma unroll
-      for (int k = D - 1; k >= 0; --k) {
-        const int id = (int)(rem % g.n_grid[k]);
-        rem /= g.n_grid[k];
-        base_l[k] = id - lb[k];
-        if (base_l[k] < 0 || base_l[k] + 1 >= ext[k]) inside = false;
-      }
-    }
-#pragma unroll
-    for (int qq = 0; qq < (1 << D); ++qq) {
-      int llin = 0;
-#pragma unroll
-      for (int k = 0; k < D; ++k) llin = llin * ext[k] + base_l[k] + ((qq >> (D - 1 - k)) & 1);
-      for (int c = 0; c < cn; ++c) {
-        const double v = cr.w[qq] * pr[(int64_t)i * np + c0 + c];
-        if (inside) atomicAdd(&acc[llin * cn + c], v);
-        else atomicAdd(&q[(int64_t)(c0 + c) * nodes + cr.node[qq]], v);
-      }
-    }
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < cells; e += blockDim.x) {
-    int rem = e;
-    int64_t lin = 0;
-    bool ok = true;
-    int sc[3] = {0, 0, 0};
-#pragma unroll
-    for (int k = D - 1; k >= 0; --k) {
-      sc[k] = lb[k] + rem % ext[k];
-      rem /= ext[k];
-      if (sc[k] < 0 || sc[k] >= g.n_grid[k]) ok = false;
-    }
-    if (!ok) continue;
-#pragma unroll
-    for (int k = 0; k < D; ++k) lin = lin * g.n_grid[k] + sc[k];
-    for (int c = 0; c < cn; ++c) {
-      const double a = acc[e * cn + c];
-      if (a != 0.0) atomicAdd(&q[(int64_t)(c0 + c) * nodes + lin], a);
-    }
-  }
-}
-
-constexpr int RSS_THREADS = 256;
-constexpr int RSS_HC = 8;  // bandwidths per block row
-
-__device__ inline double block_sum(double v, double* sh) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) sh[wid] = v;
-  __syncthreads();
-  double r = 0.0;
-  if (threadIdx.x == 0)
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += sh[w];
-  __syncthreads();
-  return r;
-}
-
-// Residual sums per (bandwidth, output column).  grid.y = bandwidth chunk,
-// grid.z = output column.  Complex columns are (re, im) pairs of m and y.
-template <int D>
-__global__ __launch_bounds__(RSS_THREADS) void k_rss(FlprGeom g, const double* __restrict__ x,
-                                                     int64_t n, const double* __restrict__ y,
-                                                     int ncol_m, int cplx,
-                                                     const double* __restrict__ m, int dh,
-                                                     int64_t nodes, double* __restrict__ rss) {
-  __shared__ double sh[RSS_THREADS / 64];
-  const int h0 = blockIdx.y * RSS_HC;
-  const int nh = min(RSS_HC, dh - h0);
-  const int col = blockIdx.z;  // output column
-  const int ncol_out = cplx ? ncol_m / 2 : ncol_m;
-  double acc[RSS_HC];
-#pragma unroll
-  for (int j = 0; j < RSS_HC; ++j) acc[j] = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const Corners<D> c = corners<D>(g, x + i * D);
-    if (cplx) {
-      const double yr = y[i * ncol_m + 2 * col], yi = y[i * ncol_m + 2 * col + 1];
-#pragma unroll
-      for (int j = 0; j < RSS_HC; ++j) {
-        if (j < nh) {
-          const double* mr = m + ((int64_t)(h0 + j) * ncol_m + 2 * col) * nodes;
-          const double* mi = mr + nodes;
-          double vr = 0.0, vi = 0.0;
-#pragma unroll
-          for (int q = 0; q < (1 << D); ++q) {
-            vr += c.w[q] * mr[c.node[q]];
-            vi += c.w[q] * mi[c.node[q]];
-          }
-          acc[j] += (yr - vr) * (yr - vr) + (yi - vi) * (yi - vi);
-        }
-      }
-    } else {
-      const double yv = y[i * ncol_m + col];
-#pragma unroll
-      for (int j = 0; j < RSS_HC; ++j) {
-        if (j < nh) {
-          const double* mm = m + ((int64_t)(h0 + j) * ncol_m + col) * nodes;
-          double v = 0.0;
-#pragma unroll
-          for (int q = 0; q < (1 << D); ++q) v += c.w[q] * mm[c.node[q]];
-          acc[j] += (yv - v) * (yv - v);
-        }
-      }
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < RSS_HC; ++j) {
-    const double s = block_sum(acc[j], sh);
-    if (threadIdx.x == 0 && j < nh) atomicAdd(&rss[(h0 + j) * ncol_out + col], s);
-  }
-}
+constexpr int EVAL_THREADS = 256;
 
 template <int D>
-__global__ __launch_bounds__(RSS_THREADS) void k_interp_eval(FlprGeom g,
-                                                             const double* __restrict__ x,
-                                                             int64_t n,
-                                                             const double* __restrict__ m,
-                                                             int ncol, int64_t nodes,
-                                                             double* __restrict__ out,
-                                                             const double* __restrict__ yvar,
-                                                             double* __restrict__ expsum) {
-  __shared__ double sh[RSS_THREADS / 64];
+__global__ __launch_bounds__(EVAL_THREADS) void k_interp_eval(FlprGeom g,
+                                                              const double* __restrict__ x,
+                                                              int64_t n,
+                                                              const double* __restrict__ m,
+                                                              int ncol, int64_t nodes,
+                                                              double* __restrict__ out,
+                                                              const double* __restrict__ yvar,
+                                                              double* __restrict__ expsum) {
+  __shared__ double sh[EVAL_THREADS / 64];
   double es = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const Corners<D> c = corners<D>(g, x + i * D);
+    const CornersT<D> c = corners<D>(g, x + i * D);
     for (int col = 0; col < ncol; ++col) {
       const double* mm = m + (int64_t)col * nodes;
       double v = 0.0;
@@ -596,40 +1005,21 @@ __global__ __launch_bounds__(RSS_THREADS) void k_interp_eval(FlprGeom g,
   }
 }
 
-__global__ void k_f64_to_f32(const double* __restrict__ s, float* __restrict__ d, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    d[i] = (float)s[i];
-}
-
-int lds_group(int64_t cells_per_channel, int channels, int64_t budget = 64 * 1024) {
+int lds_group(int64_t cells_per_channel, int channels, int cap, int64_t budget = 64 * 1024) {
   const int64_t per = cells_per_channel * (int64_t)sizeof(double);
   int cg = (int)(budget / per);
   if (cg < 1) cg = 1;
-  if (cg > MAX_CG) cg = MAX_CG;
+  if (cg > cap) cg = cap;
   if (cg > channels) cg = channels;
   return cg;
 }
 
-template <int D, int MSP>
-void launch_spread(dim3 grd, size_t lds, hipStream_t st, const FlprGeom& g, const double* xmod,
-                   const int32_t* cell, const int32_t* order, const int32_t* ts, int wo,
-                   const double* vy, int ny, const double* pr, int np, int cg, double* grid,
-                   int64_t cstride) {
-  hipLaunchKernelGGL((k_spread<D, MSP>), grd, dim3(SPREAD_THREADS), lds, st, g, xmod, cell, order,
-                     ts, wo, vy, ny, pr, np, cg, grid, cstride);
-}
-
-template <int D>
-void launch_spread_d(int msp, dim3 grd, size_t lds, hipStream_t st, const FlprGeom& g,
-                     const double* xmod, const int32_t* cell, const int32_t* order,
-                     const int32_t* ts, int wo, const double* vy, int ny, const double* pr, int np,
-                     int cg, double* grid, int64_t cstride) {
-  switch (msp) {
-    case 4: launch_spread<D, 4>(grd, lds, st, g, xmod, cell, order, ts, wo, vy, ny, pr, np, cg, grid, cstride); break;
-    case 5: launch_spread<D, 5>(grd, lds, st, g, xmod, cell, order, ts, wo, vy, ny, pr, np, cg, grid, cstride); break;
-    default: launch_spread<D, 6>(grd, lds, st, g, xmod, cell, order, ts, wo, vy, ny, pr, np, cg, grid, cstride); break;
-  }
+template <int D, int MSP, bool SPLIT>
+void launch_cells(dim3 grd, size_t lds, hipStream_t st, const FlprGeom& g, const int32_t* ks,
+                  const double* wab, const double* vals, int64_t n, int wo, int C, int cg,
+                  double* part) {
+  hipLaunchKernelGGL((k_spread_cells<D, MSP, SPLIT>), grd, dim3(SPREAD_THREADS), lds, st, g, ks,
+                     wab, vals, n, wo, C, cg, part);
 }
 
 }  // namespace
@@ -651,118 +1041,188 @@ int flpr_last_hip_error(char* buf, int32_t len) {
   return e;
 }
 
-int flpr_xstats(const double* x, int64_t n, int32_t d, double* partials, double* out,
-                void* stream) {
+int flpr_xstats(const double* x, int64_t n, int32_t d, const double* y, int32_t ny,
+                double* partials, double* out, void* stream) {
   FLPR_CLEAR_ERROR();
-  if (!x || !partials || !out || n < 1 || d < 1 || d > FLPR_MAX_DIM) return -1;
+  if (!x || !partials || !out || n < 1 || d < 1 || d > FLPR_MAX_DIM || ny < 0 || ny > MAXY ||
+      (ny > 0 && !y))
+    return -1;
   hipLaunchKernelGGL(k_xstats, dim3(XS_BLOCKS), dim3(XS_THREADS), 0, as_stream(stream), x, n, d,
-                     partials);
+                     y, ny, partials);
   FLPR_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_xstats_final, dim3(1), dim3(XS_THREADS), 0, as_stream(stream), partials,
-                     XS_BLOCKS, d, out);
+                     XS_BLOCKS, d, ny, out);
   FLPR_CHECK_LAUNCH();
   return 0;
 }
 
-int flpr_points_prepare(const double* x, int64_t n, const FlprGeom* g, double* xmod,
-                        int32_t* cell, int32_t* tile, int32_t* tile_count, void* stream) {
+int flpr_points_prepare(const double* x, int64_t n, const FlprGeom* g, int32_t* key,
+                        int32_t* key_count, void* stream) {
   FLPR_CLEAR_ERROR();
-  if (!x || !xmod || !cell || !tile || !tile_count || n < 1 || !flpr_geom_ok(g)) return -1;
-  const int64_t nt = flpr_ntiles(g);
-  if (nt > ((int64_t)1 << 30)) return -1;
-  hipLaunchKernelGGL(k_prepare, dim3(flpr_grid_for(n, PREP_THREADS, 2048)), dim3(PREP_THREADS), 0,
-                     as_stream(stream), x, n, *g, xmod, cell, tile, tile_count, (int)nt);
+  if (!x || !key || !key_count || n < 1 || !flpr_geom_ok(g)) return -1;
+  if (flpr_nkeys(g) > ((int64_t)1 << 30)) return -1;
+  hipLaunchKernelGGL(k_prepare, dim3(flpr_grid_for(n, PREP_THREADS, 4096)), dim3(PREP_THREADS), 0,
+                     as_stream(stream), x, n, *g, key, key_count);
   FLPR_CHECK_LAUNCH();
   return 0;
 }
 
-int flpr_points_bin(const int32_t* tile, int64_t n, int32_t ntiles, const int32_t* tile_count,
-                    int32_t* tile_start, int32_t* tile_cursor, int32_t* order, void* stream) {
+int flpr_points_bin(const int32_t* key, int64_t n, int32_t nkeys, const int32_t* key_count,
+                    int32_t* key_start, int32_t* key_cursor, int32_t* order, void* stream) {
   FLPR_CLEAR_ERROR();
-  if (!tile || !tile_count || !tile_start || !tile_cursor || !order || n < 1 || ntiles < 1 ||
+  if (!key || !key_count || !key_start || !key_cursor || !order || n < 1 || nkeys < 1 ||
       n > INT32_MAX)
     return -1;
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_THREADS), 0, as_stream(stream), tile_count,
-                     ntiles, tile_start);
+  // key_cursor doubles as the block-sum scratch of the scan; k_scan_apply
+  // clears each entry again before the scatter uses the cursors
+  const int nb = (nkeys + SCAN_THREADS - 1) / SCAN_THREADS;
+  if (nb > nkeys) return -3;
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(SCAN_THREADS), 0, st, key_count, nkeys,
+                     key_cursor);
   FLPR_CHECK_LAUNCH();
-  const int64_t per_block = (int64_t)BIN_THREADS * BIN_PER_THREAD;
-  const int blocks = (int)((n + per_block - 1) / per_block);
-  hipLaunchKernelGGL(k_bin, dim3(blocks), dim3(BIN_THREADS), 0, as_stream(stream), tile, n,
-                     ntiles, tile_start, tile_cursor, order);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, st, key_cursor, nb);
+  FLPR_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_THREADS), 0, st, key_count, nkeys,
+                     key_cursor, key_start);
+  FLPR_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_bin, dim3(flpr_grid_for(n, 256, 4096)), dim3(256), 0, as_stream(stream),
+                     key, n, key_start, key_cursor, order);
   FLPR_CHECK_LAUNCH();
   return 0;
 }
 
-int flpr_spread(const FlprGeom* g, const double* xmod, const int32_t* cell, const int32_t* order,
-                const int32_t* tile_start, int32_t with_ones, const double* vals_y, int32_t ny,
-                const double* probes, int32_t nprobe, double* grid, void* stream) {
+int flpr_points_pack(const FlprGeom* g, const double* x, const int32_t* order, int64_t n,
+                     const double* vals_y, int32_t ny, const double* probes, int32_t nprobe,
+                     double* wab, double* vals, int32_t* rawi, double* rawf, void* stream) {
   FLPR_CLEAR_ERROR();
-  if (!flpr_geom_ok(g) || !xmod || !cell || !order || !tile_start || !grid) return -1;
-  if (g->msp < 4) return -1;
-  if ((ny > 0 && !vals_y) || (nprobe > 0 && !probes) || ny < 0 || nprobe < 0) return -1;
+  if (!flpr_geom_ok(g) || !x || !order || !wab || !rawi || !rawf || n < 1 ||
+      ny < 0 || nprobe < 0 || ((ny + nprobe) > 0 && !vals) || (ny > 0 && !vals_y) ||
+      (nprobe > 0 && !probes))
+    return -1;
+  dim3 grd(flpr_grid_for(n, 256, 4096));
+  hipStream_t st = as_stream(stream);
+  switch (g->d) {
+    case 1: hipLaunchKernelGGL(k_pack<1>, grd, dim3(256), 0, st, *g, x, order, n, vals_y, ny, probes, nprobe, wab, vals, rawi, rawf); break;
+    case 2: hipLaunchKernelGGL(k_pack<2>, grd, dim3(256), 0, st, *g, x, order, n, vals_y, ny, probes, nprobe, wab, vals, rawi, rawf); break;
+    default: hipLaunchKernelGGL(k_pack<3>, grd, dim3(256), 0, st, *g, x, order, n, vals_y, ny, probes, nprobe, wab, vals, rawi, rawf); break;
+  }
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+int flpr_spread(const FlprGeom* g, const int32_t* key_start, const double* wab,
+                const double* vals, int64_t n, int32_t with_ones, int32_t nval, double* part,
+                double* grid, double* sums, void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!flpr_geom_ok(g) || !key_start || !wab || !part || !grid || !sums || n < 1 || nval < 0 ||
+      (nval > 0 && !vals))
+    return -1;
+  if (g->msp < 4 || g->msp > 6) return -1;
   const int wo = with_ones ? 1 : 0;
-  const int C = wo + ny + nprobe;
+  const int C = wo + nval;
   if (C < 1 || C > FLPR_MAX_CHANNELS) return -1;
-  int64_t fcells = 1, cstride = 1;
+  int64_t fcells = 1, cells = 1;
   for (int k = 0; k < g->d; ++k) {
     fcells *= g->tile[k] + 2 * g->msp;
-    cstride *= g->G[k];
+    cells *= g->G[k];
   }
-  const int cg = lds_group(fcells, C);
+  const int cg = lds_group(fcells, C, 16);
   const size_t lds = (size_t)fcells * cg * sizeof(double);
   if (lds > 64 * 1024) return -1;
-  const int ngroups = (C + cg - 1) / cg;
-  dim3 grd((unsigned)flpr_ntiles(g), (unsigned)ngroups);
+  dim3 grd((unsigned)flpr_ntiles(g), (unsigned)((C + cg - 1) / cg));
   hipStream_t st = as_stream(stream);
+  const int msp = g->msp;
+  if (g->d == 1) {
+    if (msp == 4) launch_cells<1, 4, false>(grd, lds, st, *g, key_start, wab, vals, n, wo, C, cg, part);
+    else if (msp == 5) launch_cells<1, 5, false>(grd, lds, st, *g, key_start, wab, vals, n, wo, C, cg, part);
+    else launch_cells<1, 6, false>(grd, lds, st, *g, key_start, wab, vals, n, wo, C, cg, part);
+  } else if (g->d == 2) {
+    if (msp == 4) launch_cells<2, 4, false>(grd, lds, st, *g, key_start, wab, vals, n, wo, C, cg, part);
+    else if (msp == 5) launch_cells<2, 5, true>(grd, lds, st, *g, key_start, wab, vals, n, wo, C, cg, part);
+    else launch_cells<2, 6, true>(grd, lds, st, *g, key_start, wab, vals, n, wo, C, cg, part);
+  } else {
+    if (msp == 4) hipLaunchKernelGGL(k_spread_points3<4>, grd, dim3(SPREAD_THREADS), lds, st, *g, key_start, wab, vals, n, wo, C, cg, part);
+    else if (msp == 5) hipLaunchKernelGGL(k_spread_points3<5>, grd, dim3(SPREAD_THREADS), lds, st, *g, key_start, wab, vals, n, wo, C, cg, part);
+    else hipLaunchKernelGGL(k_spread_points3<6>, grd, dim3(SPREAD_THREADS), lds, st, *g, key_start, wab, vals, n, wo, C, cg, part);
+  }
+  FLPR_CHECK_LAUNCH();
+  dim3 gg(flpr_grid_for(cells, 256, 256), (unsigned)C);
   switch (g->d) {
-    case 1: launch_spread_d<1>(g->msp, grd, lds, st, *g, xmod, cell, order, tile_start, wo, vals_y, ny, probes, nprobe, cg, grid, cstride); break;
-    case 2: launch_spread_d<2>(g->msp, grd, lds, st, *g, xmod, cell, order, tile_start, wo, vals_y, ny, probes, nprobe, cg, grid, cstride); break;
-    default: launch_spread_d<3>(g->msp, grd, lds, st, *g, xmod, cell, order, tile_start, wo, vals_y, ny, probes, nprobe, cg, grid, cstride); break;
+    case 1: hipLaunchKernelGGL(k_spread_gather<1>, gg, dim3(256), 0, st, *g, part, C, cells, grid, sums); break;
+    case 2: hipLaunchKernelGGL(k_spread_gather<2>, gg, dim3(256), 0, st, *g, part, C, cells, grid, sums); break;
+    default: hipLaunchKernelGGL(k_spread_gather<3>, gg, dim3(256), 0, st, *g, part, C, cells, grid, sums); break;
   }
   FLPR_CHECK_LAUNCH();
   return 0;
 }
 
-int flpr_adjoint_interp(const FlprGeom* g, const double* x, int64_t n, const int32_t* order,
-                        const int32_t* tile_start, const double* probes, int32_t nprobe,
-                        double* q, void* stream) {
+int flpr_grid_center(double* grid, int32_t C, int64_t cells, const double* sums, double* shift,
+                     float* grid32, void* stream) {
   FLPR_CLEAR_ERROR();
-  if (!flpr_geom_ok(g) || !x || !order || !tile_start || !probes || !q || nprobe < 1 || n < 1)
+  if (!grid || !sums || C < 1 || cells < 1) return -1;
+  dim3 grd(flpr_grid_for(cells, 256, 256), (unsigned)C);
+  hipLaunchKernelGGL(k_grid_center, grd, dim3(256), 0, as_stream(stream), grid, cells, sums, shift,
+                     grid32);
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+int flpr_adjoint(const FlprGeom* g, const int32_t* key_start, const int32_t* rawi,
+                 const double* rawf, const double* vals, int64_t n, int32_t nval, double* part,
+                 double* q, double* gram, void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!flpr_geom_ok(g) || !key_start || !rawi || !rawf || !part || n < 1 || nval < 0 ||
+      (nval > 0 && (!vals || !q)))
     return -1;
-  int64_t cells = 1;
-  for (int k = 0; k < g->d; ++k) cells *= g->tile[k] + ADJ_MARGIN;
-  const int cg = lds_group(cells, nprobe);
-  const size_t lds = (size_t)cells * cg * sizeof(double);
-  if (lds > 64 * 1024) return -1;
-  const int ngroups = (nprobe + cg - 1) / cg;
-  dim3 grd((unsigned)flpr_ntiles(g), (unsigned)ngroups);
-  const int64_t nodes = flpr_nodes(g);
-  hipStream_t st = as_stream(stream);
-  switch (g->d) {
-    case 1: hipLaunchKernelGGL(k_adjoint<1>, grd, dim3(ADJ_THREADS), lds, st, *g, x, order, tile_start, probes, nprobe, cg, q, nodes); break;
-    case 2: hipLaunchKernelGGL(k_adjoint<2>, grd, dim3(ADJ_THREADS), lds, st, *g, x, order, tile_start, probes, nprobe, cg, q, nodes); break;
-    default: hipLaunchKernelGGL(k_adjoint<3>, grd, dim3(ADJ_THREADS), lds, st, *g, x, order, tile_start, probes, nprobe, cg, q, nodes); break;
+  int64_t wcells = 1, nodes = 1, rcells = 1;
+  for (int k = 0; k < g->d; ++k) {
+    wcells *= g->tile[k] + 5;
+    nodes *= g->n_grid[k];
+    rcells *= g->n_grid[k] - 1;
   }
-  FLPR_CHECK_LAUNCH();
+  const int nq = 1 << g->d;
+  const int nw = nq * (nq + 1) / 2;
+  hipStream_t st = as_stream(stream);
+  for (int mode = 0; mode < 2; ++mode) {
+    const int nch = (mode == 0) ? nval : (gram ? nw : 0);
+    if (nch == 0) continue;
+    double* out = (mode == 0) ? q : gram;
+    const int64_t ostride = (mode == 0) ? nodes : rcells;
+    const int cg = lds_group(wcells, nch, 64);
+    const size_t lds = (size_t)wcells * cg * sizeof(double);
+    if (lds > 64 * 1024) return -1;
+    dim3 grd((unsigned)flpr_ntiles(g), (unsigned)((nch + cg - 1) / cg));
+    switch (g->d) {
+      case 1: hipLaunchKernelGGL(k_adjoint<1>, grd, dim3(ADJ_THREADS), lds, st, *g, key_start, rawi, rawf, vals, n, nch, mode, cg, out, ostride, part); break;
+      case 2: hipLaunchKernelGGL(k_adjoint<2>, grd, dim3(ADJ_THREADS), lds, st, *g, key_start, rawi, rawf, vals, n, nch, mode, cg, out, ostride, part); break;
+      default: hipLaunchKernelGGL(k_adjoint<3>, grd, dim3(ADJ_THREADS), lds, st, *g, key_start, rawi, rawf, vals, n, nch, mode, cg, out, ostride, part); break;
+    }
+    FLPR_CHECK_LAUNCH();
+    dim3 gg(flpr_grid_for(ostride, 256, 256), (unsigned)nch);
+    switch (g->d) {
+      case 1: hipLaunchKernelGGL(k_adjoint_gather<1>, gg, dim3(256), 0, st, *g, part, nch, mode, ostride, out); break;
+      case 2: hipLaunchKernelGGL(k_adjoint_gather<2>, gg, dim3(256), 0, st, *g, part, nch, mode, ostride, out); break;
+      default: hipLaunchKernelGGL(k_adjoint_gather<3>, gg, dim3(256), 0, st, *g, part, nch, mode, ostride, out); break;
+    }
+    FLPR_CHECK_LAUNCH();
+  }
   return 0;
 }
 
-int flpr_interp_rss(const FlprGeom* g, const double* x, int64_t n, const int32_t* order,
-                    const double* y, int32_t ncol_m, int32_t cplx, const double* m, int32_t dh,
-                    double* rss, void* stream) {
+int flpr_quadform(const FlprGeom* g, const double* m, int32_t ncol, int32_t dh,
+                  const double* gram, double* quad, void* stream) {
   FLPR_CLEAR_ERROR();
-  (void)order;
-  if (!flpr_geom_ok(g) || !x || !y || !m || !rss || n < 1 || ncol_m < 1 || dh < 1) return -1;
-  if (cplx && (ncol_m % 2)) return -1;
-  const int ncol_out = cplx ? ncol_m / 2 : ncol_m;
-  dim3 grd((unsigned)flpr_grid_for(n, RSS_THREADS, 1024), (unsigned)((dh + RSS_HC - 1) / RSS_HC),
-           (unsigned)ncol_out);
+  if (!flpr_geom_ok(g) || !m || !gram || !quad || ncol < 1 || dh < 1) return -1;
   const int64_t nodes = flpr_nodes(g);
+  int64_t rcells = 1;
+  for (int k = 0; k < g->d; ++k) rcells *= g->n_grid[k] - 1;
+  dim3 grd(flpr_grid_for(rcells, QF_THREADS, 64), (unsigned)(dh * ncol));
   hipStream_t st = as_stream(stream);
   switch (g->d) {
-    case 1: hipLaunchKernelGGL(k_rss<1>, grd, dim3(RSS_THREADS), 0, st, *g, x, n, y, ncol_m, cplx, m, dh, nodes, rss); break;
-    case 2: hipLaunchKernelGGL(k_rss<2>, grd, dim3(RSS_THREADS), 0, st, *g, x, n, y, ncol_m, cplx, m, dh, nodes, rss); break;
-    default: hipLaunchKernelGGL(k_rss<3>, grd, dim3(RSS_THREADS), 0, st, *g, x, n, y, ncol_m, cplx, m, dh, nodes, rss); break;
+    case 1: hipLaunchKernelGGL(k_quadform<1>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad); break;
+    case 2: hipLaunchKernelGGL(k_quadform<2>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad); break;
+    default: hipLaunchKernelGGL(k_quadform<3>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad); break;
   }
   FLPR_CHECK_LAUNCH();
   return 0;
@@ -773,24 +1233,14 @@ int flpr_interp_eval(const FlprGeom* g, const double* x, int64_t n, const double
   FLPR_CLEAR_ERROR();
   if (!flpr_geom_ok(g) || !x || !m || !out || n < 1 || ncol < 1) return -1;
   if (yvar && !expsum) return -1;
-  dim3 grd((unsigned)flpr_grid_for(n, RSS_THREADS, 2048));
+  dim3 grd((unsigned)flpr_grid_for(n, EVAL_THREADS, 2048));
   const int64_t nodes = flpr_nodes(g);
   hipStream_t st = as_stream(stream);
   switch (g->d) {
-    case 1: hipLaunchKernelGGL(k_interp_eval<1>, grd, dim3(RSS_THREADS), 0, st, *g, x, n, m, ncol, nodes, out, yvar, expsum); break;
-    case 2: hipLaunchKernelGGL(k_interp_eval<2>, grd, dim3(RSS_THREADS), 0, st, *g, x, n, m, ncol, nodes, out, yvar, expsum); break;
-    default: hipLaunchKernelGGL(k_interp_eval<3>, grd, dim3(RSS_THREADS), 0, st, *g, x, n, m, ncol, nodes, out, yvar, expsum); break;
+    case 1: hipLaunchKernelGGL(k_interp_eval<1>, grd, dim3(EVAL_THREADS), 0, st, *g, x, n, m, ncol, nodes, out, yvar, expsum); break;
+    case 2: hipLaunchKernelGGL(k_interp_eval<2>, grd, dim3(EVAL_THREADS), 0, st, *g, x, n, m, ncol, nodes, out, yvar, expsum); break;
+    default: hipLaunchKernelGGL(k_interp_eval<3>, grd, dim3(EVAL_THREADS), 0, st, *g, x, n, m, ncol, nodes, out, yvar, expsum); break;
   }
-  FLPR_CHECK_LAUNCH();
-  return 0;
-}
-
-int flpr_f64_to_f32(const double* src, float* dst, int64_t count, void* stream) {
-  FLPR_CLEAR_ERROR();
-  if (!src || !dst || count < 0) return -1;
-  if (count == 0) return 0;
-  hipLaunchKernelGGL(k_f64_to_f32, dim3(flpr_grid_for(count, 256, 4096)), dim3(256), 0,
-                     as_stream(stream), src, dst, count);
   FLPR_CHECK_LAUNCH();
   return 0;
 }

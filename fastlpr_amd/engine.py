@@ -289,19 +289,27 @@ class Stats:
     xmax: np.ndarray
     mean: np.ndarray
     std: np.ndarray
+    bad_x: int = 0
+    bad_y: int = 0
+    ysq: Optional[np.ndarray] = None
 
 
-def x_stats(x: torch.Tensor) -> Stats:
-    """Per-dimension min, max, mean and std(ddof=1) of x (n, d) float64 on the device."""
+def x_stats(x: torch.Tensor, y: Optional[torch.Tensor] = None) -> Stats:
+    """Per-dimension min, max, mean and std(ddof=1) of x (n, d) float64 on the device,
+    counts of non-finite values in x and y, and sum_i y_ic^2 per column of y."""
     n, d = x.shape
+    ny = 0 if y is None else y.shape[1]
     dev = x.device
-    part = torch.empty(5 * d * 512, dtype=torch.float64, device=dev)
-    out = torch.empty(5 * d, dtype=torch.float64, device=dev)
-    call("flpr_xstats", ptr(x), n, d, ptr(part), ptr(out), stream_handle())
-    o = out.cpu().numpy().reshape(d, 5)
-    std = np.sqrt(o[:, 3] / (n - 1)) if n > 1 else np.zeros(d)
+    part = torch.empty(512 * (5 * d + 2 + ny), dtype=torch.float64, device=dev)
+    out = torch.empty(5 * d + 2 + ny, dtype=torch.float64, device=dev)
+    call("flpr_xstats", ptr(x), n, d, ptr(y), ny, ptr(part), ptr(out), stream_handle())
+    o = out.cpu().numpy()
+    w = o[:5 * d].reshape(d, 5)
+    cnt = w[:, 4]
+    std = np.sqrt(w[:, 3] / np.maximum(cnt - 1, 1))
     std = np.where(std == 0, 1.0, std)
-    return Stats(o[:, 0].copy(), o[:, 1].copy(), o[:, 2].copy(), std)
+    return Stats(w[:, 0].copy(), w[:, 1].copy(), w[:, 2].copy(), std, int(o[5 * d]),
+                 int(o[5 * d + 1]), o[5 * d + 2:].copy())
 
 
 def padded_sizes(N: Tuple[int, ...], flag_power2: bool = True) -> Tuple[int, ...]:
@@ -470,56 +478,66 @@ class Workspace:
 def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
              probes: Optional[torch.Tensor], K: int, h: np.ndarray, order: int,
              N: Tuple[int, ...], accuracy: int, fp64: bool, regularization: float,
-             xrange=None, flag_power2: bool = True, rss_y: Optional[torch.Tensor] = None,
-             rss_cplx: bool = False, stats: Optional[Stats] = None,
-             with_ones_eps: bool = True, debug: Optional[dict] = None) -> DeviceFit:
+             xrange=None, flag_power2: bool = True, want_rss: bool = False,
+             stats: Optional[Stats] = None, with_ones_eps: bool = True,
+             debug: Optional[dict] = None) -> DeviceFit:
     """Spread, convolve and solve for all bandwidths, plus the probe dot products and
     the residual sums.
 
     x: (n, d) float64 on the device; vals_y: (n, ny) float64 or None (complex y is
-    passed as interleaved (re, im) columns); probes: (n, K) float64 or None;
-    rss_y: the (n, ny) float64 responses against which residual sums are formed
-    (None to skip).  with_ones_eps adds float64 eps to S for order 0
-    (regression.py:127; the calc_dof=False path of the reference omits it).
+    passed as interleaved (re, im) columns); probes: (n, K) float64 or None.
+    want_rss: also return sum_i (y_ic - yhat_ic(h))^2 per bandwidth and real column.
+    stats: x statistics from x_stats(x, vals_y) when the caller already has them.
+    with_ones_eps adds float64 eps to S for order 0 (regression.py:127; the
+    calc_dof=False path of the reference omits it).
     debug: when a dict is given, the intermediate device buffers are stored in it.
     """
     n, d = x.shape
     dev = x.device
     s = stream_handle()
-    st = stats if stats is not None else x_stats(x)
+    st = stats if stats is not None else x_stats(x, vals_y)
     geo = build_geometry(st, N, accuracy, xrange, flag_power2)
     h, _ihbad, warn = filter_bandwidths(geo, np.asarray(h, dtype=float), n)
     dh = h.shape[0]
-    C = 1 + ny + K
+    nval = ny + K
+    C = 1 + nval
     plan = get_plan(d, order, geo.N, geo.L, geo.msp, C, dh, fp64, dev)
     g = geo.geom
     gp = _lib.C.byref(g)
     f64, i32 = torch.float64, torch.int32
 
-    # --- points: knots, tiles, binning
+    # --- points: knots, cell keys, counting sort, sorted SoA data
     ntiles = int(np.prod([g.ntile[k] for k in range(d)]))
-    xmod = torch.empty((n, d), dtype=f64, device=dev)
-    cell = torch.empty((n, d), dtype=i32, device=dev)
-    tile = torch.empty((n,), dtype=i32, device=dev)
-    cnt = torch.zeros((2 * ntiles,), dtype=i32, device=dev)
-    tile_count, tile_cursor = cnt[:ntiles], cnt[ntiles:]
-    tile_start = torch.empty((ntiles + 1,), dtype=i32, device=dev)
+    nkeys = ntiles * int(np.prod([g.tile[k] for k in range(d)]))
+    key = torch.empty((n,), dtype=i32, device=dev)
+    cnt = torch.zeros((2 * nkeys,), dtype=i32, device=dev)
+    key_count, key_cursor = cnt[:nkeys], cnt[nkeys:]
+    key_start = torch.empty((nkeys + 1,), dtype=i32, device=dev)
     order_idx = torch.empty((n,), dtype=i32, device=dev)
-    call("flpr_points_prepare", ptr(x), n, gp, ptr(xmod), ptr(cell), ptr(tile), ptr(tile_count), s)
-    call("flpr_points_bin", ptr(tile), n, ntiles, ptr(tile_count), ptr(tile_start),
-         ptr(tile_cursor), ptr(order_idx), s)
+    call("flpr_points_prepare", ptr(x), n, gp, ptr(key), ptr(key_count), s)
+    call("flpr_points_bin", ptr(key), n, nkeys, ptr(key_count), ptr(key_start), ptr(key_cursor),
+         ptr(order_idx), s)
+    wab = torch.empty((2 * d * n,), dtype=f64, device=dev)
+    vals = torch.empty((max(nval, 1) * n,), dtype=f64, device=dev)
+    rawi = torch.empty((d * n,), dtype=i32, device=dev)
+    rawf = torch.empty((d * n,), dtype=f64, device=dev)
+    call("flpr_points_pack", gp, ptr(x), ptr(order_idx), n, ptr(vals_y), ny, ptr(probes), K,
+         ptr(wab), ptr(vals), ptr(rawi), ptr(rawf), s)
 
-    # --- spread (float64 accumulation)
+    # --- spread (float64 accumulation), centring, float32 copy
     cells = int(np.prod(plan.G))
-    grid64 = torch.zeros((C * cells,), dtype=f64, device=dev)
-    call("flpr_spread", gp, ptr(xmod), ptr(cell), ptr(order_idx), ptr(tile_start), 1,
-         ptr(vals_y), ny, ptr(probes), K, ptr(grid64), s)
+    fcells = int(np.prod([g.tile[k] + 2 * geo.msp for k in range(d)]))
+    part = torch.empty((ntiles * C * fcells,), dtype=f64, device=dev)
+    grid64 = torch.empty((C * cells,), dtype=f64, device=dev)
+    sums = torch.zeros((C,), dtype=f64, device=dev)
+    call("flpr_spread", gp, ptr(key_start), ptr(wab), ptr(vals), n, 1, nval, ptr(part),
+         ptr(grid64), ptr(sums), s)
+    del part
+    shift = torch.empty((C,), dtype=f64, device=dev)
     fdt = f64 if fp64 else torch.float32
-    if fp64:
-        grid = grid64
-    else:
-        grid = torch.empty((C * cells,), dtype=fdt, device=dev)
-        call("flpr_f64_to_f32", ptr(grid64), ptr(grid), C * cells, s)
+    grid32 = None if fp64 else torch.empty((C * cells,), dtype=fdt, device=dev)
+    call("flpr_grid_center", ptr(grid64), C, cells, ptr(sums), ptr(shift), ptr(grid32), s)
+    grid = grid64 if fp64 else grid32
 
     # --- taps and bands
     h_dev = torch.tensor(h, dtype=f64, device=dev)
@@ -546,7 +564,7 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     del inter
 
     # --- global regularisation key (regression.py:217, 243-267)
-    key = torch.zeros((1,), dtype=torch.int64, device=dev)
+    key_max = torch.zeros((1,), dtype=torch.int64, device=dev)
     per_h0 = plan.stage_per_h[0]
     if order == 0:
         offs, absval = [plan.s_off[0]], 0
@@ -556,37 +574,49 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
         offs, absval = plan.diag_off, 1
         lam_scale, eps_add = 1e-6, 0.0
     call("flpr_max_moment", int(fp64), ptr(stage0), per_h0, dh, i64_array(offs), len(offs),
-         plan.nodes, absval, ptr(key), s)
+         plan.nodes, absval, ptr(key_max), s)
 
-    # --- probe adjoint, solve (+ probe dot products), residual sums
+    # --- adjoint sums, solve (+ dots), residual quadratic form
     nodes = plan.nodes
-    q = dot = None
-    if K > 0:
-        q = torch.zeros((K * nodes,), dtype=f64, device=dev)
-        call("flpr_adjoint_interp", gp, ptr(x), n, ptr(order_idx), ptr(tile_start), ptr(probes),
-             K, ptr(q), s)
-        dot = torch.zeros((dh * K,), dtype=f64, device=dev)
+    rcells = int(np.prod([n_ - 1 for n_ in geo.N]))
+    nw = (1 << d) * ((1 << d) + 1) // 2
+    need_q = K > 0 or want_rss
+    q = dot = gram = None
+    if need_q:
+        q = torch.zeros((nval * nodes,), dtype=f64, device=dev)
+        gram = torch.zeros((nw * rcells,), dtype=f64, device=dev) if want_rss else None
+        wcells = int(np.prod([g.tile[k] + 5 for k in range(d)]))
+        apart = torch.empty((ntiles * max(nval, nw) * wcells,), dtype=f64, device=dev)
+        call("flpr_adjoint", gp, ptr(key_start), ptr(rawi), ptr(rawf), ptr(vals), n, nval,
+             ptr(apart), ptr(q), ptr(gram), s)
+        del apart
+        dot = torch.zeros((dh * nval,), dtype=f64, device=dev)
     m_y = torch.empty((dh * max(ny, 1) * nodes,), dtype=f64, device=dev)
     call("flpr_solve", int(fp64), ptr(stage0), per_h0, dh, nodes, order, plan.nt,
-         i64_array(plan.s_off), i64_array(plan.t_off), ptr(key), lam_scale, eps_add,
-         ny, 1, ptr(m_y) if ny > 0 else None, 1 + ny, K, ptr(q), ptr(dot), s)
-    rss = None
-    if rss_y is not None and ny > 0:
-        ncol_out = ny // 2 if rss_cplx else ny
-        rss = torch.zeros((dh * ncol_out,), dtype=f64, device=dev)
-        call("flpr_interp_rss", gp, ptr(x), n, ptr(order_idx), ptr(rss_y), ny, int(rss_cplx),
-             ptr(m_y), dh, ptr(rss), s)
+         i64_array(plan.s_off), i64_array(plan.t_off), ptr(key_max), lam_scale, eps_add,
+         ptr(shift), nval, ny, ptr(m_y) if ny > 0 else None, ptr(q), ptr(dot), s)
+    quad = None
+    if want_rss and ny > 0:
+        quad = torch.zeros((dh * ny,), dtype=f64, device=dev)
+        call("flpr_quadform", gp, ptr(m_y), ny, dh, ptr(gram), ptr(quad), s)
     off0 = plan.s_off[0]
     s0_all = stage0.view(dh, per_h0)[:, off0:off0 + nodes].to(f64)
     if order == 0:
         s0_all = s0_all + eps_add
     if debug is not None:
-        debug.update(grid64=grid64, stage0=stage0, key=key, q=q, dot=dot, m_y=m_y, plan=plan,
-                     geo=geo, h=h, order_idx=order_idx, tile_start=tile_start, xmod=xmod,
-                     cell=cell, lam_scale=lam_scale, eps_add=eps_add)
-    rss_h = None if rss is None else rss.cpu().numpy().reshape(dh, -1)
-    dot_h = None if dot is None else dot.cpu().numpy().reshape(dh, K)
-    return DeviceFit(h, m_y.view(dh, max(ny, 1), nodes), rss_h, dot_h, s0_all, geo, plan, warn)
+        debug.update(grid64=grid64, stage0=stage0, key=key_max, q=q, dot=dot, m_y=m_y, plan=plan,
+                     geo=geo, h=h, order_idx=order_idx, key_start=key_start,
+                     lam_scale=lam_scale, eps_add=eps_add, shift=shift, gram=gram, quad=quad)
+    dot_h = None if dot is None else dot.cpu().numpy().reshape(dh, nval)
+    rss_h = None
+    if quad is not None:
+        quad_h = quad.cpu().numpy().reshape(dh, ny)
+        ysq = st.ysq if (st.ysq is not None and len(st.ysq) == ny) else None
+        if ysq is None:
+            ysq = (vals_y * vals_y).sum(dim=0).cpu().numpy()
+        rss_h = ysq[None, :] - 2.0 * dot_h[:, :ny] + quad_h
+    probe_dot = None if (dot_h is None or K == 0) else dot_h[:, ny:]
+    return DeviceFit(h, m_y.view(dh, max(ny, 1), nodes), rss_h, probe_dot, s0_all, geo, plan, warn)
 
 
 def interp_eval(geo: Geometry, x: torch.Tensor, m: torch.Tensor, ncol: int,

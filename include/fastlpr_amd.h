@@ -61,37 +61,59 @@ typedef struct FlprGeom {
  * none); its message is copied into buf (len bytes, NUL-terminated). */
 int flpr_last_hip_error(char* buf, int32_t len);
 
-/* x statistics (model.py:58-68, core.py:83-99): per dimension
- * out[5*k+0..4] = min, max, mean, sum of squared deviations, count.
- * x is (n, d) row-major float64.  partials must hold 5*d*512 doubles. */
-int flpr_xstats(const double* x, int64_t n, int32_t d, double* partials, double* out,
-                void* stream);
+/* x statistics (model.py:58-68, core.py:83-99) and input checks
+ * (api.py:307-323).  x is (n, d) row-major float64; y (n, ny) float64 or NULL.
+ * out[5*k+0..4] = min, max, mean, sum of squared deviations, count (finite
+ * values of dimension k); out[5*d] = non-finite values in x; out[5*d+1] =
+ * non-finite values in y; out[5*d+2+c] = sum_i y_ic^2.  partials must hold
+ * 512*(5*d + 2 + ny) doubles; ny <= 32. */
+int flpr_xstats(const double* x, int64_t n, int32_t d, const double* y, int32_t ny,
+                double* partials, double* out, void* stream);
 
-/* Knot positions, spread cells and spread-tile histogram for every point
- * (convolution.py:169-199 + nufftn_type1.py:140-176, 520-543).
- *   xmod  (n, d) float64   knot position in [0, 2 pi)
- *   cell  (n, d) int32     round(xmod / hx) in [0, L]
- *   tile  (n)    int32     linear spread-tile id
- *   tile_count (ntiles) int32, zeroed by the caller, incremented here. */
-int flpr_points_prepare(const double* x, int64_t n, const FlprGeom* g, double* xmod,
-                        int32_t* cell, int32_t* tile, int32_t* tile_count, void* stream);
+/* Spread cell sort keys of every point, from the knot mapping of the
+ * reference (convolution.py:169-199 + nufftn_type1.py:140-176, 520-543):
+ *   key   (n)    int32     tile-major spread-cell index (tile, then cell in tile)
+ *   key_count (nkeys = ntiles * prod(tile)) int32, zeroed by the caller. */
+int flpr_points_prepare(const double* x, int64_t n, const FlprGeom* g, int32_t* key,
+                        int32_t* key_count, void* stream);
 
-/* Counting sort of the points by spread tile.  tile_start (ntiles+1) gets the
- * exclusive scan of tile_count; tile_cursor (ntiles) is scratch zeroed by the
- * caller; order (n) receives point indices grouped by tile. */
-int flpr_points_bin(const int32_t* tile, int64_t n, int32_t ntiles, const int32_t* tile_count,
-                    int32_t* tile_start, int32_t* tile_cursor, int32_t* order, void* stream);
+/* Counting sort of the points by key.  key_start (nkeys+1) receives the
+ * exclusive scan of key_count; key_cursor (nkeys) is scratch zeroed by the
+ * caller (left zeroed only up to the scatter's own use); order (n) receives
+ * point indices grouped by key. */
+int flpr_points_bin(const int32_t* key, int64_t n, int32_t nkeys, const int32_t* key_count,
+                    int32_t* key_start, int32_t* key_cursor, int32_t* order, void* stream);
 
-/* Gaussian-gridding spread of C channels onto the unwrapped spread grid
+/* Sorted structure-of-arrays point data for the grid kernels:
+ *   wab  (2d, n) float64  A = exp(-delta^2/(4 tau)), B = exp(2 delta hx/(4 tau)),
+ *                         delta = xmod - hx*round(xmod/hx), so that the spread weight of tap
+ *                         k is A B^k exp(-hx^2 k^2/(4 tau)) = exp(-(delta-hx k)^2/(4 tau))
+ *                         (nufftn_type1.py:226-249, spreading_fused.py:80-94)
+ *   vals (ny + nprobe, n) float64  y columns then probe columns
+ *   rawi (d, n) int32, rawf (d, n) float64  interval and fraction on the raw grid
+ *                         axis linspace(graw0, graw1, N) (RegularGridInterpolator). */
+int flpr_points_pack(const FlprGeom* g, const double* x, const int32_t* order, int64_t n,
+                     const double* vals_y, int32_t ny, const double* probes, int32_t nprobe,
+                     double* wab, double* vals, int32_t* rawi, double* rawf, void* stream);
+
+/* Gaussian-gridding spread onto the unwrapped spread grid
  * (nufftn_type1.py:565-636, spreading_fused.py:67-94, _nufft_accel.pyx:32-76).
- * Channel 0 is the constant 1 when with_ones != 0, then ny columns of vals_y
- * (n, ny) float64, then nprobe columns of probes (n, nprobe) float64.
- * grid (C, G0[, G1[, G2]]) float64 must be zeroed by the caller; values are
- * accumulated in float64. */
-int flpr_spread(const FlprGeom* g, const double* xmod, const int32_t* cell,
-                const int32_t* order, const int32_t* tile_start, int32_t with_ones,
-                const double* vals_y, int32_t ny, const double* probes, int32_t nprobe,
-                double* grid, void* stream);
+ * Channel 0 is the constant 1 when with_ones != 0, then the nval rows of vals.
+ * Every tile accumulates its footprint window in float64 and writes it to the
+ * workspace part (ntiles * C * prod(tile + 2 msp) doubles); a second pass
+ * adds the overlapping windows per cell (no atomics) into grid
+ * (C, G0[, G1[, G2]]) float64 and adds the per-channel totals into sums (C
+ * doubles, zeroed by the caller). */
+int flpr_spread(const FlprGeom* g, const int32_t* key_start, const double* wab,
+                const double* vals, int64_t n, int32_t with_ones, int32_t nval, double* part,
+                double* grid, double* sums, void* stream);
+
+/* Centre channels 1..C-1 of the spread grid on their kernel-weighted mean:
+ * shift[c] = sums[c] / sums[0], F_c -= shift[c] F_0 (float64, in place), and
+ * optionally write a float32 copy of the whole grid.  shift (C doubles) may be
+ * NULL.  The solve adds the shift back exactly (flpr_solve). */
+int flpr_grid_center(double* grid, int32_t C, int64_t cells, const double* sums, double* shift,
+                     float* grid32, void* stream);
 
 /* Kernel-moment taps (convolution.py:412-512, kernel.py:330-357,
  * design_matrix_optimized.py:216-298): for bandwidth row b (h is (dh, d)),
@@ -142,34 +164,35 @@ double flpr_key_to_double(uint64_t key);
  *   key       max key from flpr_max_moment; lam_scale multiplies it
  *             (1e-6 for order >= 1, regularization for order 0)
  *   eps_add   added to S for order 0 (regression.py:127)
- *   chan_first, nchan   T channels solved (absolute channel ids)
- *   m_out     (dh, n_out_chan, nodes) float64 for channels
- *             [chan_first, chan_first + n_out_chan)
- *   q         (nprobe, nodes) float64 adjoint-interpolated probes or NULL;
- *             channels [probe_first, probe_first + nprobe) then accumulate
- *             dot[h*nprobe + k] += sum_node q[k][node] * m_k(node). */
+ *   shift     per-channel centring of flpr_grid_center (C doubles) or NULL
+ *   nval      value channels 1..nval are solved (<= 32)
+ *   nout      the first nout of them are written to m_out (dh, nout, nodes) float64
+ *   q, dot    when q (nval, nodes) float64 is given, dot[h*nval + c] accumulates
+ *             sum_node q[c][node] m_c(h, node) (dot zeroed by the caller). */
 int flpr_solve(int32_t fp64, const void* mom, int64_t per_h, int32_t dh, int64_t nodes,
                int32_t order, int32_t nt, const int64_t* s_off, const int64_t* t_off,
-               const uint64_t* key, double lam_scale, double eps_add, int32_t n_out_chan,
-               int32_t chan_first, double* m_out, int32_t probe_first, int32_t nprobe,
-               const double* q, double* dot, void* stream);
+               const uint64_t* key, double lam_scale, double eps_add, const double* shift,
+               int32_t nval, int32_t nout, double* m_out, const double* q, double* dot,
+               void* stream);
 
-/* Adjoint of multilinear interpolation on the raw grid for the probe
- * columns: q[k][node] = sum_i p[i][k] w_i(node) (regression.py:1256-1280
- * computes p' * interp(m); this is the same sum reordered).  q (nprobe, nodes)
- * float64 must be zeroed by the caller. */
-int flpr_adjoint_interp(const FlprGeom* g, const double* x, int64_t n, const int32_t* order,
-                        const int32_t* tile_start, const double* probes, int32_t nprobe,
-                        double* q, void* stream);
+/* Adjoint interpolation sums on the raw grid, from the sorted point data:
+ *   q (nval, nodes)        q[c][node] = sum_i vals[c][i] w_i(node)
+ *   gram (nw, rcells)      per raw cell the Gram entries sum_i w_i(a) w_i(b) of
+ *                          its 2^d corner weights, a <= b row-major (nw =
+ *                          2^d (2^d + 1) / 2; rcells = prod(N_k - 1)); may be NULL
+ * with w_i the multilinear weights of RegularGridInterpolator 'linear'.  Then
+ *   sum_i v_i yhat_i = sum_node q m   and   sum_i yhat_i^2 = sum_cell m' W m,
+ * which turn the Hutchinson dots p' phat and the residual sums of
+ * regression.py:1256-1280 / api.py:686-723 into grid-sized work per bandwidth.
+ * part is workspace of ntiles * max(nval, nw) * prod(tile + 5) doubles; q and
+ * gram must be zeroed by the caller. */
+int flpr_adjoint(const FlprGeom* g, const int32_t* key_start, const int32_t* rawi,
+                 const double* rawf, const double* vals, int64_t n, int32_t nval, double* part,
+                 double* q, double* gram, void* stream);
 
-/* Residual sums for GCV (api.py:686-723, regression.py:384-396): for every
- * bandwidth h and output column c, rss[h*ncol + c] += sum_i |y_ic - yhat_ic(h)|^2
- * with yhat the multilinear interpolant (RegularGridInterpolator 'linear') of
- * m (dh, ncol_m, nodes).  Complex columns are passed as (re, im) pairs with
- * cplx != 0.  rss must be zeroed by the caller. */
-int flpr_interp_rss(const FlprGeom* g, const double* x, int64_t n, const int32_t* order,
-                    const double* y, int32_t ncol_m, int32_t cplx, const double* m, int32_t dh,
-                    double* rss, void* stream);
+/* quad[h*ncol + c] += sum_cell m' W m for grids m (dh, ncol, nodes) float64. */
+int flpr_quadform(const FlprGeom* g, const double* m, int32_t ncol, int32_t dh,
+                  const double* gram, double* quad, void* stream);
 
 /* Multilinear interpolation of one grid (ncol channels, (ncol, nodes)) at n
  * points (api.py:760-763, interp_optimized.py / scipy linear with linear
@@ -179,9 +202,6 @@ int flpr_interp_rss(const FlprGeom* g, const double* x, int64_t n, const int32_t
 int flpr_interp_eval(const FlprGeom* g, const double* x, int64_t n, const double* m,
                      int32_t ncol, double* out, const double* yvar, double* expsum,
                      void* stream);
-
-/* Float64 -> float32 copy (spread grid for the float32 convolution path). */
-int flpr_f64_to_f32(const double* src, float* dst, int64_t count, void* stream);
 
 #ifdef __cplusplus
 }
