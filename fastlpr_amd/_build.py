@@ -6,7 +6,7 @@ import shutil
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = ["csrc/flpr_points.hip", "csrc/flpr_conv.hip"]
+SOURCES = ["csrc/flpr_points.hip", "csrc/flpr_conv.hip", "csrc/flpr_select.hip"]
 OUT = os.path.join(HERE, "libfastlpr_amd.so")
 
 FLAGS = [

@@ -74,29 +74,14 @@ def _as_column_matrix(x):
     return arr.astype(float)
 
 
-def _std_ddof1(x) -> np.ndarray:
-    if isinstance(x, torch.Tensor):
-        s = torch.std(x, dim=0, correction=1).cpu().numpy() if x.shape[0] > 1 \
-            else np.full(x.shape[1], np.nan)
-    else:
-        s = np.std(x, axis=0, ddof=1)
-    return np.asarray(s, dtype=float)
-
-
-def _default_hrange(x) -> np.ndarray:
-    """[0.1, 1] x std(ddof=1) per dimension (utils/helpers.py:74-92)."""
-    s = _std_ddof1(x)
-    s = np.where(s == 0, 1.0, s)
-    return np.column_stack((0.1 * s, 1.0 * s))
-
-
 def _prepare_bandwidths(h, dims: int, options: dict, default_range) -> np.ndarray:
-    """Shape and validate bandwidths (api.py:32-70)."""
+    """Shape and validate bandwidths (api.py:32-70).  default_range is the
+    (dims, 2) range array or a callable producing it (called only when h is None)."""
     if h is None:
         dh = options.get("dh", None)
         if dh is None:
             dh = 20 if dims == 1 else [20] * dims
-        bw = get_hlist(dh, default_range)
+        bw = get_hlist(dh, default_range() if callable(default_range) else default_range)
     else:
         a = np.asarray(h.cpu() if isinstance(h, torch.Tensor) else h, dtype=float)
         if a.ndim == 0:
@@ -187,25 +172,29 @@ def _grid_shape(opts: dict, n: int, dims: int) -> Tuple[int, ...]:
 # Hutchinson probes: NumPy MT19937 stream, cached on the device
 # ---------------------------------------------------------------------------
 
-_PROBES: Dict[tuple, Tuple[torch.Tensor, np.ndarray]] = {}
+_PROBES: Dict[tuple, tuple] = {}
 
 
-def _probes(n: int, K: int, seed, random_matrix, dev) -> Tuple[torch.Tensor, np.ndarray]:
+def _probes(n: int, K: int, seed, random_matrix, dev):
     """Probe matrix p (n, K) = Generator(MT19937(seed)).standard_normal((n, K))
-    (regression.py:1184-1194) and p'p per column.  The matrix depends only on
-    (n, K, seed), not on the data, so it is generated once per key and kept on
-    the device; a caller-supplied random_matrix is used as is."""
+    (regression.py:1184-1194) and p'p per column, on the host and on the device.
+    The matrix depends only on (n, K, seed), not on the data, so it is generated
+    once per key and kept on the device; a caller-supplied random_matrix is used
+    as is.  Returns (p on the device, p'p on the host, p'p on the device)."""
     if random_matrix is not None:
         p = np.asarray(random_matrix.cpu() if isinstance(random_matrix, torch.Tensor)
                        else random_matrix, dtype=float)
-        if p.shape[0] != n:
+        if p.ndim != 2 or p.shape[0] != n:
             raise ValueError(f"random_matrix has wrong shape: expected ({n}, *), got {p.shape}")
-        return torch.as_tensor(p, device=dev).contiguous(), np.sum(p * p, axis=0)
+        pp = np.sum(p * p, axis=0)
+        return (torch.as_tensor(p, device=dev).contiguous(), pp,
+                torch.as_tensor(pp, device=dev))
     key = (n, K, int(seed), str(dev))
     hit = _PROBES.get(key)
     if hit is None:
         p = np.random.Generator(np.random.MT19937(seed)).standard_normal((n, K))
-        hit = (torch.as_tensor(p, device=dev).contiguous(), np.sum(p * p, axis=0))
+        pp = np.sum(p * p, axis=0)
+        hit = (torch.as_tensor(p, device=dev).contiguous(), pp, torch.as_tensor(pp, device=dev))
         if len(_PROBES) > 8:
             _PROBES.clear()
         _PROBES[key] = hit
@@ -213,9 +202,11 @@ def _probes(n: int, K: int, seed, random_matrix, dev) -> Tuple[torch.Tensor, np.
 
 
 def clear_caches():
-    """Drop the cached probe matrices and conv plans."""
+    """Drop the cached probe matrices, conv plans and host geometry / bandwidth tables."""
     _PROBES.clear()
     engine._PLANS.clear()
+    engine._GEOMS.clear()
+    engine._BTABS.clear()
 
 
 # ---------------------------------------------------------------------------
@@ -294,23 +285,38 @@ def _hutchinson(dot: np.ndarray, pp: np.ndarray, n: int, y_is_complex: bool):
     return dof, stderr, pen_std, pen_mean
 
 
-def _gcv(rss: np.ndarray, n: int, dof: float, pen_std: float, pen_mean, aggregation: str):
-    """GCV = mse * penalty, stderr = mse * penalty_std (regression.py:378-421)."""
+def _gcv_all(rss: np.ndarray, n: int, dof: np.ndarray, pen_std: np.ndarray, pen_mean,
+             aggregation: str):
+    """GCV = mse * penalty, stderr = mse * penalty_std (regression.py:378-421) for
+    every bandwidth at once: rss (dh, ncol), dof / pen_std / pen_mean (dh,)."""
+    rss = np.atleast_2d(np.asarray(rss, dtype=float))
+    dof = np.asarray(dof, dtype=float).reshape(-1)
+    pen_std = np.asarray(pen_std, dtype=float).reshape(-1)
     mse = rss / n
-    pd = min(max(1.0 - dof / n, 0.0), 1.0)
-    pen = float(pen_mean) if pen_mean is not None else 1.0 / max(pd, 0.01) ** 2
-    g = mse * pen
+    if pen_mean is not None:
+        pen = np.asarray(pen_mean, dtype=float).reshape(-1)
+    else:
+        pd = np.clip(1.0 - dof / n, 0.0, 1.0)
+        pen = 1.0 / np.maximum(pd, 0.01) ** 2
+    g = mse * pen[:, None]
     if aggregation == "first":
-        score, m = float(g[0]), float(mse[0])
+        score, m = g[:, 0], mse[:, 0]
     elif aggregation == "average":
-        score, m = float(np.mean(g)), float(np.mean(mse))
+        score, m = np.mean(g, axis=1), np.mean(mse, axis=1)
     elif aggregation == "sum":
-        score, m = float(np.sum(g)), float(np.sum(mse))
+        score, m = np.sum(g, axis=1), np.sum(mse, axis=1)
     else:
         raise ValueError(f"Unknown aggregation method: {aggregation}. "
                          "Use 'first', 'average', or 'sum'.")
-    se = m * pen_std if pen_std > 0 else 0.0
+    se = np.where(pen_std > 0, m * pen_std, 0.0)
     return score, se
+
+
+def _gcv(rss: np.ndarray, n: int, dof: float, pen_std: float, pen_mean, aggregation: str):
+    """One bandwidth of _gcv_all: rss (ncol,) -> (score, stderr)."""
+    sc, se = _gcv_all(np.asarray(rss, dtype=float)[None, :], n, [dof], [pen_std],
+                      None if pen_mean is None else [pen_mean], aggregation)
+    return float(sc[0]), float(se[0])
 
 
 def _y_channels(y_arr, dev):
@@ -383,7 +389,6 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
         raise NotImplementedError("the GPU engine supports 1-3 dimensions")
     if str(opts.get("kernel_type", "gaussian")).lower() not in ("gauss", "gaussian", "normal"):
         raise NotImplementedError("the GPU engine implements the Gaussian kernel")
-    bandwidths = _prepare_bandwidths(h, dims, opts, _default_hrange(x_arr))
     variance = str(opts.get("y_type_out", "mean")) == "variance"
     fp64 = str(opts.get("precision", "fp32")).lower() == "fp64"
     N = _grid_shape(opts, n, dims)
@@ -408,6 +413,9 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
         _check_finite(x_arr, "Input x", "cv_fastlpr")
     if stats.bad_y and isinstance(y_arr, torch.Tensor):
         _check_finite(y_arr, "Response y", "cv_fastlpr")
+    # default range 0.1..1 x std(ddof=1) from the device statistics (helpers.py:74-92)
+    bandwidths = _prepare_bandwidths(
+        h, dims, opts, lambda: np.column_stack((0.1 * stats.std, 1.0 * stats.std)))
     reg = float(opts["regularization"])
     calc_dof = bool(opts.get("calc_dof", True))
     xrange = opts.get("xrange", None)
@@ -421,8 +429,7 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
         fit = engine.run_grid(xd, yd, ny, None, 0, bandwidths, order, N, int(opts["accuracy"]),
                               fp64, reg, xrange, flag_power2, stats=stats, with_ones_eps=False)
         sel = 0
-        gcv_result = None
-        dof_sel = dof_err_sel = np.nan
+        sel_dev = sel_out = None
     else:
         for key in ("dof_interp_method", "gcv_interp_method"):
             if opts.get(key, lin_default) != "linear":
@@ -431,55 +438,51 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
                     "5000 points) is not implemented on the GPU yet; pass "
                     f"options['{key}']='linear'")
         K = int(opts.get("num_dof_sample", 10))
+        if K < 1:
+            raise ValueError(
+                "All GCV scores are NaN or Inf. Regression failed for all bandwidths. "
+                "num_dof_sample must be at least 1 for the GCV degrees of freedom.")
         rm = opts.get("random_matrix", None)
-        probes, pp = _probes(n, K, opts.get("dof_seed", 42), rm, dev)
+        probes, _pp, pp_dev = _probes(n, K, opts.get("dof_seed", 42), rm, dev)
         K = probes.shape[1]
         fit = engine.run_grid(xd, yd, ny, probes, K, bandwidths, order, N, int(opts["accuracy"]),
-                              fp64, reg, xrange, flag_power2, want_rss=True, stats=stats)
-        rss = fit.rss
-        if y_is_complex:
-            rss = rss[:, 0::2] + rss[:, 1::2]
-        dof_all, dof_se, pen_std, pen_mean = _hutchinson(fit.dot, pp, n, y_is_complex)
+                              fp64, reg, xrange, flag_power2, want_rss=True, stats=stats,
+                              readback=False)
         agg = str(opts.get("gcv_selection", "first"))
-        scores, errs = [], []
-        for ih in range(fit.h.shape[0]):
-            s_, e_ = _gcv(rss[ih], n, dof_all[ih], pen_std[ih],
-                          None if pen_mean is None else pen_mean[ih], agg)
-            scores.append(s_)
-            errs.append(e_)
-        scores, errs = np.asarray(scores), np.asarray(errs)
-        sel = _one_se_rule(scores, errs, fit.h, True, float(opts.get("dstd", 0.0)))
-        idmin = int(np.argmin(scores))
-        gcv_result = {"bandwidths": fit.h, "gcv_m": scores, "gcv_sd": errs, "idmin": idmin,
-                      "id1se": sel, "h1se": fit.h[sel], "hmin": fit.h[idmin],
-                      "dof": dof_all, "dof_stderr": dof_se}
-        dof_sel, dof_err_sel = float(dof_all[sel]), float(dof_se[sel])
+        # Hutchinson DOF, GCV and the 1-SE choice stay on the device; the
+        # interpolation at the chosen bandwidth is queued behind them and the
+        # host reads everything back once, at the end
+        sel_out, sel_dev = engine.select(fit, stats.ysq_dev, pp_dev, n, ny, K, y_is_complex,
+                                         agg, float(opts.get("dstd", 0.0)))
     if fit.warn:
         warnings.warn(fit.warn, stacklevel=2)
 
     geo = fit.geo
-    grid_vals = fit.m_y[sel]                          # (ny, nodes)
+    if sel_dev is None:
+        grid_vals = fit.m_y[sel]                      # (ny, nodes)
+        s0 = fit.s0_all[sel].reshape(N)
+    else:
+        idx = sel_dev.long()
+        grid_vals = fit.m_y.index_select(0, idx)[0]
+        s0 = fit.s0_all.index_select(0, idx)[0].reshape(N)
     out, expsum = engine.interp_eval(geo, xd, grid_vals, ny,
                                      yvar=y_raw_orig if variance else None)
     d_norm = None
+    fac = None
     if variance:
-        d_norm = 1.0 / ((1.0 / n) * float(expsum.item()))  # api.py:771-778
-        out = torch.exp(out) / d_norm
-        grid_vals = torch.exp(grid_vals) / d_norm
+        # d = 1 / (mean_i y_i / exp(log-variance fit)) (api.py:771-778); applied on
+        # the device, read back once at the end
+        fac = expsum * (1.0 / n)
+        out = torch.exp(out) * fac
+        grid_vals = torch.exp(grid_vals) * fac
     if y_is_complex:
         yhat = torch.complex(out[:, 0::2], out[:, 1::2])
     else:
         yhat = out
     if dy == 1:
         yhat = yhat[:, 0]
-    s0 = fit.s0_all[sel].reshape(N)
     interp = GridInterpolant(geo, geo.axes_raw, grid_vals.contiguous(), N, y_is_complex, dy,
                              as_numpy, dy == 1)
-    if gcv_result is None:
-        gcv_result = {"bandwidths": fit.h, "gcv_m": np.array([np.nan]),
-                      "gcv_sd": np.array([np.nan]), "idmin": 0, "id1se": 0,
-                      "h1se": fit.h[0], "hmin": fit.h[0], "dof": np.array([np.nan]),
-                      "dof_stderr": np.array([np.nan])}
     yref = y_raw_orig if variance else (torch.complex(yd[:, 0::2], yd[:, 1::2]) if y_is_complex
                                         else yd)
     resid = yref.reshape(-1) - yhat.reshape(-1) if dy == 1 or variance else \
@@ -487,6 +490,35 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
 
     def conv(t):
         return t.cpu().numpy() if as_numpy else t
+
+    # the one host read of the fit: selection results (and the variance scale)
+    if sel_out is not None:
+        dh = fit.h.shape[0]
+        if fac is not None:
+            sel_out[4 * dh + 3:].copy_(fac.reshape(1))
+        hv = sel_out.cpu().numpy()
+        if fac is not None:
+            d_norm = 1.0 / float(hv[4 * dh + 3])
+        if int(hv[4 * dh + 2]) == 0:
+            raise ValueError(
+                "All GCV scores are NaN or Inf. Regression failed for all bandwidths. "
+                "Try: (1) Use larger bandwidth h, (2) Increase grid points, "
+                "(3) Check for data quality issues")
+        scores, errs = hv[:dh].copy(), hv[dh:2 * dh].copy()
+        dof_all, dof_se = hv[2 * dh:3 * dh].copy(), hv[3 * dh:4 * dh].copy()
+        sel, idmin = int(hv[4 * dh]), int(hv[4 * dh + 1])
+        gcv_result = {"bandwidths": fit.h, "gcv_m": scores, "gcv_sd": errs, "idmin": idmin,
+                      "id1se": sel, "h1se": fit.h[sel], "hmin": fit.h[idmin],
+                      "dof": dof_all, "dof_stderr": dof_se}
+        dof_sel, dof_err_sel = float(dof_all[sel]), float(dof_se[sel])
+    else:
+        if fac is not None:
+            d_norm = 1.0 / float(fac.item())
+        gcv_result = {"bandwidths": fit.h, "gcv_m": np.array([np.nan]),
+                      "gcv_sd": np.array([np.nan]), "idmin": 0, "id1se": 0,
+                      "h1se": fit.h[0], "hmin": fit.h[0], "dof": np.array([np.nan]),
+                      "dof_stderr": np.array([np.nan])}
+        dof_sel = dof_err_sel = np.nan
 
     metadata = {"options": opts, "order": order, "bandwidths_tested": fit.h,
                 "fitted_at_samples": conv(yhat), "residuals": conv(resid),

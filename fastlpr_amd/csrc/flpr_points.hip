@@ -336,10 +336,14 @@ __device__ inline void raw_locate(double v, double a, double b, int N, int* idx,
   *frac = (v - lo) / (hi - lo);
 }
 
-// Sorted structure-of-arrays copy of what the grid kernels need per point:
-//   wab  [2d][n]  A, B of the tap-weight factorisation per dimension
-//   vals [nv][n]  y columns then probe columns
-//   rawi [d][n]   raw-grid interval, rawf [d][n] fraction in it
+// Sorted point data for the grid kernels, point-major so that the work items
+// of one point (one per channel) read the same addresses:
+//   wab  [n][d][2]  (A_k, B_k): the spread weight of footprint offset j - MSP
+//                   (j = 0 .. 2 MSP) along axis k is A_k B_k^j G_k[j], with
+//                   A = exp(-(delta^2 + 2 MSP hx delta) c), B = exp(2 delta hx c),
+//                   G[j] = exp(-hx^2 (j - MSP)^2 c), c = 1 / (4 tau)
+//   vals [n][nv]    y columns then probe columns
+//   rawi [n][d]     raw-grid interval, rawf [n][d] fraction in it
 template <int D>
 __global__ __launch_bounds__(256) void k_pack(FlprGeom g, const double* __restrict__ x,
                                               const int32_t* __restrict__ order, int64_t n,
@@ -348,6 +352,7 @@ __global__ __launch_bounds__(256) void k_pack(FlprGeom g, const double* __restri
                                               double* __restrict__ wab, double* __restrict__ vals,
                                               int32_t* __restrict__ rawi,
                                               double* __restrict__ rawf) {
+  const int nv = ny + np;
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
        p += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = order[p];
@@ -357,16 +362,25 @@ __global__ __launch_bounds__(256) void k_pack(FlprGeom g, const double* __restri
       int m;
       const double xm = knot_of(g, k, xv, &m);
       const double delta = xm - g.hx[k] * (double)m;
-      wab[(2 * k) * n + p] = exp(-(delta * delta * g.inv4tau[k]));
-      wab[(2 * k + 1) * n + p] = exp(2.0 * delta * g.hx[k] * g.inv4tau[k]);
+      const double c = g.inv4tau[k];
+      wab[(p * D + k) * 2] = exp(-((delta * delta + 2.0 * g.msp * g.hx[k] * delta) * c));
+      wab[(p * D + k) * 2 + 1] = exp(2.0 * delta * g.hx[k] * c);
       int ri;
       double rf;
       raw_locate(xv, g.graw0[k], g.graw1[k], g.n_grid[k], &ri, &rf);
-      rawi[k * n + p] = ri;
-      rawf[k * n + p] = rf;
+      rawi[p * D + k] = ri;
+      rawf[p * D + k] = rf;
     }
-    for (int c = 0; c < ny; ++c) vals[c * n + p] = vy[i * ny + c];
-    for (int c = 0; c < np; ++c) vals[(int64_t)(ny + c) * n + p] = pr[i * np + c];
+  }
+  // values: one thread per (point, column), so that consecutive lanes write
+  // consecutive addresses of the point-major table
+  const int64_t nvals = n * nv;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nvals;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = e / nv;
+    const int c = (int)(e - p * nv);
+    const int64_t i = order[p];
+    vals[e] = (c < ny) ? vy[i * ny + c] : pr[i * np + (c - ny)];
   }
 }
 
@@ -388,206 +402,145 @@ __device__ inline void local_coords(const FlprGeom& g, int lc, int* l) {
   }
 }
 
-// w[j] = exp(-(delta - hx (j - MSP))^2 c) from A, B and the constants G[j].
+// Footprint weights along one axis: w[j] = A B^j G[j], j = 0 .. 2 MSP.
 template <int MSP>
 __device__ inline void tap_weights(double A, double B, const double* G, double* w) {
-  w[MSP] = A * G[MSP];
-  double bp = 1.0;
+  double bp = A;
 #pragma unroll
-  for (int k = 1; k <= MSP; ++k) {
+  for (int j = 0; j <= 2 * MSP; ++j) {
+    w[j] = bp * G[j];
     bp *= B;
-    w[MSP + k] = A * bp * G[MSP + k];
-  }
-  const double binv = 1.0 / B;
-  bp = 1.0;
-#pragma unroll
-  for (int k = 1; k <= MSP; ++k) {
-    bp *= binv;
-    w[MSP - k] = A * bp * G[MSP - k];
   }
 }
 
 constexpr int SPREAD_THREADS = 256;
+constexpr int SPREAD_MAX_TAPS = 13;
 
-// Register-accumulated spread (d = 1, 2).  Item = (cell, channel[, row j0]);
-// SPLIT items carry one footprint row (used when (2 MSP + 1)^2 float64
-// accumulators would not fit in registers).
-template <int D, int MSP, bool SPLIT>
+// G[k][j] = exp(-hx_k^2 (j - MSP)^2 / (4 tau_k)); passed by value so that the
+// kernels read it through the scalar unit.
+struct SpreadTab {
+  double G[FLPR_MAX_DIM][SPREAD_MAX_TAPS];
+};
+
+// Register-accumulated spread.  Work item = (cell, split, row, channel), the
+// channel fastest so that the lanes sharing a point read the same point data.
+// An item accumulates its share (split s of nsplit) of the cell's points into
+// a footprint block held in float64 registers: the last F = d - FIX axes of
+// the (2 MSP + 1)^d footprint, at fixed offsets ("row") along the first FIX
+// axes (FIX = 0 for d = 1 and for d = 2 with MSP = 4, so the whole footprint
+// is one block; otherwise enough axes are fixed to keep the block at
+// (2 MSP + 1)^2 registers or fewer).  The block is added once into the tile's
+// LDS window, and the window is written to the tile's partial slot
+// (k_spread_gather sums the overlapping windows per cell, without atomics).
+template <int D, int MSP>
 __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_cells(
-    FlprGeom g, const int32_t* __restrict__ key_start, const double* __restrict__ wab,
-    const double* __restrict__ vals, int64_t n, int with_ones, int C, int cg,
-    double* __restrict__ part) {
+    FlprGeom g, SpreadTab tab, const int32_t* __restrict__ key_start,
+    const double* __restrict__ wab, const double* __restrict__ vals, int64_t n, int with_ones,
+    int C, int cg, int nsplit, double* __restrict__ part) {
   extern __shared__ double lds[];
   constexpr int ntap = 2 * MSP + 1;
+  constexpr int FIX = (D == 1) ? 0 : (D == 2 ? (MSP > 4 ? 1 : 0) : (MSP > 4 ? 2 : 1));
+  constexpr int F = D - FIX;
+  constexpr int ROWS = (FIX == 0) ? 1 : (FIX == 1 ? ntap : ntap * ntap);
+  constexpr int NACC = (F == 1) ? ntap : ntap * ntap;
+  const int nv = C - with_ones;
   const int c0 = blockIdx.y * cg;
   const int cn = min(cg, C - c0);
   int t[3] = {0, 0, 0};
   tile_coords<D>(g, blockIdx.x, t);
-  int fp[3] = {1, 1, 1}, org[3] = {0, 0, 0};
+  int fp[3] = {1, 1, 1};
   int fcells = 1, tc = 1;
 #pragma unroll
   for (int k = 0; k < D; ++k) {
     fp[k] = g.tile[k] + 2 * MSP;
-    org[k] = t[k] * g.tile[k] - MSP;
     fcells *= fp[k];
     tc *= g.tile[k];
   }
   for (int i = threadIdx.x; i < fcells * cn; i += blockDim.x) lds[i] = 0.0;
-  double G[D][ntap];
-#pragma unroll
-  for (int k = 0; k < D; ++k) {
-    const double c2 = g.hx[k] * g.hx[k] * g.inv4tau[k];
-#pragma unroll
-    for (int j = 0; j < ntap; ++j) G[k][j] = exp(-c2 * (double)((j - MSP) * (j - MSP)));
-  }
   __syncthreads();
-  const int nsplit = SPLIT ? ntap : 1;
-  const int nitems = tc * cn * nsplit;
   const int kbase = blockIdx.x * tc;
+  const int nitems = tc * nsplit * ROWS * cn;
   for (int item = threadIdx.x; item < nitems; item += blockDim.x) {
-    const int lc = item % tc;
-    const int rest = item / tc;
-    const int c = rest % cn;
-    const int js = rest / cn;  // footprint row of a SPLIT item
+    const int c = item % cn;
+    int rest = item / cn;
+    const int row = rest % ROWS;
+    rest /= ROWS;
+    const int sp = rest % nsplit;
+    const int lc = rest / nsplit;
     const int p0 = key_start[kbase + lc], p1 = key_start[kbase + lc + 1];
-    if (p0 == p1) continue;
+    const int64_t len = p1 - p0;
+    const int q0 = p0 + (int)((len * sp) / nsplit);
+    const int q1 = p0 + (int)((len * (sp + 1)) / nsplit);
+    if (q0 == q1) continue;
     const int ch = c0 + c;
-    const double* vch = (ch >= with_ones) ? vals + (int64_t)(ch - with_ones) * n : nullptr;
-    int l[3] = {0, 0, 0};
-    local_coords<D>(g, lc, l);
-    if constexpr (D == 1) {
-      double acc[ntap];
+    const bool ones = ch < with_ones;
+    const double* vp = vals + (ones ? 0 : ch - with_ones);
+    int pos[3] = {0, 0, 0};
+    local_coords<D>(g, lc, pos);
+    int r[2] = {0, 0};  // fixed offsets along axes 0 .. FIX-1
+    if constexpr (FIX == 1) r[0] = row;
+    if constexpr (FIX == 2) {
+      r[0] = row / ntap;
+      r[1] = row % ntap;
+    }
+    double acc[NACC];
 #pragma unroll
-      for (int j = 0; j < ntap; ++j) acc[j] = 0.0;
-      for (int p = p0; p < p1; ++p) {
-        const double v = vch ? vch[p] : 1.0;
-        double w[ntap];
-        tap_weights<MSP>(wab[p], wab[n + p], G[0], w);
+    for (int a = 0; a < NACC; ++a) acc[a] = 0.0;
+    // software pipeline: the next point's data is loaded before the current
+    // point's footprint is accumulated
+    double nxt[2 * D];
 #pragma unroll
-        for (int j = 0; j < ntap; ++j) acc[j] += v * w[j];
+    for (int k = 0; k < 2 * D; ++k) nxt[k] = wab[(int64_t)q0 * (2 * D) + k];
+    double nxv = ones ? 1.0 : vp[(int64_t)q0 * nv];
+    for (int p = q0; p < q1; ++p) {
+      double cur[2 * D];
+#pragma unroll
+      for (int k = 0; k < 2 * D; ++k) cur[k] = nxt[k];
+      double s = nxv;
+      if (p + 1 < q1) {
+#pragma unroll
+        for (int k = 0; k < 2 * D; ++k) nxt[k] = wab[(int64_t)(p + 1) * (2 * D) + k];
+        nxv = ones ? 1.0 : vp[(int64_t)(p + 1) * nv];
       }
+      double w[D][ntap];
 #pragma unroll
-      for (int j = 0; j < ntap; ++j) atomicAdd(&lds[(l[0] + j) * cn + c], acc[j]);
-    } else if constexpr (!SPLIT) {
-      double acc[ntap][ntap];
+      for (int k = 0; k < D; ++k) tap_weights<MSP>(cur[2 * k], cur[2 * k + 1], tab.G[k], w[k]);
 #pragma unroll
-      for (int a = 0; a < ntap; ++a)
+      for (int k = 0; k < FIX; ++k) {
+        double sel = 0.0;
 #pragma unroll
-        for (int b = 0; b < ntap; ++b) acc[a][b] = 0.0;
-      for (int p = p0; p < p1; ++p) {
-        const double v = vch ? vch[p] : 1.0;
-        double w0[ntap], w1[ntap];
-        tap_weights<MSP>(wab[p], wab[n + p], G[0], w0);
-        tap_weights<MSP>(wab[2 * n + p], wab[3 * n + p], G[1], w1);
+        for (int j = 0; j < ntap; ++j) sel = (j == r[k]) ? w[k][j] : sel;
+        s *= sel;
+      }
+      if constexpr (F == 1) {
+#pragma unroll
+        for (int j = 0; j < ntap; ++j) acc[j] += s * w[D - 1][j];
+      } else {
 #pragma unroll
         for (int a = 0; a < ntap; ++a) {
-          const double vw = v * w0[a];
+          const double sa = s * w[D - 2][a];
 #pragma unroll
-          for (int b = 0; b < ntap; ++b) acc[a][b] += vw * w1[b];
+          for (int b = 0; b < ntap; ++b) acc[a * ntap + b] += sa * w[D - 1][b];
         }
       }
+    }
+    // window cell of the block's first entry, then the free axes' strides
+    int base = 0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) base = base * fp[k] + pos[k] + (k < FIX ? r[k] : 0);
+    if constexpr (F == 1) {
+#pragma unroll
+      for (int j = 0; j < ntap; ++j) atomicAdd(&lds[(base + j) * cn + c], acc[j]);
+    } else {
+      const int sa = fp[D - 1];
 #pragma unroll
       for (int a = 0; a < ntap; ++a)
 #pragma unroll
         for (int b = 0; b < ntap; ++b)
-          atomicAdd(&lds[((l[0] + a) * fp[1] + (l[1] + b)) * cn + c], acc[a][b]);
-    } else {
-      double acc[ntap];
-#pragma unroll
-      for (int b = 0; b < ntap; ++b) acc[b] = 0.0;
-      for (int p = p0; p < p1; ++p) {
-        const double v = vch ? vch[p] : 1.0;
-        double w0[ntap], w1[ntap];
-        tap_weights<MSP>(wab[p], wab[n + p], G[0], w0);
-        tap_weights<MSP>(wab[2 * n + p], wab[3 * n + p], G[1], w1);
-        double sel = 0.0;
-#pragma unroll
-        for (int a = 0; a < ntap; ++a) sel = (a == js) ? w0[a] : sel;
-        const double vw = v * sel;
-#pragma unroll
-        for (int b = 0; b < ntap; ++b) acc[b] += vw * w1[b];
-      }
-#pragma unroll
-      for (int b = 0; b < ntap; ++b)
-        atomicAdd(&lds[((l[0] + js) * fp[1] + (l[1] + b)) * cn + c], acc[b]);
+          atomicAdd(&lds[(base + a * sa + b) * cn + c], acc[a * ntap + b]);
     }
   }
   __syncthreads();
-  // the tile's window goes to its own partial slot; k_spread_gather adds the
-  // overlapping windows per cell without atomics
-  double* dst = part + ((int64_t)blockIdx.x * C + c0) * fcells;
-  for (int e = threadIdx.x; e < fcells * cn; e += blockDim.x) {
-    const int c = e / fcells, cell = e % fcells;
-    dst[(int64_t)c * fcells + cell] = lds[cell * cn + c];
-  }
-}
-
-// 3-D spread: one thread per point scatters its (2 MSP + 1)^3 footprint into
-// the tile's LDS window with float64 atomics.
-template <int MSP>
-__global__ __launch_bounds__(SPREAD_THREADS) void k_spread_points3(
-    FlprGeom g, const int32_t* __restrict__ key_start, const double* __restrict__ wab,
-    const double* __restrict__ vals, int64_t n, int with_ones, int C, int cg,
-    double* __restrict__ part) {
-  extern __shared__ double lds[];
-  constexpr int D = 3;
-  constexpr int ntap = 2 * MSP + 1;
-  const int c0 = blockIdx.y * cg;
-  const int cn = min(cg, C - c0);
-  int t[3];
-  tile_coords<D>(g, blockIdx.x, t);
-  int fp[3], org[3];
-  int fcells = 1, tc = 1;
-  for (int k = 0; k < D; ++k) {
-    fp[k] = g.tile[k] + 2 * MSP;
-    org[k] = t[k] * g.tile[k] - MSP;
-    fcells *= fp[k];
-    tc *= g.tile[k];
-  }
-  for (int i = threadIdx.x; i < fcells * cn; i += blockDim.x) lds[i] = 0.0;
-  double G[D][ntap];
-#pragma unroll
-  for (int k = 0; k < D; ++k) {
-    const double c2 = g.hx[k] * g.hx[k] * g.inv4tau[k];
-#pragma unroll
-    for (int j = 0; j < ntap; ++j) G[k][j] = exp(-c2 * (double)((j - MSP) * (j - MSP)));
-  }
-  __syncthreads();
-  const int kbase = blockIdx.x * tc;
-  const int p_begin = key_start[kbase], p_end = key_start[kbase + tc];
-  for (int p = p_begin + threadIdx.x; p < p_end; p += blockDim.x) {
-    // cell of this point: binary search of the tile's key table
-    int lo = 0, hi = tc - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (key_start[kbase + mid] <= p) lo = mid;
-      else hi = mid - 1;
-    }
-    int l[3];
-    local_coords<D>(g, lo, l);
-    double w0[ntap], w1[ntap], w2[ntap];
-    tap_weights<MSP>(wab[p], wab[n + p], G[0], w0);
-    tap_weights<MSP>(wab[2 * n + p], wab[3 * n + p], G[1], w1);
-    tap_weights<MSP>(wab[4 * n + p], wab[5 * n + p], G[2], w2);
-    for (int c = 0; c < cn; ++c) {
-      const int ch = c0 + c;
-      const double v = (ch >= with_ones) ? vals[(int64_t)(ch - with_ones) * n + p] : 1.0;
-#pragma unroll
-      for (int a = 0; a < ntap; ++a) {
-        const double va = v * w0[a];
-#pragma unroll
-        for (int b = 0; b < ntap; ++b) {
-          const double vab = va * w1[b];
-          const int base = (((l[0] + a) * fp[1] + (l[1] + b)) * fp[2] + l[2]) * cn + c;
-#pragma unroll
-          for (int e = 0; e < ntap; ++e) atomicAdd(&lds[base + e * cn], vab * w2[e]);
-        }
-      }
-    }
-  }
-  __syncthreads();
-  // the tile's window goes to its own partial slot; k_spread_gather adds the
-  // overlapping windows per cell without atomics
   double* dst = part + ((int64_t)blockIdx.x * C + c0) * fcells;
   for (int e = threadIdx.x; e < fcells * cn; e += blockDim.x) {
     const int c = e / fcells, cell = e % fcells;
@@ -694,19 +647,20 @@ __device__ inline void gram_pair(int e, int* qa, int* qb) {
 
 constexpr int ADJ_THREADS = 256;
 
-// Adjoint interpolation sums on the raw grid for one channel group, per
-// spread tile:
+// Adjoint interpolation sums on the raw grid, per spread tile:
 //   mode 0: q[c][node]   += v_ic w_i(node)          (value channels)
 //   mode 1: W[e][rcell]  += w_i(qa) w_i(qb)          (Gram entries)
-// Item = (spread cell, channel); the points of a spread cell touch at most
-// three raw intervals per dimension.  Accumulated in registers, then in the
-// tile's LDS window, which is written to the tile's partial slot
-// (k_adjoint_gather adds the overlapping windows).
+// Work item = (spread cell, split, channel) with the channel fastest; the
+// points of a spread cell touch at most three raw intervals per dimension, so
+// an item accumulates 4^d node (mode 0) or 3^d cell (mode 1) sums in
+// registers, then adds them into the tile's LDS window, which is written to
+// the tile's partial slot (k_adjoint_gather adds the overlapping windows).
 template <int D>
 __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(
     FlprGeom g, const int32_t* __restrict__ key_start, const int32_t* __restrict__ rawi,
     const double* __restrict__ rawf, const double* __restrict__ vals, int64_t n, int nch,
-    int mode, int cg, double* __restrict__ out, int64_t out_stride, double* __restrict__ part) {
+    int mode, int cg, int nsplit, double* __restrict__ out, int64_t out_stride,
+    double* __restrict__ part) {
   extern __shared__ double lds[];
   constexpr int NA = (D == 1) ? 4 : (D == 2 ? 16 : 64);   // 4^D node accumulators
   constexpr int NC = (D == 1) ? 3 : (D == 2 ? 9 : 27);    // 3^D cell accumulators
@@ -714,25 +668,29 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(
   const int cn = min(cg, nch - c0);
   int t[3] = {0, 0, 0};
   tile_coords<D>(g, blockIdx.x, t);
-  int ext[3] = {1, 1, 1}, lb[3] = {0, 0, 0}, lim[3] = {1, 1, 1};
+  int ext[3] = {1, 1, 1}, lb[3] = {0, 0, 0};
   int wcells = 1, tc = 1;
 #pragma unroll
   for (int k = 0; k < D; ++k) {
     ext[k] = g.tile[k] + 5;
     lb[k] = t[k] * g.tile[k] - g.E[k] + (g.n_grid[k] - 1) / 2 - 2;
-    lim[k] = (mode == 0) ? g.n_grid[k] : g.n_grid[k] - 1;
     wcells *= ext[k];
     tc *= g.tile[k];
   }
   for (int i = threadIdx.x; i < wcells * cn; i += blockDim.x) lds[i] = 0.0;
   __syncthreads();
   const int kbase = blockIdx.x * tc;
-  const int nitems = tc * cn;
+  const int nitems = tc * nsplit * cn;
   for (int item = threadIdx.x; item < nitems; item += blockDim.x) {
-    const int lc = item % tc;
-    const int c = item / tc;
+    const int c = item % cn;
+    const int rest = item / cn;
+    const int sp = rest % nsplit;
+    const int lc = rest / nsplit;
     const int p0 = key_start[kbase + lc], p1 = key_start[kbase + lc + 1];
-    if (p0 == p1) continue;
+    const int64_t len = p1 - p0;
+    const int q0 = p0 + (int)((len * sp) / nsplit);
+    const int q1 = p0 + (int)((len * (sp + 1)) / nsplit);
+    if (q0 == q1) continue;
     int l[3] = {0, 0, 0};
     local_coords<D>(g, lc, l);
     int ib[3] = {0, 0, 0};  // lowest raw interval the cell's points can fall in
@@ -745,18 +703,19 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(
     double acc[NA];
 #pragma unroll
     for (int a = 0; a < NA; ++a) acc[a] = 0.0;
-    for (int p = p0; p < p1; ++p) {
-      int rel[3] = {0, 0, 0};
+    for (int p = q0; p < q1; ++p) {
+      int ri[3] = {0, 0, 0}, rel[3] = {0, 0, 0};
       double f[3] = {0, 0, 0};
       bool inside = true;
 #pragma unroll
       for (int k = 0; k < D; ++k) {
-        rel[k] = rawi[k * n + p] - ib[k];
-        f[k] = rawf[k * n + p];
+        ri[k] = rawi[(int64_t)p * D + k];
+        f[k] = rawf[(int64_t)p * D + k];
+        rel[k] = ri[k] - ib[k];
         if (rel[k] < 0 || rel[k] > 2) inside = false;
       }
       if (mode == 0) {
-        const double v = vals[(int64_t)ch * n + p];
+        const double v = vals[(int64_t)p * nch + ch];
         if (!inside) {
 #pragma unroll
           for (int q = 0; q < (1 << D); ++q) {
@@ -766,7 +725,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(
             for (int k = 0; k < D; ++k) {
               const int bit = (q >> (D - 1 - k)) & 1;
               w *= bit ? f[k] : 1.0 - f[k];
-              node = node * g.n_grid[k] + rawi[k * n + p] + bit;
+              node = node * g.n_grid[k] + ri[k] + bit;
             }
             atomicAdd(&out[(int64_t)ch * out_stride + node], w);
           }
@@ -810,7 +769,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(
         if (!inside) {
           int64_t rc = 0;
 #pragma unroll
-          for (int k = 0; k < D; ++k) rc = rc * (g.n_grid[k] - 1) + rawi[k * n + p];
+          for (int k = 0; k < D; ++k) rc = rc * (g.n_grid[k] - 1) + ri[k];
           atomicAdd(&out[(int64_t)ch * out_stride + rc], val);
           continue;
         }
@@ -1014,12 +973,44 @@ int lds_group(int64_t cells_per_channel, int channels, int cap, int64_t budget =
   return cg;
 }
 
-template <int D, int MSP, bool SPLIT>
-void launch_cells(dim3 grd, size_t lds, hipStream_t st, const FlprGeom& g, const int32_t* ks,
-                  const double* wab, const double* vals, int64_t n, int wo, int C, int cg,
-                  double* part) {
-  hipLaunchKernelGGL((k_spread_cells<D, MSP, SPLIT>), grd, dim3(SPREAD_THREADS), lds, st, g, ks,
-                     wab, vals, n, wo, C, cg, part);
+template <int D>
+void launch_spread(int msp, dim3 grd, size_t lds, hipStream_t st, const FlprGeom& g,
+                   const SpreadTab& tab, const int32_t* ks, const double* wab, const double* vals,
+                   int64_t n, int wo, int C, int cg, int nsplit, double* part) {
+  if (msp == 4)
+    hipLaunchKernelGGL((k_spread_cells<D, 4>), grd, dim3(SPREAD_THREADS), lds, st, g, tab, ks, wab,
+                       vals, n, wo, C, cg, nsplit, part);
+  else if (msp == 5)
+    hipLaunchKernelGGL((k_spread_cells<D, 5>), grd, dim3(SPREAD_THREADS), lds, st, g, tab, ks, wab,
+                       vals, n, wo, C, cg, nsplit, part);
+  else
+    hipLaunchKernelGGL((k_spread_cells<D, 6>), grd, dim3(SPREAD_THREADS), lds, st, g, tab, ks, wab,
+                       vals, n, wo, C, cg, nsplit, part);
+}
+
+// Points per work item the spread and adjoint aim for when a cell holds many
+// points (a cell's points are split over several items), and the number of
+// blocks below which channels are split over blocks.
+constexpr int ITEM_POINTS_1D = 8;
+constexpr int ITEM_POINTS = 16;
+constexpr int MIN_BLOCKS = 512;
+
+int item_splits(const FlprGeom* g, int64_t n) {
+  const int64_t nodes = flpr_nodes(g);
+  const int target = (g->d == 1) ? ITEM_POINTS_1D : ITEM_POINTS;
+  int64_t s = (n + nodes * target - 1) / (nodes * target);
+  if (s < 1) s = 1;
+  if (s > 64) s = 64;
+  return (int)s;
+}
+
+// Channels per block: all of them when the tiles alone fill the device, else
+// split so that tiles x groups reaches MIN_BLOCKS; capped by the LDS budget.
+int channel_group(int64_t ntiles, int64_t cells_per_channel, int channels) {
+  int64_t want = (MIN_BLOCKS + ntiles - 1) / ntiles;
+  if (want < 1) want = 1;
+  int cg = (int)((channels + want - 1) / want);
+  return lds_group(cells_per_channel, channels, cg < 1 ? 1 : cg);
 }
 
 }  // namespace
@@ -1127,24 +1118,25 @@ int flpr_spread(const FlprGeom* g, const int32_t* key_start, const double* wab,
     fcells *= g->tile[k] + 2 * g->msp;
     cells *= g->G[k];
   }
-  const int cg = lds_group(fcells, C, 16);
+  const int64_t ntiles = flpr_ntiles(g);
+  const int cg = channel_group(ntiles, fcells, C);
   const size_t lds = (size_t)fcells * cg * sizeof(double);
   if (lds > 64 * 1024) return -1;
-  dim3 grd((unsigned)flpr_ntiles(g), (unsigned)((C + cg - 1) / cg));
+  SpreadTab tab;
+  for (int k = 0; k < FLPR_MAX_DIM; ++k)
+    for (int j = 0; j < SPREAD_MAX_TAPS; ++j) {
+      const double c2 = (k < g->d) ? g->hx[k] * g->hx[k] * g->inv4tau[k] : 0.0;
+      const double o = (double)(j - g->msp);
+      tab.G[k][j] = exp(-c2 * o * o);
+    }
+  const int nsplit = item_splits(g, n);
+  dim3 grd((unsigned)ntiles, (unsigned)((C + cg - 1) / cg));
   hipStream_t st = as_stream(stream);
   const int msp = g->msp;
-  if (g->d == 1) {
-    if (msp == 4) launch_cells<1, 4, false>(grd, lds, st, *g, key_start, wab, vals, n, wo, C, cg, part);
-    else if (msp == 5) launch_cells<1, 5, false>(grd, lds, st, *g, key_start, wab, vals, n, wo, C, cg, part);
-    else launch_cells<1, 6, false>(grd, lds, st, *g, key_start, wab, vals, n, wo, C, cg, part);
-  } else if (g->d == 2) {
-    if (msp == 4) launch_cells<2, 4, false>(grd, lds, st, *g, key_start, wab, vals, n, wo, C, cg, part);
-    else if (msp == 5) launch_cells<2, 5, true>(grd, lds, st, *g, key_start, wab, vals, n, wo, C, cg, part);
-    else launch_cells<2, 6, true>(grd, lds, st, *g, key_start, wab, vals, n, wo, C, cg, part);
-  } else {
-    if (msp == 4) hipLaunchKernelGGL(k_spread_points3<4>, grd, dim3(SPREAD_THREADS), lds, st, *g, key_start, wab, vals, n, wo, C, cg, part);
-    else if (msp == 5) hipLaunchKernelGGL(k_spread_points3<5>, grd, dim3(SPREAD_THREADS), lds, st, *g, key_start, wab, vals, n, wo, C, cg, part);
-    else hipLaunchKernelGGL(k_spread_points3<6>, grd, dim3(SPREAD_THREADS), lds, st, *g, key_start, wab, vals, n, wo, C, cg, part);
+  switch (g->d) {
+    case 1: launch_spread<1>(msp, grd, lds, st, *g, tab, key_start, wab, vals, n, wo, C, cg, nsplit, part); break;
+    case 2: launch_spread<2>(msp, grd, lds, st, *g, tab, key_start, wab, vals, n, wo, C, cg, nsplit, part); break;
+    default: launch_spread<3>(msp, grd, lds, st, *g, tab, key_start, wab, vals, n, wo, C, cg, nsplit, part); break;
   }
   FLPR_CHECK_LAUNCH();
   dim3 gg(flpr_grid_for(cells, 256, 256), (unsigned)C);
@@ -1183,20 +1175,21 @@ int flpr_adjoint(const FlprGeom* g, const int32_t* key_start, const int32_t* raw
   }
   const int nq = 1 << g->d;
   const int nw = nq * (nq + 1) / 2;
+  const int nsplit = item_splits(g, n);
   hipStream_t st = as_stream(stream);
   for (int mode = 0; mode < 2; ++mode) {
     const int nch = (mode == 0) ? nval : (gram ? nw : 0);
     if (nch == 0) continue;
     double* out = (mode == 0) ? q : gram;
     const int64_t ostride = (mode == 0) ? nodes : rcells;
-    const int cg = lds_group(wcells, nch, 64);
+    const int cg = channel_group(flpr_ntiles(g), wcells, nch);
     const size_t lds = (size_t)wcells * cg * sizeof(double);
     if (lds > 64 * 1024) return -1;
     dim3 grd((unsigned)flpr_ntiles(g), (unsigned)((nch + cg - 1) / cg));
     switch (g->d) {
-      case 1: hipLaunchKernelGGL(k_adjoint<1>, grd, dim3(ADJ_THREADS), lds, st, *g, key_start, rawi, rawf, vals, n, nch, mode, cg, out, ostride, part); break;
-      case 2: hipLaunchKernelGGL(k_adjoint<2>, grd, dim3(ADJ_THREADS), lds, st, *g, key_start, rawi, rawf, vals, n, nch, mode, cg, out, ostride, part); break;
-      default: hipLaunchKernelGGL(k_adjoint<3>, grd, dim3(ADJ_THREADS), lds, st, *g, key_start, rawi, rawf, vals, n, nch, mode, cg, out, ostride, part); break;
+      case 1: hipLaunchKernelGGL(k_adjoint<1>, grd, dim3(ADJ_THREADS), lds, st, *g, key_start, rawi, rawf, vals, n, nch, mode, cg, nsplit, out, ostride, part); break;
+      case 2: hipLaunchKernelGGL(k_adjoint<2>, grd, dim3(ADJ_THREADS), lds, st, *g, key_start, rawi, rawf, vals, n, nch, mode, cg, nsplit, out, ostride, part); break;
+      default: hipLaunchKernelGGL(k_adjoint<3>, grd, dim3(ADJ_THREADS), lds, st, *g, key_start, rawi, rawf, vals, n, nch, mode, cg, nsplit, out, ostride, part); break;
     }
     FLPR_CHECK_LAUNCH();
     dim3 gg(flpr_grid_for(ostride, 256, 256), (unsigned)nch);

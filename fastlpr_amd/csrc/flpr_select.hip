@@ -1,0 +1,207 @@
+// Bandwidth selection on the device: Hutchinson degrees of freedom, GCV
+// scores and the 1-SE rule, from the per-bandwidth sums the solve and the
+// quadratic form leave behind.  Running it on the device lets the final
+// interpolation at the selected bandwidth be queued behind it, so one fit
+// needs no host round trip between the solve and the result.
+//
+// Reference dataflow restated here (fastLPR_py/src/fastlpr/...):
+//   regression.py:1276-1354  tr(H) ~ n (p'phat)/(p'p), NaN -> n, clip [0, n],
+//                            penalty 1/max(pdof, .01)^2, mean / std(ddof=1)
+//   regression.py:378-421    GCV = mse * penalty, stderr = mse * penalty_std,
+//                            aggregation first / average / sum over columns
+//   api.py:73-164            1-SE rule, larger sum(h) preferred, ties broken
+//                            by the smaller score
+// The arithmetic follows NumPy's operation order (sequential sums over the
+// probe axis, NumPy's pairwise sum over columns) without contraction, so the
+// scores and the selected index equal a NumPy evaluation of the same sums.
+#include "flpr_common.h"
+
+namespace {
+
+constexpr int SEL_THREADS = 256;
+
+__device__ inline double clip(double v, double lo, double hi) {
+  // np.clip: NaN stays NaN
+  return (v != v) ? v : (v < lo ? lo : (v > hi ? hi : v));
+}
+
+__device__ inline double np_maximum(double a, double b) {
+  // np.maximum propagates NaN
+  if (a != a) return a;
+  if (b != b) return b;
+  return a > b ? a : b;
+}
+
+// NumPy's pairwise summation of v[0..n) (n <= 128: eight strided partial sums,
+// combined pairwise, then the remainder); n < 8 is a plain loop.
+__device__ inline double np_sum(const double* v, int n) {
+#pragma clang fp contract(off)
+  if (n < 8) {
+    double r = -0.0;
+    for (int i = 0; i < n; ++i) r += v[i];
+    return r;
+  }
+  double r[8];
+  for (int j = 0; j < 8; ++j) r[j] = v[j];
+  int i = 8;
+  for (; i < n - (n % 8); i += 8)
+    for (int j = 0; j < 8; ++j) r[j] += v[i + j];
+  double res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+  for (; i < n; ++i) res += v[i];
+  return res;
+}
+
+// One thread per bandwidth for the scores, then thread 0 runs the selection.
+//   red     (dh, nval) dots [y columns, probes] then (dh, ny) quadratic forms
+//   ysq     (ny) sum_i y_ic^2; pp (K) p'p per probe; hsum (dh) sum_k h_bk
+//   out     scores, errs, dof, dof_stderr (dh each), then sel, idmin, nvalid
+__global__ __launch_bounds__(SEL_THREADS) void k_select(
+    const double* __restrict__ red, int dh, int nval, int ny, int K,
+    const double* __restrict__ ysq, const double* __restrict__ pp,
+    const double* __restrict__ hsum, double n, int cplx, int agg, double dstd,
+    double* __restrict__ out, int32_t* __restrict__ sel_out) {
+#pragma clang fp contract(off)
+  double* scores = out;
+  double* errs = out + dh;
+  double* dofs = out + 2 * dh;
+  double* dses = out + 3 * dh;
+  const int ncol = cplx ? ny / 2 : ny;
+  for (int b = threadIdx.x; b < dh; b += blockDim.x) {
+    const double* dot = red + (int64_t)b * nval;
+    const double* quad = red + (int64_t)dh * nval + (int64_t)b * ny;
+    // --- Hutchinson over the K probes (sequential sums, as NumPy over axis 0)
+    double tr[FLPR_MAX_CHANNELS];
+    double s = 0.0;
+    for (int k = 0; k < K; ++k) {
+      double t = n * (dot[ny + k] / pp[k]);
+      if (t != t) t = n;
+      t = clip(t, 0.0, n);
+      tr[k] = t;
+      s += t;
+    }
+    double dof = s / (double)K;
+    double dse = 0.0, pen_std = 0.0, pen_mean = 0.0;
+    const bool many = K > 1;
+    if (many) {
+      double ss = 0.0;
+      for (int k = 0; k < K; ++k) {
+        const double e = tr[k] - dof;
+        ss += e * e;
+      }
+      dse = sqrt(ss / (double)(K - 1)) / sqrt((double)K);
+      double pen[FLPR_MAX_CHANNELS];
+      double pcs = 0.0, ps = 0.0;
+      for (int k = 0; k < K; ++k) {
+        double pd = 1.0 - tr[k] / n;
+        if (cplx) pd = 2.0 * pd - 1.0;
+        const double pc = clip(pd, 0.0, 1.0);
+        const double m = np_maximum(pc, 0.01);
+        pen[k] = 1.0 / (m * m);
+        pcs += pc;
+        ps += pen[k];
+      }
+      pen_mean = ps / (double)K;
+      double es = 0.0;
+      for (int k = 0; k < K; ++k) {
+        const double e = pen[k] - pen_mean;
+        es += e * e;
+      }
+      pen_std = sqrt(es / (double)(K - 1));
+      if (cplx) dof = n * (1.0 - pcs / (double)K);
+    }
+    // --- residual sums per real column, complex columns paired
+    double mse[FLPR_MAX_CHANNELS], g[FLPR_MAX_CHANNELS];
+    double pen;
+    if (many) {
+      pen = pen_mean;
+    } else {
+      const double pd = clip(1.0 - dof / n, 0.0, 1.0);
+      const double m = np_maximum(pd, 0.01);
+      pen = 1.0 / (m * m);
+    }
+    for (int c = 0; c < ncol; ++c) {
+      double rss;
+      if (cplx) {
+        const double r0 = (ysq[2 * c] - 2.0 * dot[2 * c]) + quad[2 * c];
+        const double r1 = (ysq[2 * c + 1] - 2.0 * dot[2 * c + 1]) + quad[2 * c + 1];
+        rss = r0 + r1;
+      } else {
+        rss = (ysq[c] - 2.0 * dot[c]) + quad[c];
+      }
+      mse[c] = rss / n;
+      g[c] = mse[c] * pen;
+    }
+    double score, m;
+    if (agg == 0) {
+      score = g[0];
+      m = mse[0];
+    } else if (agg == 1) {
+      score = np_sum(g, ncol) / (double)ncol;
+      m = np_sum(mse, ncol) / (double)ncol;
+    } else {
+      score = np_sum(g, ncol);
+      m = np_sum(mse, ncol);
+    }
+    scores[b] = score;
+    errs[b] = (pen_std > 0.0) ? m * pen_std : 0.0;
+    dofs[b] = dof;
+    dses[b] = many ? dse : 0.0;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  // --- idmin: np.argmin (the first NaN when there is one)
+  int idmin = 0;
+  for (int b = 0; b < dh; ++b) {
+    const double v = scores[b], w = scores[idmin];
+    if (v != v) {
+      idmin = b;
+      break;
+    }
+    if (v < w) idmin = b;
+  }
+  // --- 1-SE rule over the finite scores, prefer the larger sum(h)
+  int best = -1, nvalid = 0;
+  for (int b = 0; b < dh; ++b) {
+    if (!isfinite(scores[b])) continue;
+    ++nvalid;
+    if (best < 0 || scores[b] < scores[best]) best = b;
+  }
+  int sel = 0;
+  if (best >= 0) {
+    const double thr = scores[best] + dstd * errs[best];
+    double smax = -INFINITY;
+    for (int b = 0; b < dh; ++b)
+      if (isfinite(scores[b]) && scores[b] <= thr && hsum[b] > smax) smax = hsum[b];
+    sel = -1;
+    for (int b = 0; b < dh; ++b) {
+      if (!isfinite(scores[b]) || !(scores[b] <= thr) || hsum[b] != smax) continue;
+      if (sel < 0 || scores[b] < scores[sel]) sel = b;
+    }
+    if (sel < 0) sel = best;
+  }
+  out[4 * dh] = (double)sel;
+  out[4 * dh + 1] = (double)idmin;
+  out[4 * dh + 2] = (double)nvalid;
+  if (sel_out) sel_out[0] = sel;
+}
+
+}  // namespace
+
+extern "C" {
+
+int flpr_select(const double* red, int32_t dh, int32_t nval, int32_t ny, int32_t nprobe,
+                const double* ysq, const double* pp, const double* hsum, int64_t n,
+                int32_t cplx, int32_t agg, double dstd, double* out, int32_t* sel,
+                void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!red || !ysq || !pp || !hsum || !out || dh < 1 || ny < 1 || nprobe < 1 ||
+      nprobe > FLPR_MAX_CHANNELS || ny > FLPR_MAX_CHANNELS || nval != ny + nprobe || n < 1 ||
+      agg < 0 || agg > 2 || (cplx && (ny % 2)))
+    return -1;
+  hipLaunchKernelGGL(k_select, dim3(1), dim3(SEL_THREADS), 0, as_stream(stream), red, dh, nval,
+                     ny, nprobe, ysq, pp, hsum, (double)n, cplx, agg, dstd, out, sel);
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+}  // extern "C"

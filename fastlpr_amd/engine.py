@@ -44,12 +44,19 @@ import torch
 from . import _lib
 from ._lib import FlprGeom, call, i64_array
 
-TILE = {1: 256, 2: 16, 3: 8}
+TILE = {1: 16, 2: 8, 3: 8}
 # |u| beyond which the Gaussian taps are dropped: exp(-u^2/2) u^4 relative to the
 # peak is 1.7e-14 at u = 9 (float32 path) and 1e-27 at u = 12 (float64 path).
 BAND_CUT = {False: 9.0, True: 12.0}
 # intermediate (stage >= 1) memory budget per bandwidth chunk, bytes
 STAGE_BUDGET = 6 << 30
+# Bandwidths narrower than this many grid cells (smallest over the axes) are
+# convolved in float64 even in the float32 form: below it the local moment
+# matrices of order 1 / 2 are so ill-conditioned that float32 moments move
+# GCV by more than 1e-5 (the reference's own complex64 pipeline is off by
+# 4e-4 at 0.23 cells and 3.5e-3 at 0.18 cells for order 2).  Order 0 solves a
+# scalar and never needs it.
+FP64_BELOW_CELLS = {0: 0.0, 1: 0.25, 2: 0.4}
 
 
 def stream_handle() -> int:
@@ -74,7 +81,9 @@ class KernelTimer:
     def stop(self):
         self.enabled = False
 
-    def timed(self, name: str, work: float, fn, *args):
+    def timed(self, name: str, work, fn, *args):
+        """work: the launch's algorithmic work, or a callable returning it
+        (evaluated in collect(), outside the timed region)."""
         if not self.enabled:
             return fn(*args)
         a = torch.cuda.Event(enable_timing=True)
@@ -89,7 +98,8 @@ class KernelTimer:
         torch.cuda.synchronize()
         res: Dict[str, list] = {}
         for name, work, a, b in self._pending:
-            res.setdefault(name, []).append((a.elapsed_time(b), work))
+            w = work() if callable(work) else work
+            res.setdefault(name, []).append((a.elapsed_time(b), w))
         self._pending = []
         return res
 
@@ -99,6 +109,15 @@ TIMER = KernelTimer()
 
 def ptr(t: Optional[torch.Tensor]):
     return None if t is None else t.data_ptr()
+
+
+def upload(a: np.ndarray, dtype: torch.dtype, dev) -> torch.Tensor:
+    """Host array -> device tensor through pinned memory, without a stream
+    synchronisation (a pageable copy would drain the stream first)."""
+    t = torch.from_numpy(np.ascontiguousarray(a)).to(dtype)
+    if dev.type != "cuda":
+        return t.to(dev)
+    return t.pin_memory().to(dev, non_blocking=True)
 
 
 # ---------------------------------------------------------------------------
@@ -292,6 +311,7 @@ class Stats:
     bad_x: int = 0
     bad_y: int = 0
     ysq: Optional[np.ndarray] = None
+    ysq_dev: Optional[torch.Tensor] = None   # the same sums, left on the device
 
 
 def x_stats(x: torch.Tensor, y: Optional[torch.Tensor] = None) -> Stats:
@@ -309,7 +329,7 @@ def x_stats(x: torch.Tensor, y: Optional[torch.Tensor] = None) -> Stats:
     std = np.sqrt(w[:, 3] / np.maximum(cnt - 1, 1))
     std = np.where(std == 0, 1.0, std)
     return Stats(w[:, 0].copy(), w[:, 1].copy(), w[:, 2].copy(), std, int(o[5 * d]),
-                 int(o[5 * d + 1]), o[5 * d + 2:].copy())
+                 int(o[5 * d + 1]), o[5 * d + 2:].copy(), out[5 * d + 2:])
 
 
 def padded_sizes(N: Tuple[int, ...], flag_power2: bool = True) -> Tuple[int, ...]:
@@ -430,20 +450,88 @@ def filter_bandwidths(geo: Geometry, h: np.ndarray, n: int):
     return h, ihbad, warn
 
 
-def tap_bands(geo: Geometry, h: np.ndarray, fp64: bool) -> np.ndarray:
-    """[t_lo, t_hi] per (bandwidth, axis): taps with |u| <= cut."""
+def precision_groups(geo: Geometry, h: np.ndarray, order: int, fp64: bool):
+    """Contiguous runs [a, b) of bandwidths with their convolution precision
+    (True = float64).  The float32 form sends bandwidths narrower than
+    FP64_BELOW_CELLS[order] grid cells to float64."""
+    dh = h.shape[0]
+    if fp64:
+        return [(0, dh, True)]
     d = len(geo.N)
-    cut = BAND_CUT[fp64]
-    out = np.zeros((h.shape[0], d, 2), dtype=np.int32)
-    for b in range(h.shape[0]):
-        for k in range(d):
-            u = np.abs(geo.axes_scaled[k] / h[b, k])
-            idx = np.nonzero(u <= cut)[0]
-            if idx.size:
-                out[b, k] = (max(0, idx[0] - 1), min(len(u) - 1, idx[-1] + 1))
-            else:
-                out[b, k] = (0, -1)
+    dx = np.array([abs(geo.axes_scaled[k][1] - geo.axes_scaled[k][0]) if geo.N[k] > 1 else 1.0
+                   for k in range(d)])
+    cells = np.min(h / dx[None, :], axis=1)
+    wide = cells >= FP64_BELOW_CELLS.get(order, 0.0)
+    groups = []
+    a = 0
+    for b in range(1, dh + 1):
+        if b == dh or wide[b] != wide[a]:
+            groups.append((a, b, not bool(wide[a])))
+            a = b
+    return groups
+
+
+def tap_bands(geo: Geometry, h: np.ndarray, fp64) -> np.ndarray:
+    """[t_lo, t_hi] per (bandwidth, axis): taps with |u| <= cut, widened by one
+    on each side and clipped to the axis; (0, -1) when no tap is inside.  fp64
+    is one flag or one flag per bandwidth."""
+    d = len(geo.N)
+    dh = h.shape[0]
+    flags = np.broadcast_to(np.asarray(fp64, dtype=bool), (dh,))
+    cut = np.where(flags, BAND_CUT[True], BAND_CUT[False])
+    out = np.zeros((dh, d, 2), dtype=np.int32)
+    for k in range(d):
+        ax = geo.axes_scaled[k]
+        inside = np.abs(ax[None, :] / h[:, k:k + 1]) <= cut[:, None]
+        anyin = inside.any(axis=1)
+        first = np.argmax(inside, axis=1)
+        last = len(ax) - 1 - np.argmax(inside[:, ::-1], axis=1)
+        out[:, k, 0] = np.where(anyin, np.maximum(0, first - 1), 0)
+        out[:, k, 1] = np.where(anyin, np.minimum(len(ax) - 1, last + 1), -1)
     return out
+
+
+# Host-side tables are pure functions of (statistics, grid, bandwidths, order,
+# precision); repeated fits on the same x (mean and variance passes, several y
+# columns) reuse them instead of recomputing them between device launches.
+_GEOMS: Dict[tuple, Geometry] = {}
+_BTABS: Dict[tuple, tuple] = {}
+_MEMO_MAX = 32
+
+
+def _memo_put(table: dict, key, value):
+    if len(table) >= _MEMO_MAX:
+        table.pop(next(iter(table)))
+    table[key] = value
+    return value
+
+
+def geometry_for(st: Stats, N, accuracy: int, xrange=None, flag_power2: bool = True):
+    """build_geometry, memoised on the exact statistics and grid options.
+    Returns (geometry, key)."""
+    xr = None if xrange is None else tuple(tuple(float(v) for v in r) for r in xrange)
+    key = (st.xmin.tobytes(), st.xmax.tobytes(), st.mean.tobytes(), st.std.tobytes(),
+           tuple(int(v) for v in N), int(accuracy), xr, bool(flag_power2))
+    geo = _GEOMS.get(key)
+    if geo is None:
+        geo = _memo_put(_GEOMS, key, build_geometry(st, N, accuracy, xrange, flag_power2))
+    return geo, key
+
+
+def bandwidth_tables(geo: Geometry, gkey, h: np.ndarray, n: int, order: int, fp64: bool):
+    """Filtered bandwidths, warning, precision groups and tap bands, memoised."""
+    h = np.ascontiguousarray(h, dtype=float)
+    key = (gkey, h.shape, h.tobytes(), int(n), int(order), bool(fp64))
+    hit = _BTABS.get(key)
+    if hit is None:
+        hf, _ihbad, warn = filter_bandwidths(geo, h, n)
+        groups = precision_groups(geo, hf, order, fp64)
+        flags = np.zeros(hf.shape[0], dtype=bool)
+        for (a, b, gf) in groups:
+            flags[a:b] = gf
+        band_np = tap_bands(geo, hf, flags).reshape(-1)
+        hit = _memo_put(_BTABS, key, (hf, warn, groups, band_np))
+    return hit
 
 
 # ---------------------------------------------------------------------------
@@ -460,19 +548,7 @@ class DeviceFit:
     geo: Geometry
     plan: Plan
     warn: Optional[str]
-
-
-class Workspace:
-    """Per-fit device buffers (allocated through torch's caching allocator)."""
-
-    def __init__(self, device):
-        self.device = device
-
-    def empty(self, shape, dtype):
-        return torch.empty(shape, dtype=dtype, device=self.device)
-
-    def zeros(self, shape, dtype):
-        return torch.zeros(shape, dtype=dtype, device=self.device)
+    red: Optional[torch.Tensor] = None   # device (dh, nval) dots then (dh, ny) quad forms
 
 
 def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
@@ -480,7 +556,7 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
              N: Tuple[int, ...], accuracy: int, fp64: bool, regularization: float,
              xrange=None, flag_power2: bool = True, want_rss: bool = False,
              stats: Optional[Stats] = None, with_ones_eps: bool = True,
-             debug: Optional[dict] = None) -> DeviceFit:
+             debug: Optional[dict] = None, readback: bool = True) -> DeviceFit:
     """Spread, convolve and solve for all bandwidths, plus the probe dot products and
     the residual sums.
 
@@ -491,17 +567,16 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     with_ones_eps adds float64 eps to S for order 0 (regression.py:127; the
     calc_dof=False path of the reference omits it).
     debug: when a dict is given, the intermediate device buffers are stored in it.
+    readback: copy the dots / residual sums to the host (rss, dot); when False
+    they stay on the device in DeviceFit.red and nothing waits for the device.
     """
     n, d = x.shape
     dev = x.device
     s = stream_handle()
     st = stats if stats is not None else x_stats(x, vals_y)
-    geo = build_geometry(st, N, accuracy, xrange, flag_power2)
-    h, _ihbad, warn = filter_bandwidths(geo, np.asarray(h, dtype=float), n)
-    dh = h.shape[0]
+    geo, gkey = geometry_for(st, N, accuracy, xrange, flag_power2)
     nval = ny + K
     C = 1 + nval
-    plan = get_plan(d, order, geo.N, geo.L, geo.msp, C, dh, fp64, dev)
     g = geo.geom
     gp = _lib.C.byref(g)
     f64, i32 = torch.float64, torch.int32
@@ -525,7 +600,7 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
          ptr(wab), ptr(vals), ptr(rawi), ptr(rawf), s)
 
     # --- spread (float64 accumulation), centring, float32 copy
-    cells = int(np.prod(plan.G))
+    cells = int(np.prod([g.G[k] for k in range(d)]))
     fcells = int(np.prod([g.tile[k] + 2 * geo.msp for k in range(d)]))
     part = torch.empty((ntiles * C * fcells,), dtype=f64, device=dev)
     grid64 = torch.empty((C * cells,), dtype=f64, device=dev)
@@ -533,55 +608,75 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     call("flpr_spread", gp, ptr(key_start), ptr(wab), ptr(vals), n, 1, nval, ptr(part),
          ptr(grid64), ptr(sums), s)
     del part
+    # bandwidth tables on the host while the device spreads
+    h, warn, groups, band_np = bandwidth_tables(geo, gkey, np.asarray(h, dtype=float), n, order,
+                                                fp64)
+    dh = h.shape[0]
+    any32 = any(not gf for (_a, _b, gf) in groups)
     shift = torch.empty((C,), dtype=f64, device=dev)
-    fdt = f64 if fp64 else torch.float32
-    grid32 = None if fp64 else torch.empty((C * cells,), dtype=fdt, device=dev)
+    grid32 = torch.empty((C * cells,), dtype=torch.float32, device=dev) if any32 else None
     call("flpr_grid_center", ptr(grid64), C, cells, ptr(sums), ptr(shift), ptr(grid32), s)
-    grid = grid64 if fp64 else grid32
 
-    # --- taps and bands
-    h_dev = torch.tensor(h, dtype=f64, device=dev)
-    ntap = dh * d * plan.npow * plan.nmax
-    tap64 = torch.empty((ntap,), dtype=f64, device=dev)
-    tap32 = None if fp64 else torch.empty((ntap,), dtype=torch.float32, device=dev)
-    call("flpr_taps", gp, ptr(h_dev), dh, plan.npow, plan.nmax, float(n), ptr(tap64), ptr(tap32), s)
-    tap = tap64 if fp64 else tap32
-    band_np = tap_bands(geo, h, fp64).reshape(-1)
-    bands = torch.tensor(band_np, dtype=i32, device=dev)
-
-    # --- convolution passes (bandwidth chunks share the intermediate buffers)
-    stage0 = torch.empty((dh * plan.stage_per_h[0],), dtype=fdt, device=dev)
-    inter = {k: torch.empty((plan.chunk * plan.stage_per_h[k],), dtype=fdt, device=dev)
-             for k in range(1, d)}
-    for ci in range(len(plan.chunks)):
-        for ps in plan.pass_tables[ci]:
-            src = grid if ps.in_stage == d else inter[ps.in_stage]
-            dst = stage0 if ps.out_stage == 0 else inter[ps.out_stage]
-            work = conv_flops(ps, band_np) if TIMER.enabled else 0.0
-            TIMER.timed("flpr_conv", work, call, "flpr_conv", ps.mode, int(fp64), ptr(src),
-                        ptr(dst), ptr(tap), ptr(ps.jobs), ptr(bands), int(ps.jobs.shape[0]),
-                        ps.max_ro, ps.K, ps.nout, ps.inner, ps.base, s)
-    del inter
+    # --- taps and bands (uploaded from pinned host memory without a stream sync)
+    plan0 = get_plan(d, order, geo.N, geo.L, geo.msp, C, groups[0][1] - groups[0][0],
+                     groups[0][2], dev)
+    npow, nmax = plan0.npow, plan0.nmax
+    per_tap = d * npow * nmax
+    h_dev = upload(h, f64, dev)
+    tap64 = torch.empty((dh * per_tap,), dtype=f64, device=dev)
+    tap32 = torch.empty((dh * per_tap,), dtype=torch.float32, device=dev) if any32 else None
+    call("flpr_taps", gp, ptr(h_dev), dh, npow, nmax, float(n), ptr(tap64), ptr(tap32), s)
+    bands = upload(band_np, i32, dev)
 
     # --- global regularisation key (regression.py:217, 243-267)
-    key_max = torch.zeros((1,), dtype=torch.int64, device=dev)
-    per_h0 = plan.stage_per_h[0]
     if order == 0:
-        offs, absval = [plan.s_off[0]], 0
         lam_scale = float(regularization)
         eps_add = float(np.finfo(float).eps) if with_ones_eps else 0.0
     else:
-        offs, absval = plan.diag_off, 1
         lam_scale, eps_add = 1e-6, 0.0
-    call("flpr_max_moment", int(fp64), ptr(stage0), per_h0, dh, i64_array(offs), len(offs),
-         plan.nodes, absval, ptr(key_max), s)
+    key_max = torch.zeros((1,), dtype=torch.int64, device=dev)
+
+    # --- convolution passes per precision group (bandwidth chunks share the
+    # intermediate buffers); every group feeds the one global key
+    stage0s = []
+    for (a, b, gf) in groups:
+        plan = get_plan(d, order, geo.N, geo.L, geo.msp, C, b - a, gf, dev)
+        fdt = f64 if gf else torch.float32
+        src_grid = grid64 if gf else grid32
+        tap = tap64 if gf else tap32
+        tap_ptr = tap.data_ptr() + a * per_tap * tap.element_size()
+        band_ptr = bands.data_ptr() + a * d * 2 * 4
+        bands_g = band_np[a * d * 2:b * d * 2]
+        stage0 = torch.empty(((b - a) * plan.stage_per_h[0],), dtype=fdt, device=dev)
+        inter = {k: torch.empty((plan.chunk * plan.stage_per_h[k],), dtype=fdt, device=dev)
+                 for k in range(1, d)}
+        for ci in range(len(plan.chunks)):
+            for ps in plan.pass_tables[ci]:
+                src = src_grid if ps.in_stage == d else inter[ps.in_stage]
+                dst = stage0 if ps.out_stage == 0 else inter[ps.out_stage]
+                TIMER.timed("flpr_conv", (lambda ps=ps, bg=bands_g: conv_flops(ps, bg)), call, "flpr_conv", ps.mode, int(gf), ptr(src),
+                            ptr(dst), tap_ptr, ptr(ps.jobs), band_ptr, int(ps.jobs.shape[0]),
+                            ps.max_ro, ps.K, ps.nout, ps.inner, ps.base, s)
+        del inter
+        if order == 0:
+            offs, absval = [plan.s_off[0]], 0
+        else:
+            offs, absval = plan.diag_off, 1
+        call("flpr_max_moment", int(gf), ptr(stage0), plan.stage_per_h[0], b - a,
+             i64_array(offs), len(offs), plan.nodes, absval, ptr(key_max), s)
+        stage0s.append((a, b, gf, plan, stage0))
 
     # --- adjoint sums, solve (+ dots), residual quadratic form
-    nodes = plan.nodes
+    nodes = plan0.nodes
+    per_h0 = plan0.stage_per_h[0]
     rcells = int(np.prod([n_ - 1 for n_ in geo.N]))
     nw = (1 << d) * ((1 << d) + 1) // 2
     need_q = K > 0 or want_rss
-    q = dot = gram = None
+    q = gram = None
+    # one float64 readback buffer: probe / y dots (dh, nval) then quadratic forms (dh, ny)
+    nq_out = dh * ny if (want_rss and ny > 0) else 0
+    red = torch.zeros((dh * nval + nq_out,), dtype=f64, device=dev) if need_q else None
+    dot = red[:dh * nval] if need_q else None
     if need_q:
         q = torch.zeros((nval * nodes,), dtype=f64, device=dev)
         gram = torch.zeros((nw * rcells,), dtype=f64, device=dev) if want_rss else None
@@ -590,33 +685,65 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
         call("flpr_adjoint", gp, ptr(key_start), ptr(rawi), ptr(rawf), ptr(vals), n, nval,
              ptr(apart), ptr(q), ptr(gram), s)
         del apart
-        dot = torch.zeros((dh * nval,), dtype=f64, device=dev)
-    m_y = torch.empty((dh * max(ny, 1) * nodes,), dtype=f64, device=dev)
-    call("flpr_solve", int(fp64), ptr(stage0), per_h0, dh, nodes, order, plan.nt,
-         i64_array(plan.s_off), i64_array(plan.t_off), ptr(key_max), lam_scale, eps_add,
-         ptr(shift), nval, ny, ptr(m_y) if ny > 0 else None, ptr(q), ptr(dot), s)
+    nyc = max(ny, 1)
+    m_y = torch.empty((dh * nyc * nodes,), dtype=f64, device=dev)
+    for (a, b, gf, plan, stage0) in stage0s:
+        call("flpr_solve", int(gf), ptr(stage0), per_h0, b - a, nodes, order, plan.nt,
+             i64_array(plan.s_off), i64_array(plan.t_off), ptr(key_max), lam_scale, eps_add,
+             ptr(shift), nval, ny, (m_y.data_ptr() + a * nyc * nodes * 8) if ny > 0 else None,
+             ptr(q), (dot.data_ptr() + a * nval * 8) if dot is not None else None, s)
     quad = None
-    if want_rss and ny > 0:
-        quad = torch.zeros((dh * ny,), dtype=f64, device=dev)
+    if nq_out:
+        quad = red[dh * nval:]
         call("flpr_quadform", gp, ptr(m_y), ny, dh, ptr(gram), ptr(quad), s)
-    off0 = plan.s_off[0]
-    s0_all = stage0.view(dh, per_h0)[:, off0:off0 + nodes].to(f64)
+    off0 = plan0.s_off[0]
+    s0_parts = [st0.view(b - a, per_h0)[:, off0:off0 + nodes].to(f64)
+                for (a, b, _gf, _p, st0) in stage0s]
+    s0_all = s0_parts[0] if len(s0_parts) == 1 else torch.cat(s0_parts, dim=0)
     if order == 0:
         s0_all = s0_all + eps_add
     if debug is not None:
-        debug.update(grid64=grid64, stage0=stage0, key=key_max, q=q, dot=dot, m_y=m_y, plan=plan,
-                     geo=geo, h=h, order_idx=order_idx, key_start=key_start,
-                     lam_scale=lam_scale, eps_add=eps_add, shift=shift, gram=gram, quad=quad)
-    dot_h = None if dot is None else dot.cpu().numpy().reshape(dh, nval)
+        debug.update(grid64=grid64, stage0=stage0s[0][4], stage0s=stage0s, key=key_max, q=q,
+                     dot=dot, m_y=m_y, plan=plan0, geo=geo, h=h, order_idx=order_idx,
+                     key_start=key_start, lam_scale=lam_scale, eps_add=eps_add, shift=shift,
+                     gram=gram, quad=quad, groups=groups)
+    if not readback:
+        return DeviceFit(h, m_y.view(dh, nyc, nodes), None, None, s0_all, geo, plan0, warn, red)
+    red_h = None if red is None else red.cpu().numpy()
+    dot_h = None if red_h is None else red_h[:dh * nval].reshape(dh, nval)
     rss_h = None
     if quad is not None:
-        quad_h = quad.cpu().numpy().reshape(dh, ny)
+        quad_h = red_h[dh * nval:].reshape(dh, ny)
         ysq = st.ysq if (st.ysq is not None and len(st.ysq) == ny) else None
         if ysq is None:
             ysq = (vals_y * vals_y).sum(dim=0).cpu().numpy()
         rss_h = ysq[None, :] - 2.0 * dot_h[:, :ny] + quad_h
     probe_dot = None if (dot_h is None or K == 0) else dot_h[:, ny:]
-    return DeviceFit(h, m_y.view(dh, max(ny, 1), nodes), rss_h, probe_dot, s0_all, geo, plan, warn)
+    return DeviceFit(h, m_y.view(dh, nyc, nodes), rss_h, probe_dot, s0_all, geo, plan0, warn, red)
+
+
+AGGREGATIONS = {"first": 0, "average": 1, "sum": 2}
+
+
+def select(fit: DeviceFit, ysq_dev: torch.Tensor, pp_dev: torch.Tensor, n: int, ny: int,
+           K: int, cplx: bool, aggregation: str, dstd: float):
+    """Hutchinson DOF, GCV scores and the 1-SE choice on the device (flpr_select).
+
+    Returns (out, sel): out (4 dh + 4) float64 on the device holding scores,
+    stderrs, dof, dof stderrs, then selected index, argmin index, number of
+    finite scores and one spare slot; sel (1,) int32 on the device."""
+    if aggregation not in AGGREGATIONS:
+        raise ValueError(f"Unknown aggregation method: {aggregation}. "
+                         "Use 'first', 'average', or 'sum'.")
+    dh = fit.h.shape[0]
+    dev = fit.m_y.device
+    hsum = upload(fit.h.sum(axis=1), torch.float64, dev)
+    out = torch.empty((4 * dh + 4,), dtype=torch.float64, device=dev)
+    sel = torch.empty((1,), dtype=torch.int32, device=dev)
+    call("flpr_select", ptr(fit.red), dh, ny + K, ny, K, ptr(ysq_dev), ptr(pp_dev), ptr(hsum),
+         n, int(cplx), AGGREGATIONS[aggregation], float(dstd), ptr(out), ptr(sel),
+         stream_handle())
+    return out, sel
 
 
 def interp_eval(geo: Geometry, x: torch.Tensor, m: torch.Tensor, ncol: int,

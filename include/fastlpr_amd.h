@@ -84,13 +84,16 @@ int flpr_points_prepare(const double* x, int64_t n, const FlprGeom* g, int32_t* 
 int flpr_points_bin(const int32_t* key, int64_t n, int32_t nkeys, const int32_t* key_count,
                     int32_t* key_start, int32_t* key_cursor, int32_t* order, void* stream);
 
-/* Sorted structure-of-arrays point data for the grid kernels:
- *   wab  (2d, n) float64  A = exp(-delta^2/(4 tau)), B = exp(2 delta hx/(4 tau)),
- *                         delta = xmod - hx*round(xmod/hx), so that the spread weight of tap
- *                         k is A B^k exp(-hx^2 k^2/(4 tau)) = exp(-(delta-hx k)^2/(4 tau))
+/* Sorted point data for the grid kernels, point-major (point p's entries are
+ * contiguous):
+ *   wab  (n, d, 2) float64  (A, B) per dimension with delta = xmod - hx*round(xmod/hx),
+ *                         c = 1/(4 tau): A = exp(-(delta^2 + 2 msp hx delta) c),
+ *                         B = exp(2 delta hx c), so that the spread weight of footprint
+ *                         offset j - msp (j = 0..2 msp) is
+ *                         A B^j exp(-hx^2 (j-msp)^2 c) = exp(-(delta - hx (j-msp))^2 c)
  *                         (nufftn_type1.py:226-249, spreading_fused.py:80-94)
- *   vals (ny + nprobe, n) float64  y columns then probe columns
- *   rawi (d, n) int32, rawf (d, n) float64  interval and fraction on the raw grid
+ *   vals (n, ny + nprobe) float64  y columns then probe columns
+ *   rawi (n, d) int32, rawf (n, d) float64  interval and fraction on the raw grid
  *                         axis linspace(graw0, graw1, N) (RegularGridInterpolator). */
 int flpr_points_pack(const FlprGeom* g, const double* x, const int32_t* order, int64_t n,
                      const double* vals_y, int32_t ny, const double* probes, int32_t nprobe,
@@ -98,7 +101,7 @@ int flpr_points_pack(const FlprGeom* g, const double* x, const int32_t* order, i
 
 /* Gaussian-gridding spread onto the unwrapped spread grid
  * (nufftn_type1.py:565-636, spreading_fused.py:67-94, _nufft_accel.pyx:32-76).
- * Channel 0 is the constant 1 when with_ones != 0, then the nval rows of vals.
+ * Channel 0 is the constant 1 when with_ones != 0, then the nval columns of vals.
  * Every tile accumulates its footprint window in float64 and writes it to the
  * workspace part (ntiles * C * prod(tile + 2 msp) doubles); a second pass
  * adds the overlapping windows per cell (no atomics) into grid
@@ -176,7 +179,7 @@ int flpr_solve(int32_t fp64, const void* mom, int64_t per_h, int32_t dh, int64_t
                void* stream);
 
 /* Adjoint interpolation sums on the raw grid, from the sorted point data:
- *   q (nval, nodes)        q[c][node] = sum_i vals[c][i] w_i(node)
+ *   q (nval, nodes)        q[c][node] = sum_i vals[i][c] w_i(node)
  *   gram (nw, rcells)      per raw cell the Gram entries sum_i w_i(a) w_i(b) of
  *                          its 2^d corner weights, a <= b row-major (nw =
  *                          2^d (2^d + 1) / 2; rcells = prod(N_k - 1)); may be NULL
@@ -202,6 +205,25 @@ int flpr_quadform(const FlprGeom* g, const double* m, int32_t ncol, int32_t dh,
 int flpr_interp_eval(const FlprGeom* g, const double* x, int64_t n, const double* m,
                      int32_t ncol, double* out, const double* yvar, double* expsum,
                      void* stream);
+
+/* Bandwidth selection from the per-bandwidth sums, on the device
+ * (regression.py:1276-1354 Hutchinson DOF and penalties, regression.py:378-421
+ * GCV with aggregation agg = 0 first / 1 average / 2 sum, api.py:73-164 1-SE
+ * rule with dstd, preferring the larger sum(h), ties to the smaller score).
+ *   red   (dh, nval) dots [y columns (ny), probes (nprobe)] then (dh, ny) sums
+ *         sum_cell m' W m, as flpr_solve / flpr_quadform leave them
+ *   ysq   (ny) sum_i y_ic^2; pp (nprobe) p'p per probe; hsum (dh) sum_k h_bk
+ *   cplx  y columns are (re, im) pairs: residual sums pair up, penalties use
+ *         2 pdof - 1 (regression.py:1310-1317)
+ * out (4 dh + 3) float64: GCV scores, their standard errors, dof, dof standard
+ * errors (dh each), then the selected index, the index of the smallest score
+ * (np.argmin) and the number of finite scores; *sel (device int32, may be
+ * NULL) also receives the selected index, so work at that bandwidth can be
+ * queued behind this call without a host round trip. */
+int flpr_select(const double* red, int32_t dh, int32_t nval, int32_t ny, int32_t nprobe,
+                const double* ysq, const double* pp, const double* hsum, int64_t n,
+                int32_t cplx, int32_t agg, double dstd, double* out, int32_t* sel,
+                void* stream);
 
 #ifdef __cplusplus
 }

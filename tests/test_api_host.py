@@ -77,6 +77,18 @@ def test_gcv_arithmetic_matches_oracle():
             np.testing.assert_allclose(a, b[:2])
     with pytest.raises(ValueError):
         api._gcv(rss, 300, 12.0, 0.3, None, "median")
+    # all bandwidths at once equals the per-bandwidth oracle
+    dh = 6
+    yhs = [y + s * rs.standard_normal((300, 2)) for s in np.linspace(0.01, 0.5, dh)]
+    rss_all = np.array([np.sum((y - yh_) ** 2, axis=0) for yh_ in yhs])
+    dof = np.linspace(3.0, 250.0, dh)
+    ps = np.linspace(0.0, 0.4, dh)
+    for agg in ("first", "average", "sum"):
+        for pm in (None, np.linspace(1.1, 9.0, dh)):
+            sc, se = api._gcv_all(rss_all, 300, dof, ps, pm, agg)
+            for i in range(dh):
+                b = R.gcv_score(y, yhs[i], dof[i], ps[i], None if pm is None else pm[i], agg)
+                np.testing.assert_allclose([sc[i], se[i]], b[:2], rtol=1e-13)
 
 
 @pytest.mark.parametrize("bad", ["nan_x", "inf_y", "order", "shape", "dims"])
