@@ -258,67 +258,6 @@ class GridInterpolant:
 # cv_fastlpr
 # ---------------------------------------------------------------------------
 
-def _hutchinson(dot: np.ndarray, pp: np.ndarray, n: int, y_is_complex: bool):
-    """tr(H) ~ n (p'phat)/(p'p) per probe and bandwidth, NaN -> n, clip [0, n];
-    penalty 1/max(pdof, .01)^2 mean and std (regression.py:1276-1354)."""
-    K = dot.shape[1]
-    tr = n * (dot.T / pp[:, None])          # (K, dh)
-    tr = np.where(np.isnan(tr), n, tr)
-    tr = np.clip(tr, 0.0, n)
-    dof = np.mean(tr, axis=0)
-    dh = tr.shape[1]
-    if K > 1:
-        stderr = np.std(tr, axis=0, ddof=1) / np.sqrt(K)
-        pd = 1 - tr / n
-        if y_is_complex:
-            pd = 2 * pd - 1
-        pc = np.clip(pd, 0.0, 1.0)
-        pen = 1.0 / np.maximum(pc, 0.01) ** 2
-        pen_std = np.std(pen, axis=0, ddof=1)
-        pen_mean = np.mean(pen, axis=0)
-        if y_is_complex:
-            dof = n * (1 - np.mean(pc, axis=0))
-    else:
-        stderr = np.zeros(dh)
-        pen_std = np.zeros(dh)
-        pen_mean = None
-    return dof, stderr, pen_std, pen_mean
-
-
-def _gcv_all(rss: np.ndarray, n: int, dof: np.ndarray, pen_std: np.ndarray, pen_mean,
-             aggregation: str):
-    """GCV = mse * penalty, stderr = mse * penalty_std (regression.py:378-421) for
-    every bandwidth at once: rss (dh, ncol), dof / pen_std / pen_mean (dh,)."""
-    rss = np.atleast_2d(np.asarray(rss, dtype=float))
-    dof = np.asarray(dof, dtype=float).reshape(-1)
-    pen_std = np.asarray(pen_std, dtype=float).reshape(-1)
-    mse = rss / n
-    if pen_mean is not None:
-        pen = np.asarray(pen_mean, dtype=float).reshape(-1)
-    else:
-        pd = np.clip(1.0 - dof / n, 0.0, 1.0)
-        pen = 1.0 / np.maximum(pd, 0.01) ** 2
-    g = mse * pen[:, None]
-    if aggregation == "first":
-        score, m = g[:, 0], mse[:, 0]
-    elif aggregation == "average":
-        score, m = np.mean(g, axis=1), np.mean(mse, axis=1)
-    elif aggregation == "sum":
-        score, m = np.sum(g, axis=1), np.sum(mse, axis=1)
-    else:
-        raise ValueError(f"Unknown aggregation method: {aggregation}. "
-                         "Use 'first', 'average', or 'sum'.")
-    se = np.where(pen_std > 0, m * pen_std, 0.0)
-    return score, se
-
-
-def _gcv(rss: np.ndarray, n: int, dof: float, pen_std: float, pen_mean, aggregation: str):
-    """One bandwidth of _gcv_all: rss (ncol,) -> (score, stderr)."""
-    sc, se = _gcv_all(np.asarray(rss, dtype=float)[None, :], n, [dof], [pen_std],
-                      None if pen_mean is None else [pen_mean], aggregation)
-    return float(sc[0]), float(se[0])
-
-
 def _y_channels(y_arr, dev):
     """(n, dy) real/complex -> (n, ny) float64 device, complex as interleaved (re, im)."""
     if isinstance(y_arr, torch.Tensor):
@@ -526,3 +465,90 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
     return RegressionOutput(yhat=conv(yhat), grid=tuple(geo.axes_raw), h=fit.h[sel],
                             gcv_yhat=gcv_result, fpp_yhat=interp, metadata=metadata,
                             dof=dof_sel, dof_stderr=dof_err_sel, d=d_norm, s0=conv(s0))
+
+
+# ---------------------------------------------------------------------------
+# cv_fastkde
+# ---------------------------------------------------------------------------
+
+K0_GAUSS = 1.0 / np.sqrt(2.0 * np.pi)
+
+
+def cv_fastkde(x, h: Optional[Sequence[Sequence[float]]] = None,
+               options: Optional[dict] = None) -> KDEOutput:
+    """Kernel density estimation with likelihood cross-validation (api.py:835-1184),
+    computed on the GPU.
+
+    The density grid of every bandwidth is the order-0 moment S_00 (the same
+    spread and convolution as cv_fastlpr, with only the ones channel),
+    normalised to integrate to one on the raw grid and clipped at zero.  LCV
+    uses the leave-one-out identity f_-i(x_i) = n/(n-1) (f(x_i) - K(0)/(n prod h))
+    and the 1-SE rule preferring the larger sum(h) with dstd = 1 (the
+    reference passes no dstd to its selection).
+    """
+    as_numpy = not _is_dev(x)
+    x_arr = _as_column_matrix(x)
+    n, dims = x_arr.shape
+    if n < 2:
+        raise ValueError(f"Input x must have at least 2 observations. Got {n} observations.")
+    if not isinstance(x_arr, torch.Tensor):
+        _check_finite(x_arr, "Input x", "cv_fastkde")
+    opts = set_defaults(options, order=0, calc_dof=False, y_type_out="mean")
+    opts = set_defaults(opts, kernel_type="gaussian", verbose=False, dh=20)
+    if opts.get("accuracy") is None:
+        opts["accuracy"] = max(6 - int(np.ceil(np.log10(n))), 4)
+    if opts.get("N") is None:
+        an = int(np.ceil(n ** (1.0 / dims)))
+        opts["N"] = [an] * dims if dims > 1 else an
+    if dims > 3:
+        raise NotImplementedError("the GPU engine supports 1-3 dimensions")
+    if str(opts.get("kernel_type", "gaussian")).lower() not in ("gauss", "gaussian", "normal"):
+        raise NotImplementedError("the GPU engine implements the Gaussian kernel")
+    if opts.get("output_interp_method", "linear") != "linear":
+        raise NotImplementedError("only linear interpolation is implemented on the GPU")
+    fp64 = str(opts.get("precision", "fp32")).lower() == "fp64"
+    N = _grid_shape(opts, n, dims)
+    dev = x.device if _is_dev(x) else _device()
+    xd = (x_arr.to(dev) if isinstance(x_arr, torch.Tensor)
+          else torch.as_tensor(x_arr, device=dev)).to(torch.float64).contiguous()
+    stats = engine.x_stats(xd, None)
+    if stats.bad_x:
+        _check_finite(x_arr, "Input x", "cv_fastkde")
+    bandwidths = _prepare_bandwidths(
+        h, dims, opts, lambda: np.column_stack((0.1 * stats.std, 1.0 * stats.std)))
+
+    fit = engine.run_grid(xd, None, 0, None, 0, bandwidths, 0, N, int(opts["accuracy"]), fp64,
+                          0.0, None, bool(opts.get("flag_power2", True)), stats=stats,
+                          moments_only=True, kde=True)
+    if fit.warn:
+        warnings.warn(fit.warn, stacklevel=2)
+    geo = fit.geo
+    hl = fit.h
+    dh = hl.shape[0]
+    # normalise so that sum(f) dx = 1 on the raw grid, clip at zero (api.py:1062-1096)
+    dx_grid = float(np.prod([ax[1] - ax[0] for ax in geo.axes_raw]))
+    s = fit.s0_all
+    sdx = s.sum(dim=1, keepdim=True) * dx_grid
+    dens = torch.where(sdx > 0, s / sdx, s).clamp_min(0.0)
+    k0h = K0_GAUSS / np.prod(hl, axis=1)
+    lv = engine.lcv(geo, xd, dens, k0h).cpu().numpy()
+    scores, errs = lv[:dh].copy(), lv[dh:].copy()
+    sel = _one_se_rule(scores, errs, hl, prefer_smaller=False)
+    idmax = int(np.argmax(scores))
+    unique_axes = [np.unique(hl[:, k]) for k in range(dims)]
+    shape_lcv = tuple(len(a) for a in unique_axes)
+    matrix = scores.reshape(shape_lcv) if int(np.prod(shape_lcv)) == len(scores) else None
+    lcv_result = {"hlist": hl, "lcv_m": scores, "lcv_sd": errs, "idmax": idmax, "id1se": sel,
+                  "hmax": hl[idmax], "h1se": hl[sel], "unique_axes": unique_axes,
+                  "matrix": matrix}
+    fsel = dens[sel]
+    fhat_all = torch.movedim(dens.reshape((dh,) + tuple(N)), 0, -1)
+    interp = GridInterpolant(geo, geo.axes_raw, fsel.reshape(1, -1).contiguous(), N, False, 1,
+                             as_numpy, True)
+
+    def conv(t):
+        return t.cpu().numpy() if as_numpy else t
+
+    metadata = {"options": opts, "bandwidths_tested": hl, "fhat_all": conv(fhat_all)}
+    return KDEOutput(fhat=conv(fsel.reshape(N)), grid=tuple(geo.axes_raw), h=hl[sel],
+                     lcv=lcv_result, fpp=interp, metadata=metadata)

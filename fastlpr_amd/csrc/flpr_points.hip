@@ -964,6 +964,101 @@ __global__ __launch_bounds__(EVAL_THREADS) void k_interp_eval(FlprGeom g,
   }
 }
 
+// Likelihood cross-validation of the density grids (api.py:1111-1132): for
+// every bandwidth b and data point i
+//   f_i = linear interpolation of dens[b] at x_i
+//   l_i = log(max((n / (n - 1)) (f_i - k0h[b] / n), 1e-10))
+// and per block the count, sum and centred sum of squares of l_i (Welford in a
+// thread, Chan's merge across threads), written to part[(block * dh + b) * 3].
+// A block owns a contiguous run of points and loops over the bandwidths, so
+// the run's coordinates stay in cache across them.
+constexpr int LCV_THREADS = 256;
+constexpr int LCV_BLOCKS = 1024;
+
+struct Moments3 {
+  double n, s, m2;  // count, sum, sum of squared deviations from the mean
+};
+
+__device__ inline Moments3 m3_merge(Moments3 a, Moments3 b) {
+  if (a.n == 0) return b;
+  if (b.n == 0) return a;
+  Moments3 r;
+  r.n = a.n + b.n;
+  const double d = b.s / b.n - a.s / a.n;
+  r.s = a.s + b.s;
+  r.m2 = a.m2 + b.m2 + d * d * (a.n * b.n / r.n);
+  return r;
+}
+
+template <int D>
+__global__ __launch_bounds__(LCV_THREADS) void k_lcv(FlprGeom g, const double* __restrict__ x,
+                                                     int64_t n, const double* __restrict__ dens,
+                                                     int dh, int64_t nodes,
+                                                     const double* __restrict__ k0h,
+                                                     double* __restrict__ part) {
+  __shared__ Moments3 sh[LCV_THREADS / 64];
+  const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = (int64_t)blockIdx.x * per;
+  const int64_t i1 = min(n, i0 + per);
+  const double nd = (double)n;
+  const double ratio = nd / (nd - 1.0);
+  for (int b = 0; b < dh; ++b) {
+    const double* f = dens + (int64_t)b * nodes;
+    const double c = k0h[b] / nd;
+    Moments3 acc = {0.0, 0.0, 0.0};
+    double mean = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+      const CornersT<D> cr = corners<D>(g, x + i * D);
+      double fa = 0.0;
+#pragma unroll
+      for (int q = 0; q < (1 << D); ++q) fa += cr.w[q] * f[cr.node[q]];
+      double lo = ratio * (fa - c);
+      lo = (lo != lo) ? lo : (lo > 1e-10 ? lo : 1e-10);
+      const double l = log(lo);
+      acc.n += 1.0;
+      acc.s += l;
+      const double dl = l - mean;
+      mean += dl / acc.n;
+      acc.m2 += dl * (l - mean);
+    }
+    // wave then block merge
+    for (int off = 32; off > 0; off >>= 1) {
+      Moments3 o;
+      o.n = __shfl_xor(acc.n, off);
+      o.s = __shfl_xor(acc.s, off);
+      o.m2 = __shfl_xor(acc.m2, off);
+      acc = m3_merge(acc, o);
+    }
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      Moments3 r = sh[0];
+      for (int w = 1; w < LCV_THREADS / 64; ++w) r = m3_merge(r, sh[w]);
+      double* o = part + ((int64_t)blockIdx.x * dh + b) * 3;
+      o[0] = r.n;
+      o[1] = r.s;
+      o[2] = r.m2;
+    }
+    __syncthreads();
+  }
+}
+
+// out[b] = sum_i l_i (the LCV score), out[dh + b] = std(l) sqrt(n) (its
+// standard error, np.std with ddof = 0), merged over the blocks in order.
+__global__ __launch_bounds__(256) void k_lcv_final(const double* __restrict__ part, int nblocks,
+                                                   int dh, double* __restrict__ out) {
+  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < dh; b += gridDim.x * blockDim.x) {
+    Moments3 r = {0.0, 0.0, 0.0};
+    for (int k = 0; k < nblocks; ++k) {
+      const double* o = part + ((int64_t)k * dh + b) * 3;
+      Moments3 m = {o[0], o[1], o[2]};
+      r = m3_merge(r, m);
+    }
+    out[b] = r.s;
+    out[dh + b] = sqrt(r.m2 / r.n) * sqrt(r.n);
+  }
+}
+
 int lds_group(int64_t cells_per_channel, int channels, int cap, int64_t budget = 64 * 1024) {
   const int64_t per = cells_per_channel * (int64_t)sizeof(double);
   int cg = (int)(budget / per);
@@ -1234,6 +1329,25 @@ int flpr_interp_eval(const FlprGeom* g, const double* x, int64_t n, const double
     case 2: hipLaunchKernelGGL(k_interp_eval<2>, grd, dim3(EVAL_THREADS), 0, st, *g, x, n, m, ncol, nodes, out, yvar, expsum); break;
     default: hipLaunchKernelGGL(k_interp_eval<3>, grd, dim3(EVAL_THREADS), 0, st, *g, x, n, m, ncol, nodes, out, yvar, expsum); break;
   }
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+int flpr_lcv(const FlprGeom* g, const double* x, int64_t n, const double* dens, int32_t dh,
+             const double* k0h, double* part, double* out, void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!flpr_geom_ok(g) || !x || !dens || !k0h || !part || !out || n < 2 || dh < 1) return -1;
+  const int64_t nodes = flpr_nodes(g);
+  hipStream_t st = as_stream(stream);
+  dim3 grd(LCV_BLOCKS);
+  switch (g->d) {
+    case 1: hipLaunchKernelGGL(k_lcv<1>, grd, dim3(LCV_THREADS), 0, st, *g, x, n, dens, dh, nodes, k0h, part); break;
+    case 2: hipLaunchKernelGGL(k_lcv<2>, grd, dim3(LCV_THREADS), 0, st, *g, x, n, dens, dh, nodes, k0h, part); break;
+    default: hipLaunchKernelGGL(k_lcv<3>, grd, dim3(LCV_THREADS), 0, st, *g, x, n, dens, dh, nodes, k0h, part); break;
+  }
+  FLPR_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_lcv_final, dim3((dh + 255) / 256), dim3(256), 0, st, part, LCV_BLOCKS, dh,
+                     out);
   FLPR_CHECK_LAUNCH();
   return 0;
 }

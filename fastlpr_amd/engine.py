@@ -355,7 +355,7 @@ class Geometry:
 
 
 def build_geometry(st: Stats, N: Tuple[int, ...], accuracy: int, xrange=None,
-                   flag_power2: bool = True) -> Geometry:
+                   flag_power2: bool = True, kde: bool = False) -> Geometry:
     """Knot-mapping constants exactly as the reference computes them in float64
     (model.py:58-72, convolution.py:169-199, nufftn_type1.py:140-176,
     kernel.py:97-156); the extreme points give the min / max of every
@@ -415,15 +415,22 @@ def build_geometry(st: Stats, N: Tuple[int, ...], accuracy: int, xrange=None,
         g.ks[k] = ks
         g.idx0[k] = idx0
         g.shift[k] = shift
-        if xrange is None:
-            g0, g1 = float(st.xmin[k]), float(st.xmax[k])
+        ax_s = np.linspace(lo, hi, n_k)
+        if kde:
+            # cv_fastkde maps the scaled axis back to raw units (api.py:1036-1057)
+            ax_r = (ax_s + c) * std + mean
+            g0, g1 = float(ax_r[0]), float(ax_r[-1])
         else:
-            g0, g1 = float(xrange[k][0]), float(xrange[k][1])
+            if xrange is None:
+                g0, g1 = float(st.xmin[k]), float(st.xmax[k])
+            else:
+                g0, g1 = float(xrange[k][0]), float(xrange[k][1])
+            ax_r = np.linspace(g0, g1, n_k)
         g.graw0[k] = g0
         g.graw1[k] = g1
         centre[k], xmin_s[k], xmax_s[k] = c, lo, hi
-        axes_raw.append(np.linspace(g0, g1, n_k))
-        axes_scaled.append(np.linspace(lo, hi, n_k))
+        axes_raw.append(ax_r)
+        axes_scaled.append(ax_s)
     return Geometry(g, tuple(N), L, msp, xmin_s, xmax_s, axes_raw, axes_scaled,
                     st.mean.copy(), st.std.copy(), centre)
 
@@ -506,15 +513,16 @@ def _memo_put(table: dict, key, value):
     return value
 
 
-def geometry_for(st: Stats, N, accuracy: int, xrange=None, flag_power2: bool = True):
+def geometry_for(st: Stats, N, accuracy: int, xrange=None, flag_power2: bool = True,
+                 kde: bool = False):
     """build_geometry, memoised on the exact statistics and grid options.
     Returns (geometry, key)."""
     xr = None if xrange is None else tuple(tuple(float(v) for v in r) for r in xrange)
     key = (st.xmin.tobytes(), st.xmax.tobytes(), st.mean.tobytes(), st.std.tobytes(),
-           tuple(int(v) for v in N), int(accuracy), xr, bool(flag_power2))
+           tuple(int(v) for v in N), int(accuracy), xr, bool(flag_power2), bool(kde))
     geo = _GEOMS.get(key)
     if geo is None:
-        geo = _memo_put(_GEOMS, key, build_geometry(st, N, accuracy, xrange, flag_power2))
+        geo = _memo_put(_GEOMS, key, build_geometry(st, N, accuracy, xrange, flag_power2, kde))
     return geo, key
 
 
@@ -556,7 +564,8 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
              N: Tuple[int, ...], accuracy: int, fp64: bool, regularization: float,
              xrange=None, flag_power2: bool = True, want_rss: bool = False,
              stats: Optional[Stats] = None, with_ones_eps: bool = True,
-             debug: Optional[dict] = None, readback: bool = True) -> DeviceFit:
+             debug: Optional[dict] = None, readback: bool = True,
+             moments_only: bool = False, kde: bool = False) -> DeviceFit:
     """Spread, convolve and solve for all bandwidths, plus the probe dot products and
     the residual sums.
 
@@ -569,12 +578,14 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     debug: when a dict is given, the intermediate device buffers are stored in it.
     readback: copy the dots / residual sums to the host (rss, dot); when False
     they stay on the device in DeviceFit.red and nothing waits for the device.
+    moments_only: stop after the convolution (S_00 per bandwidth in s0_all, the
+    density path of cv_fastkde); kde selects cv_fastkde's raw grid axes.
     """
     n, d = x.shape
     dev = x.device
     s = stream_handle()
     st = stats if stats is not None else x_stats(x, vals_y)
-    geo, gkey = geometry_for(st, N, accuracy, xrange, flag_power2)
+    geo, gkey = geometry_for(st, N, accuracy, xrange, flag_power2, kde)
     nval = ny + K
     C = 1 + nval
     g = geo.geom
@@ -654,21 +665,32 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
             for ps in plan.pass_tables[ci]:
                 src = src_grid if ps.in_stage == d else inter[ps.in_stage]
                 dst = stage0 if ps.out_stage == 0 else inter[ps.out_stage]
-                TIMER.timed("flpr_conv", (lambda ps=ps, bg=bands_g: conv_flops(ps, bg)), call, "flpr_conv", ps.mode, int(gf), ptr(src),
-                            ptr(dst), tap_ptr, ptr(ps.jobs), band_ptr, int(ps.jobs.shape[0]),
-                            ps.max_ro, ps.K, ps.nout, ps.inner, ps.base, s)
+                TIMER.timed("flpr_conv", (lambda ps=ps, bg=bands_g: conv_flops(ps, bg)),
+                            call, "flpr_conv", ps.mode, int(gf), ptr(src), ptr(dst), tap_ptr,
+                            ptr(ps.jobs), band_ptr, int(ps.jobs.shape[0]), ps.max_ro, ps.K,
+                            ps.nout, ps.inner, ps.base, s)
         del inter
-        if order == 0:
-            offs, absval = [plan.s_off[0]], 0
-        else:
-            offs, absval = plan.diag_off, 1
-        call("flpr_max_moment", int(gf), ptr(stage0), plan.stage_per_h[0], b - a,
-             i64_array(offs), len(offs), plan.nodes, absval, ptr(key_max), s)
+        if not moments_only:
+            if order == 0:
+                offs, absval = [plan.s_off[0]], 0
+            else:
+                offs, absval = plan.diag_off, 1
+            call("flpr_max_moment", int(gf), ptr(stage0), plan.stage_per_h[0], b - a,
+                 i64_array(offs), len(offs), plan.nodes, absval, ptr(key_max), s)
         stage0s.append((a, b, gf, plan, stage0))
 
-    # --- adjoint sums, solve (+ dots), residual quadratic form
     nodes = plan0.nodes
     per_h0 = plan0.stage_per_h[0]
+    off0 = plan0.s_off[0]
+    if moments_only:
+        s0_parts = [st0.view(b - a, per_h0)[:, off0:off0 + nodes].to(f64)
+                    for (a, b, _gf, _p, st0) in stage0s]
+        s0_all = s0_parts[0] if len(s0_parts) == 1 else torch.cat(s0_parts, dim=0)
+        if order == 0:
+            s0_all = s0_all + eps_add
+        return DeviceFit(h, None, None, None, s0_all, geo, plan0, warn, None)
+
+    # --- adjoint sums, solve (+ dots), residual quadratic form
     rcells = int(np.prod([n_ - 1 for n_ in geo.N]))
     nw = (1 << d) * ((1 << d) + 1) // 2
     need_q = K > 0 or want_rss
@@ -696,7 +718,6 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     if nq_out:
         quad = red[dh * nval:]
         call("flpr_quadform", gp, ptr(m_y), ny, dh, ptr(gram), ptr(quad), s)
-    off0 = plan0.s_off[0]
     s0_parts = [st0.view(b - a, per_h0)[:, off0:off0 + nodes].to(f64)
                 for (a, b, _gf, _p, st0) in stage0s]
     s0_all = s0_parts[0] if len(s0_parts) == 1 else torch.cat(s0_parts, dim=0)
@@ -720,6 +741,20 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
         rss_h = ysq[None, :] - 2.0 * dot_h[:, :ny] + quad_h
     probe_dot = None if (dot_h is None or K == 0) else dot_h[:, ny:]
     return DeviceFit(h, m_y.view(dh, nyc, nodes), rss_h, probe_dot, s0_all, geo, plan0, warn, red)
+
+
+def lcv(geo: Geometry, x: torch.Tensor, dens: torch.Tensor, k0h: np.ndarray):
+    """LCV score and standard error per bandwidth (flpr_lcv): dens (dh, nodes)
+    float64 on the device.  Returns a (2 dh,) device tensor [scores, stderrs]."""
+    n = x.shape[0]
+    dh = dens.shape[0]
+    dev = x.device
+    k0 = upload(np.asarray(k0h, dtype=float), torch.float64, dev)
+    part = torch.empty((1024 * dh * 3,), dtype=torch.float64, device=dev)
+    out = torch.empty((2 * dh,), dtype=torch.float64, device=dev)
+    call("flpr_lcv", _lib.C.byref(geo.geom), ptr(x), n, ptr(dens.contiguous()), dh, ptr(k0),
+         ptr(part), ptr(out), stream_handle())
+    return out
 
 
 AGGREGATIONS = {"first": 0, "average": 1, "sum": 2}

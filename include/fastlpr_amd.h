@@ -206,6 +206,16 @@ int flpr_interp_eval(const FlprGeom* g, const double* x, int64_t n, const double
                      int32_t ncol, double* out, const double* yvar, double* expsum,
                      void* stream);
 
+/* Likelihood cross-validation of density grids (api.py:1111-1132): dens
+ * (dh, nodes) float64 densities on the raw grid, x (n, d) float64 data points,
+ * k0h (dh) K(0) / prod_k h_bk.  For every bandwidth b,
+ *   l_i = log(max((n/(n-1)) (f_b(x_i) - k0h[b]/n), 1e-10))
+ * with f_b the linear interpolant (RegularGridInterpolator 'linear');
+ * out[b] = sum_i l_i and out[dh + b] = std(l) sqrt(n) (ddof = 0).
+ * part is workspace of 1024 * dh * 3 doubles. */
+int flpr_lcv(const FlprGeom* g, const double* x, int64_t n, const double* dens, int32_t dh,
+             const double* k0h, double* part, double* out, void* stream);
+
 /* Bandwidth selection from the per-bandwidth sums, on the device
  * (regression.py:1276-1354 Hutchinson DOF and penalties, regression.py:378-421
  * GCV with aggregation agg = 0 first / 1 average / 2 sum, api.py:73-164 1-SE
