@@ -14,10 +14,11 @@ Contents
              ``mode="ref"`` mirrors the reference's accuracy<=4 arithmetic
              (complex64 spread grid and spectrum products, float32 S/T/solve);
              ``mode="f64"`` runs the same discrete estimator in float64.
-``direct``   Brute-force float64 evaluations used to pin ``restate`` on tiny
-             inputs: the spread as an explicit double loop over points and
-             taps, the convolution as an explicit sum over grid cells, and the
-             textbook local-polynomial estimator at arbitrary nodes.
+``direct``   Brute-force float64 definitions that pin ``restate`` where the
+             reference's fixtures cannot (2-D / 3-D order 2): the spread as an
+             explicit loop over points and footprint taps, the moments as
+             explicit sums over grid cells with the full d-dimensional kernel,
+             and Gauss-Jordan solves per node.
 
 Parity status of each function is stated in its docstring ("pinned by ...").
 """
