@@ -101,32 +101,57 @@ __global__ __launch_bounds__(TAP_THREADS) void k_taps(FlprGeom g, const double* 
 // banded Toeplitz GEMM on MFMA
 // ---------------------------------------------------------------------------
 
-constexpr int CT = 256;  // 4 waves
-constexpr int TM = 64;   // block tile rows
-constexpr int TN = 64;   // block tile cols
-constexpr int KC = 32;   // K chunk staged in LDS
-constexpr int WM = 32;   // wave tile rows (2 x 16)
-constexpr int WN = 32;   // wave tile cols (2 x 16)
-constexpr int RB = WM / 16;
-constexpr int CB = WN / 16;
+constexpr int CT = 256;        // 4 waves
+constexpr int LT = 64;         // Toeplitz extent of a block tile (four 16-wide blocks)
+constexpr int NDB = 1;         // dense 16-blocks of one wave
+constexpr int WD = NDB * 16;   // dense extent of one wave
+constexpr int LD = 4 * WD;     // dense extent of a block tile
+constexpr int NQ = LT / 16;    // Toeplitz 16-blocks, shared by the four waves
 
-// Banded Toeplitz GEMM.  One block computes a TM x TN output tile; its four
-// waves own 32 x 32 sub-tiles (2 x 2 MFMA 16x16 blocks).  The dense operand of
-// chunk k+1 is fetched into registers while chunk k is multiplied from LDS
-// (double-buffered LDS, one barrier per chunk).  The Toeplitz operand is
-// generated from a 1-D strip of taps: element (i, s) is tap[i + base - s]; a
-// 16-wide block of the band whose taps miss a K step skips its MFMAs.
-// Float32 chunks are folded into float64 totals.
+// K chunk staged in LDS per precision (double buffering of the 256-wide dense
+// tile: 2 x 35 KB for float, 2 x 35 KB for double with the shorter chunk).
+template <typename T>
+struct ConvCfg;
+template <>
+struct ConvCfg<float> {
+  static constexpr int KC = 32;
+};
+template <>
+struct ConvCfg<double> {
+  static constexpr int KC = 16;
+};
+
+// Banded Toeplitz GEMM.  A block owns a 64-wide stretch of the convolved
+// (Toeplitz) axis and a 256-wide stretch of the other (dense) axis; each of
+// its four waves owns 64 of the dense extent and all 64 Toeplitz outputs, as
+// 4 x 4 MFMA 16x16 blocks, so one K step costs 8 LDS reads for 16 MFMAs.  The
+// dense operand of chunk k+1 is fetched into registers while chunk k is
+// multiplied from LDS (double-buffered, one barrier per chunk).  The Toeplitz
+// operand is generated from a 1-D strip of taps: element (i, s) is
+// tap[i + base - s].  Float32 chunk sums are folded into float64 totals.
+//
+// Dense rows (ROW pass) or columns (COL pass) past the job only feed outputs
+// that are never stored, and inputs past the chunk's band end are replaced by
+// zero, so the loads carry no bounds branches; the input buffer must stay
+// readable LD*K + KC*inner + LD elements past each job's input (the caller
+// pads its allocations).
 template <typename T, int MODE>
 __global__ __launch_bounds__(CT) void k_conv(const T* __restrict__ in, T* __restrict__ out,
                                              const T* __restrict__ tap,
                                              const int64_t* __restrict__ jobs,
                                              const int32_t* __restrict__ bands, int K, int nout,
-                                             int inner, int base, int ntn, int ntm) {
-  constexpr int DENSE = (MODE == 0) ? TM * (KC + 1) : KC * (TN + 1);
-  constexpr int NSTRIP = (MODE == 0) ? TN + KC - 1 : TM + KC - 1;
-  constexpr int PER = ((MODE == 0) ? TM * KC : KC * TN) / CT;
-  constexpr int LTOE = (MODE == 0) ? TN : TM;  // extent of the tile along the convolved axis
+                                             int inner, int base, int ntd, int ntt) {
+  constexpr int KC = ConvCfg<T>::KC;
+  // LDS pitch of the dense tile: a K step reads 16 rows x 4 K (ROW pass) or
+  // 4 K rows x 16 columns (COL pass); the pads spread each half-wave over 32 banks
+  constexpr int PD = (MODE == 0) ? KC + 2 : LD + 16;
+  static_assert(CT % LD == 0 && CT % KC == 0, "staging layout");
+  constexpr int DENSE = (MODE == 0) ? LD * PD : KC * PD;
+  constexpr int NSTRIP = LT + KC - 1;
+  constexpr int PER = LD * KC / CT;   // dense elements staged per thread and chunk
+  constexpr int RSTEP = CT / KC;      // ROW pass: row stride between a thread's elements
+  constexpr int KSTEP = CT / LD;      // COL pass: K-row stride between a thread's elements
+  constexpr bool F32 = sizeof(T) == 4;
   __shared__ T dense[2][DENSE];
   __shared__ T strip[2][NSTRIP + 1];
   const int64_t* jb = jobs + 8 * (int64_t)blockIdx.y;
@@ -134,147 +159,189 @@ __global__ __launch_bounds__(CT) void k_conv(const T* __restrict__ in, T* __rest
   const int band = (int)jb[3], ro = (int)jb[4];
   const int t_lo = bands[2 * band], t_hi = bands[2 * band + 1];
   int bx = blockIdx.x;
-  const int tn = bx % ntn;
-  bx /= ntn;
-  int tm, o, M, N;
+  const int td = bx % ntd;
+  bx /= ntd;
+  const int tt = bx % ntt;
+  const int o = bx / ntt;
+  const int d0 = td * LD, i0 = tt * LT;
+  int ndense, ntoe;
   if (MODE == 0) {
-    tm = bx;
-    o = 0;
-    M = ro;
-    N = nout;
-    if (tm * TM >= M) return;
+    ndense = ro;
+    ntoe = nout;
   } else {
-    tm = bx % ntm;
-    o = bx / ntm;
-    M = nout;
-    N = inner;
+    ndense = inner;
+    ntoe = nout;
     if (o >= ro) return;
   }
-  const int m0 = tm * TM, n0 = tn * TN;
-  if (n0 >= N) return;
-  const int i0 = (MODE == 0) ? n0 : m0;
+  if (d0 >= ndense || i0 >= ntoe) return;
   const int s_begin = max(i0 + base - t_hi, 0);
-  const int s_end = min(i0 + LTOE - 1 + base - t_lo, K - 1);
+  const int s_end = min(i0 + LT - 1 + base - t_lo, K - 1);
   const int nchunks = (s_end >= s_begin) ? (s_end - s_begin) / KC + 1 : 0;
   const T* src = in + in_off + (MODE == 0 ? 0 : (int64_t)o * K * inner);
   T* dst = out + out_off + (MODE == 0 ? 0 : (int64_t)o * nout * inner);
   const T* tp = tap + tap_off;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int wr = (wave >> 1) * WM, wc = (wave & 1) * WN;
+  const int dw = wave * WD;  // the wave's offset along the dense extent
+  // staging: ROW pass thread owns K column kc0 of rows r0 + i*RSTEP;
+  // COL pass thread owns dense column cc0 of K rows k0 + i*KSTEP
+  const int kc0 = tid % KC, r0 = tid / KC;
+  const int cc0 = tid % LD, k0 = tid / LD;
+  const T* sbase = (MODE == 0) ? src + (int64_t)(d0 + r0) * K + kc0
+                               : src + (int64_t)k0 * inner + d0 + cc0;
+  const int64_t sstep = (MODE == 0) ? (int64_t)RSTEP * K : (int64_t)KSTEP * inner;
 
-  typename Mfma<T>::acc acc[RB][CB];
-  double tot[RB][CB][4];
+  typename Mfma<T>::acc acc[NQ][NDB];
+  double tot[F32 ? NQ : 1][F32 ? NDB : 1][4];
 #pragma unroll
-  for (int rb = 0; rb < RB; ++rb)
+  for (int q = 0; q < NQ; ++q)
 #pragma unroll
-    for (int cb = 0; cb < CB; ++cb)
+    for (int db = 0; db < NDB; ++db)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        acc[rb][cb][j] = T(0);
-        tot[rb][cb][j] = 0.0;
-      }
+      for (int j = 0; j < 4; ++j) acc[q][db][j] = T(0);
+  if constexpr (F32) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int db = 0; db < NDB; ++db)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) tot[q][db][j] = 0.0;
+  }
 
-  T pre[PER];
-  T pst = T(0);
-  auto gload = [&](int s0) {
+  // Two register sets of prefetched raw values keep two chunks in flight;
+  // the zeroing past the band end is applied when a set is stored to LDS, so
+  // the loads are not waited for before the MFMAs.
+  T preA[PER], preB[PER];
+  T pstA = T(0), pstB = T(0);
+  auto gload = [&](T (&pre)[PER], T& pst, int s0) {
+    const T* p = sbase + (MODE == 0 ? (int64_t)s0 : (int64_t)s0 * inner);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) pre[i] = p[i * sstep];
+    if (tid < NSTRIP) {
+      const int t = i0 + base - s0 - (KC - 1) + tid;
+      pst = tp[max(min(t, t_hi), 0)];  // any in-strip address; masked when stored
+    }
+  };
+  auto sstore = [&](const T (&pre)[PER], T pst, int s0, int buf) {
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-      const int e = tid + i * CT;
-      if (MODE == 0) {
-        const int r = e / KC, kk = e % KC;
-        const int gr = m0 + r, gs = s0 + kk;
-        pre[i] = (gr < M && gs <= s_end) ? src[(int64_t)gr * K + gs] : T(0);
-      } else {
-        const int kk = e / TN, c = e % TN;
-        const int gs = s0 + kk, gc = n0 + c;
-        pre[i] = (gs <= s_end && gc < N) ? src[(int64_t)gs * inner + gc] : T(0);
-      }
+      if (MODE == 0) dense[buf][(r0 + i * RSTEP) * PD + kc0] = (s0 + kc0 <= s_end) ? pre[i] : T(0);
+      else dense[buf][(k0 + i * KSTEP) * PD + cc0] = (s0 + k0 + i * KSTEP <= s_end) ? pre[i] : T(0);
     }
     if (tid < NSTRIP) {
       const int t = i0 + base - s0 - (KC - 1) + tid;
-      pst = (t >= t_lo && t <= t_hi) ? tp[t] : T(0);
+      strip[buf][tid] = (t >= t_lo && t <= t_hi) ? pst : T(0);
     }
   };
-  auto sstore = [&](int buf) {
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int e = tid + i * CT;
-      if (MODE == 0) dense[buf][(e / KC) * (KC + 1) + (e % KC)] = pre[i];
-      else dense[buf][(e / TN) * (TN + 1) + (e % TN)] = pre[i];
-    }
-    if (tid < NSTRIP) strip[buf][tid] = pst;
-  };
-
-  if (nchunks > 0) {
-    gload(s_begin);
-    sstore(0);
-  }
-  __syncthreads();
-  for (int ch = 0; ch < nchunks; ++ch) {
-    const int buf = ch & 1;
-    const bool more = ch + 1 < nchunks;
-    if (more) gload(s_begin + (ch + 1) * KC);
+  // Multiply chunk ch (staged in LDS buffer buf) and fold it.  All operands
+  // of the chunk are read into registers first.  The Toeplitz operand of K
+  // step st and block q is strip[16 q + (lane & 15) - 4 st - (lane >> 4) + KC - 1],
+  // which depends on 4 q - st only: NTV distinct values per lane per chunk.
+  constexpr int NST = KC / 4;
+  constexpr int NTV = 4 * (NQ - 1) + NST;
+  auto compute = [&](int ch, int buf) {
     const T* dn = dense[buf];
     const T* sp = strip[buf];
-    const int sc0 = s_begin + ch * KC;  // absolute input index of the chunk's first column
+    const int sc0 = s_begin + ch * KC;  // absolute input index of the chunk's first K
+    // Toeplitz blocks that meet the band anywhere in this chunk: a block of
+    // outputs starting at i needs inputs in [i + base - t_hi, i + 15 + base - t_lo].
+    // A scalar per chunk; MFMAs on the band's zero taps add zeros.
+    int live = 0;
 #pragma unroll
-    for (int kk = 0; kk < KC; kk += 4) {
-      const int kl = kk + (lane >> 4);
-      // absolute s of this K step: [sc0 + kk, sc0 + kk + 3]; a 16-wide block of
-      // Toeplitz outputs starting at i needs s in [i + base - t_hi, i + 15 + base - t_lo]
-      bool live[2];
-#pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const int ib = (MODE == 0) ? (n0 + wc + q * 16) : (m0 + wr + q * 16);
-        live[q] = !(sc0 + kk + 3 < ib + base - t_hi || sc0 + kk > ib + 15 + base - t_lo);
-      }
-      T a[RB], b[CB];
-#pragma unroll
-      for (int rb = 0; rb < RB; ++rb) {
-        const int r = wr + rb * 16 + (lane & 15);
-        a[rb] = (MODE == 0) ? dn[r * (KC + 1) + kl] : sp[r - kl + KC - 1];
-      }
-#pragma unroll
-      for (int cb = 0; cb < CB; ++cb) {
-        const int c = wc + cb * 16 + (lane & 15);
-        b[cb] = (MODE == 0) ? sp[c - kl + KC - 1] : dn[kl * (TN + 1) + c];
-      }
-#pragma unroll
-      for (int rb = 0; rb < RB; ++rb)
-#pragma unroll
-        for (int cb = 0; cb < CB; ++cb) {
-          const bool on = (MODE == 0) ? live[cb] : live[rb];
-          if (on) acc[rb][cb] = Mfma<T>::run(a[rb], b[cb], acc[rb][cb]);
-        }
+    for (int q = 0; q < NQ; ++q) {
+      const int ib = i0 + q * 16;
+      if (!(sc0 + KC - 1 < ib + base - t_hi || sc0 > ib + 15 + base - t_lo)) live |= 1 << q;
     }
-    if constexpr (sizeof(T) == 4) {
+    live = __builtin_amdgcn_readfirstlane(live);
+    if (live != 0) {
+      const int sb = (lane & 15) - (lane >> 4) + KC - 1 - 4 * (NST - 1);
+      T tv[NTV], dv[NST][NDB];
+#pragma unroll
+      for (int m = 0; m < NTV; ++m) tv[m] = sp[sb + 4 * m];  // m = 4 q - st + NST - 1
+#pragma unroll
+      for (int st = 0; st < NST; ++st) {
+        const int kl = st * 4 + (lane >> 4);
+#pragma unroll
+        for (int db = 0; db < NDB; ++db) {
+          const int dd = dw + db * 16 + (lane & 15);
+          dv[st][db] = (MODE == 0) ? dn[dd * PD + kl] : dn[kl * PD + dd];
+        }
+      }
+      auto mac = [&](int qmask) {
+#pragma unroll
+        for (int st = 0; st < NST; ++st)
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            if (!(qmask & (1 << q))) continue;
+            const T tq = tv[4 * q - st + NST - 1];
+#pragma unroll
+            for (int db = 0; db < NDB; ++db)
+              acc[q][db] = (MODE == 0) ? Mfma<T>::run(dv[st][db], tq, acc[q][db])
+                                       : Mfma<T>::run(tq, dv[st][db], acc[q][db]);
+          }
+      };
+      if (live == (1 << NQ) - 1) mac((1 << NQ) - 1);
+      else mac(live);
+    }
+    if constexpr (F32) {
       // the float32 MFMA accumulator only ever holds KC terms
 #pragma unroll
-      for (int rb = 0; rb < RB; ++rb)
+      for (int q = 0; q < NQ; ++q)
 #pragma unroll
-        for (int cb = 0; cb < CB; ++cb)
+        for (int db = 0; db < NDB; ++db)
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
-            tot[rb][cb][j] += (double)acc[rb][cb][j];
-            acc[rb][cb][j] = T(0);
+            tot[q][db][j] += (double)acc[q][db][j];
+            acc[q][db][j] = T(0);
           }
     }
-    if (more) sstore(buf ^ 1);
-    __syncthreads();
+  };
+
+  // chunk c lives in LDS buffer c & 1 and, before that, in register set A
+  // (odd c) or B (even c, c >= 2)
+  if (nchunks > 0) {
+    gload(preA, pstA, s_begin);
+    sstore(preA, pstA, s_begin, 0);
+    if (nchunks > 1) gload(preA, pstA, s_begin + KC);
+    if (nchunks > 2) gload(preB, pstB, s_begin + 2 * KC);
   }
-  const int ld = (MODE == 0) ? nout : inner;
+  __syncthreads();
+  for (int ch = 0; ch < nchunks; ch += 2) {
+    compute(ch, 0);
+    if (ch + 1 < nchunks) {
+      sstore(preA, pstA, s_begin + (ch + 1) * KC, 1);
+      if (ch + 3 < nchunks) gload(preA, pstA, s_begin + (ch + 3) * KC);
+    }
+    __syncthreads();
+    if (ch + 1 < nchunks) {
+      compute(ch + 1, 1);
+      if (ch + 2 < nchunks) {
+        sstore(preB, pstB, s_begin + (ch + 2) * KC, 0);
+        if (ch + 4 < nchunks) gload(preB, pstB, s_begin + (ch + 4) * KC);
+      }
+      __syncthreads();
+    }
+  }
 #pragma unroll
-  for (int rb = 0; rb < RB; ++rb)
+  for (int q = 0; q < NQ; ++q)
 #pragma unroll
-    for (int cb = 0; cb < CB; ++cb)
+    for (int db = 0; db < NDB; ++db)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int row = m0 + wr + rb * 16 + Mfma<T>::row(lane, j);
-        const int col = n0 + wc + cb * 16 + (lane & 15);
-        if (row < M && col < N) {
-          const double v = (sizeof(T) == 4) ? tot[rb][cb][j] : (double)acc[rb][cb][j];
-          dst[(int64_t)row * ld + col] = (T)v;
+        // MFMA result (row(lane, j), lane & 15): rows follow the first operand
+        const int rr = Mfma<T>::row(lane, j), cc = lane & 15;
+        int dd, ii;
+        if (MODE == 0) {
+          dd = d0 + dw + db * 16 + rr;
+          ii = i0 + q * 16 + cc;
+        } else {
+          ii = i0 + q * 16 + rr;
+          dd = d0 + dw + db * 16 + cc;
+        }
+        if (dd < ndense && ii < ntoe) {
+          const double v = F32 ? tot[F32 ? q : 0][F32 ? db : 0][j] : (double)acc[q][db][j];
+          if (MODE == 0) dst[(int64_t)dd * nout + ii] = (T)v;
+          else dst[(int64_t)ii * inner + dd] = (T)v;
         }
       }
 }
@@ -500,26 +567,26 @@ int flpr_conv(int32_t mode, int32_t fp64, const void* in, void* out, const void*
   if (!in || !out || !tap || !jobs || !bands || nj < 1 || nj > 65535 || max_rows_or_outer < 1 || K < 1 ||
       nout < 1 || (mode != 0 && mode != 1) || (mode == 1 && inner < 1))
     return -1;
-  const int ntn = (mode == 0) ? (nout + TN - 1) / TN : (inner + TN - 1) / TN;
-  const int ntm = (mode == 0) ? (max_rows_or_outer + TM - 1) / TM : (nout + TM - 1) / TM;
-  const int64_t gx = (mode == 0) ? (int64_t)ntn * ntm : (int64_t)ntn * ntm * max_rows_or_outer;
+  const int ntd = (mode == 0) ? (max_rows_or_outer + LD - 1) / LD : (inner + LD - 1) / LD;
+  const int ntt = (nout + LT - 1) / LT;
+  const int64_t gx = (mode == 0) ? (int64_t)ntd * ntt : (int64_t)ntd * ntt * max_rows_or_outer;
   if (gx > INT32_MAX) return -1;
   dim3 grd((unsigned)gx, (unsigned)nj);
   hipStream_t st = as_stream(stream);
   if (fp64) {
     if (mode == 0)
       hipLaunchKernelGGL((k_conv<double, 0>), grd, dim3(CT), 0, st, (const double*)in,
-                         (double*)out, (const double*)tap, jobs, bands, K, nout, inner, base, ntn, ntm);
+                         (double*)out, (const double*)tap, jobs, bands, K, nout, inner, base, ntd, ntt);
     else
       hipLaunchKernelGGL((k_conv<double, 1>), grd, dim3(CT), 0, st, (const double*)in,
-                         (double*)out, (const double*)tap, jobs, bands, K, nout, inner, base, ntn, ntm);
+                         (double*)out, (const double*)tap, jobs, bands, K, nout, inner, base, ntd, ntt);
   } else {
     if (mode == 0)
       hipLaunchKernelGGL((k_conv<float, 0>), grd, dim3(CT), 0, st, (const float*)in, (float*)out,
-                         (const float*)tap, jobs, bands, K, nout, inner, base, ntn, ntm);
+                         (const float*)tap, jobs, bands, K, nout, inner, base, ntd, ntt);
     else
       hipLaunchKernelGGL((k_conv<float, 1>), grd, dim3(CT), 0, st, (const float*)in, (float*)out,
-                         (const float*)tap, jobs, bands, K, nout, inner, base, ntn, ntm);
+                         (const float*)tap, jobs, bands, K, nout, inner, base, ntd, ntt);
   }
   FLPR_CHECK_LAUNCH();
   return 0;

@@ -457,6 +457,14 @@ def filter_bandwidths(geo: Geometry, h: np.ndarray, n: int):
     return h, ihbad, warn
 
 
+def conv_pad(geo: Geometry) -> int:
+    """Elements flpr_conv may read past a job's input: 256 K + 64 inner + 512 for
+    the longest axis and the largest inner extent."""
+    d = len(geo.N)
+    inner = int(np.prod(geo.N[1:])) if d > 1 else 1
+    return 256 * int(max(geo.geom.G[k] for k in range(d))) + 64 * inner + 512
+
+
 def precision_groups(geo: Geometry, h: np.ndarray, order: int, fp64: bool):
     """Contiguous runs [a, b) of bandwidths with their convolution precision
     (True = float64).  The float32 form sends bandwidths narrower than
@@ -614,7 +622,10 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     cells = int(np.prod([g.G[k] for k in range(d)]))
     fcells = int(np.prod([g.tile[k] + 2 * geo.msp for k in range(d)]))
     part = torch.empty((ntiles * C * fcells,), dtype=f64, device=dev)
-    grid64 = torch.empty((C * cells,), dtype=f64, device=dev)
+    # convolution inputs are read in whole tiles past each job's end (the
+    # values there are discarded or zeroed): pad them
+    pad = conv_pad(geo)
+    grid64 = torch.empty((C * cells + pad,), dtype=f64, device=dev)
     sums = torch.zeros((C,), dtype=f64, device=dev)
     call("flpr_spread", gp, ptr(key_start), ptr(wab), ptr(vals), n, 1, nval, ptr(part),
          ptr(grid64), ptr(sums), s)
@@ -625,7 +636,7 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     dh = h.shape[0]
     any32 = any(not gf for (_a, _b, gf) in groups)
     shift = torch.empty((C,), dtype=f64, device=dev)
-    grid32 = torch.empty((C * cells,), dtype=torch.float32, device=dev) if any32 else None
+    grid32 = torch.empty((C * cells + pad,), dtype=torch.float32, device=dev) if any32 else None
     call("flpr_grid_center", ptr(grid64), C, cells, ptr(sums), ptr(shift), ptr(grid32), s)
 
     # --- taps and bands (uploaded from pinned host memory without a stream sync)
@@ -659,7 +670,7 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
         band_ptr = bands.data_ptr() + a * d * 2 * 4
         bands_g = band_np[a * d * 2:b * d * 2]
         stage0 = torch.empty(((b - a) * plan.stage_per_h[0],), dtype=fdt, device=dev)
-        inter = {k: torch.empty((plan.chunk * plan.stage_per_h[k],), dtype=fdt, device=dev)
+        inter = {k: torch.empty((plan.chunk * plan.stage_per_h[k] + pad,), dtype=fdt, device=dev)
                  for k in range(1, d)}
         for ci in range(len(plan.chunks)):
             for ps in plan.pass_tables[ci]:

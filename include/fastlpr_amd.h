@@ -139,7 +139,10 @@ int flpr_taps(const FlprGeom* g, const double* h, int32_t dh, int32_t npow, int3
  * array) are treated as zero.  K = input axis length, nout = output axis
  * length, inner = contiguous extent for mode 1, rows_or_outer = rows (mode 0)
  * or outer batch count (mode 1) of the job, max_rows_or_outer its maximum.
- * fp64 != 0 selects the float64 MFMA path (in/out/tap are double*). */
+ * fp64 != 0 selects the float64 MFMA path (in/out/tap are double*).
+ * Tiles are read whole: in must stay readable for 256*K + 64*inner + 512
+ * elements past the end of every job's input (those values are discarded or
+ * zeroed, never used). */
 int flpr_conv(int32_t mode, int32_t fp64, const void* in, void* out, const void* tap,
               const int64_t* jobs, const int32_t* bands, int32_t nj, int32_t max_rows_or_outer,
               int32_t K, int32_t nout, int32_t inner, int32_t base, void* stream);
