@@ -10,6 +10,8 @@
 // operand is the Toeplitz band, generated in LDS from a 1-D tap strip; the
 // GEMM runs on v_mfma_f32_16x16x4f32 (float32 path) or v_mfma_f64_16x16x4f64
 // (float64 path).
+#include <type_traits>
+
 #include "flpr_common.h"
 
 namespace {
@@ -267,21 +269,37 @@ __global__ __launch_bounds__(CT) void k_conv(const T* __restrict__ in, T* __rest
           dv[st][db] = (MODE == 0) ? dn[dd * PD + kl] : dn[kl * PD + dd];
         }
       }
-      auto mac = [&](int qmask) {
+      // one Toeplitz block q over the whole chunk
+      auto mac_q = [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+#pragma unroll
+        for (int st = 0; st < NST; ++st) {
+          const T tq = tv[4 * q - st + NST - 1];
+#pragma unroll
+          for (int db = 0; db < NDB; ++db)
+            acc[q][db] = (MODE == 0) ? Mfma<T>::run(dv[st][db], tq, acc[q][db])
+                                     : Mfma<T>::run(tq, dv[st][db], acc[q][db]);
+        }
+      };
+      if (live == (1 << NQ) - 1) {
+        // all four blocks, interleaved per K step
 #pragma unroll
         for (int st = 0; st < NST; ++st)
 #pragma unroll
           for (int q = 0; q < NQ; ++q) {
-            if (!(qmask & (1 << q))) continue;
             const T tq = tv[4 * q - st + NST - 1];
 #pragma unroll
             for (int db = 0; db < NDB; ++db)
               acc[q][db] = (MODE == 0) ? Mfma<T>::run(dv[st][db], tq, acc[q][db])
                                        : Mfma<T>::run(tq, dv[st][db], acc[q][db]);
           }
-      };
-      if (live == (1 << NQ) - 1) mac((1 << NQ) - 1);
-      else mac(live);
+      } else {
+        // the live blocks are contiguous (the band is): one scalar test per block
+        if (live & 1) mac_q(std::integral_constant<int, 0>{});
+        if (live & 2) mac_q(std::integral_constant<int, 1>{});
+        if (live & 4) mac_q(std::integral_constant<int, 2>{});
+        if (live & 8) mac_q(std::integral_constant<int, 3>{});
+      }
     }
     if constexpr (F32) {
       // the float32 MFMA accumulator only ever holds KC terms
