@@ -36,6 +36,8 @@ WORKLOAD = {"workload": "cv_fastlpr 2-D order-2 heteroscedastic pair (mean + var
             "seed": 0}
 PEAK_MFMA_FP32 = 157.3e12
 PEAK_MFMA_FP64 = 78.6e12
+COPY_BYTES = 256 << 20            # calibration copy: 256 MiB read + 256 MiB written
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "flpr_conv_traffic_2d_o2_n1e6.txt")
 
 
 def make_inputs(n: int, seed: int = 0):
@@ -63,24 +65,74 @@ def step(cv, xd, yd, hl, prec):
 
 def cpu_baseline(x, y, hl):
     """Oracle (NumPy restatement of the reference dataflow, float32 'ref' mode),
-    one thread, mean pass with 1 and 2 bandwidths; fits/s extrapolated to 30."""
+    one thread, mean pass: one untimed call with 1 bandwidth (FFT plans, probe
+    matrix), then timed calls with 2 and 4 of the 30 bandwidths; the per-
+    bandwidth slope extrapolates to 30.  About 20 s of CPU work."""
     from threadpoolctl import threadpool_limits
     from oracle import restate as R
     xs, ys = x.astype(np.float64), y.astype(np.float64)
+    opt = {"order": 2, "N": [256, 256]}
     times = {}
     with threadpool_limits(1):
-        for k in (1, 2):
-            sub = hl[14:14 + k]
+        R.cv_fastlpr(xs, ys, hl[14:15], opt, mode="ref")
+        for k in (2, 4):
+            sub = hl[13:13 + k]
             t = time.perf_counter()
-            R.cv_fastlpr(xs, ys, sub, {"order": 2, "N": [256, 256]}, mode="ref")
+            R.cv_fastlpr(xs, ys, sub, opt, mode="ref")
             times[k] = time.perf_counter() - t
-    slope = times[2] - times[1]
-    per_fit = times[1] + (hl.shape[0] - 1) * slope
+    slope = max(times[4] - times[2], 0.0) / 2.0
+    per_fit = times[2] + (hl.shape[0] - 2) * slope
     return {"value": 1.0 / per_fit, "unit": "fits/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle restatement of the mean pass on the same 1e6 points, 256x256 "
-                       f"grid, with 1 and 2 of the 30 bandwidths ({times[1]:.1f} s, "
-                       f"{times[2]:.1f} s), extrapolated linearly to 30 bandwidths: "
+            "sample": (f"oracle restatement (reference dataflow, float32 mode) of the mean pass "
+                       f"on the same 1e6 points and 256x256 grid, single thread, with 2 and 4 "
+                       f"of the 30 bandwidths ({times[2]:.1f} s, {times[4]:.1f} s) after one "
+                       f"untimed warm-up call; extrapolated linearly to 30 bandwidths: "
                        f"{per_fit:.1f} s per fit")}
+
+
+def pmc_summary(dbs):
+    """Per-kernel averages of the counters in rocprofv3 --pmc databases (rocpd
+    format), and the HBM traffic of flpr_conv per launch scaled by the
+    calibration copy (a plain 256 MiB device copy, --calibrate-copy)."""
+    import sqlite3
+    tot, cnt = {}, {}
+    for f in dbs:
+        db = sqlite3.connect(f)
+        rows = db.execute("select kernel_name, counter_name, value, dispatch_id "
+                          "from counters_collection").fetchall()
+        for name, cname, val, did in rows:
+            k = name.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "")
+            tot.setdefault(k, {}).setdefault(cname, 0.0)
+            tot[k][cname] += val
+            cnt.setdefault(k, {}).setdefault(cname, set()).add((f, did))
+    avg = {k: {c: v / len(cnt[k][c]) for c, v in d.items()} for k, d in tot.items()}
+    lines = []
+    for k in sorted(avg):
+        for c, v in sorted(avg[k].items()):
+            lines.append(f"{k[:70]:70s} {c:12s} {v:14.1f} KiB/dispatch  "
+                         f"({len(cnt[k][c])} dispatches)")
+    copy = [v for k, v in avg.items() if "neg_kernel" in k]
+    conv = [v for k, v in avg.items() if k.startswith("k_conv")]
+    out = {}
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        cal = max((d.get(c, 0.0) for d in copy), default=0.0)
+        conv_vals = [d[c] for d in conv if c in d]
+        if cal > 0 and conv_vals:
+            factor = (COPY_BYTES / 1024.0) / cal
+            out[c] = (float(np.mean(conv_vals)), factor)
+    return lines, out
+
+
+def read_traffic():
+    """HBM bytes per flpr_conv launch from the committed counter summary."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -90,7 +142,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the float64 plain form")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the timed steps, run a plain 256 MiB streaming kernel (negation) "
+                         "three times: the known-size reference for FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--pmc-summary", nargs="+", metavar="DB",
+                    help="summarise rocprofv3 --pmc databases (no GPU) and exit")
     args = ap.parse_args()
+    if args.pmc_summary:
+        lines, out = pmc_summary(args.pmc_summary)
+        print("\n".join(lines))
+        if "FETCH_SIZE" in out and "WRITE_SIZE" in out:
+            (f, ff), (w, fw) = out["FETCH_SIZE"], out["WRITE_SIZE"]
+            print(f"conv_fetch_kib_per_launch: {f:.1f} (copy scale {ff:.4f})")
+            print(f"conv_write_kib_per_launch: {w:.1f} (copy scale {fw:.4f})")
+            print(f"traffic_bytes_per_launch: {(f * ff + w * fw) * 1024.0:.0f}")
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(the hot path shards by bandwidth, see DESIGN.md)", file=sys.stderr)
@@ -121,6 +187,13 @@ def main():
     t1 = time.perf_counter()
     engine.TIMER.stop()
     kt = engine.TIMER.collect()
+    if args.calibrate_copy:
+        a = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.neg(a, out=b)  # a plain streaming kernel: 256 MiB in, 256 MiB out
+        torch.cuda.synchronize()
+        del a, b
     ms_step = (t1 - t0) * 1e3 / args.steps
     fits_s = 2.0 * args.steps / (t1 - t0)
     print(f"[bench] chosen: {ms_step:.2f} ms/step, {fits_s:.1f} fits/s", file=sys.stderr,
@@ -177,7 +250,7 @@ def main():
         "roofline": {"bound": "mfma", "kernel": "flpr_conv",
                      "achieved": achieved, "peak": PEAK_MFMA_FP32 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / (PEAK_MFMA_FP32 / 1e12) if conv else None,
-                     "traffic": None, "avg_launch_ms": conv_ms,
+                     "traffic": read_traffic(), "avg_launch_ms": conv_ms,
                      "flop_per_launch": conv_flop, "launches_timed": len(conv),
                      "share_of_step_time": conv_share},
         "cpu_baseline": cpu,
