@@ -258,6 +258,36 @@ class GridInterpolant:
 # cv_fastlpr
 # ---------------------------------------------------------------------------
 
+def _point_sums(fit, xd, yd, probes, ny: int, K: int, gcv_method: str, dof_method: str):
+    """Probe dots and residual sums from the grid solutions interpolated at the
+    points (the reference's direct route, regression.py:1256-1280 and
+    api.py:661-723), used when either interpolation is cubic.  Returns the fit
+    with its m_y narrowed to the y columns and its red buffer filled in the
+    layout flpr_select reads (y dots zero, residual sums in place of the
+    quadratic forms), and the matching all-zero sum of squares."""
+    dh, nval, nodes = fit.m_y.shape
+    n = xd.shape[0]
+    geo = fit.geo
+
+    def at_points(lo, hi, method):
+        grids = fit.m_y[:, lo:hi, :].reshape(-1, nodes).contiguous()
+        if method == "cubic":
+            out = engine.spline_eval(geo, xd, engine.spline_coefficients(geo, grids))
+        else:
+            out, _ = engine.interp_eval(geo, xd, grids, grids.shape[0])
+        return out.view(n, dh, hi - lo)
+
+    yh = at_points(0, ny, gcv_method)
+    rss = ((yd[:, None, :] - yh) ** 2).sum(dim=0)              # (dh, ny)
+    ph = at_points(ny, ny + K, dof_method)
+    dots = (ph * probes[:, None, :]).sum(dim=0)                 # (dh, K)
+    zeros = torch.zeros((dh, ny), dtype=torch.float64, device=xd.device)
+    red = torch.cat([torch.cat([zeros, dots], dim=1).reshape(-1), rss.reshape(-1)])
+    fit.m_y = fit.m_y[:, :ny, :]
+    fit.red = red
+    return fit, torch.zeros((ny,), dtype=torch.float64, device=xd.device)
+
+
 def _y_channels(y_arr, dev):
     """(n, dy) real/complex -> (n, ny) float64 device, complex as interleaved (re, im)."""
     if isinstance(y_arr, torch.Tensor):
@@ -370,12 +400,16 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
         sel = 0
         sel_dev = sel_out = None
     else:
-        for key in ("dof_interp_method", "gcv_interp_method"):
-            if opts.get(key, lin_default) != "linear":
-                raise NotImplementedError(
-                    f"{key}='cubic' (the reference default for 2-D/3-D problems with fewer than "
-                    "5000 points) is not implemented on the GPU yet; pass "
-                    f"options['{key}']='linear'")
+        dof_method = str(opts.get("dof_interp_method", lin_default))
+        gcv_method = str(opts.get("gcv_interp_method", lin_default))
+        for key, meth in (("dof_interp_method", dof_method), ("gcv_interp_method", gcv_method)):
+            if meth not in ("linear", "cubic"):
+                raise NotImplementedError(f"{key}={meth!r}: the GPU engine implements "
+                                          "'linear' and 'cubic'")
+        cubic = "cubic" in (dof_method, gcv_method)
+        if cubic and xrange is not None:
+            raise NotImplementedError("cubic GCV/DOF interpolation with options['xrange'] "
+                                      "is not implemented on the GPU")
         K = int(opts.get("num_dof_sample", 10))
         if K < 1:
             raise ValueError(
@@ -386,12 +420,15 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
         K = probes.shape[1]
         fit = engine.run_grid(xd, yd, ny, probes, K, bandwidths, order, N, int(opts["accuracy"]),
                               fp64, reg, xrange, flag_power2, want_rss=True, stats=stats,
-                              readback=False)
+                              readback=False, keep_all=cubic)
+        ysq_dev = stats.ysq_dev
+        if cubic:
+            fit, ysq_dev = _point_sums(fit, xd, yd, probes, ny, K, gcv_method, dof_method)
         agg = str(opts.get("gcv_selection", "first"))
         # Hutchinson DOF, GCV and the 1-SE choice stay on the device; the
         # interpolation at the chosen bandwidth is queued behind them and the
         # host reads everything back once, at the end
-        sel_out, sel_dev = engine.select(fit, stats.ysq_dev, pp_dev, n, ny, K, y_is_complex,
+        sel_out, sel_dev = engine.select(fit, ysq_dev, pp_dev, n, ny, K, y_is_complex,
                                          agg, float(opts.get("dstd", 0.0)))
     if fit.warn:
         warnings.warn(fit.warn, stacklevel=2)

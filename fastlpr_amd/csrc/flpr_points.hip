@@ -1059,6 +1059,123 @@ __global__ __launch_bounds__(256) void k_lcv_final(const double* __restrict__ pa
   }
 }
 
+// Cubic tensor-product spline evaluation (RegularGridInterpolator 'cubic',
+// i.e. scipy's make_interp_spline(k=3) with not-a-knot ends along every axis,
+// api.py:588-594, 651-657).  The caller supplies B-spline coefficients
+// c (ngrid, N_0, ..., N_{d-1}); along axis k the knot vector is
+//   x_0 x_0 x_0 x_0 x_2 x_3 ... x_{N-3} x_{N-1} x_{N-1} x_{N-1} x_{N-1}
+// over the raw axis x_j = linspace(graw0, graw1, N)[j].  Outside the grid the
+// end polynomial pieces extrapolate (BSpline extrapolate=True).
+__device__ inline double np_linspace_at(double a, double b, int N, int j) {
+  if (j <= 0) return a;
+  if (j >= N - 1) return b;
+  const double step = (b - a) / (double)(N - 1);
+  return (double)j * step + a;
+}
+
+// knot t[i] of the not-a-knot cubic knot vector (length N + 4)
+__device__ inline double nak_knot(double a, double b, int N, int i) {
+  if (i <= 3) return a;
+  if (i >= N) return b;
+  return np_linspace_at(a, b, N, i - 2);
+}
+
+// The four nonzero cubic B-splines at v: basis[r] = B_{l-3+r}(v), l the knot
+// interval (de Boor's triangular recurrence).
+__device__ inline int cubic_basis(double v, double a, double b, int N, double* basis) {
+  // interval l in [3, N-1] with t[l] <= v < t[l+1], clamped at the ends
+  int l;
+  if (!(v >= a)) {
+    l = 3;
+  } else if (!(v < b)) {
+    l = N - 1;
+  } else {
+    const double step = (b - a) / (double)(N - 1);
+    int j = (int)floor((v - a) / step);  // raw interval, then knot interval
+    j = max(0, min(j, N - 2));
+    // move to the interval that contains v exactly (linspace rounding)
+    while (j > 0 && v < np_linspace_at(a, b, N, j)) --j;
+    while (j < N - 2 && v >= np_linspace_at(a, b, N, j + 1)) ++j;
+    l = min(max(j + 2, 3), N - 1);  // raw interval j lies in knot interval j+2 (ends merged)
+  }
+  double left[4], right[4];
+  basis[0] = 1.0;
+#pragma unroll
+  for (int j = 1; j <= 3; ++j) {
+    left[j] = v - nak_knot(a, b, N, l + 1 - j);
+    right[j] = nak_knot(a, b, N, l + j) - v;
+    double saved = 0.0;
+#pragma unroll
+    for (int r = 0; r < j; ++r) {
+      const double temp = basis[r] / (right[r + 1] + left[j - r]);
+      basis[r] = saved + right[r + 1] * temp;
+      saved = left[j - r] * temp;
+    }
+    basis[j] = saved;
+  }
+  return l - 3;  // first coefficient index
+}
+
+constexpr int SPL_THREADS = 256;
+constexpr int SPL_GROUP = 8;  // grids per work item
+
+// out[i][gidx] = sum over the 4^d coefficients around x_i of c[gidx] times the
+// tensor B-spline weights.  Work item = (point, group of SPL_GROUP grids).
+template <int D>
+__global__ __launch_bounds__(SPL_THREADS) void k_spline_eval(FlprGeom g,
+                                                             const double* __restrict__ x,
+                                                             int64_t n,
+                                                             const double* __restrict__ c,
+                                                             int ngrid, int64_t nodes,
+                                                             double* __restrict__ out) {
+  const int ngroups = (ngrid + SPL_GROUP - 1) / SPL_GROUP;
+  const int64_t items = n * ngroups;
+  for (int64_t it = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; it < items;
+       it += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = it / ngroups;
+    const int gq = (int)(it - i * ngroups);
+    double bw[3][4];
+    int first[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+      first[k] = cubic_basis(x[i * D + k], g.graw0[k], g.graw1[k], g.n_grid[k], bw[k]);
+    const int g0 = gq * SPL_GROUP, g1 = min(ngrid, g0 + SPL_GROUP);
+    for (int gi = g0; gi < g1; ++gi) {
+      const double* cg = c + (int64_t)gi * nodes;
+      double v = 0.0;
+      if constexpr (D == 1) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) v += bw[0][a] * cg[first[0] + a];
+      } else if constexpr (D == 2) {
+        const int N1 = g.n_grid[1];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          double r = 0.0;
+#pragma unroll
+          for (int b = 0; b < 4; ++b) r += bw[1][b] * cg[(int64_t)(first[0] + a) * N1 + first[1] + b];
+          v += bw[0][a] * r;
+        }
+      } else {
+        const int N1 = g.n_grid[1], N2 = g.n_grid[2];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          double r = 0.0;
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            double q = 0.0;
+            const int64_t row = ((int64_t)(first[0] + a) * N1 + first[1] + b) * N2 + first[2];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) q += bw[2][e] * cg[row + e];
+            r += bw[1][b] * q;
+          }
+          v += bw[0][a] * r;
+        }
+      }
+      out[i * ngrid + gi] = v;
+    }
+  }
+}
+
 int lds_group(int64_t cells_per_channel, int channels, int cap, int64_t budget = 64 * 1024) {
   const int64_t per = cells_per_channel * (int64_t)sizeof(double);
   int cg = (int)(budget / per);
@@ -1348,6 +1465,25 @@ int flpr_lcv(const FlprGeom* g, const double* x, int64_t n, const double* dens, 
   FLPR_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_lcv_final, dim3((dh + 255) / 256), dim3(256), 0, st, part, LCV_BLOCKS, dh,
                      out);
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+int flpr_spline_eval(const FlprGeom* g, const double* x, int64_t n, const double* coef,
+                     int32_t ngrid, double* out, void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!flpr_geom_ok(g) || !x || !coef || !out || n < 1 || ngrid < 1) return -1;
+  for (int k = 0; k < g->d; ++k)
+    if (g->n_grid[k] < 4) return -1;
+  const int64_t nodes = flpr_nodes(g);
+  const int64_t items = n * ((ngrid + SPL_GROUP - 1) / SPL_GROUP);
+  dim3 grd(flpr_grid_for(items, SPL_THREADS, 4096));
+  hipStream_t st = as_stream(stream);
+  switch (g->d) {
+    case 1: hipLaunchKernelGGL(k_spline_eval<1>, grd, dim3(SPL_THREADS), 0, st, *g, x, n, coef, ngrid, nodes, out); break;
+    case 2: hipLaunchKernelGGL(k_spline_eval<2>, grd, dim3(SPL_THREADS), 0, st, *g, x, n, coef, ngrid, nodes, out); break;
+    default: hipLaunchKernelGGL(k_spline_eval<3>, grd, dim3(SPL_THREADS), 0, st, *g, x, n, coef, ngrid, nodes, out); break;
+  }
   FLPR_CHECK_LAUNCH();
   return 0;
 }

@@ -573,7 +573,8 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
              xrange=None, flag_power2: bool = True, want_rss: bool = False,
              stats: Optional[Stats] = None, with_ones_eps: bool = True,
              debug: Optional[dict] = None, readback: bool = True,
-             moments_only: bool = False, kde: bool = False) -> DeviceFit:
+             moments_only: bool = False, kde: bool = False,
+             keep_all: bool = False) -> DeviceFit:
     """Spread, convolve and solve for all bandwidths, plus the probe dot products and
     the residual sums.
 
@@ -588,6 +589,9 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     they stay on the device in DeviceFit.red and nothing waits for the device.
     moments_only: stop after the convolution (S_00 per bandwidth in s0_all, the
     density path of cv_fastkde); kde selects cv_fastkde's raw grid axes.
+    keep_all: return the grid solutions of every channel (y columns and probes,
+    DeviceFit.m_y is (dh, ny + K, nodes)) instead of the adjoint sums, for
+    callers that interpolate them at the points themselves (cubic).
     """
     n, d = x.shape
     dev = x.device
@@ -704,10 +708,10 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     # --- adjoint sums, solve (+ dots), residual quadratic form
     rcells = int(np.prod([n_ - 1 for n_ in geo.N]))
     nw = (1 << d) * ((1 << d) + 1) // 2
-    need_q = K > 0 or want_rss
+    need_q = (K > 0 or want_rss) and not keep_all
     q = gram = None
     # one float64 readback buffer: probe / y dots (dh, nval) then quadratic forms (dh, ny)
-    nq_out = dh * ny if (want_rss and ny > 0) else 0
+    nq_out = dh * ny if (want_rss and ny > 0 and not keep_all) else 0
     red = torch.zeros((dh * nval + nq_out,), dtype=f64, device=dev) if need_q else None
     dot = red[:dh * nval] if need_q else None
     if need_q:
@@ -718,12 +722,13 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
         call("flpr_adjoint", gp, ptr(key_start), ptr(rawi), ptr(rawf), ptr(vals), n, nval,
              ptr(apart), ptr(q), ptr(gram), s)
         del apart
-    nyc = max(ny, 1)
+    nout = nval if keep_all else ny
+    nyc = max(nout, 1)
     m_y = torch.empty((dh * nyc * nodes,), dtype=f64, device=dev)
     for (a, b, gf, plan, stage0) in stage0s:
         call("flpr_solve", int(gf), ptr(stage0), per_h0, b - a, nodes, order, plan.nt,
              i64_array(plan.s_off), i64_array(plan.t_off), ptr(key_max), lam_scale, eps_add,
-             ptr(shift), nval, ny, (m_y.data_ptr() + a * nyc * nodes * 8) if ny > 0 else None,
+             ptr(shift), nval, nout, (m_y.data_ptr() + a * nyc * nodes * 8) if nout > 0 else None,
              ptr(q), (dot.data_ptr() + a * nval * 8) if dot is not None else None, s)
     quad = None
     if nq_out:
@@ -765,6 +770,89 @@ def lcv(geo: Geometry, x: torch.Tensor, dens: torch.Tensor, k0h: np.ndarray):
     out = torch.empty((2 * dh,), dtype=torch.float64, device=dev)
     call("flpr_lcv", _lib.C.byref(geo.geom), ptr(x), n, ptr(dens.contiguous()), dh, ptr(k0),
          ptr(part), ptr(out), stream_handle())
+    return out
+
+
+# ---------------------------------------------------------------------------
+# cubic interpolation (RegularGridInterpolator 'cubic': not-a-knot cubic
+# splines along every axis, api.py:588-594, 651-657)
+# ---------------------------------------------------------------------------
+
+def _nak_knots(ax: np.ndarray) -> np.ndarray:
+    """Knot vector of scipy's make_interp_spline(k=3) (not-a-knot): N + 4 knots."""
+    return np.r_[(ax[0],) * 4, ax[2:-2], (ax[-1],) * 4]
+
+
+def _cubic_basis(t: np.ndarray, v: np.ndarray, N: int):
+    """The four nonzero cubic B-splines at v (de Boor's recurrence), and the
+    index of the first; v inside [t[3], t[N]]."""
+    l = np.clip(np.searchsorted(t, v, side="right") - 1, 3, N - 1)
+    basis = np.zeros((v.size, 4))
+    basis[:, 0] = 1.0
+    left = np.zeros((v.size, 4))
+    right = np.zeros((v.size, 4))
+    for j in range(1, 4):
+        left[:, j] = v - t[l + 1 - j]
+        right[:, j] = t[l + j] - v
+        saved = np.zeros(v.size)
+        for r in range(j):
+            temp = basis[:, r] / (right[:, r + 1] + left[:, j - r])
+            basis[:, r] = saved + right[:, r + 1] * temp
+            saved = left[:, j - r] * temp
+        basis[:, j] = saved
+    return basis, l - 3
+
+
+_CUBIC: Dict[tuple, List[torch.Tensor]] = {}
+
+
+def cubic_tables(geo: Geometry, dev) -> List[torch.Tensor]:
+    """Per axis, the inverse of the not-a-knot collocation matrix A[i, j] =
+    B_j(x_i) on the raw axis (float64, on the device): coefficients = A^-1 values."""
+    key = (tuple(float(a[0]) for a in geo.axes_raw), tuple(float(a[-1]) for a in geo.axes_raw),
+           tuple(geo.N), str(dev))
+    hit = _CUBIC.get(key)
+    if hit is None:
+        hit = []
+        for ax in geo.axes_raw:
+            N = len(ax)
+            if N < 4:
+                raise ValueError("cubic interpolation needs at least 4 grid nodes per dimension")
+            t = _nak_knots(ax)
+            basis, first = _cubic_basis(t, ax, N)
+            A = np.zeros((N, N))
+            for r in range(4):
+                A[np.arange(N), first + r] += basis[:, r]
+            hit.append(torch.tensor(np.linalg.inv(A), dtype=torch.float64, device=dev))
+        if len(_CUBIC) >= _MEMO_MAX:
+            _CUBIC.clear()
+        _CUBIC[key] = hit
+    return hit
+
+
+def spline_coefficients(geo: Geometry, grids: torch.Tensor) -> torch.Tensor:
+    """B-spline coefficients of grids (G, nodes) float64: A_k^-1 applied along
+    every axis (float64 GEMMs)."""
+    inv = cubic_tables(geo, grids.device)
+    N = tuple(geo.N)
+    G = grids.shape[0]
+    c = grids.reshape((G,) + N)
+    d = len(N)
+    for k in range(d):
+        # move axis k last, contract with A_k^-1, move back
+        c = torch.movedim(c, 1 + k, -1).contiguous()
+        c = torch.matmul(c, inv[k].t().contiguous())
+        c = torch.movedim(c, -1, 1 + k)
+    return c.contiguous().reshape(G, -1)
+
+
+def spline_eval(geo: Geometry, x: torch.Tensor, coef: torch.Tensor) -> torch.Tensor:
+    """(n, G) values of the cubic splines with coefficients coef (G, nodes) at x (n, d)."""
+    n = x.shape[0]
+    G = coef.shape[0]
+    out = torch.empty((n, G), dtype=torch.float64, device=x.device)
+    call("flpr_spline_eval", _lib.C.byref(geo.geom), ptr(x), n, ptr(coef), G, ptr(out),
+         stream_handle())
     return out
 
 

@@ -209,6 +209,17 @@ int flpr_interp_eval(const FlprGeom* g, const double* x, int64_t n, const double
                      int32_t ncol, double* out, const double* yvar, double* expsum,
                      void* stream);
 
+/* Cubic spline interpolation (RegularGridInterpolator 'cubic', i.e. scipy's
+ * make_interp_spline(k=3) with not-a-knot ends along every axis; the
+ * reference's GCV / DOF interpolation for 2-D and 3-D data with fewer than
+ * 5000 points, api.py:588-594, 651-657, regression.py:929-1092).  coef
+ * (ngrid, N_0, ..., N_{d-1}) float64 B-spline coefficients on the raw grid
+ * axes linspace(graw0, graw1, N), knots x_0 (four times), x_2 .. x_{N-3}, x_{N-1}
+ * (four times);
+ * out (n, ngrid) float64 values at x (n, d); N_k >= 4. */
+int flpr_spline_eval(const FlprGeom* g, const double* x, int64_t n, const double* coef,
+                     int32_t ngrid, double* out, void* stream);
+
 /* Likelihood cross-validation of density grids (api.py:1111-1132): dens
  * (dh, nodes) float64 densities on the raw grid, x (n, d) float64 data points,
  * k0h (dh) K(0) / prod_k h_bk.  For every bandwidth b,

@@ -450,10 +450,21 @@ def interp_linear(axes, values, pts):
                                    bounds_error=False, fill_value=None)(pts)
 
 
-def interp_dof(axes, values, pts, method):
+def _spline_kwargs(method, mode):
+    """RegularGridInterpolator 'cubic' solves for the spline coefficients
+    iteratively (scipy make_ndbspl with gcrotmk, relative residual 1e-5), as
+    the reference does; mode "f64" solves the same system directly, which is
+    the exact not-a-knot tensor spline."""
+    if method == "cubic" and mode == "f64":
+        from scipy.sparse.linalg import spsolve
+        return {"solver": spsolve}
+    return {}
+
+
+def interp_dof(axes, values, pts, method, mode="ref"):
     """_interpolate_with_linear_extrapolation for in-bounds points (regression.py:929-1092)."""
-    return RegularGridInterpolator(tuple(axes), values, method=method,
-                                   bounds_error=False, fill_value=None)(pts)
+    return RegularGridInterpolator(tuple(axes), values, method=method, bounds_error=False,
+                                   fill_value=None, **_spline_kwargs(method, mode))(pts)
 
 
 def hutchinson(p, phat, n, y_is_complex):
@@ -639,7 +650,7 @@ def cv_fastlpr(x, y, h=None, options=None, mode="ref", return_trace=False) -> Fi
     dof_method = opts.get("dof_interp_method", dflt)
     Tp = P.weighted(p, True)
     mp = solve(S, Tp, order, reg, P.nt, mode)           # (grid..., dh, K)
-    phat = interp_dof(P.axes_raw, mp, x_arr, dof_method)  # (n, dh, K)
+    phat = interp_dof(P.axes_raw, mp, x_arr, dof_method, mode)  # (n, dh, K)
     phat = np.real(np.transpose(phat, (0, 2, 1)))
     dof, dstd, pen_std, pen_mean, pph, pp = hutchinson(p, phat, n, not real_y)
     trace.update(p_dot_phat=pph, p_dot_p=pp, m_probe=mp)
@@ -650,7 +661,8 @@ def cv_fastlpr(x, y, h=None, options=None, mode="ref", return_trace=False) -> Fi
     C = m_all.shape[-1]
     ms = m_all if C > 1 else m_all[..., 0]
     yh_all = RegularGridInterpolator(tuple(P.axes_raw), ms, method=gcv_method,
-                                     bounds_error=False, fill_value=None)(x_arr)
+                                     bounds_error=False, fill_value=None,
+                                     **_spline_kwargs(gcv_method, mode))(x_arr)
     agg = str(opts.get("gcv_selection", "first"))
     scores, errs, rss_all = [], [], []
     for ih in range(P.dh):

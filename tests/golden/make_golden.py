@@ -3,7 +3,7 @@
 Run in the development container only (the reference checkout is not present
 where the GPU tests run):
 
-    FASTLPR_QUIET=1 python tests/golden/make_golden.py /root/reference/fastLPR_py/src
+    FASTLPR_QUIET=1 python tests/golden/make_golden.py /root/reference/fastLPR_py/src [names...]
 
 Each fixture stores the inputs (or, for the large case, the seeded recipe and a
 hash of the inputs), the options, and the reference outputs of
@@ -75,7 +75,7 @@ def run_kde(fl, name):
                 else np.asarray(np.concatenate(r.grid)))
 
 
-def main(ref_src: str):
+def main(ref_src: str, only=None):
     sys.path.insert(0, ref_src)
     os.environ.setdefault("FASTLPR_QUIET", "1")
     import fastlpr as fl
@@ -85,17 +85,26 @@ def main(ref_src: str):
                 backend=select_accumulation_backend())
     sys.path.insert(0, HERE)
     import recipes
-    for name in recipes.LPR_SMALL:
+    def wanted(name):
+        return not only or name in only
+
+    for name in recipes.LPR_SMALL + recipes.LPR_CUBIC:
+        if not wanted(name):
+            continue
         out = run_lpr(fl, name)
         out["meta"] = json.dumps(meta)
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
         print("wrote", name)
     for name in recipes.LPR_LARGE:
+        if not wanted(name):
+            continue
         out = run_lpr(fl, name, store_inputs=False, n_sub=4096)
         out["meta"] = json.dumps(meta)
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
         print("wrote", name)
     for name in recipes.KDE:
+        if not wanted(name):
+            continue
         out = run_kde(fl, name)
         out["meta"] = json.dumps(meta)
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
@@ -103,4 +112,4 @@ def main(ref_src: str):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference/fastLPR_py/src")
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference/fastLPR_py/src", sys.argv[2:])

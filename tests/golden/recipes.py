@@ -40,6 +40,19 @@ def make(name: str):
         s = 0.05 + 0.3 * x[:, 0]
         r2 = (s * rng.standard_normal(n)) ** 2
         return x, r2, _logspace_h(10, 0.05, 1.0)[:, None], {"order": 1, "N": 64, "y_type_out": "variance"}
+    if name == "lpr2d_o1_cubic":
+        # fewer than 5000 points in 2-D: the reference interpolates GCV / DOF cubically
+        rng = np.random.default_rng(22)
+        n = 1500
+        x = rng.uniform(0, 1, (n, 2))
+        y = np.sin(2 * np.pi * x[:, 0]) * np.cos(2 * np.pi * x[:, 1]) + 0.1 * rng.standard_normal(n)
+        return x, y, _iso(_logspace_h(8, 0.03, 0.5), 2), {"order": 1, "N": [24, 24]}
+    if name == "lpr3d_o1_cubic":
+        rng = np.random.default_rng(32)
+        n = 1500
+        x = rng.uniform(0, 1, (n, 3))
+        y = np.sum((x - 0.5) ** 2, axis=1) + 0.1 * rng.standard_normal(n)
+        return x, y, _iso(_logspace_h(5, 0.1, 0.6), 3), {"order": 1, "N": [10, 10, 10]}
     if name.startswith("lpr2d_o"):
         # lpr2d_o{order}_m{M0}x{M1}
         order = int(name[len("lpr2d_o")])
@@ -79,5 +92,6 @@ LPR_SMALL = [
     "lpr2d_o0_m32x32", "lpr2d_o1_m32x32", "lpr2d_o1_m31x33",
     "lpr3d_o1",
 ]
+LPR_CUBIC = ["lpr2d_o1_cubic", "lpr3d_o1_cubic"]
 LPR_LARGE = ["cfg1_1d_o1_n1e5"]
 KDE = ["kde1d", "kde2d", "kde3d"]

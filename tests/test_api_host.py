@@ -81,3 +81,33 @@ def test_cv_fastkde_validation_errors():
         fl.cv_fastkde(np.array([[0.3], [np.nan], [0.1]]))
     with pytest.raises(ValueError, match="Inf"):
         fl.cv_fastkde(np.array([[0.3], [np.inf], [0.1]]))
+
+
+def test_cubic_tables_match_scipy_regular_grid_interpolator():
+    """The not-a-knot collocation inverse and the de Boor basis the GPU kernel
+    mirrors reproduce scipy's cubic interpolation: make_interp_spline(k=3) and
+    RegularGridInterpolator(method='cubic') with a direct solve to rounding;
+    RegularGridInterpolator's default iterative solve (what the reference
+    runs) only to its 1e-5 residual."""
+    from scipy.interpolate import RegularGridInterpolator, make_interp_spline
+    from scipy.sparse.linalg import spsolve
+    from fastlpr_amd import engine as E
+    rs = np.random.default_rng(5)
+    for N in (4, 5, 7, 16, 40):
+        ax = np.linspace(-0.3, 1.7, N)
+        t = E._nak_knots(ax)
+        basis, first = E._cubic_basis(t, ax, N)
+        A = np.zeros((N, N))
+        for r in range(4):
+            A[np.arange(N), first + r] += basis[:, r]
+        y = rs.standard_normal(N)
+        c = np.linalg.solve(A, y)
+        pts = np.r_[ax, rs.uniform(ax[0], ax[-1], 50)]
+        b, f0 = E._cubic_basis(t, pts, N)
+        mine = np.sum(b * c[f0[:, None] + np.arange(4)[None, :]], axis=1)
+        np.testing.assert_allclose(mine, make_interp_spline(ax, y, k=3)(pts), rtol=1e-12,
+                                   atol=1e-12)
+        exact = RegularGridInterpolator((ax,), y, method="cubic", solver=spsolve)(pts[:, None])
+        np.testing.assert_allclose(mine, exact, rtol=1e-12, atol=1e-12)
+        loose = RegularGridInterpolator((ax,), y, method="cubic")(pts[:, None])
+        assert np.max(np.abs(mine - loose)) < 1e-4 * np.max(np.abs(y))
