@@ -1,7 +1,8 @@
 """fastlpr_amd: MI355X engine for the fastLPR cv_fastlpr / cv_fastkde hot path.
 
 Public API (same names and semantics as the reference package ``fastlpr``):
-``cv_fastlpr``, ``cv_fastkde``, ``get_hlist``, ``RegressionOutput``, ``KDEOutput``.
+``cv_fastlpr``, ``cv_fastkde``, ``get_hlist``, ``fastlpr_predict``,
+``fastlpr_interval``, ``RegressionOutput``, ``KDEOutput``.
 """
 from .bandwidth import get_hlist
 from .structures import KDEOutput, RegressionOutput
@@ -11,11 +12,12 @@ __version__ = "0.1.0"
 
 def __getattr__(name):
     # the API pulls in torch and the HIP library lazily
-    if name in ("cv_fastlpr", "cv_fastkde", "clear_caches", "GridInterpolant"):
+    if name in ("cv_fastlpr", "cv_fastkde", "clear_caches", "GridInterpolant",
+                "fastlpr_predict", "fastlpr_interval"):
         from . import api
         return getattr(api, name)
     raise AttributeError(name)
 
 
 __all__ = ["cv_fastlpr", "cv_fastkde", "get_hlist", "RegressionOutput", "KDEOutput",
-           "clear_caches"]
+           "clear_caches", "fastlpr_predict", "fastlpr_interval"]

@@ -15,6 +15,7 @@ Extra option (no counterpart in the reference):
 """
 from __future__ import annotations
 
+import math
 import warnings
 from typing import Dict, Optional, Sequence, Tuple
 
@@ -235,6 +236,14 @@ class GridInterpolant:
         if self._cplx:
             v = v[0::2] + 1j * v[1::2]
         v = np.moveaxis(v.reshape((v.shape[0],) + self._shape), 0, -1)
+        return v[..., 0] if self._squeeze else v
+
+    def device_values(self) -> torch.Tensor:
+        """The grid values as a device tensor, laid out like ``values``."""
+        v = self._vals
+        if self._cplx:
+            v = torch.complex(v[0::2], v[1::2])
+        v = torch.movedim(v.reshape((v.shape[0],) + self._shape), 0, -1)
         return v[..., 0] if self._squeeze else v
 
     def __call__(self, points):
@@ -589,3 +598,121 @@ def cv_fastkde(x, h: Optional[Sequence[Sequence[float]]] = None,
     metadata = {"options": opts, "bandwidths_tested": hl, "fhat_all": conv(fhat_all)}
     return KDEOutput(fhat=conv(fsel.reshape(N)), grid=tuple(geo.axes_raw), h=hl[sel],
                      lcv=lcv_result, fpp=interp, metadata=metadata)
+
+
+# ---------------------------------------------------------------------------
+# prediction and pointwise intervals
+# ---------------------------------------------------------------------------
+
+def fastlpr_predict(regression: RegressionOutput, x_new):
+    """Evaluate a fitted regression at new points through its interpolant
+    (predict.py:15-122): linear interpolation of the selected grid values with
+    linear extrapolation, on the GPU; complex fits stay complex."""
+    if regression.fpp_yhat is None:
+        raise ValueError("Regression output does not contain an interpolant (fpp).")
+    if isinstance(x_new, torch.Tensor):
+        pts = x_new if x_new.ndim == 2 else x_new[:, None]
+        return regression.fpp_yhat(pts)
+    pts = np.asarray(x_new, dtype=float)
+    if pts.ndim == 1:
+        pts = pts[:, None]
+    return np.asarray(regression.fpp_yhat(pts))
+
+
+def _gauss_moment(k: int, squared: bool) -> float:
+    """int phi(u) u^k du (= (k-1)!! for even k) or int phi(u)^2 u^k du
+    (= Gamma((k+1)/2) / (2 pi) for even k); 0 for odd k."""
+    if k % 2:
+        return 0.0
+    if squared:
+        return math.gamma((k + 1) / 2.0) / (2.0 * math.pi)
+    out = 1.0
+    for j in range(k - 1, 0, -2):
+        out *= j
+    return out
+
+
+def _nu(d: int, order: int) -> float:
+    """Asymptotic variance constant e1' M_K^-1 M_K2 M_K^-1 e1 of the local
+    polynomial estimator at an interior point for the product Gaussian kernel
+    (confidence.py:24-46), from the kernel moments of each dimension."""
+    if d not in (1, 2, 3) or order not in (0, 1, 2):
+        raise ValueError(f"nu constant not available for d={d}, order={order}. "
+                         "Supported: d in {1,2,3}, order in {0,1,2}.")
+    pw = engine.poly_powers(d, order)
+    nt = len(pw)
+    MK = np.ones((nt, nt))
+    MK2 = np.ones((nt, nt))
+    for i in range(nt):
+        for j in range(nt):
+            for k in range(d):
+                e = pw[i][k] + pw[j][k]
+                MK[i, j] *= _gauss_moment(e, False)
+                MK2[i, j] *= _gauss_moment(e, True)
+    Minv = np.linalg.inv(MK)
+    return float((Minv @ MK2 @ Minv)[0, 0])
+
+
+def fastlpr_interval(mu: RegressionOutput, sigma, alpha: float = 0.05,
+                     type: str = "confidence"):
+    """Pointwise confidence (mean) or prediction (new observation) intervals on
+    the grid (confidence.py:60-216): se^2 = sigma nu / (prod(h) s0), half width
+    z se (confidence) or z sqrt(sigma + se^2) (prediction), z the standard
+    normal 1 - alpha/2 quantile.  sigma is the variance fit
+    (y_type_out='variance') on the same grid, or a number / array.  Returns
+    (grid..., 2) [upper, lower], on the device when mu's grids are."""
+    if not isinstance(mu, RegressionOutput):
+        raise TypeError("mu must be a RegressionOutput object from cv_fastlpr")
+    if mu.yhat is None:
+        raise ValueError("mu must have .yhat field (fitted values)")
+    if not isinstance(sigma, (RegressionOutput, float, int, np.ndarray, torch.Tensor)):
+        raise TypeError("sigma must be RegressionOutput, float, or ndarray")
+    if isinstance(sigma, RegressionOutput) and sigma.yhat is None:
+        raise ValueError("sigma must have .yhat field (fitted values)")
+    if not (0 < alpha < 1):
+        raise ValueError(f"alpha must be in (0, 1), got {alpha}")
+    kind = type.lower()
+    if kind not in ("confidence", "prediction"):
+        raise ValueError(f"type must be 'confidence' or 'prediction', got '{type}'")
+    if mu.s0 is None:
+        raise ValueError("mu.s0 is None. The RegressionOutput must contain s_0 "
+                         "(zero-th kernel moment).")
+    from statistics import NormalDist
+    z = NormalDist().inv_cdf(1 - alpha / 2)
+    dev = isinstance(mu.s0, torch.Tensor)
+
+    def grid_of(r):
+        if r.fpp_yhat is not None and hasattr(r.fpp_yhat, "device_values"):
+            v = r.fpp_yhat.device_values()
+            return v if dev else v.cpu().numpy()
+        return r.yhat
+
+    mu_values = grid_of(mu)
+    if isinstance(sigma, RegressionOutput):
+        sigma_values = grid_of(sigma)
+    elif dev:
+        sigma_values = torch.as_tensor(sigma, dtype=torch.float64, device=mu.s0.device)
+    else:
+        sigma_values = np.asarray(sigma, dtype=float)
+    xp = torch if dev else np
+    if dev:
+        sigma_pos = torch.clamp_min(sigma_values, 0.0)
+        s0 = torch.clamp_min(mu.s0, 1e-10)
+    else:
+        sigma_pos = np.maximum(sigma_values, 0)
+        s0 = np.maximum(mu.s0, 1e-10)
+    if tuple(sigma_pos.shape) != tuple(s0.shape):
+        raise ValueError(f"Shape mismatch: sigma has shape {tuple(sigma_pos.shape)} but s0 has "
+                         f"shape {tuple(s0.shape)}. Ensure mu and sigma use the same grid.")
+    if tuple(mu_values.shape) != tuple(s0.shape):
+        raise ValueError(f"Shape mismatch: mu has shape {tuple(mu_values.shape)} but s0 has "
+                         f"shape {tuple(s0.shape)}. Ensure mu grid values and s0 are aligned.")
+    d = len(mu.grid)
+    ell = int(mu.metadata["order"]) if mu.metadata else 0
+    prod_h = float(np.prod(np.atleast_1d(np.asarray(mu.h, dtype=float))))
+    se_sq = sigma_pos * _nu(d, ell) / (prod_h * s0)
+    if kind == "confidence":
+        half = z * xp.sqrt(se_sq)
+    else:
+        half = z * xp.sqrt(sigma_pos + se_sq)
+    return xp.stack([mu_values + half, mu_values - half], -1)

@@ -752,3 +752,26 @@ def cv_fastkde(x, h=None, options=None, mode="ref") -> KDEFit:
     lcv = dict(hlist=bw, lcv_m=scores, lcv_sd=errs, idmax=idmax, id1se=sel,
                hmax=bw[idmax], h1se=bw[sel])
     return KDEFit(dens[sel], tuple(axes_raw), bw[sel], lcv, np.stack(dens, axis=-1))
+
+
+# ----------------------------------------------------------------------------
+# prediction and intervals
+# ----------------------------------------------------------------------------
+
+NU_TABLE = {  # confidence.py:33-43 (e1' M_K^-1 M_K2 M_K^-1 e1, product Gaussian kernel)
+    (1, 0): 0.282094791774, (1, 1): 0.282094791774, (1, 2): 0.476034961118,
+    (2, 0): 0.079577471546, (2, 1): 0.079577471546, (2, 2): 0.198943678865,
+    (3, 0): 0.022448390266, (3, 1): 0.022448390266, (3, 2): 0.077166341538,
+}
+
+
+def interval(mu_grid, sigma_grid, s0, h, order, alpha=0.05, kind="confidence"):
+    """fastlpr_interval (confidence.py:60-216): [upper, lower] on the grid."""
+    from scipy.stats import norm
+    z = norm.ppf(1 - alpha / 2)
+    sigma_pos = np.maximum(np.asarray(sigma_grid, float), 0)
+    s0 = np.maximum(np.asarray(s0, float), 1e-10)
+    nu = NU_TABLE[(s0.ndim, int(order))]
+    se_sq = sigma_pos * nu / (float(np.prod(np.atleast_1d(h))) * s0)
+    half = z * np.sqrt(se_sq) if kind == "confidence" else z * np.sqrt(sigma_pos + se_sq)
+    return np.stack([mu_grid + half, mu_grid - half], axis=-1)

@@ -111,3 +111,20 @@ def test_cubic_tables_match_scipy_regular_grid_interpolator():
         np.testing.assert_allclose(mine, exact, rtol=1e-12, atol=1e-12)
         loose = RegularGridInterpolator((ax,), y, method="cubic")(pts[:, None])
         assert np.max(np.abs(mine - loose)) < 1e-4 * np.max(np.abs(y))
+
+
+def test_interval_nu_constants():
+    """nu(d, order) from the Gaussian kernel moments equals the closed forms
+    and the reference's table (confidence.py:33-43)."""
+    import math
+    from fastlpr_amd import api as A
+    ref = {(1, 0): 0.282094791774, (1, 1): 0.282094791774, (1, 2): 0.476034961118,
+           (2, 0): 0.079577471546, (2, 1): 0.079577471546, (2, 2): 0.198943678865,
+           (3, 0): 0.022448390266, (3, 1): 0.022448390266, (3, 2): 0.077166341538}
+    for (d, o), v in ref.items():
+        assert abs(A._nu(d, o) - v) < 1e-11, (d, o)
+    assert abs(A._nu(1, 0) - 1 / (2 * math.sqrt(math.pi))) < 1e-15
+    assert abs(A._nu(1, 2) - 27 / (32 * math.sqrt(math.pi))) < 1e-15
+    assert abs(A._nu(3, 1) - 1 / (8 * math.pi ** 1.5)) < 1e-15
+    with pytest.raises(ValueError):
+        A._nu(4, 1)
