@@ -427,9 +427,14 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
         rm = opts.get("random_matrix", None)
         probes, _pp, pp_dev = _probes(n, K, opts.get("dof_seed", 42), rm, dev)
         K = probes.shape[1]
-        fit = engine.run_grid(xd, yd, ny, probes, K, bandwidths, order, N, int(opts["accuracy"]),
+        # S and the probes are spread at the sample-size accuracy whatever the
+        # option says (api.py:569-576 calls compute_design_matrix without one,
+        # regression.py:86-87, 756-758); only y takes the option's (api.py:632-642)
+        acc_n = max(6 - int(np.ceil(np.log10(n))), 4)
+        fit = engine.run_grid(xd, yd, ny, probes, K, bandwidths, order, N, acc_n,
                               fp64, reg, xrange, flag_power2, want_rss=True, stats=stats,
-                              readback=False, keep_all=cubic)
+                              readback=False, keep_all=cubic,
+                              accuracy_y=int(opts["accuracy"]))
         ysq_dev = stats.ysq_dev
         if cubic:
             fit, ysq_dev = _point_sums(fit, xd, yd, probes, ny, K, gcv_method, dof_method)

@@ -1189,15 +1189,22 @@ template <int D>
 void launch_spread(int msp, dim3 grd, size_t lds, hipStream_t st, const FlprGeom& g,
                    const SpreadTab& tab, const int32_t* ks, const double* wab, const double* vals,
                    int64_t n, int wo, int C, int cg, int nsplit, double* part) {
-  if (msp == 4)
-    hipLaunchKernelGGL((k_spread_cells<D, 4>), grd, dim3(SPREAD_THREADS), lds, st, g, tab, ks, wab,
-                       vals, n, wo, C, cg, nsplit, part);
-  else if (msp == 5)
-    hipLaunchKernelGGL((k_spread_cells<D, 5>), grd, dim3(SPREAD_THREADS), lds, st, g, tab, ks, wab,
-                       vals, n, wo, C, cg, nsplit, part);
-  else
-    hipLaunchKernelGGL((k_spread_cells<D, 6>), grd, dim3(SPREAD_THREADS), lds, st, g, tab, ks, wab,
-                       vals, n, wo, C, cg, nsplit, part);
+#define FLPR_SPREAD_CASE(M)                                                                    \
+  case M:                                                                                      \
+    hipLaunchKernelGGL((k_spread_cells<D, M>), grd, dim3(SPREAD_THREADS), lds, st, g, tab, ks, \
+                       wab, vals, n, wo, C, cg, nsplit, part);                                 \
+    break;
+  switch (msp) {
+    FLPR_SPREAD_CASE(0)
+    FLPR_SPREAD_CASE(1)
+    FLPR_SPREAD_CASE(2)
+    FLPR_SPREAD_CASE(3)
+    FLPR_SPREAD_CASE(4)
+    FLPR_SPREAD_CASE(5)
+    default:
+      FLPR_SPREAD_CASE(6)
+  }
+#undef FLPR_SPREAD_CASE
 }
 
 // Points per work item the spread and adjoint aim for when a cell holds many
@@ -1321,7 +1328,7 @@ int flpr_spread(const FlprGeom* g, const int32_t* key_start, const double* wab,
   if (!flpr_geom_ok(g) || !key_start || !wab || !part || !grid || !sums || n < 1 || nval < 0 ||
       (nval > 0 && !vals))
     return -1;
-  if (g->msp < 4 || g->msp > 6) return -1;
+  if (g->msp < 0 || g->msp > 6) return -1;
   const int wo = with_ones ? 1 : 0;
   const int C = wo + nval;
   if (C < 1 || C > FLPR_MAX_CHANNELS) return -1;

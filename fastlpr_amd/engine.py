@@ -344,7 +344,7 @@ class Geometry:
     geom: FlprGeom
     N: Tuple[int, ...]
     L: Tuple[int, ...]
-    msp: int
+    msp: int                         # spread footprint half-width (cells)
     xmin_s: np.ndarray
     xmax_s: np.ndarray
     axes_raw: List[np.ndarray]
@@ -352,21 +352,29 @@ class Geometry:
     mean: np.ndarray
     std: np.ndarray
     centre: np.ndarray
+    margin: int = 0                  # grid margin beyond the nodes (>= msp)
 
 
 def build_geometry(st: Stats, N: Tuple[int, ...], accuracy: int, xrange=None,
-                   flag_power2: bool = True, kde: bool = False) -> Geometry:
+                   flag_power2: bool = True, kde: bool = False,
+                   margin: Optional[int] = None) -> Geometry:
     """Knot-mapping constants exactly as the reference computes them in float64
     (model.py:58-72, convolution.py:169-199, nufftn_type1.py:140-176,
     kernel.py:97-156); the extreme points give the min / max of every
-    monotone float64 step, so no extra pass over the data is needed."""
+    monotone float64 step, so no extra pass over the data is needed.
+
+    margin: cells of spread grid kept beyond the nodes on each side (at least
+    the footprint half-width); two geometries with the same margin share one
+    grid, so channels spread at different accuracies can be convolved
+    together."""
     d = len(N)
     if accuracy > 6:
         raise NotImplementedError("accuracy > 6 (oversampled NUFFT with deconvolution) is not "
                                   "supported on the GPU path")
     msp = int(accuracy)
-    if msp < 4:
-        raise NotImplementedError("accuracy < 4 is not supported on the GPU path")
+    if msp < 0:
+        raise ValueError("accuracy must be >= 0")
+    margin = msp if margin is None else max(msp, int(margin))
     L = padded_sizes(N, flag_power2)
     g = FlprGeom()
     g.d = d
@@ -395,7 +403,7 @@ def build_geometry(st: Stats, N: Tuple[int, ...], accuracy: int, xrange=None,
         kmin = (kmin_pt + kmax_pt) * 0.5 - 0.5
         shift = -0.5 - kmin
         tau = (math.pi * msp) / (L_k * L_k * 1 * (1 - 0.5))
-        E = n_k // 2 + 1 + msp
+        E = n_k // 2 + 1 + margin
         G = 2 * E + 1
         tile = TILE[d]
         g.n_grid[k] = n_k
@@ -405,7 +413,9 @@ def build_geometry(st: Stats, N: Tuple[int, ...], accuracy: int, xrange=None,
         g.tile[k] = tile
         g.ntile[k] = -(-G // tile)
         g.hx[k] = 2 * np.pi / L_k
-        g.inv4tau[k] = 1.0 / (4 * tau)
+        # accuracy 0 bins every point to its nearest cell (nufftn_type1.py:442-516):
+        # footprint weight exp(0) = 1
+        g.inv4tau[k] = 1.0 / (4 * tau) if msp > 0 else 0.0
         g.mean[k] = mean
         g.std_[k] = std
         g.centre[k] = c
@@ -432,7 +442,7 @@ def build_geometry(st: Stats, N: Tuple[int, ...], accuracy: int, xrange=None,
         axes_raw.append(ax_r)
         axes_scaled.append(ax_s)
     return Geometry(g, tuple(N), L, msp, xmin_s, xmax_s, axes_raw, axes_scaled,
-                    st.mean.copy(), st.std.copy(), centre)
+                    st.mean.copy(), st.std.copy(), centre, margin)
 
 
 def filter_bandwidths(geo: Geometry, h: np.ndarray, n: int):
@@ -522,15 +532,17 @@ def _memo_put(table: dict, key, value):
 
 
 def geometry_for(st: Stats, N, accuracy: int, xrange=None, flag_power2: bool = True,
-                 kde: bool = False):
+                 kde: bool = False, margin: Optional[int] = None):
     """build_geometry, memoised on the exact statistics and grid options.
     Returns (geometry, key)."""
     xr = None if xrange is None else tuple(tuple(float(v) for v in r) for r in xrange)
+    mg = int(accuracy) if margin is None else max(int(accuracy), int(margin))
     key = (st.xmin.tobytes(), st.xmax.tobytes(), st.mean.tobytes(), st.std.tobytes(),
-           tuple(int(v) for v in N), int(accuracy), xr, bool(flag_power2), bool(kde))
+           tuple(int(v) for v in N), int(accuracy), xr, bool(flag_power2), bool(kde), mg)
     geo = _GEOMS.get(key)
     if geo is None:
-        geo = _memo_put(_GEOMS, key, build_geometry(st, N, accuracy, xrange, flag_power2, kde))
+        geo = _memo_put(_GEOMS, key, build_geometry(st, N, accuracy, xrange, flag_power2, kde,
+                                                    mg))
     return geo, key
 
 
@@ -574,7 +586,7 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
              stats: Optional[Stats] = None, with_ones_eps: bool = True,
              debug: Optional[dict] = None, readback: bool = True,
              moments_only: bool = False, kde: bool = False,
-             keep_all: bool = False) -> DeviceFit:
+             keep_all: bool = False, accuracy_y: Optional[int] = None) -> DeviceFit:
     """Spread, convolve and solve for all bandwidths, plus the probe dot products and
     the residual sums.
 
@@ -592,12 +604,18 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     keep_all: return the grid solutions of every channel (y columns and probes,
     DeviceFit.m_y is (dh, ny + K, nodes)) instead of the adjoint sums, for
     callers that interpolate them at the points themselves (cubic).
+    accuracy_y: spread accuracy of the y columns when it differs from the one
+    of the ones channel and the probes (the reference's cv_fastlpr with DOF
+    estimation spreads S and the probes at the sample-size accuracy and only
+    y at the option's, api.py:569-576 / regression.py:86-87, 756-758).
     """
     n, d = x.shape
     dev = x.device
     s = stream_handle()
     st = stats if stats is not None else x_stats(x, vals_y)
-    geo, gkey = geometry_for(st, N, accuracy, xrange, flag_power2, kde)
+    split_acc = accuracy_y is not None and int(accuracy_y) != int(accuracy) and ny > 0
+    margin = max(int(accuracy), int(accuracy_y)) if split_acc else None
+    geo, gkey = geometry_for(st, N, accuracy, xrange, flag_power2, kde, margin)
     nval = ny + K
     C = 1 + nval
     g = geo.geom
@@ -624,7 +642,7 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
 
     # --- spread (float64 accumulation), centring, float32 copy
     cells = int(np.prod([g.G[k] for k in range(d)]))
-    fcells = int(np.prod([g.tile[k] + 2 * geo.msp for k in range(d)]))
+    fcells = int(np.prod([g.tile[k] + 2 * geo.margin for k in range(d)]))
     part = torch.empty((ntiles * C * fcells,), dtype=f64, device=dev)
     # convolution inputs are read in whole tiles past each job's end (the
     # values there are discarded or zeroed): pad them
@@ -633,6 +651,19 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     sums = torch.zeros((C,), dtype=f64, device=dev)
     call("flpr_spread", gp, ptr(key_start), ptr(wab), ptr(vals), n, 1, nval, ptr(part),
          ptr(grid64), ptr(sums), s)
+    if split_acc:
+        # the y columns again, with the footprint and width of their own
+        # accuracy, over the channels the first spread wrote
+        geo_y, _ = geometry_for(st, N, int(accuracy_y), xrange, flag_power2, kde, margin)
+        gy = _lib.C.byref(geo_y.geom)
+        wab_y = torch.empty((2 * d * n,), dtype=f64, device=dev)
+        vals_yp = torch.empty((ny * n,), dtype=f64, device=dev)
+        call("flpr_points_pack", gy, ptr(x), ptr(order_idx), n, ptr(vals_y), ny, None, 0,
+             ptr(wab_y), ptr(vals_yp), ptr(rawi), ptr(rawf), s)
+        sums[1:1 + ny].zero_()
+        call("flpr_spread", gy, ptr(key_start), ptr(wab_y), ptr(vals_yp), n, 0, ny, ptr(part),
+             ptr(grid64) + cells * 8, ptr(sums) + 8, s)
+        del wab_y, vals_yp
     del part
     # bandwidth tables on the host while the device spreads
     h, warn, groups, band_np = bandwidth_tables(geo, gkey, np.asarray(h, dtype=float), n, order,
@@ -644,7 +675,7 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     call("flpr_grid_center", ptr(grid64), C, cells, ptr(sums), ptr(shift), ptr(grid32), s)
 
     # --- taps and bands (uploaded from pinned host memory without a stream sync)
-    plan0 = get_plan(d, order, geo.N, geo.L, geo.msp, C, groups[0][1] - groups[0][0],
+    plan0 = get_plan(d, order, geo.N, geo.L, geo.margin, C, groups[0][1] - groups[0][0],
                      groups[0][2], dev)
     npow, nmax = plan0.npow, plan0.nmax
     per_tap = d * npow * nmax
@@ -666,7 +697,7 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     # intermediate buffers); every group feeds the one global key
     stage0s = []
     for (a, b, gf) in groups:
-        plan = get_plan(d, order, geo.N, geo.L, geo.msp, C, b - a, gf, dev)
+        plan = get_plan(d, order, geo.N, geo.L, geo.margin, C, b - a, gf, dev)
         fdt = f64 if gf else torch.float32
         src_grid = grid64 if gf else grid32
         tap = tap64 if gf else tap32

@@ -348,7 +348,11 @@ def spread(xmod: np.ndarray, values: np.ndarray, L: np.ndarray, acc: int, mode: 
     for d in range(dims):
         mj = m[:, d:d + 1] + taps[d][None, :]
         diff = xmod[:, d:d + 1] - hx[d] * mj
-        wd.append(diff * diff / (4 * tau[d]))
+        if acc == 0:
+            # accuracy 0: every point binned to its nearest cell (nufftn_type1.py:442-516)
+            wd.append(np.zeros_like(diff))
+        else:
+            wd.append(diff * diff / (4 * tau[d]))
         idxd.append(np.mod(mj, Mr[d]))
     # build the full tensor-product neighbourhood
     lin = np.zeros((n, 1), dtype=np.int64)
@@ -365,7 +369,8 @@ def spread(xmod: np.ndarray, values: np.ndarray, L: np.ndarray, acc: int, mode: 
         if is_c:
             acc_im[:, c] = np.bincount(flat, weights=(w * v.imag[:, None]).ravel(), minlength=size)
     grid = acc_re + 1j * acc_im
-    cdt = np.complex64 if (mode == "ref" and acc < 5) else np.complex128
+    # the binning path (acc 0) keeps its complex128 grid (nufftn_type1.py:474-488)
+    cdt = np.complex64 if (mode == "ref" and 0 < acc < 5) else np.complex128
     return grid.astype(cdt).reshape(tuple(Mr) + (C,)), (Msp, Mr, tau, R)
 
 
@@ -568,24 +573,25 @@ class Problem:
         self.kdf = [kernel_spectrum(k, self.L, self.N) for k in kd]
         self.xmod = knot_positions(xs, self.N, self.L)
 
-    def data_spectrum(self, values):
+    def data_spectrum(self, values, acc=None):
         vals = np.asarray(values)
         if vals.ndim == 1:
             vals = vals[:, None]
         vals = vals.astype(np.complex128)
-        return nufft_type1(self.xmod, vals, self.L, self.acc, self.mode)
+        return nufft_type1(self.xmod, vals, self.L, self.acc if acc is None else acc, self.mode)
 
-    def design(self, add_eps=True):
-        """S entries (regression.py:31-159)."""
-        ones_ft = self.data_spectrum(np.ones(self.n))
+    def design(self, add_eps=True, acc=None):
+        """S entries (regression.py:31-159) at accuracy acc (default: the
+        option's)."""
+        ones_ft = self.data_spectrum(np.ones(self.n), acc)
         S = [convolve(ones_ft, k, self.N, self.qout0, True)[..., 0] for k in self.kdf]
         if self.order == 0 and add_eps:
             S[0] = S[0] + np.finfo(float).eps
         return S
 
-    def weighted(self, values, real):
+    def weighted(self, values, real, acc=None):
         """T entries for the first nt kernels (regression.py:1497-1533)."""
-        y_ft = self.data_spectrum(values)
+        y_ft = self.data_spectrum(values, acc)
         return [convolve(y_ft, self.kdf[i], self.N, self.qout0, real) for i in range(self.nt)]
 
 
@@ -634,7 +640,13 @@ def cv_fastlpr(x, y, h=None, options=None, mode="ref", return_trace=False) -> Fi
         return Fit(yh, tuple(P.axes_raw), P.h[0], gcv, gv, m_all, np.real(S[0][..., 0]), d,
                    np.nan, np.nan, trace)
 
-    S = P.design(add_eps=True)
+    # With DOF estimation, S and the probe responses use the sample-size
+    # accuracy whatever the option says: api.py:569-576 calls
+    # compute_design_matrix without an accuracy (regression.py:86-87) and the
+    # probe regression fixes it (regression.py:756-758); only the response
+    # T (api.py:632-642) takes the option's accuracy.
+    acc_n = accuracy_for(n)
+    S = P.design(add_eps=True, acc=acc_n)
     # DOF (regression.py:1095-1354)
     K = int(opts.get("num_dof_sample", 10))
     rm = opts.get("random_matrix", None)
@@ -648,7 +660,7 @@ def cv_fastlpr(x, y, h=None, options=None, mode="ref", return_trace=False) -> Fi
     else:
         dflt = "cubic"
     dof_method = opts.get("dof_interp_method", dflt)
-    Tp = P.weighted(p, True)
+    Tp = P.weighted(p, True, acc=acc_n)
     mp = solve(S, Tp, order, reg, P.nt, mode)           # (grid..., dh, K)
     phat = interp_dof(P.axes_raw, mp, x_arr, dof_method, mode)  # (n, dh, K)
     phat = np.real(np.transpose(phat, (0, 2, 1)))

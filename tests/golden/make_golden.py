@@ -88,7 +88,7 @@ def main(ref_src: str, only=None):
     def wanted(name):
         return not only or name in only
 
-    for name in recipes.LPR_SMALL + recipes.LPR_CUBIC:
+    for name in recipes.LPR_SMALL + recipes.LPR_CUBIC + recipes.LPR_ACC:
         if not wanted(name):
             continue
         out = run_lpr(fl, name)

@@ -40,6 +40,15 @@ def make(name: str):
         s = 0.05 + 0.3 * x[:, 0]
         r2 = (s * rng.standard_normal(n)) ** 2
         return x, r2, _logspace_h(10, 0.05, 1.0)[:, None], {"order": 1, "N": 64, "y_type_out": "variance"}
+    if name.startswith("acc"):
+        # acc{a}_{d}d: accuracy a (0 = nearest-cell binning), order 1
+        a, dd = int(name[3]), int(name.split("_")[1][0])
+        rng = np.random.default_rng(50 + a + 10 * dd)
+        n = 3000 if dd == 1 else 6000
+        x = rng.uniform(0, 1, (n, dd))
+        y = np.sin(2 * np.pi * x[:, 0]) + 0.2 * rng.standard_normal(n)
+        N = 100 if dd == 1 else [30, 33]
+        return x, y, _iso(_logspace_h(6, 0.04, 0.6), dd), {"order": 1, "N": N, "accuracy": a}
     if name == "lpr2d_o1_cubic":
         # fewer than 5000 points in 2-D: the reference interpolates GCV / DOF cubically
         rng = np.random.default_rng(22)
@@ -93,5 +102,6 @@ LPR_SMALL = [
     "lpr3d_o1",
 ]
 LPR_CUBIC = ["lpr2d_o1_cubic", "lpr3d_o1_cubic"]
+LPR_ACC = ["acc0_1d", "acc2_1d", "acc0_2d", "acc3_2d"]
 LPR_LARGE = ["cfg1_1d_o1_n1e5"]
 KDE = ["kde1d", "kde2d", "kde3d"]

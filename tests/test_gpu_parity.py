@@ -42,6 +42,12 @@ def _cases():
     out["1d_cplx_o1"] = (x1, (0.5 + 2j) * y1, h1, {"order": 1, "N": 64})
     r2 = (0.05 + 0.3 * x1[:, 0]) ** 2 * rng.standard_normal(n) ** 2
     out["1d_var_o1"] = (x1, r2, h1, {"order": 1, "N": 64, "y_type_out": "variance"})
+    # explicit accuracies: S and the probes stay at the sample-size accuracy
+    # (4 here), y is spread at the option's (0 = nearest-cell binning)
+    for a in (0, 2, 6):
+        out[f"1d_o1_acc{a}"] = (x1, y1, h1, {"order": 1, "N": 128, "accuracy": a})
+    out["1d_o0_acc1_nodof"] = (x1, y1, h1, {"order": 0, "N": 101, "accuracy": 1,
+                                            "calc_dof": False})
     out["1d_multi_o1"] = (x1, np.column_stack([y1, np.cos(3 * x1[:, 0])]), h1,
                           {"order": 1, "N": 64, "gcv_selection": "average"})
     n2 = 8000
@@ -53,6 +59,9 @@ def _cases():
         out[f"2d_o{o}_m40"] = (x2, y2, h2, {"order": o, "N": [40, 40]})
     out["2d_o2_m33x40"] = (x2, y2, h2, {"order": 2, "N": [33, 40]})
     out["2d_o1_nodof"] = (x2, y2, h2, {"order": 1, "N": [40, 40], "calc_dof": False})
+    out["2d_o2_acc0"] = (x2, y2, h2, {"order": 2, "N": [40, 40], "accuracy": 0})
+    out["2d_o1_acc3_nodof"] = (x2, y2, h2, {"order": 1, "N": [40, 40], "accuracy": 3,
+                                            "calc_dof": False})
     # fewer than 5000 points in 2-D / 3-D: GCV and DOF interpolate cubically
     n4 = 1500
     x4 = rng.uniform(0, 1, (n4, 2))
@@ -122,7 +131,7 @@ def test_fp32_path_within_1e5_of_fp64_oracle(fl, name):
         assert _rel(r.gcv_yhat["gcv_m"], o.gcv["gcv_m"]) < 1e-5
 
 
-@pytest.mark.parametrize("name", recipes.LPR_SMALL + recipes.LPR_CUBIC)
+@pytest.mark.parametrize("name", recipes.LPR_SMALL + recipes.LPR_CUBIC + recipes.LPR_ACC)
 def test_gpu_matches_reference_golden(fl, name):
     g = np.load(os.path.join(GOLDEN, f"{name}.npz"), allow_pickle=False)
     x, y, h, opt = recipes.make(name)

@@ -28,7 +28,7 @@ def _relmax(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
-@pytest.mark.parametrize("name", recipes.LPR_SMALL + recipes.LPR_CUBIC)
+@pytest.mark.parametrize("name", recipes.LPR_SMALL + recipes.LPR_CUBIC + recipes.LPR_ACC)
 def test_oracle_lpr_matches_reference(name):
     g = _load(name)
     x, y, h, opt = recipes.make(name)
