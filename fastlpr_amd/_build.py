@@ -6,7 +6,8 @@ import shutil
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SOURCES = ["csrc/flpr_points.hip", "csrc/flpr_conv.hip", "csrc/flpr_select.hip"]
+SOURCES = ["csrc/flpr_points.hip", "csrc/flpr_conv.hip", "csrc/flpr_select.hip",
+           "csrc/flpr_resample.hip"]
 OUT = os.path.join(HERE, "libfastlpr_amd.so")
 
 FLAGS = [

@@ -20,6 +20,8 @@ EXPORTS = (
     "flpr_points_bin",
     "flpr_points_pack",
     "flpr_spread",
+    "flpr_resample_work",
+    "flpr_resample",
     "flpr_grid_center",
     "flpr_taps",
     "flpr_conv",
@@ -80,11 +82,13 @@ _SIGS = {
     "flpr_points_bin": [_P, _I64, _I32, _P, _P, _P, _P, _P],
     "flpr_points_pack": [_GP, _P, _P, _I64, _P, _I32, _P, _I32, _P, _P, _P, _P, _P],
     "flpr_spread": [_GP, _P, _P, _P, _I64, _I32, _I32, _P, _P, _P, _P],
+    "flpr_resample_work": [_GP, _GP, _I32, _I32, _P],
+    "flpr_resample": [_GP, _GP, _P, _I32, _I32, _P, _P, _P, _P],
     "flpr_grid_center": [_P, _I32, _I64, _P, _P, _P, _P],
     "flpr_taps": [_GP, _P, _I32, _I32, _I32, _D, _P, _P, _P],
     "flpr_conv": [_I32, _I32, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P],
     "flpr_max_moment": [_I32, _P, _I64, _I32, C.POINTER(_I64), _I32, _I64, _I32, _P, _P],
-    "flpr_solve": [_I32, _P, _I64, _I32, _I64, _I32, _I32, C.POINTER(_I64), C.POINTER(_I64), _P,
+    "flpr_solve": [_I32, _P, _I64, _I32, _I64, _I32, _I32, _I32, C.POINTER(_I64), C.POINTER(_I64), _P,
                    _D, _D, _P, _I32, _I32, _P, _P, _P, _P],
     "flpr_adjoint": [_GP, _P, _P, _P, _P, _I64, _I32, _P, _P, _P, _P],
     "flpr_quadform": [_GP, _P, _I32, _I32, _P, _P, _P],

@@ -405,7 +405,8 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
         if method != "linear":
             raise NotImplementedError("only linear interpolation is implemented on the GPU")
         fit = engine.run_grid(xd, yd, ny, None, 0, bandwidths, order, N, int(opts["accuracy"]),
-                              fp64, reg, xrange, flag_power2, stats=stats, with_ones_eps=False)
+                              fp64, reg, xrange, flag_power2, stats=stats, with_ones_eps=False,
+                              y_pairs=y_is_complex)
         sel = 0
         sel_dev = sel_out = None
     else:
@@ -434,7 +435,7 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
         fit = engine.run_grid(xd, yd, ny, probes, K, bandwidths, order, N, acc_n,
                               fp64, reg, xrange, flag_power2, want_rss=True, stats=stats,
                               readback=False, keep_all=cubic,
-                              accuracy_y=int(opts["accuracy"]))
+                              accuracy_y=int(opts["accuracy"]), y_pairs=y_is_complex)
         ysq_dev = stats.ysq_dev
         if cubic:
             fit, ysq_dev = _point_sums(fit, xd, yd, probes, ny, K, gcv_method, dof_method)

@@ -33,7 +33,7 @@ static inline int flpr_grid_for(int64_t n, int threads, int cap) {
 }
 
 static inline bool flpr_geom_ok(const FlprGeom* g) {
-  if (!g || g->d < 1 || g->d > FLPR_MAX_DIM || g->msp < 0 || g->msp > 6) return false;
+  if (!g || g->d < 1 || g->d > FLPR_MAX_DIM || g->msp < 0 || g->msp > FLPR_MAX_MSP) return false;
   for (int k = 0; k < g->d; ++k)
     if (g->n_grid[k] < 2 || g->tile[k] < 1 || g->ntile[k] < 1 || g->G[k] != 2 * g->E[k] + 1 ||
         (int64_t)g->ntile[k] * g->tile[k] < g->G[k] || g->L[k] < 2)

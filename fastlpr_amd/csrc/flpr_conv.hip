@@ -413,15 +413,20 @@ constexpr int MAXV = 32;  // value channels reduced per solve launch
 
 // One thread per (node, bandwidth); block row blockIdx.y = bandwidth.
 // Order >= 1: Cholesky of S + lam I in float64 (symmetric positive definite
-// for lam > 0), z = (S + lam I)^-1 e1, and m_c = z . T_c for every value
-// channel (the intercept e1' S^-1 T that the reference's Cramer formulas
-// evaluate).  Order 0: m_c = T_c / max(S + eps, thr).  Channels were centred on
+// for lam > 0 when the spread weights are positive), z = (S + lam I)^-1 e1,
+// and m_c = z . T_c for every value channel (the intercept e1' S^-1 T that the
+// reference's Cramer formulas evaluate).  The deconvolved grids of accuracy
+// > 6 can make S indefinite at a few boundary nodes: with PIV, a node whose
+// Cholesky meets a non-positive pivot is solved again by Gaussian elimination
+// with partial pivoting (LAPACK gesv's method); without PIV such a node gives
+// NaN (a separate instantiation keeps the fallback's registers out of the
+// positive-definite case).  Order 0: m_c = T_c / max(S + eps, thr).  Channels were centred on
 // the grid by shift_c; since T_c(y) = T_c(y - shift_c) + shift_c T_0, the
 // solve adds shift_c z . T_0 (shift_c S_00 for order 0) back exactly.
 // The first nout value channels are written to m_out; for channels with an
 // adjoint row in q, dot[h][c] += sum_node q_c(node) m_c(node) is reduced in
 // registers over the block's nodes and then once per block.
-template <typename T, int NT>
+template <typename T, int NT, bool PIV>
 __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(
     const T* __restrict__ mom, int64_t per_h, int64_t nodes, SolveTab tab,
     const uint64_t* __restrict__ key, double lam_scale, double eps_add,
@@ -459,11 +464,13 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(
         }
 #pragma unroll
       for (int i = 0; i < NT; ++i) A[i][i] += lam;
+      bool indefinite = false;
 #pragma unroll
       for (int j = 0; j < NT; ++j) {
         double dsum = A[j][j];
 #pragma unroll
         for (int k = 0; k < j; ++k) dsum -= A[j][k] * A[j][k];
+        indefinite |= !(dsum > 0.0);
         const double ljj = sqrt(dsum);
         A[j][j] = ljj;
         const double inv = 1.0 / ljj;
@@ -489,6 +496,63 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(
 #pragma unroll
         for (int k = i + 1; k < NT; ++k) s -= A[k][i] * z[k];
         z[i] = s / A[i][i];
+      }
+      if (PIV && indefinite) {
+        // reload S + lam I; rows are exchanged with selects so that every
+        // register index stays static
+        p = 0;
+#pragma unroll
+        for (int i = 0; i < NT; ++i)
+#pragma unroll
+          for (int j = i; j < NT; ++j) {
+            const double v = (double)b[tab.s_off[p] + node] + (i == j ? lam : 0.0);
+            A[i][j] = v;
+            A[j][i] = v;
+            ++p;
+          }
+#pragma unroll
+        for (int i = 0; i < NT; ++i) y[i] = (i == 0) ? 1.0 : 0.0;
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          int piv = j;
+          double best = fabs(A[j][j]);
+#pragma unroll
+          for (int i = j + 1; i < NT; ++i) {
+            const double a = fabs(A[i][j]);
+            if (a > best) {
+              best = a;
+              piv = i;
+            }
+          }
+#pragma unroll
+          for (int i = j + 1; i < NT; ++i) {
+            const bool sw = (i == piv);
+#pragma unroll
+            for (int k = j; k < NT; ++k) {
+              const double u = A[j][k], v = A[i][k];
+              A[j][k] = sw ? v : u;
+              A[i][k] = sw ? u : v;
+            }
+            const double u = y[j], v = y[i];
+            y[j] = sw ? v : u;
+            y[i] = sw ? u : v;
+          }
+          const double inv = 1.0 / A[j][j];
+#pragma unroll
+          for (int i = j + 1; i < NT; ++i) {
+            const double f = A[i][j] * inv;
+#pragma unroll
+            for (int k = j + 1; k < NT; ++k) A[i][k] -= f * A[j][k];
+            y[i] -= f * y[j];
+          }
+        }
+#pragma unroll
+        for (int i = NT - 1; i >= 0; --i) {
+          double s = y[i];
+#pragma unroll
+          for (int k = i + 1; k < NT; ++k) s -= A[i][k] * z[k];
+          z[i] = s / A[i][i];
+        }
       }
       // z . T_0 (ones channel) for the centring correction
 #pragma unroll
@@ -530,23 +594,28 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(
 }
 
 template <typename T, int NT>
-void launch_solve(hipStream_t st, dim3 grd, const T* mom, int64_t per_h, int64_t nodes,
-                  const SolveTab& tab, const uint64_t* key, double lam_scale, double eps_add,
-                  const double* shift, int nval, int nout, double* m_out, const double* q,
-                  double* dot) {
-  hipLaunchKernelGGL((k_solve<T, NT>), grd, dim3(SOLVE_THREADS), 0, st, mom, per_h, nodes, tab,
-                     key, lam_scale, eps_add, shift, nval, nout, m_out, q, dot);
+void launch_solve(hipStream_t st, dim3 grd, bool piv, const T* mom, int64_t per_h,
+                  int64_t nodes, const SolveTab& tab, const uint64_t* key, double lam_scale,
+                  double eps_add, const double* shift, int nval, int nout, double* m_out,
+                  const double* q, double* dot) {
+  if (piv && NT > 1)
+    hipLaunchKernelGGL((k_solve<T, NT, true>), grd, dim3(SOLVE_THREADS), 0, st, mom, per_h,
+                       nodes, tab, key, lam_scale, eps_add, shift, nval, nout, m_out, q, dot);
+  else
+    hipLaunchKernelGGL((k_solve<T, NT, false>), grd, dim3(SOLVE_THREADS), 0, st, mom, per_h,
+                       nodes, tab, key, lam_scale, eps_add, shift, nval, nout, m_out, q, dot);
 }
 
 template <typename T>
-int dispatch_solve(int nt, hipStream_t st, dim3 grd, const T* mom, int64_t per_h, int64_t nodes,
+int dispatch_solve(int nt, hipStream_t st, dim3 grd, bool piv, const T* mom, int64_t per_h,
+                   int64_t nodes,
                    const SolveTab& tab, const uint64_t* key, double lam_scale, double eps_add,
                    const double* shift, int nval, int nout, double* m_out, const double* q,
                    double* dot) {
 #define FLPR_SOLVE_CASE(N)                                                                   \
   case N:                                                                                    \
-    launch_solve<T, N>(st, grd, mom, per_h, nodes, tab, key, lam_scale, eps_add, shift, nval, \
-                       nout, m_out, q, dot);                                                 \
+    launch_solve<T, N>(st, grd, piv, mom, per_h, nodes, tab, key, lam_scale, eps_add, shift, \
+                       nval, nout, m_out, q, dot);                                           \
     return 0;
   switch (nt) {
     FLPR_SOLVE_CASE(1)
@@ -634,10 +703,10 @@ int flpr_max_moment(int32_t fp64, const void* src, int64_t per_h, int32_t dh, co
 double flpr_key_to_double(uint64_t key) { return flpr_decode(key); }
 
 int flpr_solve(int32_t fp64, const void* mom, int64_t per_h, int32_t dh, int64_t nodes,
-               int32_t order, int32_t nt, const int64_t* s_off, const int64_t* t_off,
-               const uint64_t* key, double lam_scale, double eps_add, const double* shift,
-               int32_t nval, int32_t nout, double* m_out, const double* q, double* dot,
-               void* stream) {
+               int32_t order, int32_t nt, int32_t pivot, const int64_t* s_off,
+               const int64_t* t_off, const uint64_t* key, double lam_scale, double eps_add,
+               const double* shift, int32_t nval, int32_t nout, double* m_out, const double* q,
+               double* dot, void* stream) {
   FLPR_CLEAR_ERROR();
   if (!mom || !s_off || !t_off || !key || dh < 1 || dh > 65535 || nodes < 1 || nt < 1 ||
       nt > FLPR_MAX_TERMS || nval < 0 || nval > MAXV || nout < 0 || nout > nval)
@@ -654,10 +723,10 @@ int flpr_solve(int32_t fp64, const void* mom, int64_t per_h, int32_t dh, int64_t
   hipStream_t st = as_stream(stream);
   int rc;
   if (fp64)
-    rc = dispatch_solve<double>(nt, st, grd, (const double*)mom, per_h, nodes, tab, key,
+    rc = dispatch_solve<double>(nt, st, grd, pivot != 0, (const double*)mom, per_h, nodes, tab, key,
                                 lam_scale, eps_add, shift, nval, nout, m_out, q, dot);
   else
-    rc = dispatch_solve<float>(nt, st, grd, (const float*)mom, per_h, nodes, tab, key,
+    rc = dispatch_solve<float>(nt, st, grd, pivot != 0, (const float*)mom, per_h, nodes, tab, key,
                                lam_scale, eps_add, shift, nval, nout, m_out, q, dot);
   if (rc) return rc;
   FLPR_CHECK_LAUNCH();

@@ -365,11 +365,13 @@ __global__ __launch_bounds__(256) void k_pack(FlprGeom g, const double* __restri
       const double c = g.inv4tau[k];
       wab[(p * D + k) * 2] = exp(-((delta * delta + 2.0 * g.msp * g.hx[k] * delta) * c));
       wab[(p * D + k) * 2 + 1] = exp(2.0 * delta * g.hx[k] * c);
-      int ri;
-      double rf;
-      raw_locate(xv, g.graw0[k], g.graw1[k], g.n_grid[k], &ri, &rf);
-      rawi[p * D + k] = ri;
-      rawf[p * D + k] = rf;
+      if (rawi) {
+        int ri;
+        double rf;
+        raw_locate(xv, g.graw0[k], g.graw1[k], g.n_grid[k], &ri, &rf);
+        rawi[p * D + k] = ri;
+        rawf[p * D + k] = rf;
+      }
     }
   }
   // values: one thread per (point, column), so that consecutive lanes write
@@ -414,7 +416,7 @@ __device__ inline void tap_weights(double A, double B, const double* G, double* 
 }
 
 constexpr int SPREAD_THREADS = 256;
-constexpr int SPREAD_MAX_TAPS = 13;
+constexpr int SPREAD_MAX_TAPS = 2 * FLPR_MAX_MSP + 1;
 
 // G[k][j] = exp(-hx_k^2 (j - MSP)^2 / (4 tau_k)); passed by value so that the
 // kernels read it through the scalar unit.
@@ -1201,8 +1203,21 @@ void launch_spread(int msp, dim3 grd, size_t lds, hipStream_t st, const FlprGeom
     FLPR_SPREAD_CASE(3)
     FLPR_SPREAD_CASE(4)
     FLPR_SPREAD_CASE(5)
+    FLPR_SPREAD_CASE(6)
     default:
-      FLPR_SPREAD_CASE(6)
+      break;
+  }
+  if constexpr (D < 3) {
+    switch (msp) {
+      FLPR_SPREAD_CASE(7)
+      FLPR_SPREAD_CASE(8)
+      FLPR_SPREAD_CASE(9)
+      FLPR_SPREAD_CASE(10)
+      FLPR_SPREAD_CASE(11)
+      FLPR_SPREAD_CASE(12)
+      default:
+        break;
+    }
   }
 #undef FLPR_SPREAD_CASE
 }
@@ -1306,7 +1321,7 @@ int flpr_points_pack(const FlprGeom* g, const double* x, const int32_t* order, i
                      const double* vals_y, int32_t ny, const double* probes, int32_t nprobe,
                      double* wab, double* vals, int32_t* rawi, double* rawf, void* stream) {
   FLPR_CLEAR_ERROR();
-  if (!flpr_geom_ok(g) || !x || !order || !wab || !rawi || !rawf || n < 1 ||
+  if (!flpr_geom_ok(g) || !x || !order || !wab || (!rawi != !rawf) || n < 1 ||
       ny < 0 || nprobe < 0 || ((ny + nprobe) > 0 && !vals) || (ny > 0 && !vals_y) ||
       (nprobe > 0 && !probes))
     return -1;
@@ -1328,7 +1343,9 @@ int flpr_spread(const FlprGeom* g, const int32_t* key_start, const double* wab,
   if (!flpr_geom_ok(g) || !key_start || !wab || !part || !grid || !sums || n < 1 || nval < 0 ||
       (nval > 0 && !vals))
     return -1;
-  if (g->msp < 0 || g->msp > 6) return -1;
+  // footprints wider than 13 cells only in 1-D and 2-D (the 3-D window would
+  // not fit the LDS)
+  if (g->msp < 0 || g->msp > (g->d == 3 ? 6 : FLPR_MAX_MSP)) return -1;
   const int wo = with_ones ? 1 : 0;
   const int C = wo + nval;
   if (C < 1 || C > FLPR_MAX_CHANNELS) return -1;

@@ -369,8 +369,8 @@ def build_geometry(st: Stats, N: Tuple[int, ...], accuracy: int, xrange=None,
     together."""
     d = len(N)
     if accuracy > 6:
-        raise NotImplementedError("accuracy > 6 (oversampled NUFFT with deconvolution) is not "
-                                  "supported on the GPU path")
+        raise ValueError("the spread grid geometry takes accuracy <= 6; accuracy > 6 spreads "
+                         "onto fine_geometry's oversampled grid")
     msp = int(accuracy)
     if msp < 0:
         raise ValueError("accuracy must be >= 0")
@@ -443,6 +443,103 @@ def build_geometry(st: Stats, N: Tuple[int, ...], accuracy: int, xrange=None,
         axes_scaled.append(ax_s)
     return Geometry(g, tuple(N), L, msp, xmin_s, xmax_s, axes_raw, axes_scaled,
                     st.mean.copy(), st.std.copy(), centre, margin)
+
+
+MAX_FINE_ACCURACY = 12
+
+
+def fine_margin(N: Tuple[int, ...]) -> int:
+    """Grid margin that makes E >= N on every axis: the deconvolved grid of
+    accuracy > 6 is not compactly supported, so the unwrapped grid must hold
+    every cell a node's kernel window reaches."""
+    return max(int(n_) - int(n_) // 2 - 1 for n_ in N)
+
+
+def fine_geometry(geo: Geometry, accuracy: int) -> FlprGeom:
+    """Oversampled spread geometry of accuracy > 6 (kernel.py:97-156): R = 2
+    (accuracy <= 11) or 3, Mr = R L, Msp = accuracy,
+    tau = pi Msp / (L^2 R (R - 1/2)); the knot mapping is the coarse one."""
+    acc = int(accuracy)
+    d = len(geo.N)
+    if acc > MAX_FINE_ACCURACY:
+        raise NotImplementedError(f"accuracy > {MAX_FINE_ACCURACY} is not supported on the GPU")
+    if d == 3:
+        raise NotImplementedError("accuracy > 6 in 3-D is not supported on the GPU (the "
+                                  "oversampled footprint does not fit the spread tile)")
+    R = 2 if acc <= 11 else 3
+    f = FlprGeom.from_buffer_copy(geo.geom)
+    f.msp = acc
+    for k in range(d):
+        n_k, L_k = int(geo.N[k]), int(geo.L[k])
+        tau = (math.pi * acc) / (L_k * L_k * R * (R - 0.5))
+        Mr = R * L_k
+        E = R * (n_k // 2 + 1) + acc
+        f.L[k] = Mr
+        f.hx[k] = 2 * np.pi / Mr
+        f.inv4tau[k] = 1.0 / (4 * tau)
+        f.E[k] = E
+        f.G[k] = 2 * E + 1
+        f.tile[k] = TILE[d]
+        f.ntile[k] = -(-(2 * E + 1) // TILE[d])
+    return f
+
+
+def spread_fine(geo: Geometry, accuracy: int, x: torch.Tensor, n: int,
+                vals_y: Optional[torch.Tensor], ny: int, y_pairs: bool,
+                probes: Optional[torch.Tensor], K: int, with_ones: bool,
+                grid64: torch.Tensor, sums: torch.Tensor, c0: int, s) -> None:
+    """Spread channels (ones?, y, probes) onto the oversampled grid of accuracy
+    > 6 (nufftn_type1.py:520-636) and resample them, deconvolved, into
+    channels c0.. of the coarse unwrapped grid (nufftn_type1.py:693-770); sums
+    receives their totals (flpr_resample)."""
+    d = len(geo.N)
+    dev = x.device
+    gf = fine_geometry(geo, accuracy)
+    gfp = _lib.C.byref(gf)
+    gp = _lib.C.byref(geo.geom)
+    f64, i32 = torch.float64, torch.int32
+    ntiles = int(np.prod([gf.ntile[k] for k in range(d)]))
+    nkeys = ntiles * int(np.prod([gf.tile[k] for k in range(d)]))
+    key = torch.empty((n,), dtype=i32, device=dev)
+    cnt = torch.zeros((2 * nkeys,), dtype=i32, device=dev)
+    key_start = torch.empty((nkeys + 1,), dtype=i32, device=dev)
+    order_idx = torch.empty((n,), dtype=i32, device=dev)
+    call("flpr_points_prepare", ptr(x), n, gfp, ptr(key), ptr(cnt[:nkeys]), s)
+    call("flpr_points_bin", ptr(key), n, nkeys, ptr(cnt[:nkeys]), ptr(key_start),
+         ptr(cnt[nkeys:]), ptr(order_idx), s)
+    del key
+    nv = ny + K
+    wab = torch.empty((2 * d * n,), dtype=f64, device=dev)
+    vals = torch.empty((max(nv, 1) * n,), dtype=f64, device=dev)
+    call("flpr_points_pack", gfp, ptr(x), ptr(order_idx), n, ptr(vals_y) if ny else None, ny,
+         ptr(probes) if K else None, K, ptr(wab), ptr(vals), None, None, s)
+    wo = 1 if with_ones else 0
+    Cf = wo + nv
+    cells_f = int(np.prod([gf.G[k] for k in range(d)]))
+    fcells = int(np.prod([gf.tile[k] + 2 * gf.msp for k in range(d)]))
+    part = torch.empty((ntiles * Cf * fcells,), dtype=f64, device=dev)
+    fine = torch.empty((Cf * cells_f,), dtype=f64, device=dev)
+    sums_f = torch.zeros((Cf,), dtype=f64, device=dev)
+    call("flpr_spread", gfp, ptr(key_start), ptr(wab), ptr(vals), n, wo, nv, ptr(part),
+         ptr(fine), ptr(sums_f), s)
+    del part, wab, vals
+    cells = int(np.prod([geo.geom.G[k] for k in range(d)]))
+    groups = []
+    if wo:
+        groups.append((0, 1, 0))
+    if ny:
+        groups.append((wo, ny, 1 if y_pairs else 0))
+    if K:
+        groups.append((wo + ny, K, 0))
+    need = 0
+    for (_f0, nch, pairs) in groups:
+        w = _lib.C.c_int64(0)
+        call("flpr_resample_work", gfp, gp, nch, pairs, _lib.C.addressof(w))
+        need = max(need, int(w.value))
+    work = torch.empty((need,), dtype=f64, device=dev)
+    for (f0, nch, pairs) in groups:
+        call("flpr_resample", gfp, gp, fine.data_ptr() + f0 * cells_f * 8, nch, pairs, ptr(work),
+             grid64.data_ptr() + (c0 + f0) * cells * 8, sums.data_ptr() + (c0 + f0) * 8, s)
 
 
 def filter_bandwidths(geo: Geometry, h: np.ndarray, n: int):
@@ -586,7 +683,8 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
              stats: Optional[Stats] = None, with_ones_eps: bool = True,
              debug: Optional[dict] = None, readback: bool = True,
              moments_only: bool = False, kde: bool = False,
-             keep_all: bool = False, accuracy_y: Optional[int] = None) -> DeviceFit:
+             keep_all: bool = False, accuracy_y: Optional[int] = None,
+             y_pairs: bool = False) -> DeviceFit:
     """Spread, convolve and solve for all bandwidths, plus the probe dot products and
     the residual sums.
 
@@ -608,14 +706,24 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     of the ones channel and the probes (the reference's cv_fastlpr with DOF
     estimation spreads S and the probes at the sample-size accuracy and only
     y at the option's, api.py:569-576 / regression.py:86-87, 756-758).
+    Accuracies above 6 spread onto the oversampled grid and deconvolve
+    (spread_fine); y_pairs says that the y columns are (re, im) pairs of
+    complex responses, which that deconvolution must keep together.
     """
     n, d = x.shape
     dev = x.device
     s = stream_handle()
     st = stats if stats is not None else x_stats(x, vals_y)
-    split_acc = accuracy_y is not None and int(accuracy_y) != int(accuracy) and ny > 0
-    margin = max(int(accuracy), int(accuracy_y)) if split_acc else None
-    geo, gkey = geometry_for(st, N, accuracy, xrange, flag_power2, kde, margin)
+    acc_s = int(accuracy)
+    acc_y = acc_s if (accuracy_y is None or ny == 0) else int(accuracy_y)
+    fine_s, fine_y = acc_s > 6, acc_y > 6
+    split_acc = acc_y != acc_s
+    if fine_s or fine_y:
+        margin = max(fine_margin(N), min(acc_s, 6), min(acc_y, 6))
+    else:
+        margin = max(acc_s, acc_y) if split_acc else None
+    # with accuracy > 6 the ones channel is not spread onto this grid: msp 0
+    geo, gkey = geometry_for(st, N, 0 if fine_s else acc_s, xrange, flag_power2, kde, margin)
     nval = ny + K
     C = 1 + nval
     g = geo.geom
@@ -642,16 +750,25 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
 
     # --- spread (float64 accumulation), centring, float32 copy
     cells = int(np.prod([g.G[k] for k in range(d)]))
-    fcells = int(np.prod([g.tile[k] + 2 * geo.margin for k in range(d)]))
-    part = torch.empty((ntiles * C * fcells,), dtype=f64, device=dev)
+    # partial windows of the widest footprint spread onto this grid
+    wmsp = max(geo.msp, acc_y if (split_acc and not fine_y) else 0)
+    fcells = int(np.prod([g.tile[k] + 2 * wmsp for k in range(d)]))
+    part = (torch.empty((ntiles * C * fcells,), dtype=f64, device=dev) if not fine_s
+            else None)
     # convolution inputs are read in whole tiles past each job's end (the
     # values there are discarded or zeroed): pad them
     pad = conv_pad(geo)
     grid64 = torch.empty((C * cells + pad,), dtype=f64, device=dev)
     sums = torch.zeros((C,), dtype=f64, device=dev)
-    call("flpr_spread", gp, ptr(key_start), ptr(wab), ptr(vals), n, 1, nval, ptr(part),
-         ptr(grid64), ptr(sums), s)
-    if split_acc:
+    if fine_s:
+        # every channel through the oversampled grid (acc_y == acc_s here)
+        spread_fine(geo, acc_s, x, n, vals_y, ny, y_pairs, probes, K, True, grid64, sums, 0, s)
+    else:
+        call("flpr_spread", gp, ptr(key_start), ptr(wab), ptr(vals), n, 1, nval, ptr(part),
+             ptr(grid64), ptr(sums), s)
+    if fine_y and not fine_s:
+        spread_fine(geo, acc_y, x, n, vals_y, ny, y_pairs, None, 0, False, grid64, sums, 1, s)
+    elif split_acc and not fine_s:
         # the y columns again, with the footprint and width of their own
         # accuracy, over the channels the first spread wrote
         geo_y, _ = geometry_for(st, N, int(accuracy_y), xrange, flag_power2, kde, margin)
@@ -659,7 +776,7 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
         wab_y = torch.empty((2 * d * n,), dtype=f64, device=dev)
         vals_yp = torch.empty((ny * n,), dtype=f64, device=dev)
         call("flpr_points_pack", gy, ptr(x), ptr(order_idx), n, ptr(vals_y), ny, None, 0,
-             ptr(wab_y), ptr(vals_yp), ptr(rawi), ptr(rawf), s)
+             ptr(wab_y), ptr(vals_yp), None, None, s)
         sums[1:1 + ny].zero_()
         call("flpr_spread", gy, ptr(key_start), ptr(wab_y), ptr(vals_yp), n, 0, ny, ptr(part),
              ptr(grid64) + cells * 8, ptr(sums) + 8, s)
@@ -758,6 +875,7 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     m_y = torch.empty((dh * nyc * nodes,), dtype=f64, device=dev)
     for (a, b, gf, plan, stage0) in stage0s:
         call("flpr_solve", int(gf), ptr(stage0), per_h0, b - a, nodes, order, plan.nt,
+             int(fine_s),
              i64_array(plan.s_off), i64_array(plan.t_off), ptr(key_max), lam_scale, eps_add,
              ptr(shift), nval, nout, (m_y.data_ptr() + a * nyc * nodes * 8) if nout > 0 else None,
              ptr(q), (dot.data_ptr() + a * nval * 8) if dot is not None else None, s)

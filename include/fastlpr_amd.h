@@ -28,6 +28,7 @@ extern "C" {
 #define FLPR_MAX_TERMS 10   /* 3-D order 2 */
 #define FLPR_MAX_MOMENTS 35 /* 3-D order 4 monomials */
 #define FLPR_MAX_CHANNELS 64
+#define FLPR_MAX_MSP 12     /* spread half-width (accuracy 12; 3-D: 6) */
 
 /* Geometry of one fit.  Filled on the host from the x statistics; mirrors
  * the reference's knot mapping (convolution.py:169-199, nufftn_type1.py:140-176),
@@ -37,7 +38,9 @@ typedef struct FlprGeom {
   int32_t d;                      /* dimensions 1..3 */
   int32_t msp;                    /* spread half-width in cells (= accuracy) */
   int32_t n_grid[FLPR_MAX_DIM];   /* output grid nodes per dimension (N) */
-  int32_t L[FLPR_MAX_DIM];        /* padded FFT length of the reference (power of 2) */
+  int32_t L[FLPR_MAX_DIM];        /* periodic length of the spread grid: the reference's
+                                     padded FFT length (power of 2), or R times it for
+                                     the oversampled grid of accuracy > 6 */
   int32_t E[FLPR_MAX_DIM];        /* unwrapped spread grid covers signed cells [-E, E] */
   int32_t G[FLPR_MAX_DIM];        /* 2E+1 */
   int32_t tile[FLPR_MAX_DIM];     /* spread tile edge in cells */
@@ -94,7 +97,8 @@ int flpr_points_bin(const int32_t* key, int64_t n, int32_t nkeys, const int32_t*
  *                         (nufftn_type1.py:226-249, spreading_fused.py:80-94)
  *   vals (n, ny + nprobe) float64  y columns then probe columns
  *   rawi (n, d) int32, rawf (n, d) float64  interval and fraction on the raw grid
- *                         axis linspace(graw0, graw1, N) (RegularGridInterpolator). */
+ *                         axis linspace(graw0, graw1, N) (RegularGridInterpolator);
+ *                         both may be NULL (not written). */
 int flpr_points_pack(const FlprGeom* g, const double* x, const int32_t* order, int64_t n,
                      const double* vals_y, int32_t ny, const double* probes, int32_t nprobe,
                      double* wab, double* vals, int32_t* rawi, double* rawf, void* stream);
@@ -110,6 +114,24 @@ int flpr_points_pack(const FlprGeom* g, const double* x, const int32_t* order, i
 int flpr_spread(const FlprGeom* g, const int32_t* key_start, const double* wab,
                 const double* vals, int64_t n, int32_t with_ones, int32_t nval, double* part,
                 double* grid, double* sums, void* stream);
+
+/* Accuracy > 6 (nufftn_type1.py:693-770, kernel.py:97-156,
+ * convolution.py:235-236): the nch channels of fine_grid, spread with the
+ * oversampled geometry `fine` (L = R * coarse L, R = 2 or 3, msp = accuracy,
+ * tau = pi msp / (L^2 R (R - 1/2)) of the coarse L), are Fourier-transformed,
+ * cut to the central L frequencies, divided by the Gaussian's transform and
+ * scaled by 1/R^d, then transformed back onto the coarse unwrapped grid
+ * (coarse E >= N on every axis; cells equal modulo L hold equal values) -- all
+ * as one separable linear map in float64.  With pairs, channels (2q, 2q+1)
+ * are the real and imaginary parts of one complex channel; otherwise every
+ * channel is real and only the real part of its result is kept.  Writes the
+ * nch channels of grid (coarse cells each) and their totals into sums.  work
+ * holds flpr_resample_work(...) doubles. */
+int flpr_resample_work(const FlprGeom* fine, const FlprGeom* coarse, int32_t nch, int32_t pairs,
+                       int64_t* doubles);
+int flpr_resample(const FlprGeom* fine, const FlprGeom* coarse, const double* fine_grid,
+                  int32_t nch, int32_t pairs, double* work, double* grid, double* sums,
+                  void* stream);
 
 /* Centre channels 1..C-1 of the spread grid on their kernel-weighted mean:
  * shift[c] = sums[c] / sums[0], F_c -= shift[c] F_0 (float64, in place), and
@@ -164,6 +186,9 @@ double flpr_key_to_double(uint64_t key);
  * lwp_formulas.py:133-380), in float64.
  *   mom       stage-0 moment slabs (fp64 ? double : float)
  *   per_h     elements per bandwidth
+ *   pivot     nonzero: nodes where S + lam I is not positive definite (the
+ *             deconvolved grids of accuracy > 6) are solved by Gaussian
+ *             elimination with partial pivoting instead of giving NaN
  *   s_off[ns] slab offset of S pair (i,j>=i) (row-major pairs), channel 0
  *   t_off[nt] slab offset of term i; channel c adds c*nodes
  *             (s_off and t_off are HOST arrays copied into the kernel arguments)
@@ -176,10 +201,10 @@ double flpr_key_to_double(uint64_t key);
  *   q, dot    when q (nval, nodes) float64 is given, dot[h*nval + c] accumulates
  *             sum_node q[c][node] m_c(h, node) (dot zeroed by the caller). */
 int flpr_solve(int32_t fp64, const void* mom, int64_t per_h, int32_t dh, int64_t nodes,
-               int32_t order, int32_t nt, const int64_t* s_off, const int64_t* t_off,
-               const uint64_t* key, double lam_scale, double eps_add, const double* shift,
-               int32_t nval, int32_t nout, double* m_out, const double* q, double* dot,
-               void* stream);
+               int32_t order, int32_t nt, int32_t pivot, const int64_t* s_off,
+               const int64_t* t_off, const uint64_t* key, double lam_scale, double eps_add,
+               const double* shift, int32_t nval, int32_t nout, double* m_out, const double* q,
+               double* dot, void* stream);
 
 /* Adjoint interpolation sums on the raw grid, from the sorted point data:
  *   q (nval, nodes)        q[c][node] = sum_i vals[i][c] w_i(node)

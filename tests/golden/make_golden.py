@@ -88,7 +88,7 @@ def main(ref_src: str, only=None):
     def wanted(name):
         return not only or name in only
 
-    for name in recipes.LPR_SMALL + recipes.LPR_CUBIC + recipes.LPR_ACC:
+    for name in recipes.LPR_SMALL + recipes.LPR_CUBIC + recipes.LPR_ACC + recipes.LPR_ACC_HI:
         if not wanted(name):
             continue
         out = run_lpr(fl, name)
@@ -102,7 +102,7 @@ def main(ref_src: str, only=None):
         out["meta"] = json.dumps(meta)
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
         print("wrote", name)
-    for name in recipes.KDE:
+    for name in recipes.KDE + recipes.KDE_ACC_HI:
         if not wanted(name):
             continue
         out = run_kde(fl, name)

@@ -41,14 +41,19 @@ def make(name: str):
         r2 = (s * rng.standard_normal(n)) ** 2
         return x, r2, _logspace_h(10, 0.05, 1.0)[:, None], {"order": 1, "N": 64, "y_type_out": "variance"}
     if name.startswith("acc"):
-        # acc{a}_{d}d: accuracy a (0 = nearest-cell binning), order 1
-        a, dd = int(name[3]), int(name.split("_")[1][0])
+        # acc{a}_{d}d[_nodof]: accuracy a (0 = nearest-cell binning, > 6 =
+        # oversampled grid with deconvolution), order 1
+        parts = name.split("_")
+        a, dd = int(parts[0][3:]), int(parts[1][0])
         rng = np.random.default_rng(50 + a + 10 * dd)
         n = 3000 if dd == 1 else 6000
         x = rng.uniform(0, 1, (n, dd))
         y = np.sin(2 * np.pi * x[:, 0]) + 0.2 * rng.standard_normal(n)
         N = 100 if dd == 1 else [30, 33]
-        return x, y, _iso(_logspace_h(6, 0.04, 0.6), dd), {"order": 1, "N": N, "accuracy": a}
+        opt = {"order": 1, "N": N, "accuracy": a}
+        if "nodof" in parts:
+            opt["calc_dof"] = False
+        return x, y, _iso(_logspace_h(6, 0.04, 0.6), dd), opt
     if name == "lpr2d_o1_cubic":
         # fewer than 5000 points in 2-D: the reference interpolates GCV / DOF cubically
         rng = np.random.default_rng(22)
@@ -86,11 +91,15 @@ def make(name: str):
         y = np.sin(2 * np.pi * x[:, 0]) + 0.1 * rng.standard_normal(n)
         return x, y, _logspace_h(20, 0.01, 1.0)[:, None], {"order": 1, "N": 1024}
     if name.startswith("kde"):
+        # kde{d}d[_acc{a}]
         d = int(name[3])
         rng = np.random.default_rng(40 + d)
         n = {1: 800, 2: 1500, 3: 1500}[d]
         x = rng.standard_normal((n, d))
-        return x, None, _iso(_logspace_h(10, 0.1, 1.0), d), {}
+        opt = {}
+        if "_acc" in name:
+            opt["accuracy"] = int(name.split("_acc")[1])
+        return x, None, _iso(_logspace_h(10, 0.1, 1.0), d), opt
     raise KeyError(name)
 
 
@@ -103,5 +112,7 @@ LPR_SMALL = [
 ]
 LPR_CUBIC = ["lpr2d_o1_cubic", "lpr3d_o1_cubic"]
 LPR_ACC = ["acc0_1d", "acc2_1d", "acc0_2d", "acc3_2d"]
+LPR_ACC_HI = ["acc7_1d", "acc12_1d", "acc7_2d", "acc9_2d", "acc7_2d_nodof"]
 LPR_LARGE = ["cfg1_1d_o1_n1e5"]
 KDE = ["kde1d", "kde2d", "kde3d"]
+KDE_ACC_HI = ["kde1d_acc7", "kde2d_acc8"]

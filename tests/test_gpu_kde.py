@@ -34,7 +34,7 @@ def _oracle(x, h, opt):
         return R.cv_fastkde(x, h, dict(opt), mode="f64")
 
 
-@pytest.mark.parametrize("name", recipes.KDE)
+@pytest.mark.parametrize("name", recipes.KDE + recipes.KDE_ACC_HI)
 def test_kde_matches_reference_golden(fl, name):
     g = np.load(os.path.join(GOLDEN, f"{name}.npz"), allow_pickle=False)
     x, _, h, opt = recipes.make(name)
@@ -48,7 +48,7 @@ def test_kde_matches_reference_golden(fl, name):
     np.testing.assert_array_equal(r.h, g["h1se"])
 
 
-@pytest.mark.parametrize("name", recipes.KDE)
+@pytest.mark.parametrize("name", recipes.KDE + recipes.KDE_ACC_HI)
 def test_kde_fp64_equals_fp64_oracle(fl, name):
     x, _, h, opt = recipes.make(name)
     o = _oracle(x, h, opt)
@@ -61,7 +61,7 @@ def test_kde_fp64_equals_fp64_oracle(fl, name):
     assert _rel(r.lcv["lcv_sd"], o.lcv["lcv_sd"]) < 1e-7
 
 
-@pytest.mark.parametrize("name", recipes.KDE)
+@pytest.mark.parametrize("name", recipes.KDE + recipes.KDE_ACC_HI)
 def test_kde_fp32_within_1e5_of_fp64_oracle(fl, name):
     x, _, h, opt = recipes.make(name)
     o = _oracle(x, h, opt)

@@ -28,7 +28,8 @@ def _relmax(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
-@pytest.mark.parametrize("name", recipes.LPR_SMALL + recipes.LPR_CUBIC + recipes.LPR_ACC)
+@pytest.mark.parametrize("name", recipes.LPR_SMALL + recipes.LPR_CUBIC + recipes.LPR_ACC
+                         + recipes.LPR_ACC_HI)
 def test_oracle_lpr_matches_reference(name):
     g = _load(name)
     x, y, h, opt = recipes.make(name)
@@ -42,8 +43,11 @@ def test_oracle_lpr_matches_reference(name):
     tol = 3e-5 if name.startswith("lpr3d") else 2e-5
     assert _relmax(f.yhat, g["yhat"]) < tol
     assert _relmax(f.grid_values, g["grid_values"]) < tol
-    assert _relmax(f.gcv["gcv_m"], g["gcv_m"]) < 2e-6
-    assert _relmax(f.gcv["dof"], g["dof"]) < 1e-4
+    if opt.get("calc_dof", True):
+        assert _relmax(f.gcv["gcv_m"], g["gcv_m"]) < 2e-6
+        assert _relmax(f.gcv["dof"], g["dof"]) < 1e-4
+    else:
+        assert np.all(np.isnan(g["gcv_m"])) and np.all(np.isnan(f.gcv["gcv_m"]))
     assert _relmax(f.s0, g["s0"]) < 2e-6
     if np.isfinite(float(g["d"])):
         assert abs(f.d - float(g["d"])) / abs(float(g["d"])) < 2e-5
@@ -61,7 +65,7 @@ def test_oracle_config1_full_size_matches_reference():
     assert _relmax(f.gcv["gcv_m"], g["gcv_m"]) < 2e-6
 
 
-@pytest.mark.parametrize("name", recipes.KDE)
+@pytest.mark.parametrize("name", recipes.KDE + recipes.KDE_ACC_HI)
 def test_oracle_kde_matches_reference(name):
     g = _load(name)
     x, _, h, opt = recipes.make(name)

@@ -7,6 +7,10 @@ on a NumPy spread grid built with the same knot mapping, and the resulting
 S and T moments are compared with the oracle's FFT convolution in float64.
 A plausible indexing mistake (wrong qout/pad offset = one-cell shift, wrong
 slab offset, a dropped channel prefix) changes the moments by O(1).
+
+Accuracy > 6 adds the oversampled spread and the deconvolving resample
+(flpr_resample's formula g[j] = sum_c F[c] a[(R j - c) mod Mr]) onto the
+widened coarse grid (E >= N), emulated here with the engine's fine geometry.
 """
 import math
 
@@ -18,17 +22,61 @@ from fastlpr_amd import engine as E
 from oracle import restate as R
 
 
-def _emulate(x, y, h, order, N):
+def _fine_grid(geo, xmod, vals, acc):
+    """Oversampled spread (every footprint index checked against the fine
+    grid) and the per-axis deconvolving resample onto the coarse grid."""
+    gf = E.fine_geometry(geo, acc)
+    g = geo.geom
+    n, d = xmod.shape
+    Gf = tuple(gf.G[k] for k in range(d))
+    fine = np.zeros((vals.shape[1],) + Gf)
+    j = np.arange(-acc, acc + 1)
+    for i in range(n):
+        ws, idx = [], []
+        for k in range(d):
+            m = int(np.round(xmod[i, k] / gf.hx[k]))
+            pm = m - gf.L[k] if m > gf.L[k] // 2 else m
+            diff = xmod[i, k] - gf.hx[k] * (m + j)
+            ws.append(np.exp(-(diff * diff * gf.inv4tau[k])))
+            ix = pm + j + gf.E[k]
+            assert ix.min() >= 0 and ix.max() < gf.G[k]
+            idx.append(ix)
+        if d == 1:
+            fine[:, idx[0]] += vals[i][:, None] * ws[0][None]
+        else:
+            fine[:, idx[0][:, None], idx[1][None, :]] += \
+                vals[i][:, None, None] * np.outer(ws[0], ws[1])[None]
+    out = fine.astype(complex)
+    for k in range(d - 1, -1, -1):
+        Mr, Lc = gf.L[k], g.L[k]
+        Rr = Mr // Lc
+        tau = (math.pi * acc) / (Lc * Lc * Rr * (Rr - 0.5))
+        kap = np.arange(-Lc // 2, Lc // 2)
+        delta = np.arange(Mr)
+        a = (np.sqrt(math.pi / tau) * np.exp(tau * kap * kap.astype(float))[None, :]
+             * np.exp(2j * np.pi * np.outer(delta, kap) / Mr)).sum(1) / (Lc * Rr)
+        jc = np.arange(g.G[k]) - g.E[k]
+        cf = np.arange(gf.G[k]) - gf.E[k]
+        A = a[np.mod(Rr * jc[:, None] - cf[None, :], Mr)]
+        out = np.moveaxis(np.tensordot(A, np.moveaxis(out, k + 1, 0), axes=(1, 0)), 0, k + 1)
+    return out.real
+
+
+def _emulate(x, y, h, order, N, acc=None):
     n, d = x.shape
-    acc = R.accuracy_for(n)
+    acc = R.accuracy_for(n) if acc is None else acc
     sd = x.std(0, ddof=1)
     st = E.Stats(x.min(0), x.max(0), x.mean(0), np.where(sd == 0, 1, sd))
-    geo = E.build_geometry(st, tuple(N), acc)
+    if acc > 6:
+        geo = E.build_geometry(st, tuple(N), 0, margin=E.fine_margin(tuple(N)))
+        assert all(geo.geom.E[k] >= N[k] for k in range(d))
+    else:
+        geo = E.build_geometry(st, tuple(N), acc)
     g = geo.geom
     h, _, _ = E.filter_bandwidths(geo, h, n)
     dh = h.shape[0]
     C = 2
-    plan = E.Plan(d, order, geo.N, geo.L, geo.msp, C, dh, True, "cpu")
+    plan = E.Plan(d, order, geo.N, geo.L, geo.margin, C, dh, True, "cpu")
     xmod = np.zeros((n, d))
     cell = np.zeros((n, d), int)
     for k in range(d):
@@ -43,7 +91,9 @@ def _emulate(x, y, h, order, N):
     msp = geo.msp
     vals = np.column_stack([np.ones(n), y])
     j = np.arange(-msp, msp + 1)
-    for i in range(n):
+    if acc > 6:
+        grid = _fine_grid(geo, xmod, vals, acc)
+    for i in range(n if acc <= 6 else 0):
         ws, idx = [], []
         for k in range(d):
             m = cell[i, k]
@@ -92,14 +142,17 @@ CASES = [(1, o, [40]) for o in (0, 1, 2)] + [(1, 1, [33])] + \
         [(2, o, [12, 10]) for o in (0, 1, 2)] + [(2, 2, [11, 9])]
 
 
-@pytest.mark.parametrize("d,order,N", CASES)
-def test_banded_plan_reproduces_fft_moments(d, order, N):
+CASES_HI = [(1, 1, [40], 7), (1, 2, [33], 12), (2, 1, [12, 10], 7), (2, 2, [11, 9], 9)]
+
+
+@pytest.mark.parametrize("d,order,N,acc", [c + (None,) for c in CASES] + CASES_HI)
+def test_banded_plan_reproduces_fft_moments(d, order, N, acc):
     rng = np.random.default_rng(3 + d + order)
     n = 300
     x = rng.uniform(0, 1, (n, d))
     y = np.sin(6 * x[:, 0]) + 0.1 * rng.standard_normal(n)
     h = np.array([[0.05] * d, [0.2] * d, [0.6] * d]) if d == 1 else np.array([[0.1, 0.1], [0.3, 0.5]])
-    plan, st0, h, acc = _emulate(x, y, h, order, N)
+    plan, st0, h, acc = _emulate(x, y, h, order, N, acc)
     P = R.Problem(x, h, {"order": order, "accuracy": acc, "N": N}, "f64")
     S = P.design(add_eps=False)
     T = P.weighted(y, True)
