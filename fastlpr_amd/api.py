@@ -152,6 +152,18 @@ def _check_finite(a, what: str, fn: str):
                          f"before calling {fn}.")
 
 
+def _check_accuracy(acc: int, dims: int) -> None:
+    """NUFFT accuracies the engine runs: 0 (nearest-cell binning) to 6 on the
+    L-periodic grid, 7..12 on the oversampled grid in 1-D and 2-D."""
+    if acc < 0:
+        raise ValueError("options['accuracy'] must be a non-negative integer.")
+    if acc > 12:
+        raise NotImplementedError("accuracy > 12 is not supported on the GPU")
+    if acc > 6 and dims == 3:
+        raise NotImplementedError("accuracy > 6 in 3-D is not supported on the GPU (the "
+                                  "oversampled footprint does not fit the spread tile)")
+
+
 def _grid_shape(opts: dict, n: int, dims: int) -> Tuple[int, ...]:
     """Grid nodes per dimension: options['N'] or ceil(n^(1/d)) (model.py:78-83, core.py:76-82)."""
     N = opts.get("N", None)
@@ -367,6 +379,7 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
         raise NotImplementedError("the GPU engine supports 1-3 dimensions")
     if str(opts.get("kernel_type", "gaussian")).lower() not in ("gauss", "gaussian", "normal"):
         raise NotImplementedError("the GPU engine implements the Gaussian kernel")
+    _check_accuracy(int(opts["accuracy"]), dims)
     variance = str(opts.get("y_type_out", "mean")) == "variance"
     fp64 = str(opts.get("precision", "fp32")).lower() == "fp64"
     N = _grid_shape(opts, n, dims)
@@ -558,6 +571,7 @@ def cv_fastkde(x, h: Optional[Sequence[Sequence[float]]] = None,
         raise NotImplementedError("the GPU engine implements the Gaussian kernel")
     if opts.get("output_interp_method", "linear") != "linear":
         raise NotImplementedError("only linear interpolation is implemented on the GPU")
+    _check_accuracy(int(opts["accuracy"]), dims)
     fp64 = str(opts.get("precision", "fp32")).lower() == "fp64"
     N = _grid_shape(opts, n, dims)
     dev = x.device if _is_dev(x) else _device()

@@ -781,9 +781,12 @@ def interval(mu_grid, sigma_grid, s0, h, order, alpha=0.05, kind="confidence"):
     """fastlpr_interval (confidence.py:60-216): [upper, lower] on the grid."""
     from scipy.stats import norm
     z = norm.ppf(1 - alpha / 2)
-    sigma_pos = np.maximum(np.asarray(sigma_grid, float), 0)
-    s0 = np.maximum(np.asarray(s0, float), 1e-10)
+    # NumPy keeps the grids' dtypes (the reference's s0 and mean grid are
+    # float32; a Python float times a float32 array stays float32)
+    sigma_pos = np.maximum(np.asarray(sigma_grid), 0)
+    s0 = np.maximum(np.asarray(s0), 1e-10)
     nu = NU_TABLE[(s0.ndim, int(order))]
-    se_sq = sigma_pos * nu / (float(np.prod(np.atleast_1d(h))) * s0)
+    se_sq = sigma_pos * nu / (float(np.prod(np.atleast_1d(h).astype(float))) * s0)
     half = z * np.sqrt(se_sq) if kind == "confidence" else z * np.sqrt(sigma_pos + se_sq)
+    mu_grid = np.asarray(mu_grid)
     return np.stack([mu_grid + half, mu_grid - half], axis=-1)

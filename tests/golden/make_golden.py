@@ -7,7 +7,7 @@ where the GPU tests run):
 
 Each fixture stores the inputs (or, for the large case, the seeded recipe and a
 hash of the inputs), the options, and the reference outputs of
-``cv_fastlpr`` / ``cv_fastkde``: yhat (or a seeded subsample of it plus its
+``cv_fastlpr`` / ``cv_fastkde`` (and ``fastlpr_interval`` / ``fastlpr_predict``): yhat (or a seeded subsample of it plus its
 hash), gcv_m, gcv_sd, dof, dof_stderr, idmin, id1se, the selected bandwidth,
 the grid values at the selected bandwidth, s0 and the variance factor d.
 These end-to-end compositions (probe stream x interpolation x float32
@@ -75,6 +75,26 @@ def run_kde(fl, name):
                 else np.asarray(np.concatenate(r.grid)))
 
 
+def run_interval(fl, name):
+    """Mean fit, variance fit of the squared residuals, fastlpr_interval
+    (confidence and prediction, alpha 0.05) and fastlpr_predict at new points."""
+    import recipes
+    x, y, h, opt = recipe_inputs(name)
+    xn = recipes.predict_points(name)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        mu = fl.cv_fastlpr(x, y, h, dict(opt))
+        sg = fl.cv_fastlpr(x, (y - mu.yhat) ** 2, h, dict(opt, y_type_out="variance"))
+        ci = fl.fastlpr_interval(mu, sg, 0.05, type="confidence")
+        pi = fl.fastlpr_interval(mu, sg, 0.05, type="prediction")
+        pred = fl.fastlpr_predict(mu, xn)
+    return dict(kind="interval", name=name, options=json.dumps(opt), hlist=np.asarray(h, float),
+                x=x, y=y, x_new=xn, pred=np.asarray(pred), ci=np.asarray(ci),
+                pi=np.asarray(pi), mu_grid=np.asarray(mu.fpp_yhat.values),
+                sigma_grid=np.asarray(sg.fpp_yhat.values), s0=np.asarray(mu.s0),
+                h_mu=np.asarray(mu.h), h_sigma=np.asarray(sg.h))
+
+
 def main(ref_src: str, only=None):
     sys.path.insert(0, ref_src)
     os.environ.setdefault("FASTLPR_QUIET", "1")
@@ -99,6 +119,13 @@ def main(ref_src: str, only=None):
         if not wanted(name):
             continue
         out = run_lpr(fl, name, store_inputs=False, n_sub=4096)
+        out["meta"] = json.dumps(meta)
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+        print("wrote", name)
+    for name in recipes.IVL:
+        if not wanted(name):
+            continue
+        out = run_interval(fl, name)
         out["meta"] = json.dumps(meta)
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
         print("wrote", name)

@@ -90,6 +90,18 @@ def make(name: str):
         x = rng.uniform(0, 1, (n, 1))
         y = np.sin(2 * np.pi * x[:, 0]) + 0.1 * rng.standard_normal(n)
         return x, y, _logspace_h(20, 0.01, 1.0)[:, None], {"order": 1, "N": 1024}
+    if name.startswith("ivl"):
+        # ivl{d}d_o{order}: mean fit, variance fit of its squared residuals,
+        # then intervals and predictions (x_new from predict_points)
+        d, order = int(name[3]), int(name.split("_o")[1])
+        rng = np.random.default_rng(70 + d + 10 * order)
+        n = 2000 if d == 1 else 6000
+        x = rng.uniform(0, 1, (n, d))
+        sd = 0.05 + 0.3 * x[:, 0]
+        y = np.sin(2 * np.pi * x[:, 0]) + (np.cos(2 * np.pi * x[:, -1]) if d > 1 else 0) \
+            + sd * rng.standard_normal(n)
+        N = 100 if d == 1 else [30, 28]
+        return x, y, _iso(_logspace_h(8, 0.03, 0.6), d), {"order": order, "N": N}
     if name.startswith("kde"):
         # kde{d}d[_acc{a}]
         d = int(name[3])
@@ -114,5 +126,16 @@ LPR_CUBIC = ["lpr2d_o1_cubic", "lpr3d_o1_cubic"]
 LPR_ACC = ["acc0_1d", "acc2_1d", "acc0_2d", "acc3_2d"]
 LPR_ACC_HI = ["acc7_1d", "acc12_1d", "acc7_2d", "acc9_2d", "acc7_2d_nodof"]
 LPR_LARGE = ["cfg1_1d_o1_n1e5"]
+IVL = ["ivl1d_o1", "ivl1d_o2", "ivl2d_o1"]
+
+
+def predict_points(name: str):
+    """New predictor locations inside the data range for the ivl recipes."""
+    x = make(name)[0]
+    rng = np.random.default_rng(99)
+    lo, hi = x.min(0), x.max(0)
+    return lo + (hi - lo) * rng.uniform(0, 1, (500, x.shape[1]))
+
+
 KDE = ["kde1d", "kde2d", "kde3d"]
 KDE_ACC_HI = ["kde1d_acc7", "kde2d_acc8"]

@@ -73,6 +73,35 @@ def test_cv_fastlpr_validation_errors(bad):
         fastlpr_amd.cv_fastlpr(x, y, [[0.2]] if bad != "dims" else None, opts)
 
 
+@pytest.mark.parametrize("acc,dims,exc", [(-1, 1, ValueError), (13, 1, NotImplementedError),
+                                          (7, 3, NotImplementedError)])
+def test_unsupported_accuracy_raises_before_device_work(acc, dims, exc):
+    x = np.random.default_rng(0).uniform(size=(60, dims))
+    with pytest.raises(exc):
+        fastlpr_amd.cv_fastlpr(x, x[:, 0], [[0.3] * dims], {"order": 1, "accuracy": acc})
+    with pytest.raises(exc):
+        fastlpr_amd.cv_fastkde(x, [[0.3] * dims], {"accuracy": acc})
+
+
+def test_fine_geometry_oversampling_constants():
+    """kernel.py:97-156: R = 2 up to accuracy 11, 3 above; Mr = R L; the fine
+    grid holds the data support plus the footprint on every axis."""
+    import math
+    from fastlpr_amd import engine as E
+    st = E.Stats(np.array([0.0, 0.0]), np.array([1.0, 1.0]), np.array([0.5, 0.5]),
+                 np.array([0.29, 0.29]))
+    geo = E.build_geometry(st, (40, 33), 0, margin=E.fine_margin((40, 33)))
+    assert all(geo.geom.E[k] >= (40, 33)[k] for k in range(2))
+    for acc, R in ((7, 2), (11, 2), (12, 3)):
+        f = E.fine_geometry(geo, acc)
+        for k in range(2):
+            L = geo.L[k]
+            assert f.L[k] == R * L and f.msp == acc
+            tau = math.pi * acc / (L * L * R * (R - 0.5))
+            assert abs(f.inv4tau[k] * 4 * tau - 1) < 1e-15
+            assert f.G[k] == 2 * (R * ((40, 33)[k] // 2 + 1) + acc) + 1
+
+
 def test_cv_fastkde_validation_errors():
     import fastlpr_amd as fl
     with pytest.raises(ValueError, match="at least 2 observations"):

@@ -77,3 +77,27 @@ def test_predict_matches_linear_interpolation_of_the_grid(d):
                                   bounds_error=False, fill_value=None)(xn)
     np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-10)
     np.testing.assert_allclose(fl.fastlpr_predict(r, x), r.yhat, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("name", ["ivl1d_o1", "ivl1d_o2", "ivl2d_o1"])
+def test_interval_and_predict_match_reference_golden(name):
+    """The reference's fastlpr_interval / fastlpr_predict on its own fits
+    (tests/golden/<name>.npz) against this engine's fits of the same data: the
+    same selected bandwidths, intervals within the float32 spread of the two
+    fits, predictions to the grid-value tolerance of the regression fixtures."""
+    import os
+    import recipes
+    import fastlpr_amd as fl
+    g = np.load(os.path.join(os.path.dirname(recipes.__file__), f"{name}.npz"),
+                allow_pickle=False)
+    x, y, h, opt = recipes.make(name)
+    mu = fl.cv_fastlpr(x, y, h, dict(opt))
+    sg = fl.cv_fastlpr(x, (y - mu.yhat) ** 2, h, dict(opt, y_type_out="variance"))
+    np.testing.assert_array_equal(np.atleast_1d(mu.h), g["h_mu"])
+    np.testing.assert_array_equal(np.atleast_1d(sg.h), g["h_sigma"])
+    for kind, key in (("confidence", "ci"), ("prediction", "pi")):
+        ci = fl.fastlpr_interval(mu, sg, 0.05, kind)
+        assert ci.shape == g[key].shape
+        assert np.max(np.abs(ci - g[key])) / np.max(np.abs(g[key])) < 2e-4
+    pred = fl.fastlpr_predict(mu, g["x_new"])
+    assert np.max(np.abs(pred - g["pred"])) / np.max(np.abs(g["pred"])) < 5e-5

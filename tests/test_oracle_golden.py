@@ -74,3 +74,23 @@ def test_oracle_kde_matches_reference(name):
     assert int(g["idmax"]) == f.lcv["idmax"]
     assert _relmax(f.fhat, g["fhat"]) < 2e-5
     assert _relmax(f.lcv["lcv_m"], g["lcv_m"]) < 2e-6
+
+
+@pytest.mark.parametrize("name", recipes.IVL)
+def test_oracle_interval_and_predict_match_reference(name):
+    """fastlpr_interval's formula (confidence.py:60-216) and fastlpr_predict's
+    linear interpolation (predict.py:15), applied to the reference's own mean
+    and variance grids, reproduce the reference's intervals and predictions."""
+    g = _load(name)
+    x, y, h, opt = recipes.make(name)
+    np.testing.assert_array_equal(g["x"], x)
+    np.testing.assert_array_equal(g["x_new"], recipes.predict_points(name))
+    for kind, key in (("confidence", "ci"), ("prediction", "pi")):
+        got = R.interval(g["mu_grid"], g["sigma_grid"], g["s0"], g["h_mu"], opt["order"], 0.05,
+                         kind)
+        assert got.shape == g[key].shape
+        assert _relmax(got, g[key]) < 1e-12
+    axes = R.raw_grid(x, list(np.atleast_1d(opt["N"])) * (x.shape[1] if np.isscalar(opt["N"])
+                                                          else 1))
+    pred = R.interp_linear(axes, g["mu_grid"], g["x_new"])
+    assert _relmax(pred, g["pred"]) < 1e-12
