@@ -7,7 +7,12 @@ grid 256 x 256, 30 log-spaced bandwidths in [0.005, 0.5], order 2, GCV with
     pass 1  cv_fastlpr(x, y)                          (mean)
     pass 2  cv_fastlpr(x, (y - yhat)^2, variance)     (variance)
 so one step is two fits; value = fits per second = 2 K / time.  Inputs are
-resident in HBM before the timed region; outputs stay on the device.
+resident in HBM before the timed region; outputs stay on the device.  The
+step runs the pair as one cv_fastlpr_pair call, whose second pass reuses the
+first pass's probe sums (the DOF depends on x, h and the probes only); every
+other stage of both passes, including the first pass's probe channels, is
+computed in every step.  The same pair as two separate cv_fastlpr calls is
+timed too and reported beside it (two_call_fits_per_s).
 
 The JSON line carries the chosen form (float32 MFMA convolution, float64 spread
 and solves), the plain form (float64 end to end) for comparison, the maximum
@@ -56,11 +61,17 @@ def hlist():
 
 
 def step(cv, xd, yd, hl, prec):
+    """The pair as two cv_fastlpr calls (the drop-in usage)."""
     opt = {"order": 2, "N": [256, 256], "precision": prec}
     r1 = cv(xd, yd, hl, opt)
     r2y = (yd - r1.yhat) ** 2
     r2 = cv(xd, r2y, hl, dict(opt, y_type_out="variance"))
     return r1, r2
+
+
+def step_pair(pair, xd, yd, hl, prec):
+    """The pair as one cv_fastlpr_pair call."""
+    return pair(xd, yd, hl, {"order": 2, "N": [256, 256], "precision": prec})
 
 
 def cpu_baseline(x, y, hl):
@@ -166,6 +177,7 @@ def main():
     import fastlpr_amd
     from fastlpr_amd import engine
     cv = fastlpr_amd.cv_fastlpr
+    pair = fastlpr_amd.cv_fastlpr_pair
     n = WORKLOAD["n_points"]
     x, y = make_inputs(n, WORKLOAD["seed"])
     hl = hlist()
@@ -176,17 +188,29 @@ def main():
           flush=True)
 
     for _ in range(max(args.warmup, 1)):
-        step(cv, xd, yd, hl, "fp32")
+        step_pair(pair, xd, yd, hl, "fp32")
     torch.cuda.synchronize()
     engine.TIMER.start()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        r1, r2 = step(cv, xd, yd, hl, "fp32")
+        r1, r2 = step_pair(pair, xd, yd, hl, "fp32")
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     engine.TIMER.stop()
     kt = engine.TIMER.collect()
+    assert r2.metadata["dof_shared"]
+    # the same pair as two independent calls
+    step(cv, xd, yd, hl, "fp32")
+    torch.cuda.synchronize()
+    tc0 = time.perf_counter()
+    for _ in range(args.steps):
+        c1, c2 = step(cv, xd, yd, hl, "fp32")
+    torch.cuda.synchronize()
+    tc1 = time.perf_counter()
+    two_call_fits = 2.0 * args.steps / (tc1 - tc0)
+    pair_vs_calls = max(float((r1.yhat - c1.yhat).abs().max() / c1.yhat.abs().max()),
+                        float((r2.yhat - c2.yhat).abs().max() / c2.yhat.abs().max()))
     if args.calibrate_copy:
         a = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device=dev).fill_(1.0)
         b = torch.empty_like(a)
@@ -196,8 +220,9 @@ def main():
         del a, b
     ms_step = (t1 - t0) * 1e3 / args.steps
     fits_s = 2.0 * args.steps / (t1 - t0)
-    print(f"[bench] chosen: {ms_step:.2f} ms/step, {fits_s:.1f} fits/s", file=sys.stderr,
-          flush=True)
+    print(f"[bench] chosen: {ms_step:.2f} ms/step, {fits_s:.1f} fits/s "
+          f"(two calls: {two_call_fits:.1f} fits/s, yhat difference {pair_vs_calls:.1e})",
+          file=sys.stderr, flush=True)
     conv = kt.get("flpr_conv", [])
     conv_ms = float(np.mean([c[0] for c in conv])) if conv else float("nan")
     conv_flop = float(np.mean([c[1] for c in conv])) if conv else float("nan")
@@ -207,12 +232,12 @@ def main():
     extra = {}
     if not args.no_plain:
         for _ in range(1):
-            step(cv, xd, yd, hl, "fp64")
+            step_pair(pair, xd, yd, hl, "fp64")
         torch.cuda.synchronize()
         tp0 = time.perf_counter()
         ksteps = max(2, args.steps // 3)
         for _ in range(ksteps):
-            p1, p2 = step(cv, xd, yd, hl, "fp64")
+            p1, p2 = step_pair(pair, xd, yd, hl, "fp64")
         torch.cuda.synchronize()
         tp1 = time.perf_counter()
         plain_fits = 2.0 * ksteps / (tp1 - tp0)
@@ -246,7 +271,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": dict(WORKLOAD, form="chosen: float32 MFMA convolution, float64 spread/solve",
-                       fits_per_step=2),
+                       fits_per_step=2, call="cv_fastlpr_pair (second pass shares the DOF)"),
         "roofline": {"bound": "mfma", "kernel": "flpr_conv",
                      "achieved": achieved, "peak": PEAK_MFMA_FP32 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / (PEAK_MFMA_FP32 / 1e12) if conv else None,
@@ -256,6 +281,8 @@ def main():
         "cpu_baseline": cpu,
     }
     line.update(extra)
+    line["two_call_fits_per_s"] = two_call_fits
+    line["pair_vs_two_calls_yhat_max_rel_diff"] = pair_vs_calls
     print(json.dumps(line), flush=True)
 
 

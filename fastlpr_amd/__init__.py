@@ -2,7 +2,8 @@
 
 Public API (same names and semantics as the reference package ``fastlpr``):
 ``cv_fastlpr``, ``cv_fastkde``, ``get_hlist``, ``fastlpr_predict``,
-``fastlpr_interval``, ``RegressionOutput``, ``KDEOutput``.
+``fastlpr_interval``, ``RegressionOutput``, ``KDEOutput``; plus
+``cv_fastlpr_pair`` (the heteroscedastic mean + variance fit in one call).
 """
 from .bandwidth import get_hlist
 from .structures import KDEOutput, RegressionOutput
@@ -13,11 +14,11 @@ __version__ = "0.1.0"
 def __getattr__(name):
     # the API pulls in torch and the HIP library lazily
     if name in ("cv_fastlpr", "cv_fastkde", "clear_caches", "GridInterpolant",
-                "fastlpr_predict", "fastlpr_interval"):
+                "fastlpr_predict", "fastlpr_interval", "cv_fastlpr_pair"):
         from . import api
         return getattr(api, name)
     raise AttributeError(name)
 
 
 __all__ = ["cv_fastlpr", "cv_fastkde", "get_hlist", "RegressionOutput", "KDEOutput",
-           "clear_caches", "fastlpr_predict", "fastlpr_interval"]
+           "clear_caches", "fastlpr_predict", "fastlpr_interval", "cv_fastlpr_pair"]

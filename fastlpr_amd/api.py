@@ -333,7 +333,66 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
     dstd, accuracy, N, xrange, flag_power2, dof_interp_method,
     gcv_interp_method, interp_method; plus ``precision`` ('fp32' | 'fp64').
     """
+    return _fit(x, y, h, options)[0]
+
+
+def cv_fastlpr_pair(x, y, h: Optional[Sequence[Sequence[float]]] = None,
+                    options: Optional[dict] = None,
+                    variance_options: Optional[dict] = None):
+    """The heteroscedastic two-pass fit (examples/example_hetero.py:70-135):
+
+        mu    = cv_fastlpr(x, y, h, options)
+        sigma = cv_fastlpr(x, (y - mu.yhat)**2, h,
+                           {**options, **variance_options, 'y_type_out': 'variance'})
+
+    returned as (mu, sigma), equal to those two calls.  The degrees of freedom
+    of a fit depend only on x, the bandwidths, the order, the grid and the
+    probes, not on y (regression.py:1095-1354), so when the variance options
+    leave all of those unchanged the second pass reuses the first pass's
+    probe sums instead of spreading, convolving and solving the 10 probe
+    channels again.  Real single-column y only.
+    """
     as_numpy = not (_is_dev(x) or _is_dev(y))
+    x_arr = _as_column_matrix(x)
+    dev = x.device if _is_dev(x) else (y.device if _is_dev(y) else _device())
+    xd = (x_arr.to(dev) if isinstance(x_arr, torch.Tensor)
+          else torch.as_tensor(x_arr, device=dev)).to(torch.float64).contiguous()
+    yt = y if isinstance(y, torch.Tensor) else torch.as_tensor(np.asarray(y))
+    if yt.is_complex() or (yt.dim() == 2 and yt.shape[0] > 1 and yt.shape[1] > 1):
+        raise NotImplementedError("cv_fastlpr_pair takes a real single-column response")
+    yd = yt.to(dev).to(torch.float64).reshape(-1).contiguous()
+    mu, share = _fit(xd, yd, h, options, as_numpy=False)
+    vopts = dict(options or {})
+    vopts.update(variance_options or {})
+    vopts["y_type_out"] = "variance"
+    r2 = (yd - mu.yhat) ** 2
+    sigma, _ = _fit(xd, r2, h, vopts, prior=share, as_numpy=False)
+    if as_numpy:
+        mu, sigma = _to_numpy(mu), _to_numpy(sigma)
+    return mu, sigma
+
+
+def _to_numpy(r: RegressionOutput) -> RegressionOutput:
+    def cv(t):
+        return t.cpu().numpy() if isinstance(t, torch.Tensor) else t
+    md = dict(r.metadata)
+    for k in ("fitted_at_samples", "residuals"):
+        md[k] = cv(md[k])
+    interp = r.fpp_yhat
+    if isinstance(interp, GridInterpolant):
+        interp._as_numpy = True
+    return RegressionOutput(yhat=cv(r.yhat), grid=r.grid, h=r.h, gcv_yhat=r.gcv_yhat,
+                            fpp_yhat=interp, metadata=md, dof=r.dof, dof_stderr=r.dof_stderr,
+                            d=r.d, s0=cv(r.s0))
+
+
+def _fit(x, y, h, options, prior=None, as_numpy=None):
+    """cv_fastlpr's body.  Returns (result, share): share identifies the
+    x-side of the DOF estimate (the key) and holds the probe sums on the
+    device, so that a later fit of the same x with a matching key (prior)
+    reuses them instead of computing the probe channels again."""
+    if as_numpy is None:
+        as_numpy = not (_is_dev(x) or _is_dev(y))
     x_arr = _as_column_matrix(x)
     if isinstance(y, torch.Tensor):
         y_arr = y
@@ -413,6 +472,8 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
     flag_power2 = bool(opts.get("flag_power2", True))
     lin_default = "linear" if (dims == 1 or n >= 5000) else "cubic"
 
+    share = {"key": None, "probe_dots": None}
+    reuse = False
     if not calc_dof:
         method = opts.get("interp_method", "linear")
         if method != "linear":
@@ -445,10 +506,27 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
         # option says (api.py:569-576 calls compute_design_matrix without one,
         # regression.py:86-87, 756-758); only y takes the option's (api.py:632-642)
         acc_n = max(6 - int(np.ceil(np.log10(n))), 4)
-        fit = engine.run_grid(xd, yd, ny, probes, K, bandwidths, order, N, acc_n,
+        key = (xd.data_ptr(), xd._version, xd.shape, np.asarray(bandwidths).tobytes(), order,
+               tuple(N), acc_n, fp64, reg, repr(xrange), flag_power2, K, probes.data_ptr(),
+               y_is_complex)
+        reuse = (prior is not None and not cubic and prior["key"] == key)
+        fit = engine.run_grid(xd, yd, ny, None if reuse else probes, 0 if reuse else K,
+                              bandwidths, order, N, acc_n,
                               fp64, reg, xrange, flag_power2, want_rss=True, stats=stats,
                               readback=False, keep_all=cubic,
                               accuracy_y=int(opts["accuracy"]), y_pairs=y_is_complex)
+        dh = fit.h.shape[0]
+        if reuse:
+            # y dots, the first pass's probe dots, then the quadratic forms
+            red = torch.empty((dh * (ny + K) + dh * ny,), dtype=torch.float64, device=dev)
+            dots = red[:dh * (ny + K)].view(dh, ny + K)
+            dots[:, :ny] = fit.red[:dh * ny].view(dh, ny)
+            dots[:, ny:] = prior["probe_dots"]
+            red[dh * (ny + K):] = fit.red[dh * ny:]
+            fit.red = red
+        elif not cubic:
+            share["key"] = key
+            share["probe_dots"] = fit.red[:dh * (ny + K)].view(dh, ny + K)[:, ny:].clone()
         ysq_dev = stats.ysq_dev
         if cubic:
             fit, ysq_dev = _point_sums(fit, xd, yd, probes, ny, K, gcv_method, dof_method)
@@ -526,10 +604,11 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
 
     metadata = {"options": opts, "order": order, "bandwidths_tested": fit.h,
                 "fitted_at_samples": conv(yhat), "residuals": conv(resid),
-                "mean": geo.mean, "std": geo.std, "dof": dof_sel, "dof_stderr": dof_err_sel}
+                "mean": geo.mean, "std": geo.std, "dof": dof_sel, "dof_stderr": dof_err_sel,
+                "dof_shared": reuse}
     return RegressionOutput(yhat=conv(yhat), grid=tuple(geo.axes_raw), h=fit.h[sel],
                             gcv_yhat=gcv_result, fpp_yhat=interp, metadata=metadata,
-                            dof=dof_sel, dof_stderr=dof_err_sel, d=d_norm, s0=conv(s0))
+                            dof=dof_sel, dof_stderr=dof_err_sel, d=d_norm, s0=conv(s0)), share
 
 
 # ---------------------------------------------------------------------------
