@@ -361,7 +361,7 @@ def cv_fastlpr_pair(x, y, h: Optional[Sequence[Sequence[float]]] = None,
     if yt.is_complex() or (yt.dim() == 2 and yt.shape[0] > 1 and yt.shape[1] > 1):
         raise NotImplementedError("cv_fastlpr_pair takes a real single-column response")
     yd = yt.to(dev).to(torch.float64).reshape(-1).contiguous()
-    mu, share = _fit(xd, yd, h, options, as_numpy=False)
+    mu, share = _fit(xd, yd, h, options, as_numpy=False, keep_x=True)
     vopts = dict(options or {})
     vopts.update(variance_options or {})
     vopts["y_type_out"] = "variance"
@@ -369,6 +369,7 @@ def cv_fastlpr_pair(x, y, h: Optional[Sequence[Sequence[float]]] = None,
     sigma, _ = _fit(xd, r2, h, vopts, prior=share, as_numpy=False)
     if as_numpy:
         mu, sigma = _to_numpy(mu), _to_numpy(sigma)
+    share.clear()
     return mu, sigma
 
 
@@ -386,11 +387,12 @@ def _to_numpy(r: RegressionOutput) -> RegressionOutput:
                             d=r.d, s0=cv(r.s0))
 
 
-def _fit(x, y, h, options, prior=None, as_numpy=None):
+def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
     """cv_fastlpr's body.  Returns (result, share): share identifies the
-    x-side of the DOF estimate (the key) and holds the probe sums on the
-    device, so that a later fit of the same x with a matching key (prior)
-    reuses them instead of computing the probe channels again."""
+    x-side of the fit (the key) and holds the probe sums on the device, and
+    with keep_x the rest of the x-side state (engine.run_grid share_out), so
+    that a later fit of the same x with a matching key (prior) only spreads,
+    convolves and solves its own response columns (engine.run_grid_y)."""
     if as_numpy is None:
         as_numpy = not (_is_dev(x) or _is_dev(y))
     x_arr = _as_column_matrix(x)
@@ -507,14 +509,21 @@ def _fit(x, y, h, options, prior=None, as_numpy=None):
         # regression.py:86-87, 756-758); only y takes the option's (api.py:632-642)
         acc_n = max(6 - int(np.ceil(np.log10(n))), 4)
         key = (xd.data_ptr(), xd._version, xd.shape, np.asarray(bandwidths).tobytes(), order,
-               tuple(N), acc_n, fp64, reg, repr(xrange), flag_power2, K, probes.data_ptr(),
-               y_is_complex)
+               tuple(N), acc_n, int(opts["accuracy"]), fp64, reg, repr(xrange), flag_power2, K,
+               probes.data_ptr(), y_is_complex, ny)
         reuse = (prior is not None and not cubic and prior["key"] == key)
-        fit = engine.run_grid(xd, yd, ny, None if reuse else probes, 0 if reuse else K,
-                              bandwidths, order, N, acc_n,
-                              fp64, reg, xrange, flag_power2, want_rss=True, stats=stats,
-                              readback=False, keep_all=cubic,
-                              accuracy_y=int(opts["accuracy"]), y_pairs=y_is_complex)
+        xshare = {} if (keep_x and not cubic) else None
+        if reuse and prior.get("xshare"):
+            fit = engine.run_grid_y(prior["xshare"], xd, yd, ny, y_is_complex)
+        else:
+            fit = engine.run_grid(xd, yd, ny, None if reuse else probes, 0 if reuse else K,
+                                  bandwidths, order, N, acc_n,
+                                  fp64, reg, xrange, flag_power2, want_rss=True, stats=stats,
+                                  readback=False, keep_all=cubic,
+                                  accuracy_y=int(opts["accuracy"]), y_pairs=y_is_complex,
+                                  share_out=xshare)
+        if xshare:
+            share["xshare"] = xshare
         dh = fit.h.shape[0]
         if reuse:
             # y dots, the first pass's probe dots, then the quadratic forms

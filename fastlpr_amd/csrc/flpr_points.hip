@@ -353,18 +353,20 @@ __global__ __launch_bounds__(256) void k_pack(FlprGeom g, const double* __restri
                                               int32_t* __restrict__ rawi,
                                               double* __restrict__ rawf) {
   const int nv = ny + np;
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n && (wab || rawi);
        p += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = order[p];
 #pragma unroll
     for (int k = 0; k < D; ++k) {
       const double xv = x[i * D + k];
-      int m;
-      const double xm = knot_of(g, k, xv, &m);
-      const double delta = xm - g.hx[k] * (double)m;
-      const double c = g.inv4tau[k];
-      wab[(p * D + k) * 2] = exp(-((delta * delta + 2.0 * g.msp * g.hx[k] * delta) * c));
-      wab[(p * D + k) * 2 + 1] = exp(2.0 * delta * g.hx[k] * c);
+      if (wab) {
+        int m;
+        const double xm = knot_of(g, k, xv, &m);
+        const double delta = xm - g.hx[k] * (double)m;
+        const double c = g.inv4tau[k];
+        wab[(p * D + k) * 2] = exp(-((delta * delta + 2.0 * g.msp * g.hx[k] * delta) * c));
+        wab[(p * D + k) * 2 + 1] = exp(2.0 * delta * g.hx[k] * c);
+      }
       if (rawi) {
         int ri;
         double rf;
@@ -1321,7 +1323,7 @@ int flpr_points_pack(const FlprGeom* g, const double* x, const int32_t* order, i
                      const double* vals_y, int32_t ny, const double* probes, int32_t nprobe,
                      double* wab, double* vals, int32_t* rawi, double* rawf, void* stream) {
   FLPR_CLEAR_ERROR();
-  if (!flpr_geom_ok(g) || !x || !order || !wab || (!rawi != !rawf) || n < 1 ||
+  if (!flpr_geom_ok(g) || !x || !order || (!rawi != !rawf) || n < 1 ||
       ny < 0 || nprobe < 0 || ((ny + nprobe) > 0 && !vals) || (ny > 0 && !vals_y) ||
       (nprobe > 0 && !probes))
     return -1;

@@ -285,6 +285,62 @@ class Plan:
             out.append(Pass(axis, mode, K, nout, inner, base, max_ro, jt, k, k - 1, jh))
         return out
 
+    def y_pass_tables(self, ny: int):
+        """Passes that convolve only ny new value channels (a second fit of the
+        same x): input = the ny centred channels of their own spread grid,
+        intermediate stages hold only the tuples with |beta| <= order (ny
+        channels each, compact), and the final T moments are written into this
+        plan's stage-0 slabs at channels 1..ny.  Returns (per-stage compact
+        sizes per bandwidth, one pass list per chunk); memoised per ny."""
+        memo = self.__dict__.setdefault("_ypass", {})
+        if ny in memo:
+            return memo[ny]
+        d = self.d
+        yslabs: List[Dict[Tuple[int, ...], int]] = [None] * (d + 1)
+        yper = [0] * (d + 1)
+        yslabs[d] = {(): 0}
+        for k in range(d - 1, 0, -1):
+            off, slabs = 0, {}
+            for beta in yslabs[k + 1]:
+                for a in range(self.order - sum(beta) + 1):
+                    slabs[(a,) + beta] = off
+                    off += ny * self._slab_cells(k)
+            yslabs[k], yper[k] = slabs, off
+        nodes = self.nodes
+        tables = []
+        for (h0, h1) in self.chunks:
+            out: List[Pass] = []
+            for k in range(d, 0, -1):
+                axis = k - 1
+                mode = 0 if axis == d - 1 else 1
+                K = self.G[axis]
+                nout = self.N[axis]
+                inner = int(np.prod(self.N[axis + 1:])) if axis + 1 < d else 1
+                prefix = int(np.prod(self.G[:axis])) if axis > 0 else 1
+                L, N = self.L[axis], self.N[axis]
+                base = (-(-(L - N) // 2) - ((L - N) // 2 + 1)) + self.E[axis]
+                ro = ny * prefix
+                jobs = []
+                for hl in range(h1 - h0):
+                    hg = h0 + hl
+                    for beta, ioff in yslabs[k].items():
+                        for a in range(self.order - sum(beta) + 1):
+                            nb = (a,) + beta
+                            in_off = 0 if k == d else hl * yper[k] + ioff
+                            if k - 1 == 0:
+                                out_off = hg * self.stage_per_h[0] + self.stage_slabs[0][nb][0] \
+                                    + nodes
+                            else:
+                                out_off = hl * yper[k - 1] + yslabs[k - 1][nb]
+                            tap_off = ((hg * d + axis) * self.npow + a) * self.nmax
+                            jobs.append([in_off, out_off, tap_off, hg * d + axis, ro, 0, 0, 0])
+                jh = np.asarray(jobs, dtype=np.int64)
+                jt = torch.tensor(jh, device=self.device)
+                out.append(Pass(axis, mode, K, nout, inner, base, ro, jt, k, k - 1, jh))
+            tables.append(out)
+        memo[ny] = (yper, tables)
+        return memo[ny]
+
 
 _PLANS: Dict[tuple, Plan] = {}
 
@@ -684,7 +740,7 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
              debug: Optional[dict] = None, readback: bool = True,
              moments_only: bool = False, kde: bool = False,
              keep_all: bool = False, accuracy_y: Optional[int] = None,
-             y_pairs: bool = False) -> DeviceFit:
+             y_pairs: bool = False, share_out: Optional[dict] = None) -> DeviceFit:
     """Spread, convolve and solve for all bandwidths, plus the probe dot products and
     the residual sums.
 
@@ -709,6 +765,10 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     Accuracies above 6 spread onto the oversampled grid and deconvolve
     (spread_fine); y_pairs says that the y columns are (re, im) pairs of
     complex responses, which that deconvolution must keep together.
+    share_out: when a dict is given (linear interpolation paths only), it
+    receives the x-side state of the fit (sorted points, the ones channel,
+    the moment slabs, lambda, the interpolation Gram matrices) for
+    run_grid_y.
     """
     n, d = x.shape
     dev = x.device
@@ -893,6 +953,15 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
                      dot=dot, m_y=m_y, plan=plan0, geo=geo, h=h, order_idx=order_idx,
                      key_start=key_start, lam_scale=lam_scale, eps_add=eps_add, shift=shift,
                      gram=gram, quad=quad, groups=groups)
+    if share_out is not None and not keep_all and gram is not None:
+        share_out.update(
+            geo=geo, x=x, n=n, order=order, order_idx=order_idx, key_start=key_start, wab=wab,
+            rawi=rawi, rawf=rawf, grid64=grid64, sums=sums, cells=cells, pad=pad,
+            stage0s=stage0s, key_max=key_max, gram=gram, s0_all=s0_all, h=h, warn=warn,
+            tap64=tap64, tap32=tap32, per_tap=per_tap, bands=bands, band_np=band_np,
+            lam_scale=lam_scale, eps_add=eps_add, fine_s=fine_s, acc_s=acc_s, acc_y=acc_y,
+            split_acc=split_acc, margin=margin, st=st, xrange=xrange, flag_power2=flag_power2,
+            kde=kde, N=tuple(N), plan0=plan0, any32=any32)
     if not readback:
         return DeviceFit(h, m_y.view(dh, nyc, nodes), None, None, s0_all, geo, plan0, warn, red)
     red_h = None if red is None else red.cpu().numpy()
@@ -906,6 +975,111 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
         rss_h = ysq[None, :] - 2.0 * dot_h[:, :ny] + quad_h
     probe_dot = None if (dot_h is None or K == 0) else dot_h[:, ny:]
     return DeviceFit(h, m_y.view(dh, nyc, nodes), rss_h, probe_dot, s0_all, geo, plan0, warn, red)
+
+
+def run_grid_y(sh: dict, x: torch.Tensor, vals_y: torch.Tensor, ny: int,
+               y_pairs: bool = False) -> DeviceFit:
+    """A second fit of the same x, bandwidths, order, grid, accuracy and
+    precision as the run_grid call that filled sh, for new response columns
+    vals_y (n, ny), with the residual sums and without probes.  Everything
+    that depends only on x is taken from sh: the sorted points, the ones
+    channel of the spread grid, the S moments (and the ones channel's T
+    moments for the centring), lambda, the interpolation Gram matrices and
+    S_00.  The new channels are spread, centred on the ones channel,
+    convolved into the T slots of sh's moment slabs (y-only passes), solved
+    and reduced; the DeviceFit is the one run_grid(readback=False) returns,
+    with (dh, ny) dots and (dh, ny) quadratic forms in red."""
+    assert x is sh["x"] and x.shape[0] == sh["n"]
+    n, d = x.shape
+    dev = x.device
+    s = stream_handle()
+    geo = sh["geo"]
+    g = geo.geom
+    gp = _lib.C.byref(g)
+    f64 = torch.float64
+    cells, pad = sh["cells"], sh["pad"]
+    order = sh["order"]
+    h = sh["h"]
+    dh = h.shape[0]
+    C2 = 1 + ny
+    # --- the new columns in sorted order, spread next to the first fit's ones channel
+    vals = torch.empty((ny * n,), dtype=f64, device=dev)
+    acc_s, acc_y = sh["acc_s"], sh["acc_y"]
+    ntiles = int(np.prod([g.ntile[k] for k in range(d)]))
+    grid64 = torch.empty((C2 * cells + pad,), dtype=f64, device=dev)
+    grid64[:cells].copy_(sh["grid64"][:cells])
+    sums = torch.zeros((C2,), dtype=f64, device=dev)
+    sums[:1].copy_(sh["sums"][:1])
+    if acc_y > 6:
+        call("flpr_points_pack", gp, ptr(x), ptr(sh["order_idx"]), n, ptr(vals_y), ny, None, 0,
+             None, ptr(vals), None, None, s)
+        spread_fine(geo, acc_y, x, n, vals_y, ny, y_pairs, None, 0, False, grid64, sums, 1, s)
+    else:
+        if sh["split_acc"]:
+            geo_y, _ = geometry_for(sh["st"], sh["N"], acc_y, sh["xrange"], sh["flag_power2"],
+                                    sh["kde"], sh["margin"])
+            gy = _lib.C.byref(geo_y.geom)
+            wab = torch.empty((2 * d * n,), dtype=f64, device=dev)
+            call("flpr_points_pack", gy, ptr(x), ptr(sh["order_idx"]), n, ptr(vals_y), ny,
+                 None, 0, ptr(wab), ptr(vals), None, None, s)
+        else:
+            gy, wab = gp, sh["wab"]
+            call("flpr_points_pack", gp, ptr(x), ptr(sh["order_idx"]), n, ptr(vals_y), ny,
+                 None, 0, None, ptr(vals), None, None, s)
+        fcells = int(np.prod([g.tile[k] + 2 * acc_y for k in range(d)]))
+        part = torch.empty((ntiles * ny * fcells,), dtype=f64, device=dev)
+        call("flpr_spread", gy, ptr(sh["key_start"]), ptr(wab), ptr(vals), n, 0, ny, ptr(part),
+             grid64.data_ptr() + cells * 8, sums.data_ptr() + 8, s)
+        del part
+    shift = torch.empty((C2,), dtype=f64, device=dev)
+    any32 = sh["any32"]
+    grid32 = torch.empty((C2 * cells + pad,), dtype=torch.float32, device=dev) if any32 else None
+    call("flpr_grid_center", ptr(grid64), C2, cells, ptr(sums), ptr(shift), ptr(grid32), s)
+    # --- T moments of the new channels into the first fit's slabs
+    per_tap = sh["per_tap"]
+    bands = sh["bands"]
+    for (a, b, gf, plan, stage0) in sh["stage0s"]:
+        fdt = f64 if gf else torch.float32
+        src_grid = grid64 if gf else grid32
+        esz = 8 if gf else 4
+        tap = sh["tap64"] if gf else sh["tap32"]
+        tap_ptr = tap.data_ptr() + a * per_tap * tap.element_size()
+        band_ptr = bands.data_ptr() + a * d * 2 * 4
+        bands_g = sh["band_np"][a * d * 2:b * d * 2]
+        yper, tables = plan.y_pass_tables(ny)
+        inter = {k: torch.empty((plan.chunk * yper[k] + pad,), dtype=fdt, device=dev)
+                 for k in range(1, d)}
+        for ci in range(len(plan.chunks)):
+            for ps in tables[ci]:
+                src = (src_grid.data_ptr() + cells * esz) if ps.in_stage == d \
+                    else ptr(inter[ps.in_stage])
+                dst = stage0 if ps.out_stage == 0 else inter[ps.out_stage]
+                TIMER.timed("flpr_conv", (lambda ps=ps, bg=bands_g: conv_flops(ps, bg)),
+                            call, "flpr_conv", ps.mode, int(gf), src, ptr(dst), tap_ptr,
+                            ptr(ps.jobs), band_ptr, int(ps.jobs.shape[0]), ps.max_ro, ps.K,
+                            ps.nout, ps.inner, ps.base, s)
+        del inter
+    # --- adjoint sums of the new columns, solve (+ dots), residual quadratic form
+    plan0 = sh["plan0"]
+    nodes = plan0.nodes
+    per_h0 = plan0.stage_per_h[0]
+    nw = (1 << d) * ((1 << d) + 1) // 2
+    red = torch.zeros((2 * dh * ny,), dtype=f64, device=dev)
+    q = torch.zeros((ny * nodes,), dtype=f64, device=dev)
+    wcells = int(np.prod([g.tile[k] + 5 for k in range(d)]))
+    apart = torch.empty((ntiles * max(ny, nw) * wcells,), dtype=f64, device=dev)
+    call("flpr_adjoint", gp, ptr(sh["key_start"]), ptr(sh["rawi"]), ptr(sh["rawf"]), ptr(vals), n,
+         ny, ptr(apart), ptr(q), None, s)
+    del apart
+    m_y = torch.empty((dh * ny * nodes,), dtype=f64, device=dev)
+    for (a, b, gf, plan, stage0) in sh["stage0s"]:
+        call("flpr_solve", int(gf), ptr(stage0), per_h0, b - a, nodes, order, plan.nt,
+             int(sh["fine_s"]), i64_array(plan.s_off), i64_array(plan.t_off),
+             ptr(sh["key_max"]), sh["lam_scale"], sh["eps_add"], ptr(shift), ny, ny,
+             m_y.data_ptr() + a * ny * nodes * 8, ptr(q), red.data_ptr() + a * ny * 8, s)
+    call("flpr_quadform", gp, ptr(m_y), ny, dh, ptr(sh["gram"]), red.data_ptr() + dh * ny * 8, s)
+    return DeviceFit(h, m_y.view(dh, ny, nodes), None, None, sh["s0_all"], geo, plan0,
+                     sh["warn"], red)
 
 
 def lcv(geo: Geometry, x: torch.Tensor, dens: torch.Tensor, k0h: np.ndarray):

@@ -98,7 +98,8 @@ int flpr_points_bin(const int32_t* key, int64_t n, int32_t nkeys, const int32_t*
  *   vals (n, ny + nprobe) float64  y columns then probe columns
  *   rawi (n, d) int32, rawf (n, d) float64  interval and fraction on the raw grid
  *                         axis linspace(graw0, graw1, N) (RegularGridInterpolator);
- *                         both may be NULL (not written). */
+ *                         both may be NULL (not written); wab may be NULL too (a
+ *                         second set of values for points already sorted). */
 int flpr_points_pack(const FlprGeom* g, const double* x, const int32_t* order, int64_t n,
                      const double* vals_y, int32_t ny, const double* probes, int32_t nprobe,
                      double* wab, double* vals, int32_t* rawi, double* rawf, void* stream);

@@ -118,16 +118,28 @@ def _emulate(x, y, h, order, N, acc=None):
                 tap[o:o + geo.N[k]] = e * u ** a
     bands = E.tap_bands(geo, h, True).reshape(-1)
     stage0 = np.zeros(dh * plan.stage_per_h[0])
-    for ci in range(len(plan.chunks)):
-        inter = {k: np.zeros(plan.chunk * plan.stage_per_h[k]) for k in range(1, d)}
-        for ps in plan.pass_tables[ci]:
-            src = grid.reshape(-1) if ps.in_stage == d else inter[ps.in_stage]
+    _run(plan.pass_tables, [plan.chunk * plan.stage_per_h[k] for k in range(d + 1)],
+         grid.reshape(-1), stage0, tap, bands, d)
+    return plan, stage0, h, acc, dict(geo=geo, tap=tap, bands=bands, xmod=xmod, cell=cell)
+
+
+def _run(tables, inter_sizes, grid, stage0, tap, bands, d):
+    """flpr_conv semantics over a plan's pass tables; every job's input and
+    output range is checked against its buffer."""
+    for ci in range(len(tables)):
+        inter = {k: np.zeros(inter_sizes[k]) for k in range(1, d)}
+        for ps in tables[ci]:
+            src = grid if ps.in_stage == d else inter[ps.in_stage]
             dst = stage0 if ps.out_stage == 0 else inter[ps.out_stage]
             for rec in ps.jobs_host:
                 ioff, ooff, toff, band, ro = (int(v) for v in rec[:5])
                 lo, hi = bands[2 * band], bands[2 * band + 1]
                 t = np.arange(ps.nout)[:, None] + ps.base - np.arange(ps.K)[None, :]
                 A = np.where((t >= lo) & (t <= hi), tap[np.clip(toff + t, 0, len(tap) - 1)], 0.0)
+                span_in = ro * ps.K * (ps.inner if ps.mode == 1 else 1)
+                span_out = ro * ps.nout * (ps.inner if ps.mode == 1 else 1)
+                assert 0 <= ioff and ioff + span_in <= len(src)
+                assert 0 <= ooff and ooff + span_out <= len(dst)
                 if ps.mode == 0:
                     In = src[ioff:ioff + ro * ps.K].reshape(ro, ps.K)
                     dst[ooff:ooff + ro * ps.nout] = (In @ A.T).reshape(-1)
@@ -135,7 +147,6 @@ def _emulate(x, y, h, order, N, acc=None):
                     In = src[ioff:ioff + ro * ps.K * ps.inner].reshape(ro, ps.K, ps.inner)
                     dst[ooff:ooff + ro * ps.nout * ps.inner] = \
                         np.einsum("is,osc->oic", A, In).reshape(-1)
-    return plan, stage0, h, acc
 
 
 CASES = [(1, o, [40]) for o in (0, 1, 2)] + [(1, 1, [33])] + \
@@ -152,7 +163,7 @@ def test_banded_plan_reproduces_fft_moments(d, order, N, acc):
     x = rng.uniform(0, 1, (n, d))
     y = np.sin(6 * x[:, 0]) + 0.1 * rng.standard_normal(n)
     h = np.array([[0.05] * d, [0.2] * d, [0.6] * d]) if d == 1 else np.array([[0.1, 0.1], [0.3, 0.5]])
-    plan, st0, h, acc = _emulate(x, y, h, order, N, acc)
+    plan, st0, h, acc, _ = _emulate(x, y, h, order, N, acc)
     P = R.Problem(x, h, {"order": order, "accuracy": acc, "N": N}, "f64")
     S = P.design(add_eps=False)
     T = P.weighted(y, True)
@@ -165,6 +176,53 @@ def test_banded_plan_reproduces_fft_moments(d, order, N, acc):
         em = np.stack([st0[b * per_h + off + nodes:b * per_h + off + 2 * nodes] for b in range(dh)], -1)
         ref = T[i][..., 0].reshape(nodes, -1)
         assert np.max(np.abs(em - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("d,order,N", [(1, 1, [40]), (1, 2, [33]), (2, 2, [12, 10]),
+                                       (2, 1, [11, 9])])
+def test_y_only_passes_write_the_new_channel_moments(d, order, N):
+    """Plan.y_pass_tables (a second fit of the same x): the passes convolve
+    only the new channel and write its T moments into channel 1 of the first
+    fit's stage-0 slabs, leaving every S moment and the ones channel intact."""
+    rng = np.random.default_rng(30 + d + order)
+    n = 300
+    x = rng.uniform(0, 1, (n, d))
+    y = np.sin(6 * x[:, 0]) + 0.1 * rng.standard_normal(n)
+    y2 = np.cos(4 * x[:, -1]) + 0.2 * rng.standard_normal(n)
+    h = np.array([[0.05] * d, [0.2] * d, [0.6] * d]) if d == 1 else np.array([[0.1, 0.1], [0.3, 0.5]])
+    plan, st0, h, acc, aux = _emulate(x, y, h, order, N)
+    before = st0.copy()
+    geo, xmod, cell = aux["geo"], aux["xmod"], aux["cell"]
+    g = geo.geom
+    grid2 = np.zeros((1,) + plan.G)
+    j = np.arange(-geo.msp, geo.msp + 1)
+    for i in range(n):
+        ws, idx = [], []
+        for k in range(d):
+            m = cell[i, k]
+            pm = m - g.L[k] if m > g.L[k] // 2 else m
+            diff = xmod[i, k] - g.hx[k] * (m + j)
+            ws.append(np.exp(-(diff * diff * g.inv4tau[k])))
+            idx.append(pm + j + g.E[k])
+        w = ws[0] if d == 1 else np.outer(ws[0], ws[1])
+        sl = (0, idx[0]) if d == 1 else (0, idx[0][:, None], idx[1][None, :])
+        grid2[sl] += y2[i] * w
+    yper, tables = plan.y_pass_tables(1)
+    _run(tables, [plan.chunk * yper[k] for k in range(d + 1)], grid2.reshape(-1), st0,
+         aux["tap"], aux["bands"], d)
+    P = R.Problem(x, h, {"order": order, "accuracy": acc, "N": N}, "f64")
+    T2 = P.weighted(y2, True)
+    per_h, nodes, dh = plan.stage_per_h[0], plan.nodes, h.shape[0]
+    changed = np.zeros(st0.shape, bool)
+    for i, off in enumerate(plan.t_off):
+        for b in range(dh):
+            sl = slice(b * per_h + off + nodes, b * per_h + off + 2 * nodes)
+            changed[sl] = True
+        em = np.stack([st0[b * per_h + off + nodes:b * per_h + off + 2 * nodes]
+                       for b in range(dh)], -1)
+        ref = T2[i][..., 0].reshape(nodes, -1)
+        assert np.max(np.abs(em - ref)) <= 1e-12 * np.max(np.abs(ref))
+    assert np.array_equal(st0[~changed], before[~changed])
 
 
 def test_plan_stage_layout_counts():
