@@ -333,7 +333,8 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
     dstd, accuracy, N, xrange, flag_power2, dof_interp_method,
     gcv_interp_method, interp_method; plus ``precision`` ('fp32' | 'fp64').
     """
-    return _fit(x, y, h, options)[0]
+    finish, _ = _fit(x, y, h, options)
+    return finish()
 
 
 def cv_fastlpr_pair(x, y, h: Optional[Sequence[Sequence[float]]] = None,
@@ -361,15 +362,19 @@ def cv_fastlpr_pair(x, y, h: Optional[Sequence[Sequence[float]]] = None,
     if yt.is_complex() or (yt.dim() == 2 and yt.shape[0] > 1 and yt.shape[1] > 1):
         raise NotImplementedError("cv_fastlpr_pair takes a real single-column response")
     yd = yt.to(dev).to(torch.float64).reshape(-1).contiguous()
-    mu, share = _fit(xd, yd, h, options, as_numpy=False, keep_x=True)
+    # both passes are queued before either result is read back, so the device
+    # runs them back to back
+    finish_mu, share = _fit(xd, yd, h, options, as_numpy=False, keep_x=True)
     vopts = dict(options or {})
     vopts.update(variance_options or {})
     vopts["y_type_out"] = "variance"
-    r2 = (yd - mu.yhat) ** 2
-    sigma, _ = _fit(xd, r2, h, vopts, prior=share, as_numpy=False)
+    r2 = (yd - share["yhat"][:, 0]) ** 2
+    finish_sigma, _ = _fit(xd, r2, h, vopts, prior=share, as_numpy=False)
+    share.clear()
+    mu = finish_mu()
+    sigma = finish_sigma()
     if as_numpy:
         mu, sigma = _to_numpy(mu), _to_numpy(sigma)
-    share.clear()
     return mu, sigma
 
 
@@ -388,11 +393,15 @@ def _to_numpy(r: RegressionOutput) -> RegressionOutput:
 
 
 def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
-    """cv_fastlpr's body.  Returns (result, share): share identifies the
-    x-side of the fit (the key) and holds the probe sums on the device, and
-    with keep_x the rest of the x-side state (engine.run_grid share_out), so
-    that a later fit of the same x with a matching key (prior) only spreads,
-    convolves and solves its own response columns (engine.run_grid_y)."""
+    """cv_fastlpr's body, split in two: the call queues all device work and
+    returns (finish, share); finish() reads the results back (one host
+    synchronisation) and builds the RegressionOutput.  share identifies the
+    x side of the fit (the key), holds the probe sums and the fitted values
+    (share['yhat'], n x ny real columns) on the device and, with keep_x, the
+    rest of the x-side state (engine.run_grid share_out), so that a later fit
+    of the same x with a matching key (prior) only spreads, convolves and
+    solves its own response columns (engine.run_grid_y) and needs no x
+    statistics."""
     if as_numpy is None:
         as_numpy = not (_is_dev(x) or _is_dev(y))
     x_arr = _as_column_matrix(x)
@@ -457,14 +466,26 @@ def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
     ny = yd.shape[1]
     if ny > 32:
         raise NotImplementedError("the GPU engine takes at most 32 real response columns")
+    yd_in = yd
     if variance:
         yd = torch.log(yd + 1.0 / n)  # api.py:390-396
-    # one device pass: x statistics, non-finite checks of device inputs, sum y^2
-    stats = engine.x_stats(xd, yd)
-    if stats.bad_x:
-        _check_finite(x_arr, "Input x", "cv_fastlpr")
-    if stats.bad_y and isinstance(y_arr, torch.Tensor):
-        _check_finite(y_arr, "Response y", "cv_fastlpr")
+    xs = prior.get("xshare") if prior is not None else None
+    bad_y_dev = None
+    if xs and xs.get("x") is xd and prior["key"][:3] == (xd.data_ptr(), xd._version, xd.shape):
+        # the same x as the fit that left its x side in prior: its statistics,
+        # and y's sums of squares and non-finite check on the device (read
+        # back with the results)
+        s1 = xs["st"]
+        stats = engine.Stats(s1.xmin, s1.xmax, s1.mean, s1.std, 0, 0, None,
+                             (yd * yd).sum(dim=0))
+        bad_y_dev = (~torch.isfinite(yd_in)).sum().to(torch.float64)
+    else:
+        # one device pass: x statistics, non-finite checks of device inputs, sum y^2
+        stats = engine.x_stats(xd, yd)
+        if stats.bad_x:
+            _check_finite(x_arr, "Input x", "cv_fastlpr")
+        if stats.bad_y and isinstance(y_arr, torch.Tensor):
+            _check_finite(y_arr, "Response y", "cv_fastlpr")
     # default range 0.1..1 x std(ddof=1) from the device statistics (helpers.py:74-92)
     bandwidths = _prepare_bandwidths(
         h, dims, opts, lambda: np.column_stack((0.1 * stats.std, 1.0 * stats.std)))
@@ -579,15 +600,33 @@ def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
     resid = yref.reshape(-1) - yhat.reshape(-1) if dy == 1 or variance else \
         (yref - yhat).reshape(-1)
 
+    share["yhat"] = out
+    dh = fit.h.shape[0]
+    if sel_out is not None:
+        if fac is not None:
+            sel_out[4 * dh + 3:4 * dh + 4].copy_(fac.reshape(1))
+        if bad_y_dev is not None:
+            sel_out[4 * dh + 4:4 * dh + 5].copy_(bad_y_dev.reshape(1))
+
     def conv(t):
         return t.cpu().numpy() if as_numpy else t
 
-    # the one host read of the fit: selection results (and the variance scale)
+    def finish():
+        return _finish(fit, sel_out, fac, bad_y_dev, yd_in, dh, opts, order, geo, yhat, resid,
+                       s0, interp, reuse, conv)
+
+    return finish, share
+
+
+def _finish(fit, sel_out, fac, bad_y_dev, y_in, dh, opts, order, geo, yhat, resid, s0, interp,
+            reuse, conv):
+    """The one host read of a fit (selection results, the variance scale, the
+    deferred non-finite count of y) and the RegressionOutput."""
+    d_norm = None
     if sel_out is not None:
-        dh = fit.h.shape[0]
-        if fac is not None:
-            sel_out[4 * dh + 3:].copy_(fac.reshape(1))
         hv = sel_out.cpu().numpy()
+        if bad_y_dev is not None and hv[4 * dh + 4] > 0:
+            _check_finite(y_in, "Response y", "cv_fastlpr")
         if fac is not None:
             d_norm = 1.0 / float(hv[4 * dh + 3])
         if int(hv[4 * dh + 2]) == 0:
@@ -603,6 +642,9 @@ def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
                       "dof": dof_all, "dof_stderr": dof_se}
         dof_sel, dof_err_sel = float(dof_all[sel]), float(dof_se[sel])
     else:
+        sel = 0   # without DOF the first bandwidth is the result (api.py:436-470)
+        if bad_y_dev is not None and float(bad_y_dev.item()) > 0:
+            _check_finite(y_in, "Response y", "cv_fastlpr")
         if fac is not None:
             d_norm = 1.0 / float(fac.item())
         gcv_result = {"bandwidths": fit.h, "gcv_m": np.array([np.nan]),
@@ -617,7 +659,7 @@ def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
                 "dof_shared": reuse}
     return RegressionOutput(yhat=conv(yhat), grid=tuple(geo.axes_raw), h=fit.h[sel],
                             gcv_yhat=gcv_result, fpp_yhat=interp, metadata=metadata,
-                            dof=dof_sel, dof_stderr=dof_err_sel, d=d_norm, s0=conv(s0)), share
+                            dof=dof_sel, dof_stderr=dof_err_sel, d=d_norm, s0=conv(s0))
 
 
 # ---------------------------------------------------------------------------

@@ -1186,16 +1186,16 @@ def select(fit: DeviceFit, ysq_dev: torch.Tensor, pp_dev: torch.Tensor, n: int, 
            K: int, cplx: bool, aggregation: str, dstd: float):
     """Hutchinson DOF, GCV scores and the 1-SE choice on the device (flpr_select).
 
-    Returns (out, sel): out (4 dh + 4) float64 on the device holding scores,
+    Returns (out, sel): out (4 dh + 5) float64 on the device holding scores,
     stderrs, dof, dof stderrs, then selected index, argmin index, number of
-    finite scores and one spare slot; sel (1,) int32 on the device."""
+    finite scores and two spare slots; sel (1,) int32 on the device."""
     if aggregation not in AGGREGATIONS:
         raise ValueError(f"Unknown aggregation method: {aggregation}. "
                          "Use 'first', 'average', or 'sum'.")
     dh = fit.h.shape[0]
     dev = fit.m_y.device
     hsum = upload(fit.h.sum(axis=1), torch.float64, dev)
-    out = torch.empty((4 * dh + 4,), dtype=torch.float64, device=dev)
+    out = torch.empty((4 * dh + 5,), dtype=torch.float64, device=dev)
     sel = torch.empty((1,), dtype=torch.int32, device=dev)
     call("flpr_select", ptr(fit.red), dh, ny + K, ny, K, ptr(ysq_dev), ptr(pp_dev), ptr(hsum),
          n, int(cplx), AGGREGATIONS[aggregation], float(dstd), ptr(out), ptr(sel),

@@ -71,3 +71,24 @@ def test_pair_small_2d_uses_cubic_and_matches_two_calls():
         mu2 = fl.cv_fastlpr(x, y, h, dict(opt))
         sg2 = fl.cv_fastlpr(x, (y - mu2.yhat) ** 2, h, dict(opt, y_type_out="variance"))
     assert _rel(sg.yhat, sg2.yhat) < 1e-10
+
+
+def test_reused_x_side_still_rejects_non_finite_y():
+    """With the x side reused, y's non-finite check runs on the device and is
+    reported when the results are read back: the same ValueError as cv_fastlpr."""
+    import torch
+    from fastlpr_amd import api
+    x, y = _data(1, 3000)
+    h = np.logspace(np.log10(0.02), np.log10(0.5), 6)[:, None]
+    dev = torch.device("cuda", 0)
+    xd = torch.as_tensor(x, device=dev)
+    yd = torch.as_tensor(y, device=dev)
+    opt = {"order": 1, "N": 64}
+    finish, share = api._fit(xd, yd, h, opt, as_numpy=False, keep_x=True)
+    bad = yd.clone()
+    bad[17] = float("nan")
+    finish2, _ = api._fit(xd, bad, h, opt, prior=share, as_numpy=False)
+    finish()
+    with pytest.raises(ValueError, match="NaN"):
+        finish2()
+    share.clear()
