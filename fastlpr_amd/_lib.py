@@ -25,6 +25,7 @@ EXPORTS = (
     "flpr_grid_center",
     "flpr_taps",
     "flpr_conv",
+    "flpr_conv_tiles",
     "flpr_max_moment",
     "flpr_key_to_double",
     "flpr_solve",
@@ -83,10 +84,12 @@ _SIGS = {
     "flpr_points_pack": [_GP, _P, _P, _I64, _P, _I32, _P, _I32, _P, _P, _P, _P, _P],
     "flpr_spread": [_GP, _P, _P, _P, _I64, _I32, _I32, _P, _P, _P, _P],
     "flpr_resample_work": [_GP, _GP, _I32, _I32, _P],
+    "flpr_conv_tiles": [_I32, _I32, _I32, _I32, _I32, _P, _P],
     "flpr_resample": [_GP, _GP, _P, _I32, _I32, _P, _P, _P, _P],
     "flpr_grid_center": [_P, _I32, _I64, _P, _P, _P, _P],
     "flpr_taps": [_GP, _P, _I32, _I32, _I32, _D, _P, _P, _P],
-    "flpr_conv": [_I32, _I32, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P],
+    "flpr_conv": [_I32, _I32, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P,
+                  _I32, _P],
     "flpr_max_moment": [_I32, _P, _I64, _I32, C.POINTER(_I64), _I32, _I64, _I32, _P, _P],
     "flpr_solve": [_I32, _P, _I64, _I32, _I64, _I32, _I32, _I32, C.POINTER(_I64), C.POINTER(_I64), _P,
                    _D, _D, _P, _I32, _I32, _P, _P, _P, _P],

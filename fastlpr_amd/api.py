@@ -220,6 +220,7 @@ def clear_caches():
     engine._PLANS.clear()
     engine._GEOMS.clear()
     engine._BTABS.clear()
+    engine._SCHED.clear()
 
 
 # ---------------------------------------------------------------------------

@@ -137,12 +137,17 @@ struct ConvCfg<double> {
 // zero, so the loads carry no bounds branches; the input buffer must stay
 // readable LD*K + KC*inner + LD elements past each job's input (the caller
 // pads its allocations).
+//
+// Block schedule: with a block table (blocks != nullptr) block b runs tile
+// blocks[2b+1] of job blocks[2b] (a negative job is an idle block); without
+// one, the grid is (tiles of the largest job, jobs).
 template <typename T, int MODE>
 __global__ __launch_bounds__(CT) void k_conv(const T* __restrict__ in, T* __restrict__ out,
                                              const T* __restrict__ tap,
                                              const int64_t* __restrict__ jobs,
                                              const int32_t* __restrict__ bands, int K, int nout,
-                                             int inner, int base, int ntd, int ntt) {
+                                             int inner, int base, int ntd, int ntt,
+                                             const int32_t* __restrict__ blocks) {
   constexpr int KC = ConvCfg<T>::KC;
   // LDS pitch of the dense tile: a K step reads 16 rows x 4 K (ROW pass) or
   // 4 K rows x 16 columns (COL pass); the pads spread each half-wave over 32 banks
@@ -156,11 +161,16 @@ __global__ __launch_bounds__(CT) void k_conv(const T* __restrict__ in, T* __rest
   constexpr bool F32 = sizeof(T) == 4;
   __shared__ T dense[2][DENSE];
   __shared__ T strip[2][NSTRIP + 1];
-  const int64_t* jb = jobs + 8 * (int64_t)blockIdx.y;
+  int job = blockIdx.y, bx = blockIdx.x;
+  if (blocks) {
+    job = blocks[2 * blockIdx.x];
+    bx = blocks[2 * blockIdx.x + 1];
+    if (job < 0) return;
+  }
+  const int64_t* jb = jobs + 8 * (int64_t)job;
   const int64_t in_off = jb[0], out_off = jb[1], tap_off = jb[2];
   const int band = (int)jb[3], ro = (int)jb[4];
   const int t_lo = bands[2 * band], t_hi = bands[2 * band + 1];
-  int bx = blockIdx.x;
   const int td = bx % ntd;
   bx /= ntd;
   const int tt = bx % ntt;
@@ -647,33 +657,55 @@ int flpr_taps(const FlprGeom* g, const double* h, int32_t dh, int32_t npow, int3
   return 0;
 }
 
+int flpr_conv_tiles(int32_t mode, int32_t rows_or_outer, int32_t max_rows_or_outer,
+                    int32_t nout, int32_t inner, int32_t* tiles, int32_t* tile_stride) {
+  if (!tiles || !tile_stride || rows_or_outer < 1 || max_rows_or_outer < rows_or_outer ||
+      nout < 1 || (mode != 0 && mode != 1) || (mode == 1 && inner < 1))
+    return -1;
+  const int ntt = (nout + LT - 1) / LT;
+  if (mode == 0) {
+    *tile_stride = (max_rows_or_outer + LD - 1) / LD;
+    *tiles = ((rows_or_outer + LD - 1) / LD) * ntt;
+  } else {
+    *tile_stride = (inner + LD - 1) / LD;
+    *tiles = *tile_stride * ntt * rows_or_outer;
+  }
+  return 0;
+}
+
 int flpr_conv(int32_t mode, int32_t fp64, const void* in, void* out, const void* tap,
               const int64_t* jobs, const int32_t* bands, int32_t nj, int32_t max_rows_or_outer,
-              int32_t K, int32_t nout, int32_t inner, int32_t base, void* stream) {
+              int32_t K, int32_t nout, int32_t inner, int32_t base, const int32_t* blocks,
+              int32_t nblocks, void* stream) {
   FLPR_CLEAR_ERROR();
   if (!in || !out || !tap || !jobs || !bands || nj < 1 || nj > 65535 || max_rows_or_outer < 1 || K < 1 ||
-      nout < 1 || (mode != 0 && mode != 1) || (mode == 1 && inner < 1))
+      nout < 1 || (mode != 0 && mode != 1) || (mode == 1 && inner < 1) ||
+      (blocks && nblocks < 1))
     return -1;
   const int ntd = (mode == 0) ? (max_rows_or_outer + LD - 1) / LD : (inner + LD - 1) / LD;
   const int ntt = (nout + LT - 1) / LT;
   const int64_t gx = (mode == 0) ? (int64_t)ntd * ntt : (int64_t)ntd * ntt * max_rows_or_outer;
   if (gx > INT32_MAX) return -1;
-  dim3 grd((unsigned)gx, (unsigned)nj);
+  dim3 grd = blocks ? dim3((unsigned)nblocks) : dim3((unsigned)gx, (unsigned)nj);
   hipStream_t st = as_stream(stream);
   if (fp64) {
     if (mode == 0)
       hipLaunchKernelGGL((k_conv<double, 0>), grd, dim3(CT), 0, st, (const double*)in,
-                         (double*)out, (const double*)tap, jobs, bands, K, nout, inner, base, ntd, ntt);
+                         (double*)out, (const double*)tap, jobs, bands, K, nout, inner, base, ntd, ntt,
+                         blocks);
     else
       hipLaunchKernelGGL((k_conv<double, 1>), grd, dim3(CT), 0, st, (const double*)in,
-                         (double*)out, (const double*)tap, jobs, bands, K, nout, inner, base, ntd, ntt);
+                         (double*)out, (const double*)tap, jobs, bands, K, nout, inner, base, ntd, ntt,
+                         blocks);
   } else {
     if (mode == 0)
       hipLaunchKernelGGL((k_conv<float, 0>), grd, dim3(CT), 0, st, (const float*)in, (float*)out,
-                         (const float*)tap, jobs, bands, K, nout, inner, base, ntd, ntt);
+                         (const float*)tap, jobs, bands, K, nout, inner, base, ntd, ntt,
+                         blocks);
     else
       hipLaunchKernelGGL((k_conv<float, 1>), grd, dim3(CT), 0, st, (const float*)in, (float*)out,
-                         (const float*)tap, jobs, bands, K, nout, inner, base, ntd, ntt);
+                         (const float*)tap, jobs, bands, K, nout, inner, base, ntd, ntt,
+                         blocks);
   }
   FLPR_CHECK_LAUNCH();
   return 0;

@@ -168,7 +168,18 @@ int flpr_taps(const FlprGeom* g, const double* h, int32_t dh, int32_t npow, int3
  * zeroed, never used). */
 int flpr_conv(int32_t mode, int32_t fp64, const void* in, void* out, const void* tap,
               const int64_t* jobs, const int32_t* bands, int32_t nj, int32_t max_rows_or_outer,
-              int32_t K, int32_t nout, int32_t inner, int32_t base, void* stream);
+              int32_t K, int32_t nout, int32_t inner, int32_t base, const int32_t* blocks,
+              int32_t nblocks, void* stream);
+
+/* Tiles of one flpr_conv job (rows_or_outer of the job, max_rows_or_outer of
+ * the launch) and the stride of the tile numbering.  With a block table
+ * (blocks, device int32 pairs, nblocks of them) block b runs tile
+ * blocks[2b+1] in [0, tiles) of job blocks[2b] (a negative job idles): the
+ * caller chooses the order and the placement of the tiles (the device hands
+ * consecutive blocks to its XCDs in turn).  Without one (blocks NULL) the
+ * launch runs every tile of every job, largest job first in x. */
+int flpr_conv_tiles(int32_t mode, int32_t rows_or_outer, int32_t max_rows_or_outer,
+                    int32_t nout, int32_t inner, int32_t* tiles, int32_t* tile_stride);
 
 /* Largest |S_kk| over all nodes, bandwidths and diagonal moments
  * (regression.py:243-267) when absval != 0, or the largest S (order-0

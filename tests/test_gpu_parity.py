@@ -202,3 +202,22 @@ def test_spline_eval_equals_exact_not_a_knot_spline(fl, d):
         ref = RegularGridInterpolator(tuple(geo.axes_raw), grids[k], method="cubic",
                                       solver=spsolve)(pts)
         np.testing.assert_allclose(got[:, k], ref, rtol=1e-10, atol=1e-10)
+
+
+@pytest.mark.parametrize("name", ["2d_o2_m33x40", "2d_o1_m40", "1d_o2_m128", "3d_o1_m14"])
+def test_block_scheduled_convolution_equals_oracle(fl, name, monkeypatch):
+    """Every launch through flpr_conv's block table (conv_schedule, which the
+    engine otherwise keeps for launches of 4096 tiles or more): float64 form
+    against the float64 oracle at the usual 1e-8."""
+    from fastlpr_amd import engine as E
+    monkeypatch.setattr(E, "SCHED_MIN_TILES", 0)
+    E._SCHED.clear()
+    x, y, h, opt = CASES[name]
+    o = _oracle(x, y, h, opt)
+    r = fl.cv_fastlpr(x, y, h, dict(opt, precision="fp64"))
+    assert r.gcv_yhat["id1se"] == o.gcv["id1se"]
+    assert _rel(r.yhat, o.yhat) < 1e-8
+    assert _rel(r.gcv_yhat["gcv_m"], o.gcv["gcv_m"]) < 1e-8
+    r32 = fl.cv_fastlpr(x, y, h, dict(opt))
+    assert _rel(r32.yhat, o.yhat) <= 1e-5
+    E._SCHED.clear()

@@ -234,3 +234,46 @@ def test_plan_stage_layout_counts():
     assert p.stage_per_h[0] == (6 * 12 + 9) * 256 * 256
     assert len(p.s_off) == 21 and len(p.t_off) == 6
     assert [len(t.jobs_host) for t in p.pass_tables[0]] == [5 * 30, 15 * 30]
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_conv_block_schedule_covers_every_tile_once(fp64):
+    """conv_schedule (config 2: 2-D order 2, 256 x 256 nodes, 30 bandwidths,
+    12 channels): every (job, tile) of both passes appears exactly once, each
+    tile decodes inside its job (flpr_conv's tile numbering), idle entries are
+    marked -1, and XCD b mod 8 receives a balanced share of the work."""
+    from fastlpr_amd import _lib
+    from fastlpr_amd import get_hlist
+    _lib.load()
+    st = E.Stats(np.zeros(2), np.ones(2), np.full(2, 0.5), np.full(2, 0.2887))
+    geo = E.build_geometry(st, (256, 256), 4)
+    hh = get_hlist(30, [0.005, 0.5])[:, 0]
+    h = np.column_stack([hh, hh])
+    bands = E.tap_bands(geo, h, np.full(30, fp64)).reshape(-1).astype(np.int32)
+    plan = E.Plan(2, 2, geo.N, geo.L, geo.margin, 12, 30, fp64, "cpu")
+    for ps in plan.pass_tables[0]:
+        table, nb = E.conv_schedule(ps, fp64, bands)
+        assert table is not None and (ps.mode == 0 or nb % E.XCDS == 0)
+        t = table.numpy().reshape(-1, 2)
+        live = t[t[:, 0] >= 0]
+        assert np.all(t[t[:, 0] < 0][:, 1] == 0)
+        ntt = -(-ps.nout // 64)
+        seen = set()
+        for j in range(ps.jobs_host.shape[0]):
+            ro = int(ps.jobs_host[j, 4])
+            if ps.mode == 0:
+                ntd = -(-ps.max_ro // 64)
+                want = {td + ntd * tt for tt in range(ntt) for td in range(-(-ro // 64))}
+            else:
+                ntd = -(-ps.inner // 64)
+                want = set(range(ntd * ntt * ro))
+            got = live[live[:, 0] == j][:, 1]
+            assert len(got) == len(set(got.tolist())) == len(want)
+            assert set(got.tolist()) == want
+            seen.add(j)
+        assert seen == set(range(ps.jobs_host.shape[0]))
+        if ps.mode == 1:
+            per_xcd = [np.sum(t[x::E.XCDS, 0] >= 0) for x in range(E.XCDS)]
+            assert max(per_xcd) <= 1.25 * np.mean(per_xcd) + 64
+        else:
+            assert len(live) == len(t)
