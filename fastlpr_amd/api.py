@@ -463,7 +463,7 @@ def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
     y_is_complex = bool(y_arr.is_complex()) if isinstance(y_arr, torch.Tensor) \
         else bool(np.iscomplexobj(y_arr))
     yd = _y_channels(y_arr, dev)
-    y_raw_orig = yd[:, 0].clone() if variance else None
+    y_raw_orig = yd[:, 0].contiguous() if variance else None
     ny = yd.shape[1]
     if ny > 32:
         raise NotImplementedError("the GPU engine takes at most 32 real response columns")
@@ -477,9 +477,9 @@ def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
         # and y's sums of squares and non-finite check on the device (read
         # back with the results)
         s1 = xs["st"]
-        stats = engine.Stats(s1.xmin, s1.xmax, s1.mean, s1.std, 0, 0, None,
-                             (yd * yd).sum(dim=0))
-        bad_y_dev = (~torch.isfinite(yd_in)).sum().to(torch.float64)
+        ys = engine.y_stats(yd)
+        stats = engine.Stats(s1.xmin, s1.xmax, s1.mean, s1.std, 0, 0, None, ys[2:])
+        bad_y_dev = ys[1:2]
     else:
         # one device pass: x statistics, non-finite checks of device inputs, sum y^2
         stats = engine.x_stats(xd, yd)
@@ -575,7 +575,7 @@ def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
         grid_vals = fit.m_y[sel]                      # (ny, nodes)
         s0 = fit.s0_all[sel].reshape(N)
     else:
-        idx = sel_dev.long()
+        idx = sel_dev
         grid_vals = fit.m_y.index_select(0, idx)[0]
         s0 = fit.s0_all.index_select(0, idx)[0].reshape(N)
     out, expsum = engine.interp_eval(geo, xd, grid_vals, ny,

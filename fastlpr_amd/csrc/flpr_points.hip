@@ -1271,8 +1271,8 @@ int flpr_last_hip_error(char* buf, int32_t len) {
 int flpr_xstats(const double* x, int64_t n, int32_t d, const double* y, int32_t ny,
                 double* partials, double* out, void* stream) {
   FLPR_CLEAR_ERROR();
-  if (!x || !partials || !out || n < 1 || d < 1 || d > FLPR_MAX_DIM || ny < 0 || ny > MAXY ||
-      (ny > 0 && !y))
+  if ((d > 0 && !x) || !partials || !out || n < 1 || d < 0 || d > FLPR_MAX_DIM || ny < 0 ||
+      ny > MAXY || (ny > 0 && !y))
     return -1;
   hipLaunchKernelGGL(k_xstats, dim3(XS_BLOCKS), dim3(XS_THREADS), 0, as_stream(stream), x, n, d,
                      y, ny, partials);

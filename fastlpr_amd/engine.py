@@ -111,6 +111,43 @@ def ptr(t: Optional[torch.Tensor]):
     return None if t is None else t.data_ptr()
 
 
+def zeroed(dev, *specs):
+    """Several zero-filled device buffers from one allocation and one fill:
+    specs are (count, dtype) pairs (count 0 gives None); every buffer starts
+    on a 256-byte boundary."""
+    offs, total = [], 0
+    for (cnt, dt) in specs:
+        offs.append(total)
+        nb = int(cnt) * torch.empty((), dtype=dt).element_size()
+        total += (nb + 255) // 256 * 256
+    arena = torch.zeros((max(total, 256),), dtype=torch.uint8, device=dev)
+    out = []
+    for (cnt, dt), o in zip(specs, offs):
+        if int(cnt) == 0:
+            out.append(None)
+            continue
+        nb = int(cnt) * torch.empty((), dtype=dt).element_size()
+        out.append(arena[o:o + nb].view(dt))
+    return out
+
+
+_DEVTAB: Dict[tuple, torch.Tensor] = {}
+
+
+def device_table(a: np.ndarray, dtype: torch.dtype, dev) -> torch.Tensor:
+    """A small constant host table on the device, uploaded once per content
+    (bandwidths, tap bands, their sums); callers must not write to it."""
+    a = np.ascontiguousarray(a)
+    key = (a.dtype.str, a.shape, a.tobytes(), dtype, str(dev))
+    t = _DEVTAB.get(key)
+    if t is None:
+        if len(_DEVTAB) >= 256:
+            _DEVTAB.pop(next(iter(_DEVTAB)))
+        t = upload(a, dtype, dev)
+        _DEVTAB[key] = t
+    return t
+
+
 def upload(a: np.ndarray, dtype: torch.dtype, dev) -> torch.Tensor:
     """Host array -> device tensor through pinned memory, without a stream
     synchronisation (a pageable copy would drain the stream first)."""
@@ -510,6 +547,17 @@ def x_stats(x: torch.Tensor, y: Optional[torch.Tensor] = None) -> Stats:
     std = np.where(std == 0, 1.0, std)
     return Stats(w[:, 0].copy(), w[:, 1].copy(), w[:, 2].copy(), std, int(o[5 * d]),
                  int(o[5 * d + 1]), o[5 * d + 2:].copy(), out[5 * d + 2:])
+
+
+def y_stats(y: torch.Tensor) -> torch.Tensor:
+    """sum_i y_ic^2 per column and the count of non-finite values of y (n, ny)
+    float64, left on the device: out[0] = 0, out[1] = non-finite count,
+    out[2 + c] = sum of squares of column c (flpr_xstats with d = 0)."""
+    n, ny = y.shape
+    part = torch.empty(512 * (2 + ny), dtype=torch.float64, device=y.device)
+    out = torch.empty(2 + ny, dtype=torch.float64, device=y.device)
+    call("flpr_xstats", None, n, 0, ptr(y), ny, ptr(part), ptr(out), stream_handle())
+    return out
 
 
 def padded_sizes(N: Tuple[int, ...], flag_power2: bool = True) -> Tuple[int, ...]:
@@ -914,11 +962,24 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     gp = _lib.C.byref(g)
     f64, i32 = torch.float64, torch.int32
 
+    # bandwidth tables (host, memoised) and the sizes of the small zeroed buffers
+    h, warn, groups, band_np = bandwidth_tables(geo, gkey, np.asarray(h, dtype=float), n, order,
+                                                fp64)
+    dh = h.shape[0]
+    nodes_all = int(np.prod(geo.N))
+    rcells = int(np.prod([n_ - 1 for n_ in geo.N]))
+    nw = (1 << d) * ((1 << d) + 1) // 2
+    need_q = (K > 0 or want_rss) and not keep_all and not moments_only
+    nq_out = dh * ny if (want_rss and ny > 0 and not keep_all) else 0
+
     # --- points: knots, cell keys, counting sort, sorted SoA data
     ntiles = int(np.prod([g.ntile[k] for k in range(d)]))
     nkeys = ntiles * int(np.prod([g.tile[k] for k in range(d)]))
+    cnt, sums, key_max, red, q, gram = zeroed(
+        dev, (2 * nkeys, i32), (C, f64), (1, torch.int64),
+        (dh * nval + nq_out if need_q else 0, f64), (nval * nodes_all if need_q else 0, f64),
+        (nw * rcells if (need_q and want_rss) else 0, f64))
     key = torch.empty((n,), dtype=i32, device=dev)
-    cnt = torch.zeros((2 * nkeys,), dtype=i32, device=dev)
     key_count, key_cursor = cnt[:nkeys], cnt[nkeys:]
     key_start = torch.empty((nkeys + 1,), dtype=i32, device=dev)
     order_idx = torch.empty((n,), dtype=i32, device=dev)
@@ -943,7 +1004,6 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     # values there are discarded or zeroed): pad them
     pad = conv_pad(geo)
     grid64 = torch.empty((C * cells + pad,), dtype=f64, device=dev)
-    sums = torch.zeros((C,), dtype=f64, device=dev)
     if fine_s:
         # every channel through the oversampled grid (acc_y == acc_s here)
         spread_fine(geo, acc_s, x, n, vals_y, ny, y_pairs, probes, K, True, grid64, sums, 0, s)
@@ -966,10 +1026,6 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
              ptr(grid64) + cells * 8, ptr(sums) + 8, s)
         del wab_y, vals_yp
     del part
-    # bandwidth tables on the host while the device spreads
-    h, warn, groups, band_np = bandwidth_tables(geo, gkey, np.asarray(h, dtype=float), n, order,
-                                                fp64)
-    dh = h.shape[0]
     any32 = any(not gf for (_a, _b, gf) in groups)
     shift = torch.empty((C,), dtype=f64, device=dev)
     grid32 = torch.empty((C * cells + pad,), dtype=torch.float32, device=dev) if any32 else None
@@ -980,11 +1036,11 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
                      groups[0][2], dev)
     npow, nmax = plan0.npow, plan0.nmax
     per_tap = d * npow * nmax
-    h_dev = upload(h, f64, dev)
+    h_dev = device_table(h, f64, dev)
     tap64 = torch.empty((dh * per_tap,), dtype=f64, device=dev)
     tap32 = torch.empty((dh * per_tap,), dtype=torch.float32, device=dev) if any32 else None
     call("flpr_taps", gp, ptr(h_dev), dh, npow, nmax, float(n), ptr(tap64), ptr(tap32), s)
-    bands = upload(band_np, i32, dev)
+    bands = device_table(band_np, i32, dev)
 
     # --- global regularisation key (regression.py:217, 243-267)
     if order == 0:
@@ -992,7 +1048,6 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
         eps_add = float(np.finfo(float).eps) if with_ones_eps else 0.0
     else:
         lam_scale, eps_add = 1e-6, 0.0
-    key_max = torch.zeros((1,), dtype=torch.int64, device=dev)
 
     # --- convolution passes per precision group (bandwidth chunks share the
     # intermediate buffers); every group feeds the one global key
@@ -1035,17 +1090,10 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
         return DeviceFit(h, None, None, None, s0_all, geo, plan0, warn, None)
 
     # --- adjoint sums, solve (+ dots), residual quadratic form
-    rcells = int(np.prod([n_ - 1 for n_ in geo.N]))
-    nw = (1 << d) * ((1 << d) + 1) // 2
-    need_q = (K > 0 or want_rss) and not keep_all
-    q = gram = None
-    # one float64 readback buffer: probe / y dots (dh, nval) then quadratic forms (dh, ny)
-    nq_out = dh * ny if (want_rss and ny > 0 and not keep_all) else 0
-    red = torch.zeros((dh * nval + nq_out,), dtype=f64, device=dev) if need_q else None
+    # red: one float64 readback buffer, probe / y dots (dh, nval) then
+    # quadratic forms (dh, ny); q, gram: adjoint sums (all zeroed above)
     dot = red[:dh * nval] if need_q else None
     if need_q:
-        q = torch.zeros((nval * nodes,), dtype=f64, device=dev)
-        gram = torch.zeros((nw * rcells,), dtype=f64, device=dev) if want_rss else None
         wcells = int(np.prod([g.tile[k] + 5 for k in range(d)]))
         apart = torch.empty((ntiles * max(nval, nw) * wcells,), dtype=f64, device=dev)
         call("flpr_adjoint", gp, ptr(key_start), ptr(rawi), ptr(rawf), ptr(vals), n, nval,
@@ -1129,7 +1177,8 @@ def run_grid_y(sh: dict, x: torch.Tensor, vals_y: torch.Tensor, ny: int,
     ntiles = int(np.prod([g.ntile[k] for k in range(d)]))
     grid64 = torch.empty((C2 * cells + pad,), dtype=f64, device=dev)
     grid64[:cells].copy_(sh["grid64"][:cells])
-    sums = torch.zeros((C2,), dtype=f64, device=dev)
+    nodes = sh["plan0"].nodes
+    sums, red, q = zeroed(dev, (C2, f64), (2 * dh * ny, f64), (ny * nodes, f64))
     sums[:1].copy_(sh["sums"][:1])
     if acc_y > 6:
         call("flpr_points_pack", gp, ptr(x), ptr(sh["order_idx"]), n, ptr(vals_y), ny, None, 0,
@@ -1182,8 +1231,6 @@ def run_grid_y(sh: dict, x: torch.Tensor, vals_y: torch.Tensor, ny: int,
     nodes = plan0.nodes
     per_h0 = plan0.stage_per_h[0]
     nw = (1 << d) * ((1 << d) + 1) // 2
-    red = torch.zeros((2 * dh * ny,), dtype=f64, device=dev)
-    q = torch.zeros((ny * nodes,), dtype=f64, device=dev)
     wcells = int(np.prod([g.tile[k] + 5 for k in range(d)]))
     apart = torch.empty((ntiles * max(ny, nw) * wcells,), dtype=f64, device=dev)
     call("flpr_adjoint", gp, ptr(sh["key_start"]), ptr(sh["rawi"]), ptr(sh["rawf"]), ptr(vals), n,
@@ -1312,7 +1359,7 @@ def select(fit: DeviceFit, ysq_dev: torch.Tensor, pp_dev: torch.Tensor, n: int, 
                          "Use 'first', 'average', or 'sum'.")
     dh = fit.h.shape[0]
     dev = fit.m_y.device
-    hsum = upload(fit.h.sum(axis=1), torch.float64, dev)
+    hsum = device_table(fit.h.sum(axis=1), torch.float64, dev)
     out = torch.empty((4 * dh + 5,), dtype=torch.float64, device=dev)
     sel = torch.empty((1,), dtype=torch.int32, device=dev)
     call("flpr_select", ptr(fit.red), dh, ny + K, ny, K, ptr(ysq_dev), ptr(pp_dev), ptr(hsum),

@@ -68,8 +68,9 @@ int flpr_last_hip_error(char* buf, int32_t len);
  * (api.py:307-323).  x is (n, d) row-major float64; y (n, ny) float64 or NULL.
  * out[5*k+0..4] = min, max, mean, sum of squared deviations, count (finite
  * values of dimension k); out[5*d] = non-finite values in x; out[5*d+1] =
- * non-finite values in y; out[5*d+2+c] = sum_i y_ic^2.  partials must hold
- * 512*(5*d + 2 + ny) doubles; ny <= 32. */
+ * non-finite values in y; out[5*d+2+c] = sum_i y_ic^2 (finite values).
+ * partials must hold 512*(5*d + 2 + ny) doubles; ny <= 32.  d = 0 gives the
+ * y part only (x may be NULL). */
 int flpr_xstats(const double* x, int64_t n, int32_t d, const double* y, int32_t ny,
                 double* partials, double* out, void* stream);
 
