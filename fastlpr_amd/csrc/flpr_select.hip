@@ -32,22 +32,27 @@ __device__ inline double np_maximum(double a, double b) {
   return a > b ? a : b;
 }
 
-// NumPy's pairwise summation of v[0..n) (n <= 128: eight strided partial sums,
-// combined pairwise, then the remainder); n < 8 is a plain loop.
-__device__ inline double np_sum(const double* v, int n) {
+// NumPy's pairwise summation of f(0..n-1) (n <= 128: eight strided partial
+// sums, combined pairwise, then the remainder); n < 8 is a plain loop.  The
+// terms are produced on demand, so no per-thread array (scratch memory) is
+// needed.
+template <typename F>
+__device__ inline double np_sum(F f, int n) {
 #pragma clang fp contract(off)
   if (n < 8) {
     double r = -0.0;
-    for (int i = 0; i < n; ++i) r += v[i];
+    for (int i = 0; i < n; ++i) r += f(i);
     return r;
   }
   double r[8];
-  for (int j = 0; j < 8; ++j) r[j] = v[j];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) r[j] = f(j);
   int i = 8;
   for (; i < n - (n % 8); i += 8)
-    for (int j = 0; j < 8; ++j) r[j] += v[i + j];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r[j] += f(i + j);
   double res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
-  for (; i < n; ++i) res += v[i];
+  for (; i < n; ++i) res += f(i);
   return res;
 }
 
@@ -69,48 +74,49 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(
   for (int b = threadIdx.x; b < dh; b += blockDim.x) {
     const double* dot = red + (int64_t)b * nval;
     const double* quad = red + (int64_t)dh * nval + (int64_t)b * ny;
-    // --- Hutchinson over the K probes (sequential sums, as NumPy over axis 0)
-    double tr[FLPR_MAX_CHANNELS];
-    double s = 0.0;
-    for (int k = 0; k < K; ++k) {
+    // --- Hutchinson over the K probes (sequential sums, as NumPy over axis 0);
+    // the per-probe terms are recomputed in every pass instead of stored
+    auto trace = [&](int k) {
       double t = n * (dot[ny + k] / pp[k]);
       if (t != t) t = n;
-      t = clip(t, 0.0, n);
-      tr[k] = t;
-      s += t;
-    }
+      return clip(t, 0.0, n);
+    };
+    double s = 0.0;
+    for (int k = 0; k < K; ++k) s += trace(k);
     double dof = s / (double)K;
     double dse = 0.0, pen_std = 0.0, pen_mean = 0.0;
     const bool many = K > 1;
     if (many) {
       double ss = 0.0;
       for (int k = 0; k < K; ++k) {
-        const double e = tr[k] - dof;
+        const double e = trace(k) - dof;
         ss += e * e;
       }
       dse = sqrt(ss / (double)(K - 1)) / sqrt((double)K);
-      double pen[FLPR_MAX_CHANNELS];
+      auto pdof = [&](int k) {
+        double pd = 1.0 - trace(k) / n;
+        if (cplx) pd = 2.0 * pd - 1.0;
+        return clip(pd, 0.0, 1.0);
+      };
+      auto penalty = [&](int k) {
+        const double m = np_maximum(pdof(k), 0.01);
+        return 1.0 / (m * m);
+      };
       double pcs = 0.0, ps = 0.0;
       for (int k = 0; k < K; ++k) {
-        double pd = 1.0 - tr[k] / n;
-        if (cplx) pd = 2.0 * pd - 1.0;
-        const double pc = clip(pd, 0.0, 1.0);
-        const double m = np_maximum(pc, 0.01);
-        pen[k] = 1.0 / (m * m);
-        pcs += pc;
-        ps += pen[k];
+        pcs += pdof(k);
+        ps += penalty(k);
       }
       pen_mean = ps / (double)K;
       double es = 0.0;
       for (int k = 0; k < K; ++k) {
-        const double e = pen[k] - pen_mean;
+        const double e = penalty(k) - pen_mean;
         es += e * e;
       }
       pen_std = sqrt(es / (double)(K - 1));
       if (cplx) dof = n * (1.0 - pcs / (double)K);
     }
     // --- residual sums per real column, complex columns paired
-    double mse[FLPR_MAX_CHANNELS], g[FLPR_MAX_CHANNELS];
     double pen;
     if (many) {
       pen = pen_mean;
@@ -119,7 +125,7 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(
       const double m = np_maximum(pd, 0.01);
       pen = 1.0 / (m * m);
     }
-    for (int c = 0; c < ncol; ++c) {
+    auto mse = [&](int c) {
       double rss;
       if (cplx) {
         const double r0 = (ysq[2 * c] - 2.0 * dot[2 * c]) + quad[2 * c];
@@ -128,18 +134,18 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(
       } else {
         rss = (ysq[c] - 2.0 * dot[c]) + quad[c];
       }
-      mse[c] = rss / n;
-      g[c] = mse[c] * pen;
-    }
+      return rss / n;
+    };
+    auto gcv = [&](int c) { return mse(c) * pen; };
     double score, m;
     if (agg == 0) {
-      score = g[0];
-      m = mse[0];
+      score = gcv(0);
+      m = mse(0);
     } else if (agg == 1) {
-      score = np_sum(g, ncol) / (double)ncol;
+      score = np_sum(gcv, ncol) / (double)ncol;
       m = np_sum(mse, ncol) / (double)ncol;
     } else {
-      score = np_sum(g, ncol);
+      score = np_sum(gcv, ncol);
       m = np_sum(mse, ncol);
     }
     scores[b] = score;
