@@ -56,105 +56,125 @@ __device__ inline double np_sum(F f, int n) {
   return res;
 }
 
-// One thread per bandwidth for the scores, then thread 0 runs the selection.
+// Scores: one block per bandwidth; lane k computes the terms of probe k
+// (trace, pdof, penalty) into LDS, then lane 0 forms every sum sequentially in
+// probe order (as NumPy over axis 0), so the values are NumPy's bit for bit.
 //   red     (dh, nval) dots [y columns, probes] then (dh, ny) quadratic forms
-//   ysq     (ny) sum_i y_ic^2; pp (K) p'p per probe; hsum (dh) sum_k h_bk
-//   out     scores, errs, dof, dof_stderr (dh each), then sel, idmin, nvalid
-__global__ __launch_bounds__(SEL_THREADS) void k_select(
+//   ysq     (ny) sum_i y_ic^2; pp (K) p'p per probe
+//   out     scores, errs, dof, dof_stderr (dh each)
+__global__ __launch_bounds__(64) void k_select_scores(
     const double* __restrict__ red, int dh, int nval, int ny, int K,
-    const double* __restrict__ ysq, const double* __restrict__ pp,
-    const double* __restrict__ hsum, double n, int cplx, int agg, double dstd,
-    double* __restrict__ out, int32_t* __restrict__ sel_out) {
+    const double* __restrict__ ysq, const double* __restrict__ pp, double n, int cplx, int agg,
+    double* __restrict__ out) {
 #pragma clang fp contract(off)
-  double* scores = out;
-  double* errs = out + dh;
-  double* dofs = out + 2 * dh;
-  double* dses = out + 3 * dh;
-  const int ncol = cplx ? ny / 2 : ny;
-  for (int b = threadIdx.x; b < dh; b += blockDim.x) {
-    const double* dot = red + (int64_t)b * nval;
-    const double* quad = red + (int64_t)dh * nval + (int64_t)b * ny;
-    // --- Hutchinson over the K probes (sequential sums, as NumPy over axis 0);
-    // the per-probe terms are recomputed in every pass instead of stored
-    auto trace = [&](int k) {
-      double t = n * (dot[ny + k] / pp[k]);
-      if (t != t) t = n;
-      return clip(t, 0.0, n);
-    };
-    double s = 0.0;
-    for (int k = 0; k < K; ++k) s += trace(k);
-    double dof = s / (double)K;
-    double dse = 0.0, pen_std = 0.0, pen_mean = 0.0;
-    const bool many = K > 1;
-    if (many) {
-      double ss = 0.0;
-      for (int k = 0; k < K; ++k) {
-        const double e = trace(k) - dof;
-        ss += e * e;
-      }
-      dse = sqrt(ss / (double)(K - 1)) / sqrt((double)K);
-      auto pdof = [&](int k) {
-        double pd = 1.0 - trace(k) / n;
-        if (cplx) pd = 2.0 * pd - 1.0;
-        return clip(pd, 0.0, 1.0);
-      };
-      auto penalty = [&](int k) {
-        const double m = np_maximum(pdof(k), 0.01);
-        return 1.0 / (m * m);
-      };
-      double pcs = 0.0, ps = 0.0;
-      for (int k = 0; k < K; ++k) {
-        pcs += pdof(k);
-        ps += penalty(k);
-      }
-      pen_mean = ps / (double)K;
-      double es = 0.0;
-      for (int k = 0; k < K; ++k) {
-        const double e = penalty(k) - pen_mean;
-        es += e * e;
-      }
-      pen_std = sqrt(es / (double)(K - 1));
-      if (cplx) dof = n * (1.0 - pcs / (double)K);
-    }
-    // --- residual sums per real column, complex columns paired
-    double pen;
-    if (many) {
-      pen = pen_mean;
-    } else {
-      const double pd = clip(1.0 - dof / n, 0.0, 1.0);
-      const double m = np_maximum(pd, 0.01);
-      pen = 1.0 / (m * m);
-    }
-    auto mse = [&](int c) {
-      double rss;
-      if (cplx) {
-        const double r0 = (ysq[2 * c] - 2.0 * dot[2 * c]) + quad[2 * c];
-        const double r1 = (ysq[2 * c + 1] - 2.0 * dot[2 * c + 1]) + quad[2 * c + 1];
-        rss = r0 + r1;
-      } else {
-        rss = (ysq[c] - 2.0 * dot[c]) + quad[c];
-      }
-      return rss / n;
-    };
-    auto gcv = [&](int c) { return mse(c) * pen; };
-    double score, m;
-    if (agg == 0) {
-      score = gcv(0);
-      m = mse(0);
-    } else if (agg == 1) {
-      score = np_sum(gcv, ncol) / (double)ncol;
-      m = np_sum(mse, ncol) / (double)ncol;
-    } else {
-      score = np_sum(gcv, ncol);
-      m = np_sum(mse, ncol);
-    }
-    scores[b] = score;
-    errs[b] = (pen_std > 0.0) ? m * pen_std : 0.0;
-    dofs[b] = dof;
-    dses[b] = many ? dse : 0.0;
+  __shared__ double tr[FLPR_MAX_CHANNELS], pdv[FLPR_MAX_CHANNELS], penv[FLPR_MAX_CHANNELS];
+  const int b = blockIdx.x;
+  const double* dot = red + (int64_t)b * nval;
+  const double* quad = red + (int64_t)dh * nval + (int64_t)b * ny;
+  for (int k = threadIdx.x; k < K; k += blockDim.x) {
+    double t = n * (dot[ny + k] / pp[k]);
+    if (t != t) t = n;
+    t = clip(t, 0.0, n);
+    tr[k] = t;
+    double pd = 1.0 - t / n;
+    if (cplx) pd = 2.0 * pd - 1.0;
+    pd = clip(pd, 0.0, 1.0);
+    pdv[k] = pd;
+    const double m = np_maximum(pd, 0.01);
+    penv[k] = 1.0 / (m * m);
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
+  const int ncol = cplx ? ny / 2 : ny;
+  // --- Hutchinson over the K probes (sequential sums, as NumPy over axis 0)
+  double s = 0.0;
+  for (int k = 0; k < K; ++k) s += tr[k];
+  double dof = s / (double)K;
+  double dse = 0.0, pen_std = 0.0, pen_mean = 0.0;
+  const bool many = K > 1;
+  if (many) {
+    double ss = 0.0;
+    for (int k = 0; k < K; ++k) {
+      const double e = tr[k] - dof;
+      ss += e * e;
+    }
+    dse = sqrt(ss / (double)(K - 1)) / sqrt((double)K);
+    double pcs = 0.0, ps = 0.0;
+    for (int k = 0; k < K; ++k) {
+      pcs += pdv[k];
+      ps += penv[k];
+    }
+    pen_mean = ps / (double)K;
+    double es = 0.0;
+    for (int k = 0; k < K; ++k) {
+      const double e = penv[k] - pen_mean;
+      es += e * e;
+    }
+    pen_std = sqrt(es / (double)(K - 1));
+    if (cplx) dof = n * (1.0 - pcs / (double)K);
+  }
+  // --- residual sums per real column, complex columns paired
+  double pen;
+  if (many) {
+    pen = pen_mean;
+  } else {
+    const double pd = clip(1.0 - dof / n, 0.0, 1.0);
+    const double m = np_maximum(pd, 0.01);
+    pen = 1.0 / (m * m);
+  }
+  auto mse = [&](int c) {
+    double rss;
+    if (cplx) {
+      const double r0 = (ysq[2 * c] - 2.0 * dot[2 * c]) + quad[2 * c];
+      const double r1 = (ysq[2 * c + 1] - 2.0 * dot[2 * c + 1]) + quad[2 * c + 1];
+      rss = r0 + r1;
+    } else {
+      rss = (ysq[c] - 2.0 * dot[c]) + quad[c];
+    }
+    return rss / n;
+  };
+  auto gcv = [&](int c) { return mse(c) * pen; };
+  double score, m;
+  if (agg == 0) {
+    score = gcv(0);
+    m = mse(0);
+  } else if (agg == 1) {
+    score = np_sum(gcv, ncol) / (double)ncol;
+    m = np_sum(mse, ncol) / (double)ncol;
+  } else {
+    score = np_sum(gcv, ncol);
+    m = np_sum(mse, ncol);
+  }
+  out[b] = score;
+  out[dh + b] = (pen_std > 0.0) ? m * pen_std : 0.0;
+  out[2 * dh + b] = dof;
+  out[3 * dh + b] = many ? dse : 0.0;
+}
+
+constexpr int SEL_LDS = 2048;  // bandwidths held in LDS for the selection
+
+// The selection over the scores (one block; thread 0 walks the bandwidths in
+// LDS when they fit): out[4 dh] = sel, out[4 dh + 1] = idmin, out[4 dh + 2] =
+// number of finite scores.
+__global__ __launch_bounds__(SEL_THREADS) void k_select_pick(int dh,
+                                                             const double* __restrict__ hsum_g,
+                                                             double dstd,
+                                                             double* __restrict__ out,
+                                                             int32_t* __restrict__ sel_out) {
+  __shared__ double lsc[SEL_LDS], ler[SEL_LDS], lhs[SEL_LDS];
+  const bool in_lds = dh <= SEL_LDS;
+  if (in_lds) {
+    for (int b = threadIdx.x; b < dh; b += blockDim.x) {
+      lsc[b] = out[b];
+      ler[b] = out[dh + b];
+      lhs[b] = hsum_g[b];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double* scores = in_lds ? lsc : out;
+  const double* errs = in_lds ? ler : out + dh;
+  const double* hsum = in_lds ? lhs : hsum_g;
   // --- idmin: np.argmin (the first NaN when there is one)
   int idmin = 0;
   for (int b = 0; b < dh; ++b) {
@@ -204,8 +224,11 @@ int flpr_select(const double* red, int32_t dh, int32_t nval, int32_t ny, int32_t
       nprobe > FLPR_MAX_CHANNELS || ny > FLPR_MAX_CHANNELS || nval != ny + nprobe || n < 1 ||
       agg < 0 || agg > 2 || (cplx && (ny % 2)))
     return -1;
-  hipLaunchKernelGGL(k_select, dim3(1), dim3(SEL_THREADS), 0, as_stream(stream), red, dh, nval,
-                     ny, nprobe, ysq, pp, hsum, (double)n, cplx, agg, dstd, out, sel);
+  hipLaunchKernelGGL(k_select_scores, dim3((unsigned)dh), dim3(64), 0, as_stream(stream), red, dh,
+                     nval, ny, nprobe, ysq, pp, (double)n, cplx, agg, out);
+  FLPR_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_select_pick, dim3(1), dim3(SEL_THREADS), 0, as_stream(stream), dh, hsum,
+                     dstd, out, sel);
   FLPR_CHECK_LAUNCH();
   return 0;
 }
