@@ -32,6 +32,7 @@ EXPORTS = (
     "flpr_adjoint",
     "flpr_quadform",
     "flpr_interp_eval",
+    "flpr_fit_output",
     "flpr_select",
     "flpr_lcv",
     "flpr_spline_eval",
@@ -96,6 +97,7 @@ _SIGS = {
     "flpr_adjoint": [_GP, _P, _P, _P, _P, _I64, _I32, _P, _P, _P, _P],
     "flpr_quadform": [_GP, _P, _I32, _I32, _P, _P, _P],
     "flpr_interp_eval": [_GP, _P, _I64, _P, _I32, _P, _P, _P, _P],
+    "flpr_fit_output": [_GP, _P, _I64, _P, _I32, _P, _P, _I32, _P, _P, _P, _P, _P, _P],
     "flpr_spline_eval": [_GP, _P, _I64, _P, _I32, _P, _P],
     "flpr_lcv": [_GP, _P, _I64, _P, _I32, _P, _P, _P, _P],
     "flpr_select": [_P, _I32, _I32, _I32, _I32, _P, _P, _P, _I64, _I32, _I32, _D, _P, _P, _P],

@@ -571,58 +571,42 @@ def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
         warnings.warn(fit.warn, stacklevel=2)
 
     geo = fit.geo
-    if sel_dev is None:
-        grid_vals = fit.m_y[sel]                      # (ny, nodes)
-        s0 = fit.s0_all[sel].reshape(N)
-    else:
-        idx = sel_dev
-        grid_vals = fit.m_y.index_select(0, idx)[0]
-        s0 = fit.s0_all.index_select(0, idx)[0].reshape(N)
-    out, expsum = engine.interp_eval(geo, xd, grid_vals, ny,
-                                     yvar=y_raw_orig if variance else None)
-    d_norm = None
-    fac = None
-    if variance:
-        # d = 1 / (mean_i y_i / exp(log-variance fit)) (api.py:771-778); applied on
-        # the device, read back once at the end
-        fac = expsum * (1.0 / n)
-        out = torch.exp(out) * fac
-        grid_vals = torch.exp(grid_vals) * fac
+    dh = fit.h.shape[0]
+    if variance and ny != 1:
+        raise NotImplementedError("y_type_out='variance' takes a real single-column response")
+    # yhat, residuals (and the variance back-transform) at the selected
+    # bandwidth, on the device; the factor 1/d lands in the readback buffer
+    fac_slot = sel_out[4 * dh + 3:4 * dh + 4] if (variance and sel_out is not None) else None
+    out, resid_r, grid_var, fac = engine.fit_output(
+        geo, xd, fit.m_y, ny, sel_dev, y_raw_orig if variance else yd, variance, fac_slot)
     if y_is_complex:
-        yhat = torch.complex(out[:, 0::2], out[:, 1::2])
+        yhat = torch.view_as_complex(out.view(n, dy, 2))
+        resid = torch.view_as_complex(resid_r.view(n, dy, 2))
     else:
-        yhat = out
+        yhat, resid = out, resid_r
     if dy == 1:
         yhat = yhat[:, 0]
-    interp = GridInterpolant(geo, geo.axes_raw, grid_vals.contiguous(), N, y_is_complex, dy,
-                             as_numpy, dy == 1)
-    yref = y_raw_orig if variance else (torch.complex(yd[:, 0::2], yd[:, 1::2]) if y_is_complex
-                                        else yd)
-    resid = yref.reshape(-1) - yhat.reshape(-1) if dy == 1 or variance else \
-        (yref - yhat).reshape(-1)
-
+    resid = resid.reshape(-1)
     share["yhat"] = out
-    dh = fit.h.shape[0]
-    if sel_out is not None:
-        if fac is not None:
-            sel_out[4 * dh + 3:4 * dh + 4].copy_(fac.reshape(1))
-        if bad_y_dev is not None:
-            sel_out[4 * dh + 4:4 * dh + 5].copy_(bad_y_dev.reshape(1))
+    if sel_out is not None and bad_y_dev is not None:
+        sel_out[4 * dh + 4:4 * dh + 5].copy_(bad_y_dev.reshape(1))
 
     def conv(t):
         return t.cpu().numpy() if as_numpy else t
 
     def finish():
-        return _finish(fit, sel_out, fac, bad_y_dev, yd_in, dh, opts, order, geo, yhat, resid,
-                       s0, interp, reuse, conv)
+        return _finish(fit, sel_out, fac if variance else None, bad_y_dev, yd_in, dh, opts,
+                       order, geo, yhat, resid, grid_var, N, y_is_complex, dy, as_numpy,
+                       reuse, conv)
 
     return finish, share
 
 
-def _finish(fit, sel_out, fac, bad_y_dev, y_in, dh, opts, order, geo, yhat, resid, s0, interp,
-            reuse, conv):
+def _finish(fit, sel_out, fac, bad_y_dev, y_in, dh, opts, order, geo, yhat, resid, grid_var, N,
+            y_is_complex, dy, as_numpy, reuse, conv):
     """The one host read of a fit (selection results, the variance scale, the
-    deferred non-finite count of y) and the RegressionOutput."""
+    deferred non-finite count of y) and the RegressionOutput; the grid values
+    and S_00 of the selected bandwidth are views of the fit's arrays."""
     d_norm = None
     if sel_out is not None:
         hv = sel_out.cpu().numpy()
@@ -630,6 +614,7 @@ def _finish(fit, sel_out, fac, bad_y_dev, y_in, dh, opts, order, geo, yhat, resi
             _check_finite(y_in, "Response y", "cv_fastlpr")
         if fac is not None:
             d_norm = 1.0 / float(hv[4 * dh + 3])
+        sel = int(hv[4 * dh])
         if int(hv[4 * dh + 2]) == 0:
             raise ValueError(
                 "All GCV scores are NaN or Inf. Regression failed for all bandwidths. "
@@ -654,6 +639,10 @@ def _finish(fit, sel_out, fac, bad_y_dev, y_in, dh, opts, order, geo, yhat, resi
                       "dof_stderr": np.array([np.nan])}
         dof_sel = dof_err_sel = np.nan
 
+    grid_vals = grid_var.view(1, -1) if grid_var is not None else fit.m_y[sel]
+    s0 = fit.s0_all[sel].reshape(N)
+    interp = GridInterpolant(geo, geo.axes_raw, grid_vals, N, y_is_complex, dy, as_numpy,
+                             dy == 1)
     metadata = {"options": opts, "order": order, "bandwidths_tested": fit.h,
                 "fitted_at_samples": conv(yhat), "residuals": conv(resid),
                 "mean": geo.mean, "std": geo.std, "dof": dof_sel, "dof_stderr": dof_err_sel,

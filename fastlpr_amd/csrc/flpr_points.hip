@@ -968,6 +968,71 @@ __global__ __launch_bounds__(EVAL_THREADS) void k_interp_eval(FlprGeom g,
   }
 }
 
+// The fit's outputs at the selected bandwidth (index *sel on the device, or 0):
+// yhat at the points by multilinear interpolation of m_all[sel] (api.py:760-770)
+// and the residuals yref - yhat.  variance: yhat first holds the log-variance
+// fit; each block writes its share of sum_i yref_i exp(-fit_i) to part, and
+// k_fit_finish applies d = n / sum (api.py:771-786): yhat = exp(fit) / d, the
+// grid values exp(m) / d into grid, resid = yref - yhat.
+constexpr int FIT_BLOCKS = 1024;
+
+template <int D>
+__global__ __launch_bounds__(EVAL_THREADS) void k_fit_points(
+    FlprGeom g, const double* __restrict__ x, int64_t n, const double* __restrict__ m_all,
+    int ncol, int64_t nodes, const int32_t* __restrict__ sel, const double* __restrict__ yref,
+    int variance, double* __restrict__ yhat, double* __restrict__ resid,
+    double* __restrict__ part) {
+  __shared__ double sh[EVAL_THREADS / 64];
+  const double* m = m_all + (sel ? (int64_t)sel[0] : 0) * ncol * nodes;
+  double es = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const CornersT<D> c = corners<D>(g, x + i * D);
+    for (int col = 0; col < ncol; ++col) {
+      const double* mm = m + (int64_t)col * nodes;
+      double v = 0.0;
+#pragma unroll
+      for (int q = 0; q < (1 << D); ++q) v += c.w[q] * mm[c.node[q]];
+      yhat[i * ncol + col] = v;
+      if (variance) {
+        es += yref[i] * exp(-v);
+      } else if (resid) {
+        resid[i * ncol + col] = yref[i * ncol + col] - v;
+      }
+    }
+  }
+  if (variance) {
+    const double t = block_sum(es, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(EVAL_THREADS) void k_fit_finish(
+    int64_t n, const double* __restrict__ m_all, int64_t nodes, const int32_t* __restrict__ sel,
+    const double* __restrict__ yref, const double* __restrict__ part, int nparts,
+    double* __restrict__ yhat, double* __restrict__ resid, double* __restrict__ grid,
+    double* __restrict__ fac_out) {
+  __shared__ double sh[EVAL_THREADS / 64];
+  __shared__ double fac_s;
+  double p = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x) p += part[b];
+  p = block_sum(p, sh);
+  if (threadIdx.x == 0) fac_s = p * (1.0 / (double)n);  // 1 / d
+  __syncthreads();
+  const double fac = fac_s;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && fac_out) fac_out[0] = fac;
+  const double* m = m_all + (sel ? (int64_t)sel[0] : 0) * nodes;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double v = exp(yhat[i]) * fac;
+    yhat[i] = v;
+    if (resid) resid[i] = yref[i] - v;
+  }
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nodes;
+       j += (int64_t)gridDim.x * blockDim.x)
+    grid[j] = exp(m[j]) * fac;
+}
+
 // Likelihood cross-validation of the density grids (api.py:1111-1132): for
 // every bandwidth b and data point i
 //   f_i = linear interpolation of dens[b] at x_i
@@ -1473,6 +1538,32 @@ int flpr_interp_eval(const FlprGeom* g, const double* x, int64_t n, const double
     default: hipLaunchKernelGGL(k_interp_eval<3>, grd, dim3(EVAL_THREADS), 0, st, *g, x, n, m, ncol, nodes, out, yvar, expsum); break;
   }
   FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+int flpr_fit_output(const FlprGeom* g, const double* x, int64_t n, const double* m_all,
+                    int32_t ncol, const int32_t* sel, const double* yref, int32_t variance,
+                    double* yhat, double* resid, double* grid, double* part, double* fac,
+                    void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!flpr_geom_ok(g) || !x || !m_all || !yhat || n < 1 || ncol < 1 || (resid && !yref) ||
+      (variance && (ncol != 1 || !yref || !grid || !part)))
+    return -1;
+  dim3 grd((unsigned)flpr_grid_for(n, EVAL_THREADS, FIT_BLOCKS));
+  const int64_t nodes = flpr_nodes(g);
+  hipStream_t st = as_stream(stream);
+  const int var = variance ? 1 : 0;
+  switch (g->d) {
+    case 1: hipLaunchKernelGGL(k_fit_points<1>, grd, dim3(EVAL_THREADS), 0, st, *g, x, n, m_all, ncol, nodes, sel, yref, var, yhat, resid, part); break;
+    case 2: hipLaunchKernelGGL(k_fit_points<2>, grd, dim3(EVAL_THREADS), 0, st, *g, x, n, m_all, ncol, nodes, sel, yref, var, yhat, resid, part); break;
+    default: hipLaunchKernelGGL(k_fit_points<3>, grd, dim3(EVAL_THREADS), 0, st, *g, x, n, m_all, ncol, nodes, sel, yref, var, yhat, resid, part); break;
+  }
+  FLPR_CHECK_LAUNCH();
+  if (variance) {
+    hipLaunchKernelGGL(k_fit_finish, grd, dim3(EVAL_THREADS), 0, st, n, m_all, nodes, sel, yref,
+                       part, (int)grd.x, yhat, resid, grid, fac);
+    FLPR_CHECK_LAUNCH();
+  }
   return 0;
 }
 

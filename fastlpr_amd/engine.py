@@ -1368,6 +1368,32 @@ def select(fit: DeviceFit, ysq_dev: torch.Tensor, pp_dev: torch.Tensor, n: int, 
     return out, sel
 
 
+def fit_output(geo: Geometry, x: torch.Tensor, m_all: torch.Tensor, ncol: int,
+               sel: Optional[torch.Tensor], yref: torch.Tensor, variance: bool,
+               fac_out: Optional[torch.Tensor] = None):
+    """yhat (n, ncol) and residuals at the selected bandwidth (flpr_fit_output);
+    variance: also the back-transformed grid (nodes) and the factor 1/d
+    (written to fac_out, a 1-element float64 device view, when given).
+    Returns (yhat, resid, grid or None, fac tensor or None)."""
+    n = x.shape[0]
+    dev = x.device
+    nodes = int(np.prod(geo.N))
+    yhat = torch.empty((n, ncol), dtype=torch.float64, device=dev)
+    resid = torch.empty((n, ncol), dtype=torch.float64, device=dev)
+    grid = part = None
+    fac = fac_out
+    if variance:
+        grid = torch.empty((nodes,), dtype=torch.float64, device=dev)
+        part = torch.empty((1024,), dtype=torch.float64, device=dev)
+        if fac is None:
+            fac = torch.empty((1,), dtype=torch.float64, device=dev)
+    m_all = m_all.contiguous()
+    call("flpr_fit_output", _lib.C.byref(geo.geom), ptr(x), n, ptr(m_all), ncol, ptr(sel),
+         ptr(yref.contiguous()), int(variance), ptr(yhat), ptr(resid), ptr(grid), ptr(part),
+         ptr(fac), stream_handle())
+    return yhat, resid, grid, fac
+
+
 def interp_eval(geo: Geometry, x: torch.Tensor, m: torch.Tensor, ncol: int,
                 yvar: Optional[torch.Tensor] = None):
     """Multilinear interpolation of (ncol, nodes) float64 grid values at x (n, d)."""

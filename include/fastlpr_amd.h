@@ -247,6 +247,19 @@ int flpr_interp_eval(const FlprGeom* g, const double* x, int64_t n, const double
                      int32_t ncol, double* out, const double* yvar, double* expsum,
                      void* stream);
 
+/* The fit's outputs at the selected bandwidth (api.py:760-786): m_all is
+ * (dh, ncol, nodes) float64, sel a device int32 index into it (NULL: 0).
+ *   yhat  (n, ncol)  multilinear interpolation of m_all[sel] at x
+ *   resid (n, ncol)  yref - yhat (resid may be NULL; yref (n, ncol))
+ * variance != 0 (ncol = 1; yref = the raw response y of the variance fit, n):
+ * with d = n / sum_i yref_i exp(-fit_i), yhat = exp(fit) / d, grid (nodes)
+ * = exp(m_all[sel]) / d, resid = yref - yhat, fac[0] = 1 / d (may be NULL);
+ * part holds 1024 doubles of partial sums. */
+int flpr_fit_output(const FlprGeom* g, const double* x, int64_t n, const double* m_all,
+                    int32_t ncol, const int32_t* sel, const double* yref, int32_t variance,
+                    double* yhat, double* resid, double* grid, double* part, double* fac,
+                    void* stream);
+
 /* Cubic spline interpolation (RegularGridInterpolator 'cubic', i.e. scipy's
  * make_interp_spline(k=3) with not-a-knot ends along every axis; the
  * reference's GCV / DOF interpolation for 2-D and 3-D data with fewer than
