@@ -441,6 +441,9 @@ struct SolveTab {
 
 constexpr int SOLVE_THREADS = 256;
 constexpr int MAXV = 32;  // value channels reduced per solve launch
+// value channels whose moments are loaded together (a divisor of MAXV)
+template <typename T>
+constexpr int solve_group() { return 4; }
 
 // One thread per (node, bandwidth); block row blockIdx.y = bandwidth.
 // Order >= 1: Cholesky of S + lam I in float64 (symmetric positive definite
@@ -464,6 +467,8 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(
     const double* __restrict__ shift, int nval, int nout, double* __restrict__ m_out,
     const double* __restrict__ q, double* __restrict__ dot) {
   __shared__ double red[SOLVE_THREADS / 64][MAXV];
+  constexpr int CGR = solve_group<T>();
+  static_assert(MAXV % CGR == 0, "channel groups");
   const double kmax = flpr_decode(*key);
   const int h = blockIdx.y;
   const T* b = mom + (int64_t)h * per_h;
@@ -474,6 +479,26 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(
        node += (int64_t)gridDim.x * blockDim.x) {
     double z[NT];
     double den = 1.0, s00 = 0.0, zt0 = 0.0;
+    // the value channels' moments are read in groups of CGR channels, each
+    // group's loads issued together (and the first group's before the
+    // factorisation), so that a node waits for memory a few times, not once
+    // per channel
+    T tnext[CGR][NT];
+    double qnext[CGR];
+    auto load_group = [&](int g0) {
+#pragma unroll
+      for (int j = 0; j < CGR; ++j) {
+        const bool on = g0 + j < nval;
+        const int64_t c = g0 + j + 1;  // channel 0 is the ones channel
+#pragma unroll
+        for (int i = 0; i < NT; ++i) tnext[j][i] = on ? b[tab.t_off[i] + c * nodes + node] : T(0);
+        qnext[j] = (q && on) ? q[(int64_t)(g0 + j) * nodes + node] : 0.0;
+      }
+    };
+    load_group(0);
+    T t0v[NT];
+#pragma unroll
+    for (int i = 0; i < NT; ++i) t0v[i] = b[tab.t_off[i] + node];
     if (NT == 1) {
       s00 = (double)b[tab.s_off[0] + node];
       const double s = s00 + eps_add;                    // regression.py:127
@@ -587,23 +612,37 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(
       }
       // z . T_0 (ones channel) for the centring correction
 #pragma unroll
-      for (int i = 0; i < NT; ++i) zt0 += z[i] * (double)b[tab.t_off[i] + node];
+      for (int i = 0; i < NT; ++i) zt0 += z[i] * (double)t0v[i];
     }
 #pragma unroll
-    for (int cc = 0; cc < MAXV; ++cc) {
-      if (cc >= nval) continue;
-      const int c = cc + 1;  // channel 0 is the ones channel
-      const double sh = shift ? shift[c] : 0.0;
-      double m;
-      if (NT == 1) {
-        m = ((double)b[tab.t_off[0] + (int64_t)c * nodes + node] + sh * s00) / den;
-      } else {
-        m = sh * zt0;
+    for (int g0 = 0; g0 < MAXV; g0 += CGR) {
+      if (g0 >= nval) break;
+      T tcur[CGR][NT];
+      double qcur[CGR];
 #pragma unroll
-        for (int i = 0; i < NT; ++i) m += z[i] * (double)b[tab.t_off[i] + (int64_t)c * nodes + node];
+      for (int j = 0; j < CGR; ++j) {
+        qcur[j] = qnext[j];
+#pragma unroll
+        for (int i = 0; i < NT; ++i) tcur[j][i] = tnext[j][i];
       }
-      if (cc < nout) m_out[((int64_t)h * nout + cc) * nodes + node] = m;
-      if (q) dacc[cc] += q[(int64_t)cc * nodes + node] * m;
+      if (g0 + CGR < nval) load_group(g0 + CGR);
+#pragma unroll
+      for (int j = 0; j < CGR; ++j) {
+        const int cc = g0 + j;
+        if (cc >= nval) break;
+        const int c = cc + 1;
+        const double sh = shift ? shift[c] : 0.0;
+        double m;
+        if (NT == 1) {
+          m = ((double)tcur[j][0] + sh * s00) / den;
+        } else {
+          m = sh * zt0;
+#pragma unroll
+          for (int i = 0; i < NT; ++i) m += z[i] * (double)tcur[j][i];
+        }
+        if (cc < nout) m_out[((int64_t)h * nout + cc) * nodes + node] = m;
+        if (q) dacc[cc] += qcur[j] * m;
+      }
     }
   }
   if (q) {
