@@ -23,6 +23,7 @@ EXPORTS = (
     "flpr_resample_work",
     "flpr_resample",
     "flpr_grid_center",
+    "flpr_grid_periodize",
     "flpr_taps",
     "flpr_conv",
     "flpr_conv_tiles",
@@ -85,6 +86,7 @@ _SIGS = {
     "flpr_points_pack": [_GP, _P, _P, _I64, _P, _I32, _P, _I32, _P, _P, _P, _P, _P],
     "flpr_spread": [_GP, _P, _P, _P, _I64, _I32, _I32, _P, _P, _P, _P],
     "flpr_resample_work": [_GP, _GP, _I32, _I32, _P],
+    "flpr_grid_periodize": [_GP, _P, _I32, _P, _P],
     "flpr_conv_tiles": [_I32, _I32, _I32, _I32, _I32, _P, _P],
     "flpr_resample": [_GP, _GP, _P, _I32, _I32, _P, _P, _P, _P],
     "flpr_grid_center": [_P, _I32, _I64, _P, _P, _P, _P],

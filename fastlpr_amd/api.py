@@ -96,6 +96,8 @@ def _prepare_bandwidths(h, dims: int, options: dict, default_range) -> np.ndarra
                 raise ValueError(
                     "Bandwidth array must have the same dimensionality as the input data.")
             bw = a.reshape(-1, dims)
+    if bw.shape[0] == 0:
+        raise ValueError("Bandwidth array is empty: give at least one bandwidth.")
     if not np.all(np.isfinite(bw)):
         raise ValueError("Bandwidth values must be finite (no NaN or Inf).")
     if np.any(bw <= 0):
@@ -162,6 +164,27 @@ def _check_accuracy(acc: int, dims: int) -> None:
     if acc > 6 and dims == 3:
         raise NotImplementedError("accuracy > 6 in 3-D is not supported on the GPU (the "
                                   "oversampled footprint does not fit the spread tile)")
+
+
+def _check_axes(stats, xrange=None):
+    """A dimension without spread gives a raw grid axis linspace(v, v, N) that
+    is not strictly ascending; the reference's RegularGridInterpolator on it
+    (api.py:686-696, regression.py:672-713) raises this ValueError."""
+    if xrange is not None:
+        return
+    flat = np.flatnonzero(~(np.asarray(stats.xmax) > np.asarray(stats.xmin)))
+    if flat.size:
+        raise ValueError(f"The points in dimension {int(flat[0])} must be strictly ascending "
+                         "or descending")
+
+
+def _check_cubic(N):
+    """scipy's cubic RegularGridInterpolator (the reference's default for 2-D /
+    3-D under 5000 points, api.py:588-594) needs four nodes per axis."""
+    for k, nk in enumerate(N):
+        if nk < 4:
+            raise ValueError(f"There are {nk} points in dimension {k}, but method cubic "
+                             "requires at least  4 points per dimension.")
 
 
 def _grid_shape(opts: dict, n: int, dims: int) -> Tuple[int, ...]:
@@ -451,6 +474,9 @@ def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
     if str(opts.get("kernel_type", "gaussian")).lower() not in ("gauss", "gaussian", "normal"):
         raise NotImplementedError("the GPU engine implements the Gaussian kernel")
     _check_accuracy(int(opts["accuracy"]), dims)
+    # explicit bandwidths are validated before any device work; the default
+    # range needs the x statistics
+    bw_given = _prepare_bandwidths(h, dims, opts, None) if h is not None else None
     variance = str(opts.get("y_type_out", "mean")) == "variance"
     fp64 = str(opts.get("precision", "fp32")).lower() == "fp64"
     N = _grid_shape(opts, n, dims)
@@ -488,11 +514,12 @@ def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
         if stats.bad_y and isinstance(y_arr, torch.Tensor):
             _check_finite(y_arr, "Response y", "cv_fastlpr")
     # default range 0.1..1 x std(ddof=1) from the device statistics (helpers.py:74-92)
-    bandwidths = _prepare_bandwidths(
+    bandwidths = bw_given if bw_given is not None else _prepare_bandwidths(
         h, dims, opts, lambda: np.column_stack((0.1 * stats.std, 1.0 * stats.std)))
     reg = float(opts["regularization"])
     calc_dof = bool(opts.get("calc_dof", True))
     xrange = opts.get("xrange", None)
+    _check_axes(stats, xrange)
     flag_power2 = bool(opts.get("flag_power2", True))
     lin_default = "linear" if (dims == 1 or n >= 5000) else "cubic"
 
@@ -515,6 +542,8 @@ def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
                 raise NotImplementedError(f"{key}={meth!r}: the GPU engine implements "
                                           "'linear' and 'cubic'")
         cubic = "cubic" in (dof_method, gcv_method)
+        if cubic:
+            _check_cubic(N)
         if cubic and xrange is not None:
             raise NotImplementedError("cubic GCV/DOF interpolation with options['xrange'] "
                                       "is not implemented on the GPU")
@@ -692,6 +721,7 @@ def cv_fastkde(x, h: Optional[Sequence[Sequence[float]]] = None,
     if opts.get("output_interp_method", "linear") != "linear":
         raise NotImplementedError("only linear interpolation is implemented on the GPU")
     _check_accuracy(int(opts["accuracy"]), dims)
+    bw_given = _prepare_bandwidths(h, dims, opts, None) if h is not None else None
     fp64 = str(opts.get("precision", "fp32")).lower() == "fp64"
     N = _grid_shape(opts, n, dims)
     dev = x.device if _is_dev(x) else _device()
@@ -700,7 +730,8 @@ def cv_fastkde(x, h: Optional[Sequence[Sequence[float]]] = None,
     stats = engine.x_stats(xd, None)
     if stats.bad_x:
         _check_finite(x_arr, "Input x", "cv_fastkde")
-    bandwidths = _prepare_bandwidths(
+    _check_axes(stats)
+    bandwidths = bw_given if bw_given is not None else _prepare_bandwidths(
         h, dims, opts, lambda: np.column_stack((0.1 * stats.std, 1.0 * stats.std)))
 
     fit = engine.run_grid(xd, None, 0, None, 0, bandwidths, 0, N, int(opts["accuracy"]), fp64,

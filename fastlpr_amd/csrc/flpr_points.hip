@@ -636,6 +636,76 @@ __global__ __launch_bounds__(256) void k_grid_center(double* __restrict__ grid, 
   }
 }
 
+// Periodic folding of the unwrapped spread grid for grids narrower than their
+// footprint.  The reference spreads onto an L-periodic grid and convolves
+// circularly (nufftn_type1.py:565-636, convolution.py:904-951); the unwrapped
+// grid (signed cells -E..E, G = 2E+1 of them) equals it only while no two data
+// cells alias, i.e. G <= L.  For small N (G > L on some axis) every cell is set
+// to the sum over its residue class modulo L: each output's N-tap window then
+// sees every residue once, with the periodic grid's value, which is the
+// circular convolution.  Pass 1 gathers each residue's sum into per (channel,
+// residue) storage, pass 2 writes the sums back to every cell of the class.
+template <int D>
+__global__ __launch_bounds__(256) void k_periodic_sum(FlprGeom g, const double* __restrict__ grid,
+                                                      int64_t cells, int64_t pcells,
+                                                      double* __restrict__ per) {
+  const int c = blockIdx.y;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < pcells;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    int res[3] = {0, 0, 0}, cnt[3] = {1, 1, 1}, first[3] = {0, 0, 0};
+    int64_t rem = r;
+#pragma unroll
+    for (int k = D - 1; k >= 0; --k) {
+      res[k] = (int)(rem % g.L[k]);
+      rem /= g.L[k];
+      // cells s with (s - E) mod L == res: the first is s0 = the smallest s >= 0
+      int s0 = ((res[k] + g.E[k]) % g.L[k] + g.L[k]) % g.L[k];
+      first[k] = s0;
+      cnt[k] = (s0 < g.G[k]) ? (g.G[k] - 1 - s0) / g.L[k] + 1 : 0;
+    }
+    double v = 0.0;
+    for (int a = 0; a < cnt[0]; ++a) {
+      const int s0 = first[0] + a * g.L[0];
+      if constexpr (D == 1) {
+        v += grid[(int64_t)c * cells + s0];
+      } else {
+        for (int b = 0; b < cnt[1]; ++b) {
+          const int s1 = first[1] + b * g.L[1];
+          if constexpr (D == 2) {
+            v += grid[(int64_t)c * cells + (int64_t)s0 * g.G[1] + s1];
+          } else {
+            for (int e = 0; e < cnt[2]; ++e) {
+              const int s2 = first[2] + e * g.L[2];
+              v += grid[(int64_t)c * cells + ((int64_t)s0 * g.G[1] + s1) * g.G[2] + s2];
+            }
+          }
+        }
+      }
+    }
+    per[(int64_t)c * pcells + r] = v;
+  }
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_periodic_put(FlprGeom g, double* __restrict__ grid,
+                                                      int64_t cells, int64_t pcells,
+                                                      const double* __restrict__ per) {
+  const int c = blockIdx.y;
+  for (int64_t lin = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; lin < cells;
+       lin += (int64_t)gridDim.x * blockDim.x) {
+    int64_t rem = lin, r = 0, mul = 1;
+#pragma unroll
+    for (int k = D - 1; k >= 0; --k) {
+      const int s = (int)(rem % g.G[k]);
+      rem /= g.G[k];
+      const int res = ((s - g.E[k]) % g.L[k] + g.L[k]) % g.L[k];
+      r += res * mul;
+      mul *= g.L[k];
+    }
+    grid[(int64_t)c * cells + lin] = per[(int64_t)c * pcells + r];
+  }
+}
+
 // Corner pair (qa <= qb) of the e-th entry of a cell's interpolation Gram matrix.
 template <int D>
 __device__ inline void gram_pair(int e, int* qa, int* qb) {
@@ -1447,6 +1517,35 @@ int flpr_spread(const FlprGeom* g, const int32_t* key_start, const double* wab,
     case 1: hipLaunchKernelGGL(k_spread_gather<1>, gg, dim3(256), 0, st, *g, part, C, cells, grid, sums); break;
     case 2: hipLaunchKernelGGL(k_spread_gather<2>, gg, dim3(256), 0, st, *g, part, C, cells, grid, sums); break;
     default: hipLaunchKernelGGL(k_spread_gather<3>, gg, dim3(256), 0, st, *g, part, C, cells, grid, sums); break;
+  }
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+int flpr_grid_periodize(const FlprGeom* g, double* grid, int32_t C, double* work,
+                        void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!flpr_geom_ok(g) || !grid || !work || C < 1) return -1;
+  int64_t cells = 1, pcells = 1;
+  for (int k = 0; k < g->d; ++k) {
+    cells *= g->G[k];
+    pcells *= g->L[k];
+  }
+  hipStream_t st = as_stream(stream);
+  dim3 g1(flpr_grid_for(pcells, 256, 256), (unsigned)C), g2(flpr_grid_for(cells, 256, 256), (unsigned)C);
+  switch (g->d) {
+    case 1:
+      hipLaunchKernelGGL(k_periodic_sum<1>, g1, dim3(256), 0, st, *g, grid, cells, pcells, work);
+      hipLaunchKernelGGL(k_periodic_put<1>, g2, dim3(256), 0, st, *g, grid, cells, pcells, work);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_periodic_sum<2>, g1, dim3(256), 0, st, *g, grid, cells, pcells, work);
+      hipLaunchKernelGGL(k_periodic_put<2>, g2, dim3(256), 0, st, *g, grid, cells, pcells, work);
+      break;
+    default:
+      hipLaunchKernelGGL(k_periodic_sum<3>, g1, dim3(256), 0, st, *g, grid, cells, pcells, work);
+      hipLaunchKernelGGL(k_periodic_put<3>, g2, dim3(256), 0, st, *g, grid, cells, pcells, work);
+      break;
   }
   FLPR_CHECK_LAUNCH();
   return 0;

@@ -770,6 +770,21 @@ def spread_fine(geo: Geometry, accuracy: int, x: torch.Tensor, n: int,
              grid64.data_ptr() + (c0 + f0) * cells * 8, sums.data_ptr() + (c0 + f0) * 8, s)
 
 
+def periodize(geo: Geometry, grid: torch.Tensor, c0: int, nch: int) -> None:
+    """Fold channels c0..c0+nch-1 of a freshly spread unwrapped grid onto the
+    reference's L-periodic grid when some axis is narrower than the spread
+    footprint (G > L, small N; flpr_grid_periodize); otherwise nothing."""
+    g = geo.geom
+    d = len(geo.N)
+    if all(g.G[k] <= g.L[k] for k in range(d)):
+        return
+    cells = int(np.prod([g.G[k] for k in range(d)]))
+    work = torch.empty((nch * int(np.prod([g.L[k] for k in range(d)])),), dtype=torch.float64,
+                       device=grid.device)
+    call("flpr_grid_periodize", _lib.C.byref(g), grid.data_ptr() + c0 * cells * 8, nch,
+         ptr(work), stream_handle())
+
+
 def filter_bandwidths(geo: Geometry, h: np.ndarray, n: int):
     """Too-small bandwidths (kernel sum < 10 eps) are dropped; all bad -> Silverman
     (convolution.py:542-605).  The d-dimensional kernel sum is the product of
@@ -1010,9 +1025,7 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     else:
         call("flpr_spread", gp, ptr(key_start), ptr(wab), ptr(vals), n, 1, nval, ptr(part),
              ptr(grid64), ptr(sums), s)
-    if fine_y and not fine_s:
-        spread_fine(geo, acc_y, x, n, vals_y, ny, y_pairs, None, 0, False, grid64, sums, 1, s)
-    elif split_acc and not fine_s:
+    if split_acc and not fine_s and not fine_y:
         # the y columns again, with the footprint and width of their own
         # accuracy, over the channels the first spread wrote
         geo_y, _ = geometry_for(st, N, int(accuracy_y), xrange, flag_power2, kde, margin)
@@ -1025,6 +1038,10 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
         call("flpr_spread", gy, ptr(key_start), ptr(wab_y), ptr(vals_yp), n, 0, ny, ptr(part),
              ptr(grid64) + cells * 8, ptr(sums) + 8, s)
         del wab_y, vals_yp
+    if not fine_s:
+        periodize(geo, grid64, 0, C)
+    if fine_y and not fine_s:
+        spread_fine(geo, acc_y, x, n, vals_y, ny, y_pairs, None, 0, False, grid64, sums, 1, s)
     del part
     any32 = any(not gf for (_a, _b, gf) in groups)
     shift = torch.empty((C,), dtype=f64, device=dev)
@@ -1201,6 +1218,7 @@ def run_grid_y(sh: dict, x: torch.Tensor, vals_y: torch.Tensor, ny: int,
         call("flpr_spread", gy, ptr(sh["key_start"]), ptr(wab), ptr(vals), n, 0, ny, ptr(part),
              grid64.data_ptr() + cells * 8, sums.data_ptr() + 8, s)
         del part
+        periodize(geo, grid64, 1, ny)
     shift = torch.empty((C2,), dtype=f64, device=dev)
     any32 = sh["any32"]
     grid32 = torch.empty((C2 * cells + pad,), dtype=torch.float32, device=dev) if any32 else None

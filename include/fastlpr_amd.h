@@ -135,6 +135,15 @@ int flpr_resample(const FlprGeom* fine, const FlprGeom* coarse, const double* fi
                   int32_t nch, int32_t pairs, double* work, double* grid, double* sums,
                   void* stream);
 
+/* Grids narrower than their spread footprint (G > L on some axis; small N):
+ * the reference's spread grid is L-periodic and its convolution circular
+ * (nufftn_type1.py:565-636, convolution.py:904-951).  Sets every cell of the
+ * C channels of grid (the unwrapped grid, G0 x .. cells per channel) to the
+ * sum over its residue class modulo L, after which the convolution passes
+ * compute the circular convolution.  work holds C * prod(L) doubles.  Apply
+ * once to freshly spread channels (a second application would multiply). */
+int flpr_grid_periodize(const FlprGeom* g, double* grid, int32_t C, double* work, void* stream);
+
 /* Centre channels 1..C-1 of the spread grid on their kernel-weighted mean:
  * shift[c] = sums[c] / sums[0], F_c -= shift[c] F_0 (float64, in place), and
  * optionally write a float32 copy of the whole grid.  shift (C doubles) may be

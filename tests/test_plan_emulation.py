@@ -105,6 +105,8 @@ def _emulate(x, y, h, order, N, acc=None):
             grid[:, idx[0]] += vals[i][:, None] * ws[0][None]
         else:
             grid[:, idx[0][:, None], idx[1][None, :]] += vals[i][:, None, None] * np.outer(ws[0], ws[1])[None]
+    if acc <= 6:
+        grid = _periodize(grid, g, d)
     tap = np.zeros(dh * d * plan.npow * plan.nmax)
     for b in range(dh):
         for k in range(d):
@@ -121,6 +123,26 @@ def _emulate(x, y, h, order, N, acc=None):
     _run(plan.pass_tables, [plan.chunk * plan.stage_per_h[k] for k in range(d + 1)],
          grid.reshape(-1), stage0, tap, bands, d)
     return plan, stage0, h, acc, dict(geo=geo, tap=tap, bands=bands, xmod=xmod, cell=cell)
+
+
+def _periodize(grid, g, d):
+    """flpr_grid_periodize: for G > L on some axis, every cell of the unwrapped
+    grid becomes the sum over its residue class modulo L."""
+    G = [g.G[k] for k in range(d)]
+    L = [g.L[k] for k in range(d)]
+    if all(G[k] <= L[k] for k in range(d)):
+        return grid
+    res = [np.mod(np.arange(G[k]) - g.E[k], L[k]) for k in range(d)]
+    out = np.empty_like(grid)
+    for c in range(grid.shape[0]):
+        per = np.zeros(L)
+        if d == 1:
+            np.add.at(per, res[0], grid[c])
+            out[c] = per[res[0]]
+        else:
+            np.add.at(per, (res[0][:, None], res[1][None, :]), grid[c])
+            out[c] = per[res[0][:, None], res[1][None, :]]
+    return out
 
 
 def _run(tables, inter_sizes, grid, stage0, tap, bands, d):
@@ -150,7 +172,8 @@ def _run(tables, inter_sizes, grid, stage0, tap, bands, d):
 
 
 CASES = [(1, o, [40]) for o in (0, 1, 2)] + [(1, 1, [33])] + \
-        [(2, o, [12, 10]) for o in (0, 1, 2)] + [(2, 2, [11, 9])]
+        [(2, o, [12, 10]) for o in (0, 1, 2)] + [(2, 2, [11, 9])] + \
+        [(1, 1, [2]), (1, 0, [3]), (1, 2, [4]), (2, 1, [2, 3]), (2, 0, [5, 2])]
 
 
 CASES_HI = [(1, 1, [40], 7), (1, 2, [33], 12), (2, 1, [12, 10], 7), (2, 2, [11, 9], 9)]
