@@ -27,6 +27,8 @@ EXPORTS = (
     "flpr_taps",
     "flpr_conv",
     "flpr_conv_tiles",
+    "flpr_conv_strip_len",
+    "flpr_conv_strips",
     "flpr_max_moment",
     "flpr_key_to_double",
     "flpr_solve",
@@ -87,12 +89,13 @@ _SIGS = {
     "flpr_spread": [_GP, _P, _P, _P, _I64, _I32, _I32, _P, _P, _P, _P],
     "flpr_resample_work": [_GP, _GP, _I32, _I32, _P],
     "flpr_grid_periodize": [_GP, _P, _I32, _P, _P],
-    "flpr_conv_tiles": [_I32, _I32, _I32, _I32, _I32, _P, _P],
+    "flpr_conv_tiles": [_I32, _I32, _I32, _I32, _I32, _I32, _P, _P],
+    "flpr_conv_strips": [_P, _P, _I32, _I32, _I32, _P, _P],
     "flpr_resample": [_GP, _GP, _P, _I32, _I32, _P, _P, _P, _P],
     "flpr_grid_center": [_P, _I32, _I64, _P, _P, _P, _P],
     "flpr_taps": [_GP, _P, _I32, _I32, _I32, _D, _P, _P, _P],
-    "flpr_conv": [_I32, _I32, _P, _P, _P, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _P,
-                  _I32, _P],
+    "flpr_conv": [_I32, _I32, _P, _P, _P, _I32, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32,
+                  _P, _I32, _P],
     "flpr_max_moment": [_I32, _P, _I64, _I32, C.POINTER(_I64), _I32, _I64, _I32, _P, _P],
     "flpr_solve": [_I32, _P, _I64, _I32, _I64, _I32, _I32, _I32, C.POINTER(_I64), C.POINTER(_I64), _P,
                    _D, _D, _P, _I32, _I32, _P, _P, _P, _P],
@@ -128,6 +131,8 @@ def load(path: str = LIB_PATH):
         fn.restype = C.c_int
     lib.flpr_key_to_double.argtypes = [C.c_uint64]
     lib.flpr_key_to_double.restype = C.c_double
+    lib.flpr_conv_strip_len.argtypes = [C.c_int32]
+    lib.flpr_conv_strip_len.restype = C.c_int64
     _lib = lib
     return lib
 

@@ -8,8 +8,8 @@
 // same numbers are a sequence of 1-D banded Toeplitz products along each axis
 // of the unwrapped spread grid.  Each pass is a batched GEMM in which one
 // operand is the Toeplitz band, generated in LDS from a 1-D tap strip; the
-// GEMM runs on v_mfma_f32_16x16x4f32 (float32 path) or v_mfma_f64_16x16x4f64
-// (float64 path).
+// GEMM runs on v_mfma_f64_16x16x4f64 (float64 form) or on bf16 MFMA with
+// exact three-term splits of the float32 operands (float32 form).
 #include <type_traits>
 
 #include "flpr_common.h"
@@ -19,23 +19,10 @@ namespace {
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-template <typename T>
-struct Mfma;
-// Operands (both forms): lane l holds A[l & 15][l >> 4] and B[l >> 4][l & 15].
-// Results: register r of lane l is D[row(l, r)][l & 15] with
-//   f32 16x16x4: row = 4 (l >> 4) + r
-//   f64 16x16x4: row = (l >> 4) + 4 r
-// (both checked on gfx950 with a probe of known operands).
-template <>
-struct Mfma<float> {
-  typedef v4f acc;
-  __device__ static inline acc run(float a, float b, acc c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  __device__ static inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
-};
-template <>
-struct Mfma<double> {
+// v_mfma_f64_16x16x4f64: lane l holds A[l & 15][l >> 4] and B[l >> 4][l & 15];
+// result register r of lane l is D[(l >> 4) + 4 r][l & 15] (checked on gfx950
+// with a probe of known operands).
+struct Mfma64 {
   typedef v4d acc;
   __device__ static inline acc run(double a, double b, acc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -110,27 +97,14 @@ constexpr int WD = NDB * 16;   // dense extent of one wave
 constexpr int LD = 4 * WD;     // dense extent of a block tile
 constexpr int NQ = LT / 16;    // Toeplitz 16-blocks, shared by the four waves
 
-// K chunk staged in LDS per precision (double buffering of the 256-wide dense
-// tile: 2 x 35 KB for float, 2 x 35 KB for double with the shorter chunk).
-template <typename T>
-struct ConvCfg;
-template <>
-struct ConvCfg<float> {
-  static constexpr int KC = 32;
-};
-template <>
-struct ConvCfg<double> {
-  static constexpr int KC = 16;
-};
-
-// Banded Toeplitz GEMM.  A block owns a 64-wide stretch of the convolved
-// (Toeplitz) axis and a 256-wide stretch of the other (dense) axis; each of
-// its four waves owns 64 of the dense extent and all 64 Toeplitz outputs, as
-// 4 x 4 MFMA 16x16 blocks, so one K step costs 8 LDS reads for 16 MFMAs.  The
-// dense operand of chunk k+1 is fetched into registers while chunk k is
-// multiplied from LDS (double-buffered, one barrier per chunk).  The Toeplitz
+// Banded Toeplitz GEMM of the float64 form.  A block owns a 64-wide stretch
+// of the convolved (Toeplitz) axis and a 64-wide stretch of the other (dense)
+// axis; each of its four waves owns 16 of the dense extent and all 64
+// Toeplitz outputs, as 4 MFMA 16x16 blocks.  The dense operand of chunk k+1
+// is fetched into registers while chunk k is multiplied from LDS
+// (double-buffered, one barrier per chunk; K chunks of 16).  The Toeplitz
 // operand is generated from a 1-D strip of taps: element (i, s) is
-// tap[i + base - s].  Float32 chunk sums are folded into float64 totals.
+// tap[i + base - s].
 //
 // Dense rows (ROW pass) or columns (COL pass) past the job only feed outputs
 // that are never stored, and inputs past the chunk's band end are replaced by
@@ -141,14 +115,16 @@ struct ConvCfg<double> {
 // Block schedule: with a block table (blocks != nullptr) block b runs tile
 // blocks[2b+1] of job blocks[2b] (a negative job is an idle block); without
 // one, the grid is (tiles of the largest job, jobs).
-template <typename T, int MODE>
-__global__ __launch_bounds__(CT) void k_conv(const T* __restrict__ in, T* __restrict__ out,
-                                             const T* __restrict__ tap,
+template <int MODE>
+__global__ __launch_bounds__(CT) void k_conv(const double* __restrict__ in,
+                                             double* __restrict__ out,
+                                             const double* __restrict__ tap,
                                              const int64_t* __restrict__ jobs,
                                              const int32_t* __restrict__ bands, int K, int nout,
                                              int inner, int base, int ntd, int ntt,
                                              const int32_t* __restrict__ blocks) {
-  constexpr int KC = ConvCfg<T>::KC;
+  typedef double T;
+  constexpr int KC = 16;
   // LDS pitch of the dense tile: a K step reads 16 rows x 4 K (ROW pass) or
   // 4 K rows x 16 columns (COL pass); the pads spread each half-wave over 32 banks
   constexpr int PD = (MODE == 0) ? KC + 2 : LD + 16;
@@ -158,7 +134,6 @@ __global__ __launch_bounds__(CT) void k_conv(const T* __restrict__ in, T* __rest
   constexpr int PER = LD * KC / CT;   // dense elements staged per thread and chunk
   constexpr int RSTEP = CT / KC;      // ROW pass: row stride between a thread's elements
   constexpr int KSTEP = CT / LD;      // COL pass: K-row stride between a thread's elements
-  constexpr bool F32 = sizeof(T) == 4;
   __shared__ T dense[2][DENSE];
   __shared__ T strip[2][NSTRIP + 1];
   int job = blockIdx.y, bx = blockIdx.x;
@@ -203,22 +178,13 @@ __global__ __launch_bounds__(CT) void k_conv(const T* __restrict__ in, T* __rest
                                : src + (int64_t)k0 * inner + d0 + cc0;
   const int64_t sstep = (MODE == 0) ? (int64_t)RSTEP * K : (int64_t)KSTEP * inner;
 
-  typename Mfma<T>::acc acc[NQ][NDB];
-  double tot[F32 ? NQ : 1][F32 ? NDB : 1][4];
+  Mfma64::acc acc[NQ][NDB];
 #pragma unroll
   for (int q = 0; q < NQ; ++q)
 #pragma unroll
     for (int db = 0; db < NDB; ++db)
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[q][db][j] = T(0);
-  if constexpr (F32) {
-#pragma unroll
-    for (int q = 0; q < NQ; ++q)
-#pragma unroll
-      for (int db = 0; db < NDB; ++db)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) tot[q][db][j] = 0.0;
-  }
 
   // Two register sets of prefetched raw values keep two chunks in flight;
   // the zeroing past the band end is applied when a set is stored to LDS, so
@@ -287,8 +253,8 @@ __global__ __launch_bounds__(CT) void k_conv(const T* __restrict__ in, T* __rest
           const T tq = tv[4 * q - st + NST - 1];
 #pragma unroll
           for (int db = 0; db < NDB; ++db)
-            acc[q][db] = (MODE == 0) ? Mfma<T>::run(dv[st][db], tq, acc[q][db])
-                                     : Mfma<T>::run(tq, dv[st][db], acc[q][db]);
+            acc[q][db] = (MODE == 0) ? Mfma64::run(dv[st][db], tq, acc[q][db])
+                                     : Mfma64::run(tq, dv[st][db], acc[q][db]);
         }
       };
       if (live == (1 << NQ) - 1) {
@@ -300,8 +266,8 @@ __global__ __launch_bounds__(CT) void k_conv(const T* __restrict__ in, T* __rest
             const T tq = tv[4 * q - st + NST - 1];
 #pragma unroll
             for (int db = 0; db < NDB; ++db)
-              acc[q][db] = (MODE == 0) ? Mfma<T>::run(dv[st][db], tq, acc[q][db])
-                                       : Mfma<T>::run(tq, dv[st][db], acc[q][db]);
+              acc[q][db] = (MODE == 0) ? Mfma64::run(dv[st][db], tq, acc[q][db])
+                                       : Mfma64::run(tq, dv[st][db], acc[q][db]);
           }
       } else {
         // the live blocks are contiguous (the band is): one scalar test per block
@@ -310,18 +276,6 @@ __global__ __launch_bounds__(CT) void k_conv(const T* __restrict__ in, T* __rest
         if (live & 4) mac_q(std::integral_constant<int, 2>{});
         if (live & 8) mac_q(std::integral_constant<int, 3>{});
       }
-    }
-    if constexpr (F32) {
-      // the float32 MFMA accumulator only ever holds KC terms
-#pragma unroll
-      for (int q = 0; q < NQ; ++q)
-#pragma unroll
-        for (int db = 0; db < NDB; ++db)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            tot[q][db][j] += (double)acc[q][db][j];
-            acc[q][db][j] = T(0);
-          }
     }
   };
 
@@ -357,7 +311,7 @@ __global__ __launch_bounds__(CT) void k_conv(const T* __restrict__ in, T* __rest
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         // MFMA result (row(lane, j), lane & 15): rows follow the first operand
-        const int rr = Mfma<T>::row(lane, j), cc = lane & 15;
+        const int rr = Mfma64::row(lane, j), cc = lane & 15;
         int dd, ii;
         if (MODE == 0) {
           dd = d0 + dw + db * 16 + rr;
@@ -367,9 +321,351 @@ __global__ __launch_bounds__(CT) void k_conv(const T* __restrict__ in, T* __rest
           dd = d0 + dw + db * 16 + cc;
         }
         if (dd < ndense && ii < ntoe) {
-          const double v = F32 ? tot[F32 ? q : 0][F32 ? db : 0][j] : (double)acc[q][db][j];
-          if (MODE == 0) dst[(int64_t)dd * nout + ii] = (T)v;
-          else dst[(int64_t)ii * inner + dd] = (T)v;
+          if (MODE == 0) dst[(int64_t)dd * nout + ii] = acc[q][db][j];
+          else dst[(int64_t)ii * inner + dd] = acc[q][db][j];
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// float32 form: the same banded Toeplitz GEMM on bf16 MFMA with three-term splits
+// ---------------------------------------------------------------------------
+//
+// Every float32 operand x is written exactly as x = hi + mid + lo with three
+// bfloat16 values (hi = bf16(x), mid = bf16(x - hi), lo = bf16(x - hi - mid);
+// 3 x 8 significand bits hold the 24 of a float32).  A product of two split
+// operands keeps the six terms whose levels add up to at most two
+// (hi*hi, hi*mid, mid*hi, hi*lo, lo*hi, mid*mid); the three dropped ones are
+// below 2^-26 of the product.  Each term is an exact float32 product summed
+// by v_mfma_f32_16x16x32_bf16 (16 cycles for 16x16x32, against 32 cycles for
+// 16x16x4 in float32), so the six terms cost 2.67x less MFMA time than the
+// float32 product at float32 accuracy.  Chunk sums are folded into float64 as
+// in k_conv.
+//
+// Operand layout of v_mfma_f32_16x16x32_bf16 (checked on gfx950 with known
+// operands): lane l holds A[l & 15][8 (l >> 4) + j] and B[8 (l >> 4) + j][l & 15],
+// j = 0..7; result register r of lane l is D[4 (l >> 4) + r][l & 15].
+//
+// The Toeplitz operand of a chunk comes from a reversed tap window
+// R[v] = tap[t_top - v], v in [0, 95), t_top = i0 + base - s0 + 63: lane
+// (i, g) of output block q needs R[63 - 16 q - i + 8 g + j], 8 consecutive
+// values at an alignment set by i.  LDS holds eight copies of the window
+// shifted by r = 0..7 (copy r, element m = R[m + r]), so every lane reads one
+// aligned 16-byte unit of copy 7 - (i & 7).  The copies are cut from a table
+// built once per fit (flpr_conv_strips): per tap row, the reversed, band-masked,
+// split row in eight shifted copies, so that the per-chunk staging is aligned
+// 16-byte loads.
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));   // one 16-byte unit
+
+constexpr int SX_LD = 128;          // dense extent of a block tile (2 x 16 per wave)
+constexpr int SX_KC = 32;           // K chunk
+constexpr int SX_NDB = 2;           // dense 16-blocks per wave
+constexpr int SX_COPY = 88;         // elements of one shifted window copy in LDS (11 units;
+                                    // 11 odd keeps the eight copies on distinct banks)
+constexpr int SX_UNITS = SX_COPY / 8;
+constexpr int SX_TOP_PAD = 62;      // reversed table index v = (nmax + 62) - t
+
+__host__ __device__ inline int64_t split_strip_len(int nmax) {
+  // v runs over [0, nmax + 62 + 95) for the windows a kernel can ask for
+  return ((int64_t)nmax + SX_TOP_PAD + 95 + 7) / 8 * 8;
+}
+
+__device__ inline void split3(float x, __bf16& h, __bf16& m, __bf16& l) {
+  h = (__bf16)x;
+  const float r1 = x - (float)h;
+  m = (__bf16)r1;
+  const float r2 = r1 - (float)m;
+  l = (__bf16)r2;
+}
+
+// The same split for 8 values at once, two per conversion (v_cvt_pk_bf16_f32)
+// and the exact bf16 -> f32 widening done with shifts and masks.
+__device__ inline void split3x8(const float (&x)[8], u32x4& hv, u32x4& mv, u32x4& lv) {
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    float a = x[2 * p], b = x[2 * p + 1];
+    uint32_t w[3];
+#pragma unroll
+    for (int lev = 0; lev < 3; ++lev) {
+      typedef float v2f __attribute__((ext_vector_type(2)));
+      typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+      v2f ab = {a, b};
+      const uint32_t pk = __builtin_bit_cast(uint32_t, __builtin_convertvector(ab, v2bf));
+      w[lev] = pk;
+      if (lev < 2) {
+        a -= __builtin_bit_cast(float, pk << 16);
+        b -= __builtin_bit_cast(float, pk & 0xffff0000u);
+      }
+    }
+    hv[p] = w[0];
+    mv[p] = w[1];
+    lv[p] = w[2];
+  }
+}
+
+// Table row A (tap row (b*d + k)*npow + a, band A / npow), plane p, copy r:
+// strips[((A*3 + p)*8 + r)*V + e] = split plane p of R[e + r], with
+// R[v] = tap[nmax + 62 - v] inside the band and zero elsewhere.
+__global__ __launch_bounds__(256) void k_split_taps(const float* __restrict__ tap,
+                                                    const int32_t* __restrict__ bands, int ntap,
+                                                    int npow, int nmax, int64_t V,
+                                                    uint16_t* __restrict__ strips) {
+  const int64_t units = V / 8;
+  const int64_t total = (int64_t)ntap * 8 * units;
+  const int top = nmax + SX_TOP_PAD;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t u = idx % units;
+    const int r = (int)((idx / units) % 8);
+    const int A = (int)(idx / (units * 8));
+    const int band = A / npow;
+    const int t_lo = max(bands[2 * band], 0), t_hi = min(bands[2 * band + 1], nmax - 1);
+    v8bf hv, mv, lv;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int t = top - (int)(u * 8 + j + r);
+      const float x = (t >= t_lo && t <= t_hi) ? tap[(int64_t)A * nmax + t] : 0.0f;
+      __bf16 h, m, l;
+      split3(x, h, m, l);
+      hv[j] = h;
+      mv[j] = m;
+      lv[j] = l;
+    }
+    u32x4* dst = reinterpret_cast<u32x4*>(strips);
+    const int64_t row = (int64_t)A * 3 * 8 + r;
+    dst[((row + 0 * 8) * V) / 8 + u] = __builtin_bit_cast(u32x4, hv);
+    dst[((row + 1 * 8) * V) / 8 + u] = __builtin_bit_cast(u32x4, mv);
+    dst[((row + 2 * 8) * V) / 8 + u] = __builtin_bit_cast(u32x4, lv);
+  }
+}
+
+__device__ inline v4f mfma_bf(const u32x4& a, const u32x4& b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a),
+                                                 __builtin_bit_cast(v8bf, b), c, 0, 0, 0);
+}
+
+// The six split products of X (first operand) and Y, added into c.
+__device__ inline v4f mfma_split6(const u32x4 (&x)[3], const u32x4 (&y)[3], v4f c) {
+  c = mfma_bf(x[1], y[1], c);
+  c = mfma_bf(x[0], y[2], c);
+  c = mfma_bf(x[2], y[0], c);
+  c = mfma_bf(x[0], y[1], c);
+  c = mfma_bf(x[1], y[0], c);
+  c = mfma_bf(x[0], y[0], c);
+  return c;
+}
+
+// Banded Toeplitz GEMM of the float32 form (jobs, bands, tiles and the block
+// schedule as in k_conv).  A block owns 64 Toeplitz outputs x 128 dense; wave
+// w owns dense [32 w, 32 w + 32) and all 64 outputs (4 x 2 MFMA blocks).  The
+// dense chunk (128 x 32) is split into three bf16 planes in LDS, 16-byte units
+// of 8 K per dense index with the unit order swizzled by (index >> 1) & 3 so
+// that fragment reads and stores hit distinct banks.
+template <int MODE>
+__global__ __launch_bounds__(CT) void k_conv_split(const float* __restrict__ in,
+                                                   float* __restrict__ out,
+                                                   const uint16_t* __restrict__ strips, int nmax,
+                                                   const int64_t* __restrict__ jobs,
+                                                   const int32_t* __restrict__ bands, int K,
+                                                   int nout, int inner, int base, int ntd, int ntt,
+                                                   const int32_t* __restrict__ blocks) {
+  constexpr int KC = SX_KC;
+  __shared__ u32x4 dense[2][3][SX_LD][4];
+  __shared__ u32x4 strip[2][3][8][SX_UNITS];
+  int job = blockIdx.y, bx = blockIdx.x;
+  if (blocks) {
+    job = blocks[2 * blockIdx.x];
+    bx = blocks[2 * blockIdx.x + 1];
+    if (job < 0) return;
+  }
+  const int64_t* jb = jobs + 8 * (int64_t)job;
+  const int64_t in_off = jb[0], out_off = jb[1], tap_off = jb[2];
+  const int band = (int)jb[3], ro = (int)jb[4];
+  const int t_lo = bands[2 * band], t_hi = bands[2 * band + 1];
+  const int td = bx % ntd;
+  bx /= ntd;
+  const int tt = bx % ntt;
+  const int o = bx / ntt;
+  const int d0 = td * SX_LD, i0 = tt * LT;
+  const int ndense = (MODE == 0) ? ro : inner;
+  if (MODE == 1 && o >= ro) return;
+  if (d0 >= ndense || i0 >= nout) return;
+  const int s_begin = max(i0 + base - t_hi, 0);
+  const int s_end = min(i0 + LT - 1 + base - t_lo, K - 1);
+  const int nchunks = (s_end >= s_begin) ? (s_end - s_begin) / KC + 1 : 0;
+  const float* src = in + in_off + (MODE == 0 ? 0 : (int64_t)o * K * inner);
+  float* dst = out + out_off + (MODE == 0 ? 0 : (int64_t)o * nout * inner);
+  const int64_t V = split_strip_len(nmax);
+  const u32x4* sg = reinterpret_cast<const u32x4*>(strips) + (tap_off / nmax) * 3 * 8 * (V / 8);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int dw = wave * (SX_NDB * 16);
+
+  // staging: thread owns two (dense index, 8-K unit) groups
+  //   ROW: group g = tid + 256 u -> dense row g >> 2, unit g & 3 (loads along K)
+  //   COL: dense column tid & 127, unit (tid >> 7) + 2 u (coalesced along dense)
+  int gd[2], gk[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    if (MODE == 0) {
+      gd[u] = (tid + 256 * u) >> 2;
+      gk[u] = tid & 3;
+    } else {
+      gd[u] = tid & 127;
+      gk[u] = (tid >> 7) + 2 * u;
+    }
+  }
+  // strip staging: threads < 8 * SX_UNITS own (copy r, unit mc)
+  const bool sthr = tid < 8 * SX_UNITS;
+  const int sr = tid / SX_UNITS, smc = tid % SX_UNITS;
+
+  double tot[NQ][SX_NDB][4];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q)
+#pragma unroll
+    for (int db = 0; db < SX_NDB; ++db)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) tot[q][db][j] = 0.0;
+
+  float preA[2][8], preB[2][8];
+  u32x4 spA[3], spB[3];
+  auto gload = [&](float (&pre)[2][8], u32x4 (&sp)[3], int s0) __attribute__((always_inline)) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const float* p = (MODE == 0) ? src + (int64_t)(d0 + gd[u]) * K + s0 + 8 * gk[u]
+                                   : src + (int64_t)(s0 + 8 * gk[u]) * inner + d0 + gd[u];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) pre[u][j] = (MODE == 0) ? p[j] : p[(int64_t)j * inner];
+    }
+    if (sthr) {
+      // copy sr of the chunk window = table copy (vb + sr) & 7 from an aligned unit
+      const int vb = nmax - 1 - i0 - base + s0;
+      const int rr = (vb + sr) & 7;
+      const int64_t e8 = ((vb + sr) >> 3) + smc;
+#pragma unroll
+      for (int pl = 0; pl < 3; ++pl) sp[pl] = sg[((int64_t)(pl * 8 + rr) * V) / 8 + e8];
+    }
+  };
+  auto sstore = [&](const float (&pre)[2][8], const u32x4 (&sp)[3], int s0, int buf)
+      __attribute__((always_inline)) {
+    // only the last chunk reaches past the band end (inputs there may be
+    // another slab's values or padding: zeroed)
+    float x[2][8];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) x[u][j] = pre[u][j];
+    if (s0 + KC - 1 > s_end) {
+      __asm__ volatile("" ::: "memory");  // a branch, not eight selects per group
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int nv = s_end - (s0 + 8 * gk[u]);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x[u][j] = (j <= nv) ? x[u][j] : 0.0f;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      u32x4 hv, mv, lv;
+      split3x8(x[u], hv, mv, lv);
+      const int d = gd[u];
+      const int slot = gk[u] ^ ((d >> 1) & 3);
+      dense[buf][0][d][slot] = hv;
+      dense[buf][1][d][slot] = mv;
+      dense[buf][2][d][slot] = lv;
+    }
+    if (sthr) {
+#pragma unroll
+      for (int pl = 0; pl < 3; ++pl) strip[buf][pl][sr][smc] = sp[pl];
+    }
+  };
+  auto compute = [&](int ch, int buf) __attribute__((always_inline)) {
+    const int sc0 = s_begin + ch * KC;
+    int live = 0;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int ib = i0 + q * 16;
+      if (!(sc0 + KC - 1 < ib + base - t_hi || sc0 > ib + 15 + base - t_lo)) live |= 1 << q;
+    }
+    live = __builtin_amdgcn_readfirstlane(live);
+    if (live != 0) {
+      const int n = lane & 15, g = lane >> 4;
+      u32x4 dv[SX_NDB][3];
+#pragma unroll
+      for (int db = 0; db < SX_NDB; ++db) {
+        const int d = dw + db * 16 + n;
+        const int slot = g ^ ((d >> 1) & 3);
+#pragma unroll
+        for (int pl = 0; pl < 3; ++pl) dv[db][pl] = dense[buf][pl][d][slot];
+      }
+      // every Toeplitz fragment is read before the first MFMA (one LDS
+      // round trip per chunk instead of one per block)
+      const int cr = 7 - (n & 7);
+      u32x4 tv[NQ][3];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int pl = 0; pl < 3; ++pl) tv[q][pl] = strip[buf][pl][cr][7 - 2 * q - (n >> 3) + g];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        if (live & (1 << q)) {
+#pragma unroll
+          for (int db = 0; db < SX_NDB; ++db) {
+            // the chunk's float32 sum starts from zero and is folded at once
+            const v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
+            const v4f c = (MODE == 0) ? mfma_split6(dv[db], tv[q], zero)
+                                      : mfma_split6(tv[q], dv[db], zero);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) tot[q][db][j] += (double)c[j];
+          }
+        }
+      }
+    }
+  };
+
+  if (nchunks > 0) {
+    gload(preA, spA, s_begin);
+    sstore(preA, spA, s_begin, 0);
+    if (nchunks > 1) gload(preA, spA, s_begin + KC);
+    if (nchunks > 2) gload(preB, spB, s_begin + 2 * KC);
+  }
+  __syncthreads();
+  for (int ch = 0; ch < nchunks; ch += 2) {
+    compute(ch, 0);
+    if (ch + 1 < nchunks) {
+      sstore(preA, spA, s_begin + (ch + 1) * KC, 1);
+      if (ch + 3 < nchunks) gload(preA, spA, s_begin + (ch + 3) * KC);
+    }
+    __syncthreads();
+    if (ch + 1 < nchunks) {
+      compute(ch + 1, 1);
+      if (ch + 2 < nchunks) {
+        sstore(preB, spB, s_begin + (ch + 2) * KC, 0);
+        if (ch + 4 < nchunks) gload(preB, spB, s_begin + (ch + 4) * KC);
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q)
+#pragma unroll
+    for (int db = 0; db < SX_NDB; ++db)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int rr = 4 * (lane >> 4) + j, cc = lane & 15;
+        int dd, ii;
+        if (MODE == 0) {
+          dd = d0 + dw + db * 16 + rr;
+          ii = i0 + q * 16 + cc;
+        } else {
+          ii = i0 + q * 16 + rr;
+          dd = d0 + dw + db * 16 + cc;
+        }
+        if (dd < ndense && ii < nout) {
+          if (MODE == 0) dst[(int64_t)dd * nout + ii] = (float)tot[q][db][j];
+          else dst[(int64_t)ii * inner + dd] = (float)tot[q][db][j];
         }
       }
 }
@@ -717,32 +1013,48 @@ int flpr_taps(const FlprGeom* g, const double* h, int32_t dh, int32_t npow, int3
   return 0;
 }
 
-int flpr_conv_tiles(int32_t mode, int32_t rows_or_outer, int32_t max_rows_or_outer,
+int64_t flpr_conv_strip_len(int32_t nmax) { return nmax < 1 ? -1 : split_strip_len(nmax); }
+
+int flpr_conv_strips(const float* tap32, const int32_t* bands, int32_t ntap, int32_t npow,
+                     int32_t nmax, uint16_t* strips, void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!tap32 || !bands || !strips || ntap < 1 || npow < 1 || nmax < 1 || ntap % npow) return -1;
+  const int64_t V = split_strip_len(nmax);
+  const int64_t total = (int64_t)ntap * 8 * (V / 8);
+  hipLaunchKernelGGL(k_split_taps, dim3(flpr_grid_for(total, 256, 4096)), dim3(256), 0,
+                     as_stream(stream), tap32, bands, ntap, npow, nmax, V, strips);
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+int flpr_conv_tiles(int32_t mode, int32_t fp64, int32_t rows_or_outer, int32_t max_rows_or_outer,
                     int32_t nout, int32_t inner, int32_t* tiles, int32_t* tile_stride) {
   if (!tiles || !tile_stride || rows_or_outer < 1 || max_rows_or_outer < rows_or_outer ||
       nout < 1 || (mode != 0 && mode != 1) || (mode == 1 && inner < 1))
     return -1;
+  const int ld = fp64 ? LD : SX_LD;
   const int ntt = (nout + LT - 1) / LT;
   if (mode == 0) {
-    *tile_stride = (max_rows_or_outer + LD - 1) / LD;
-    *tiles = ((rows_or_outer + LD - 1) / LD) * ntt;
+    *tile_stride = (max_rows_or_outer + ld - 1) / ld;
+    *tiles = ((rows_or_outer + ld - 1) / ld) * ntt;
   } else {
-    *tile_stride = (inner + LD - 1) / LD;
+    *tile_stride = (inner + ld - 1) / ld;
     *tiles = *tile_stride * ntt * rows_or_outer;
   }
   return 0;
 }
 
 int flpr_conv(int32_t mode, int32_t fp64, const void* in, void* out, const void* tap,
-              const int64_t* jobs, const int32_t* bands, int32_t nj, int32_t max_rows_or_outer,
-              int32_t K, int32_t nout, int32_t inner, int32_t base, const int32_t* blocks,
-              int32_t nblocks, void* stream) {
+              int32_t nmax, const int64_t* jobs, const int32_t* bands, int32_t nj,
+              int32_t max_rows_or_outer, int32_t K, int32_t nout, int32_t inner, int32_t base,
+              const int32_t* blocks, int32_t nblocks, void* stream) {
   FLPR_CLEAR_ERROR();
   if (!in || !out || !tap || !jobs || !bands || nj < 1 || nj > 65535 || max_rows_or_outer < 1 || K < 1 ||
       nout < 1 || (mode != 0 && mode != 1) || (mode == 1 && inner < 1) ||
-      (blocks && nblocks < 1))
+      (blocks && nblocks < 1) || nmax < 1)
     return -1;
-  const int ntd = (mode == 0) ? (max_rows_or_outer + LD - 1) / LD : (inner + LD - 1) / LD;
+  const int ld = fp64 ? LD : SX_LD;
+  const int ntd = (mode == 0) ? (max_rows_or_outer + ld - 1) / ld : (inner + ld - 1) / ld;
   const int ntt = (nout + LT - 1) / LT;
   const int64_t gx = (mode == 0) ? (int64_t)ntd * ntt : (int64_t)ntd * ntt * max_rows_or_outer;
   if (gx > INT32_MAX) return -1;
@@ -750,21 +1062,21 @@ int flpr_conv(int32_t mode, int32_t fp64, const void* in, void* out, const void*
   hipStream_t st = as_stream(stream);
   if (fp64) {
     if (mode == 0)
-      hipLaunchKernelGGL((k_conv<double, 0>), grd, dim3(CT), 0, st, (const double*)in,
+      hipLaunchKernelGGL((k_conv<0>), grd, dim3(CT), 0, st, (const double*)in,
                          (double*)out, (const double*)tap, jobs, bands, K, nout, inner, base, ntd, ntt,
                          blocks);
     else
-      hipLaunchKernelGGL((k_conv<double, 1>), grd, dim3(CT), 0, st, (const double*)in,
+      hipLaunchKernelGGL((k_conv<1>), grd, dim3(CT), 0, st, (const double*)in,
                          (double*)out, (const double*)tap, jobs, bands, K, nout, inner, base, ntd, ntt,
                          blocks);
   } else {
     if (mode == 0)
-      hipLaunchKernelGGL((k_conv<float, 0>), grd, dim3(CT), 0, st, (const float*)in, (float*)out,
-                         (const float*)tap, jobs, bands, K, nout, inner, base, ntd, ntt,
+      hipLaunchKernelGGL((k_conv_split<0>), grd, dim3(CT), 0, st, (const float*)in, (float*)out,
+                         (const uint16_t*)tap, nmax, jobs, bands, K, nout, inner, base, ntd, ntt,
                          blocks);
     else
-      hipLaunchKernelGGL((k_conv<float, 1>), grd, dim3(CT), 0, st, (const float*)in, (float*)out,
-                         (const float*)tap, jobs, bands, K, nout, inner, base, ntd, ntt,
+      hipLaunchKernelGGL((k_conv_split<1>), grd, dim3(CT), 0, st, (const float*)in, (float*)out,
+                         (const uint16_t*)tap, nmax, jobs, bands, K, nout, inner, base, ntd, ntt,
                          blocks);
   }
   FLPR_CHECK_LAUNCH();

@@ -490,21 +490,30 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_cells(
     double acc[NACC];
 #pragma unroll
     for (int a = 0; a < NACC; ++a) acc[a] = 0.0;
-    // software pipeline: the next point's data is loaded before the current
-    // point's footprint is accumulated
-    double nxt[2 * D];
+    // software pipeline, two points deep: the data of points p + 1 and p + 2
+    // is in flight while point p's footprint is accumulated
+    double nxt[2][2 * D];
+    double nxv[2];
 #pragma unroll
-    for (int k = 0; k < 2 * D; ++k) nxt[k] = wab[(int64_t)q0 * (2 * D) + k];
-    double nxv = ones ? 1.0 : vp[(int64_t)q0 * nv];
+    for (int j = 0; j < 2; ++j) {
+      const int pj = min(q0 + j, q1 - 1);
+#pragma unroll
+      for (int k = 0; k < 2 * D; ++k) nxt[j][k] = wab[(int64_t)pj * (2 * D) + k];
+      nxv[j] = ones ? 1.0 : vp[(int64_t)pj * nv];
+    }
     for (int p = q0; p < q1; ++p) {
       double cur[2 * D];
 #pragma unroll
-      for (int k = 0; k < 2 * D; ++k) cur[k] = nxt[k];
-      double s = nxv;
-      if (p + 1 < q1) {
+      for (int k = 0; k < 2 * D; ++k) {
+        cur[k] = nxt[0][k];
+        nxt[0][k] = nxt[1][k];
+      }
+      double s = nxv[0];
+      nxv[0] = nxv[1];
+      if (p + 2 < q1) {
 #pragma unroll
-        for (int k = 0; k < 2 * D; ++k) nxt[k] = wab[(int64_t)(p + 1) * (2 * D) + k];
-        nxv = ones ? 1.0 : vp[(int64_t)(p + 1) * nv];
+        for (int k = 0; k < 2 * D; ++k) nxt[1][k] = wab[(int64_t)(p + 2) * (2 * D) + k];
+        nxv[1] = ones ? 1.0 : vp[(int64_t)(p + 2) * nv];
       }
       double w[D][ntap];
 #pragma unroll
@@ -777,19 +786,41 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(
     double acc[NA];
 #pragma unroll
     for (int a = 0; a < NA; ++a) acc[a] = 0.0;
+    // software pipeline: the next point's data is loaded before the current
+    // point is accumulated
+    int nri[3] = {0, 0, 0};
+    double nf[3] = {0, 0, 0};
+    double nv = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      nri[k] = rawi[(int64_t)q0 * D + k];
+      nf[k] = rawf[(int64_t)q0 * D + k];
+    }
+    if (mode == 0) nv = vals[(int64_t)q0 * nch + ch];
     for (int p = q0; p < q1; ++p) {
       int ri[3] = {0, 0, 0}, rel[3] = {0, 0, 0};
       double f[3] = {0, 0, 0};
+      const double v = nv;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        ri[k] = nri[k];
+        f[k] = nf[k];
+      }
+      if (p + 1 < q1) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          nri[k] = rawi[(int64_t)(p + 1) * D + k];
+          nf[k] = rawf[(int64_t)(p + 1) * D + k];
+        }
+        if (mode == 0) nv = vals[(int64_t)(p + 1) * nch + ch];
+      }
       bool inside = true;
 #pragma unroll
       for (int k = 0; k < D; ++k) {
-        ri[k] = rawi[(int64_t)p * D + k];
-        f[k] = rawf[(int64_t)p * D + k];
         rel[k] = ri[k] - ib[k];
         if (rel[k] < 0 || rel[k] > 2) inside = false;
       }
       if (mode == 0) {
-        const double v = vals[(int64_t)p * nch + ch];
         if (!inside) {
 #pragma unroll
           for (int q = 0; q < (1 << D); ++q) {

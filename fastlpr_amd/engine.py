@@ -386,6 +386,7 @@ _PLANS: Dict[tuple, Plan] = {}
 XCDS = 8
 CONV_LT = 64                       # flpr_conv's Toeplitz tile extent
 CONV_KC = {False: 32, True: 16}    # its K chunk, float32 / float64
+CONV_LD = {False: 128, True: 64}   # its dense tile extent, float32 / float64
 SCHED_MIN_TILES = 4096             # smaller launches keep the plain (tiles, jobs) grid
 _SCHED: Dict[tuple, tuple] = {}
 
@@ -417,7 +418,7 @@ def conv_schedule(ps: Pass, fp64: bool, bands: np.ndarray):
     stride = _lib.C.c_int32(0)
     cnt = _lib.C.c_int32(0)
     for r in np.unique(ro):
-        call("flpr_conv_tiles", ps.mode, int(r), ps.max_ro, ps.nout, ps.inner,
+        call("flpr_conv_tiles", ps.mode, int(bool(fp64)), int(r), ps.max_ro, ps.nout, ps.inner,
              _lib.C.addressof(cnt), _lib.C.addressof(stride))
         tiles[ro == r] = int(cnt.value)
     ntd = int(stride.value)
@@ -435,7 +436,7 @@ def conv_schedule(ps: Pass, fp64: bool, bands: np.ndarray):
     ids, works, jobs_of = [], [], []
     for j in range(nj):
         if ps.mode == 0:
-            nrow = -(-int(ro[j]) // 64)
+            nrow = -(-int(ro[j]) // CONV_LD[bool(fp64)])
             tt, td = np.divmod(np.arange(nrow * ntt), nrow)
             loc = td + ntd * tt
         else:
@@ -496,12 +497,28 @@ def conv_schedule(ps: Pass, fp64: bool, bands: np.ndarray):
     return dev_t, nmax * XCDS
 
 
-def launch_conv(ps: Pass, gf: bool, src: int, dst: int, tap_ptr: int, band_ptr: int,
+def split_strips(tap32: torch.Tensor, bands: torch.Tensor, dh: int, d: int, npow: int,
+                 nmax: int, s) -> Tuple[torch.Tensor, int]:
+    """The float32 form's tap operand (flpr_conv_strips) for all dh bandwidth
+    rows; returns (table, int16 elements per bandwidth row)."""
+    V = int(_lib.load().flpr_conv_strip_len(nmax))
+    if V < 1:
+        raise _lib.FlprError("flpr_conv_strip_len failed")
+    ntap = dh * d * npow
+    strips = torch.empty((ntap * 24 * V,), dtype=torch.int16, device=tap32.device)
+    call("flpr_conv_strips", ptr(tap32), ptr(bands), ntap, npow, nmax, ptr(strips), s)
+    return strips, d * npow * 24 * V
+
+
+def launch_conv(ps: Pass, gf: bool, src: int, dst: int, tap_ptr: int, nmax: int, band_ptr: int,
                 bands_g: np.ndarray, s) -> None:
-    """One flpr_conv pass with its block schedule (conv_schedule)."""
+    """One flpr_conv pass with its block schedule (conv_schedule).  tap_ptr:
+    the float64 taps (gf) or the split strip table (float32 form)."""
     table, nblocks = conv_schedule(ps, gf, bands_g)   # (None, 0): plain grid
-    TIMER.timed("flpr_conv", (lambda ps=ps, bg=bands_g: conv_flops(ps, bg)),
-                call, "flpr_conv", ps.mode, int(gf), src, dst, tap_ptr, ptr(ps.jobs), band_ptr,
+    TIMER.timed("flpr_conv64" if gf else "flpr_conv",
+                (lambda ps=ps, bg=bands_g: conv_flops(ps, bg)),
+                call, "flpr_conv", ps.mode, int(gf), src, dst, tap_ptr, nmax, ptr(ps.jobs),
+                band_ptr,
                 int(ps.jobs.shape[0]), ps.max_ro, ps.K, ps.nout, ps.inner, ps.base, ptr(table),
                 nblocks, s)
 
@@ -1058,6 +1075,8 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     tap32 = torch.empty((dh * per_tap,), dtype=torch.float32, device=dev) if any32 else None
     call("flpr_taps", gp, ptr(h_dev), dh, npow, nmax, float(n), ptr(tap64), ptr(tap32), s)
     bands = device_table(band_np, i32, dev)
+    strips, strip_row = (split_strips(tap32, bands, dh, d, npow, nmax, s) if any32
+                         else (None, 0))
 
     # --- global regularisation key (regression.py:217, 243-267)
     if order == 0:
@@ -1073,8 +1092,8 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
         plan = get_plan(d, order, geo.N, geo.L, geo.margin, C, b - a, gf, dev)
         fdt = f64 if gf else torch.float32
         src_grid = grid64 if gf else grid32
-        tap = tap64 if gf else tap32
-        tap_ptr = tap.data_ptr() + a * per_tap * tap.element_size()
+        tap_ptr = (tap64.data_ptr() + a * per_tap * 8) if gf else \
+            (strips.data_ptr() + a * strip_row * 2)
         band_ptr = bands.data_ptr() + a * d * 2 * 4
         bands_g = band_np[a * d * 2:b * d * 2]
         stage0 = torch.empty(((b - a) * plan.stage_per_h[0],), dtype=fdt, device=dev)
@@ -1084,7 +1103,7 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
             for ps in plan.pass_tables[ci]:
                 src = src_grid if ps.in_stage == d else inter[ps.in_stage]
                 dst = stage0 if ps.out_stage == 0 else inter[ps.out_stage]
-                launch_conv(ps, gf, ptr(src), ptr(dst), tap_ptr, band_ptr, bands_g, s)
+                launch_conv(ps, gf, ptr(src), ptr(dst), tap_ptr, nmax, band_ptr, bands_g, s)
         del inter
         if not moments_only:
             if order == 0:
@@ -1144,7 +1163,8 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
             geo=geo, x=x, n=n, order=order, order_idx=order_idx, key_start=key_start, wab=wab,
             rawi=rawi, rawf=rawf, grid64=grid64, sums=sums, cells=cells, pad=pad,
             stage0s=stage0s, key_max=key_max, gram=gram, s0_all=s0_all, h=h, warn=warn,
-            tap64=tap64, tap32=tap32, per_tap=per_tap, bands=bands, band_np=band_np,
+            tap64=tap64, strips=strips, strip_row=strip_row, per_tap=per_tap, nmax=nmax,
+            bands=bands, band_np=band_np,
             lam_scale=lam_scale, eps_add=eps_add, fine_s=fine_s, acc_s=acc_s, acc_y=acc_y,
             split_acc=split_acc, margin=margin, st=st, xrange=xrange, flag_power2=flag_power2,
             kde=kde, N=tuple(N), plan0=plan0, any32=any32)
@@ -1230,8 +1250,8 @@ def run_grid_y(sh: dict, x: torch.Tensor, vals_y: torch.Tensor, ny: int,
         fdt = f64 if gf else torch.float32
         src_grid = grid64 if gf else grid32
         esz = 8 if gf else 4
-        tap = sh["tap64"] if gf else sh["tap32"]
-        tap_ptr = tap.data_ptr() + a * per_tap * tap.element_size()
+        tap_ptr = (sh["tap64"].data_ptr() + a * per_tap * 8) if gf else \
+            (sh["strips"].data_ptr() + a * sh["strip_row"] * 2)
         band_ptr = bands.data_ptr() + a * d * 2 * 4
         bands_g = sh["band_np"][a * d * 2:b * d * 2]
         yper, tables = plan.y_pass_tables(ny)
@@ -1242,7 +1262,7 @@ def run_grid_y(sh: dict, x: torch.Tensor, vals_y: torch.Tensor, ny: int,
                 src = (src_grid.data_ptr() + cells * esz) if ps.in_stage == d \
                     else ptr(inter[ps.in_stage])
                 dst = stage0 if ps.out_stage == 0 else inter[ps.out_stage]
-                launch_conv(ps, gf, src, ptr(dst), tap_ptr, band_ptr, bands_g, s)
+                launch_conv(ps, gf, src, ptr(dst), tap_ptr, sh["nmax"], band_ptr, bands_g, s)
         del inter
     # --- adjoint sums of the new columns, solve (+ dots), residual quadratic form
     plan0 = sh["plan0"]

@@ -160,6 +160,15 @@ int flpr_grid_center(double* grid, int32_t C, int64_t cells, const double* sums,
 int flpr_taps(const FlprGeom* g, const double* h, int32_t dh, int32_t npow, int32_t nmax,
               double norm, double* tap64, float* tap32, void* stream);
 
+/* The float32 form's tap operand for flpr_conv: every float32 tap row
+ * (ntap = dh * d * npow rows of nmax, the tap32 layout of flpr_taps; row A
+ * uses band A / npow) masked to its band, reversed and split exactly into
+ * three bfloat16 planes (hi + mid + lo), in eight copies shifted by one
+ * element each: ntap * 24 * flpr_conv_strip_len(nmax) uint16 values. */
+int64_t flpr_conv_strip_len(int32_t nmax);
+int flpr_conv_strips(const float* tap32, const int32_t* bands, int32_t ntap, int32_t npow,
+                     int32_t nmax, uint16_t* strips, void* stream);
+
 /* One separable convolution pass along one axis for a batch of jobs: the
  * reference's spectrum product + inverse FFT + qout extraction
  * (convolution.py:904-951) restated as a banded Toeplitz product on the
@@ -172,14 +181,17 @@ int flpr_taps(const FlprGeom* g, const double* h, int32_t dh, int32_t npow, int3
  * array) are treated as zero.  K = input axis length, nout = output axis
  * length, inner = contiguous extent for mode 1, rows_or_outer = rows (mode 0)
  * or outer batch count (mode 1) of the job, max_rows_or_outer its maximum.
- * fp64 != 0 selects the float64 MFMA path (in/out/tap are double*).
+ * fp64 != 0: float64 MFMA, in/out/tap are double* (tap in the flpr_taps
+ * layout, row length nmax).  fp64 == 0: float32 in/out, products of the
+ * three-term bfloat16 splits on bf16 MFMA (float32 accuracy), tap = the
+ * flpr_conv_strips table of the float32 taps (row tap_off / nmax).
  * Tiles are read whole: in must stay readable for 256*K + 64*inner + 512
  * elements past the end of every job's input (those values are discarded or
  * zeroed, never used). */
 int flpr_conv(int32_t mode, int32_t fp64, const void* in, void* out, const void* tap,
-              const int64_t* jobs, const int32_t* bands, int32_t nj, int32_t max_rows_or_outer,
-              int32_t K, int32_t nout, int32_t inner, int32_t base, const int32_t* blocks,
-              int32_t nblocks, void* stream);
+              int32_t nmax, const int64_t* jobs, const int32_t* bands, int32_t nj,
+              int32_t max_rows_or_outer, int32_t K, int32_t nout, int32_t inner, int32_t base,
+              const int32_t* blocks, int32_t nblocks, void* stream);
 
 /* Tiles of one flpr_conv job (rows_or_outer of the job, max_rows_or_outer of
  * the launch) and the stride of the tile numbering.  With a block table
@@ -188,7 +200,7 @@ int flpr_conv(int32_t mode, int32_t fp64, const void* in, void* out, const void*
  * caller chooses the order and the placement of the tiles (the device hands
  * consecutive blocks to its XCDs in turn).  Without one (blocks NULL) the
  * launch runs every tile of every job, largest job first in x. */
-int flpr_conv_tiles(int32_t mode, int32_t rows_or_outer, int32_t max_rows_or_outer,
+int flpr_conv_tiles(int32_t mode, int32_t fp64, int32_t rows_or_outer, int32_t max_rows_or_outer,
                     int32_t nout, int32_t inner, int32_t* tiles, int32_t* tile_stride);
 
 /* Largest |S_kk| over all nodes, bandwidths and diagonal moments

@@ -13,7 +13,7 @@ HEADER = os.path.join(ROOT, "include", "fastlpr_amd.h")
 def _declared():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|double)\s+(flpr_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|double)\s+(flpr_\w+)\s*\(", text, flags=re.M)))
 
 
 def test_header_declares_entry_points():

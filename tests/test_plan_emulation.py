@@ -284,11 +284,12 @@ def test_conv_block_schedule_covers_every_tile_once(fp64):
         seen = set()
         for j in range(ps.jobs_host.shape[0]):
             ro = int(ps.jobs_host[j, 4])
+            ld = E.CONV_LD[fp64]
             if ps.mode == 0:
-                ntd = -(-ps.max_ro // 64)
-                want = {td + ntd * tt for tt in range(ntt) for td in range(-(-ro // 64))}
+                ntd = -(-ps.max_ro // ld)
+                want = {td + ntd * tt for tt in range(ntt) for td in range(-(-ro // ld))}
             else:
-                ntd = -(-ps.inner // 64)
+                ntd = -(-ps.inner // ld)
                 want = set(range(ntd * ntt * ro))
             got = live[live[:, 0] == j][:, 1]
             assert len(got) == len(set(got.tolist())) == len(want)
