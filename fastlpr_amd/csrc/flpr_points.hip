@@ -764,6 +764,13 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(
   __syncthreads();
   const int kbase = blockIdx.x * tc;
   const int nitems = tc * nsplit * cn;
+  // With an even number of nodes on every axis the raw nodes sit half a cell
+  // off the spread cell centres, so a cell's points share one raw interval
+  // (rel = 1 on every axis; only rounding at the data's ends differs): those
+  // points take a short path that accumulates the 2^D corner products only.
+  bool aligned = true;
+#pragma unroll
+  for (int k = 0; k < D; ++k) aligned = aligned && !(g.n_grid[k] & 1);
   for (int item = threadIdx.x; item < nitems; item += blockDim.x) {
     const int c = item % cn;
     const int rest = item / cn;
@@ -807,18 +814,60 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(
         f[k] = nf[k];
       }
       if (p + 1 < q1) {
+        const int64_t pn = p + 1;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-          nri[k] = rawi[(int64_t)(p + 1) * D + k];
-          nf[k] = rawf[(int64_t)(p + 1) * D + k];
+          nri[k] = rawi[pn * D + k];
+          nf[k] = rawf[pn * D + k];
         }
-        if (mode == 0) nv = vals[(int64_t)(p + 1) * nch + ch];
+        if (mode == 0) nv = vals[pn * nch + ch];
       }
-      bool inside = true;
+      bool inside = true, main = aligned;
 #pragma unroll
       for (int k = 0; k < D; ++k) {
         rel[k] = ri[k] - ib[k];
         if (rel[k] < 0 || rel[k] > 2) inside = false;
+        main = main && rel[k] == 1;
+      }
+      if (main) {
+        // nodes ib + 1 + bit per axis: accumulator (1 + b0, 1 + b1, ..) of the
+        // 4^D node window, or the centre of the 3^D cell window
+        if (mode == 0) {
+          if constexpr (D == 1) {
+            acc[1] += v * (1.0 - f[0]);
+            acc[2] += v * f[0];
+          } else if constexpr (D == 2) {
+            const double va = v * (1.0 - f[0]), vb = v * f[0];
+            const double g1 = 1.0 - f[1];
+            acc[5] += va * g1;
+            acc[6] += va * f[1];
+            acc[9] += vb * g1;
+            acc[10] += vb * f[1];
+          } else {
+            const double va = v * (1.0 - f[0]), vb = v * f[0];
+            const double g1 = 1.0 - f[1], g2 = 1.0 - f[2];
+            const double w00 = va * g1, w01 = va * f[1], w10 = vb * g1, w11 = vb * f[1];
+            acc[21] += w00 * g2;
+            acc[22] += w00 * f[2];
+            acc[25] += w01 * g2;
+            acc[26] += w01 * f[2];
+            acc[37] += w10 * g2;
+            acc[38] += w10 * f[2];
+            acc[41] += w11 * g2;
+            acc[42] += w11 * f[2];
+          }
+        } else {
+          double wa = 1.0, wb = 1.0;
+#pragma unroll
+          for (int k = 0; k < D; ++k) {
+            const int ba = (qa >> (D - 1 - k)) & 1, bb = (qb >> (D - 1 - k)) & 1;
+            wa *= ba ? f[k] : 1.0 - f[k];
+            wb *= bb ? f[k] : 1.0 - f[k];
+          }
+          constexpr int centre = (D == 1) ? 1 : (D == 2 ? 4 : 13);
+          acc[centre] += wa * wb;
+        }
+        continue;
       }
       if (mode == 0) {
         if (!inside) {
@@ -1406,6 +1455,15 @@ int item_splits(const FlprGeom* g, int64_t n) {
   return (int)s;
 }
 
+// Splits of a cell's points so that a block's work items cover its threads
+// (few channels per block, e.g. a single new y column): items of one split
+// number tile cells x rows x channels.
+int fill_splits(int nsplit, int64_t items_per_split, int threads) {
+  if (items_per_split < 1) return nsplit;
+  const int64_t need = (threads + items_per_split - 1) / items_per_split;
+  return (int)std::max<int64_t>(nsplit, std::min<int64_t>(need, 64));
+}
+
 // Channels per block: all of them when the tiles alone fill the device, else
 // split so that tiles x groups reaches MIN_BLOCKS; capped by the LDS budget.
 int channel_group(int64_t ntiles, int64_t cells_per_channel, int channels) {
@@ -1533,7 +1591,13 @@ int flpr_spread(const FlprGeom* g, const int32_t* key_start, const double* wab,
       const double o = (double)(j - g->msp);
       tab.G[k][j] = exp(-c2 * o * o);
     }
-  const int nsplit = item_splits(g, n);
+  int64_t tc = 1;
+  for (int k = 0; k < g->d; ++k) tc *= g->tile[k];
+  // footprint rows per item (k_spread_cells' ROWS)
+  const int ntap = 2 * g->msp + 1;
+  const int rows = (g->d == 1) ? 1 : (g->d == 2 ? (g->msp > 4 ? ntap : 1)
+                                               : (g->msp > 4 ? ntap * ntap : ntap));
+  const int nsplit = fill_splits(item_splits(g, n), tc * rows * std::min(cg, C), SPREAD_THREADS);
   dim3 grd((unsigned)ntiles, (unsigned)((C + cg - 1) / cg));
   hipStream_t st = as_stream(stream);
   const int msp = g->msp;
@@ -1608,7 +1672,7 @@ int flpr_adjoint(const FlprGeom* g, const int32_t* key_start, const int32_t* raw
   }
   const int nq = 1 << g->d;
   const int nw = nq * (nq + 1) / 2;
-  const int nsplit = item_splits(g, n);
+  const int nsplit0 = item_splits(g, n);
   hipStream_t st = as_stream(stream);
   for (int mode = 0; mode < 2; ++mode) {
     const int nch = (mode == 0) ? nval : (gram ? nw : 0);
@@ -1616,6 +1680,9 @@ int flpr_adjoint(const FlprGeom* g, const int32_t* key_start, const int32_t* raw
     double* out = (mode == 0) ? q : gram;
     const int64_t ostride = (mode == 0) ? nodes : rcells;
     const int cg = channel_group(flpr_ntiles(g), wcells, nch);
+    int64_t tc = 1;
+    for (int k = 0; k < g->d; ++k) tc *= g->tile[k];
+    const int nsplit = fill_splits(nsplit0, tc * std::min(cg, nch), ADJ_THREADS);
     const size_t lds = (size_t)wcells * cg * sizeof(double);
     if (lds > 64 * 1024) return -1;
     dim3 grd((unsigned)flpr_ntiles(g), (unsigned)((nch + cg - 1) / cg));
