@@ -362,9 +362,10 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));   // one 16-byte uni
 constexpr int SX_LD = 128;          // dense extent of a block tile (2 x 16 per wave)
 constexpr int SX_KC = 32;           // K chunk
 constexpr int SX_NDB = 2;           // dense 16-blocks per wave
-constexpr int SX_COPY = 88;         // elements of one shifted window copy in LDS (11 units;
-                                    // 11 odd keeps the eight copies on distinct banks)
+constexpr int SX_COPY = 88;         // elements of one shifted window copy in LDS (11 units)
 constexpr int SX_UNITS = SX_COPY / 8;
+constexpr int SX_PITCH = 12;        // LDS units per copy: with 12 the fragment reads of the
+                                    // eight copies measure conflict-free on gfx950 (11: 1.25x)
 constexpr int SX_TOP_PAD = 62;      // reversed table index v = (nmax + 62) - t
 
 __host__ __device__ inline int64_t split_strip_len(int nmax) {
@@ -460,9 +461,11 @@ __device__ inline v4f mfma_split6(const u32x4 (&x)[3], const u32x4 (&y)[3], v4f 
 // Banded Toeplitz GEMM of the float32 form (jobs, bands, tiles and the block
 // schedule as in k_conv).  A block owns 64 Toeplitz outputs x 128 dense; wave
 // w owns dense [32 w, 32 w + 32) and all 64 outputs (4 x 2 MFMA blocks).  The
-// dense chunk (128 x 32) is split into three bf16 planes in LDS, 16-byte units
-// of 8 K per dense index with the unit order swizzled by (index >> 1) & 3 so
-// that fragment reads and stores hit distinct banks.
+// dense operand is not shared between waves, so each wave loads its own
+// fragments straight into registers (lane (n, g) of dense block db holds the
+// 8 K values 8 g .. 8 g + 7 of dense index 16 db + n, two chunks ahead) and
+// splits them there; only the Toeplitz windows, which all four waves read,
+// go through LDS (double-buffered, one barrier per chunk).
 template <int MODE>
 __global__ __launch_bounds__(CT) void k_conv_split(const float* __restrict__ in,
                                                    float* __restrict__ out,
@@ -472,8 +475,7 @@ __global__ __launch_bounds__(CT) void k_conv_split(const float* __restrict__ in,
                                                    int nout, int inner, int base, int ntd, int ntt,
                                                    const int32_t* __restrict__ blocks) {
   constexpr int KC = SX_KC;
-  __shared__ u32x4 dense[2][3][SX_LD][4];
-  __shared__ u32x4 strip[2][3][8][SX_UNITS];
+  __shared__ u32x4 strip[2][3][8][SX_PITCH];
   int job = blockIdx.y, bx = blockIdx.x;
   if (blocks) {
     job = blocks[2 * blockIdx.x];
@@ -502,20 +504,17 @@ __global__ __launch_bounds__(CT) void k_conv_split(const float* __restrict__ in,
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int dw = wave * (SX_NDB * 16);
+  const int n = lane & 15, g = lane >> 4;
+  const int cr = 7 - (n & 7);
 
-  // staging: thread owns two (dense index, 8-K unit) groups
-  //   ROW: group g = tid + 256 u -> dense row g >> 2, unit g & 3 (loads along K)
-  //   COL: dense column tid & 127, unit (tid >> 7) + 2 u (coalesced along dense)
-  int gd[2], gk[2];
+  // this lane's dense data: index d0 + dw + 16 db + n, K = s0 + 8 g + j.  Rows
+  // (ROW pass) or columns (COL pass) past the job feed outputs that are never
+  // stored; reads stay inside the caller's padding.
+  const float* dsrc[SX_NDB];
 #pragma unroll
-  for (int u = 0; u < 2; ++u) {
-    if (MODE == 0) {
-      gd[u] = (tid + 256 * u) >> 2;
-      gk[u] = tid & 3;
-    } else {
-      gd[u] = tid & 127;
-      gk[u] = (tid >> 7) + 2 * u;
-    }
+  for (int db = 0; db < SX_NDB; ++db) {
+    const int dd = d0 + dw + 16 * db + n;
+    dsrc[db] = (MODE == 0) ? src + (int64_t)dd * K + 8 * g : src + (int64_t)(8 * g) * inner + dd;
   }
   // strip staging: threads < 8 * SX_UNITS own (copy r, unit mc)
   const bool sthr = tid < 8 * SX_UNITS;
@@ -529,15 +528,14 @@ __global__ __launch_bounds__(CT) void k_conv_split(const float* __restrict__ in,
 #pragma unroll
       for (int j = 0; j < 4; ++j) tot[q][db][j] = 0.0;
 
-  float preA[2][8], preB[2][8];
+  float preA[SX_NDB][8], preB[SX_NDB][8];
   u32x4 spA[3], spB[3];
-  auto gload = [&](float (&pre)[2][8], u32x4 (&sp)[3], int s0) __attribute__((always_inline)) {
+  auto gload = [&](float (&pre)[SX_NDB][8], u32x4 (&sp)[3], int s0) __attribute__((always_inline)) {
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const float* p = (MODE == 0) ? src + (int64_t)(d0 + gd[u]) * K + s0 + 8 * gk[u]
-                                   : src + (int64_t)(s0 + 8 * gk[u]) * inner + d0 + gd[u];
+    for (int db = 0; db < SX_NDB; ++db) {
+      const float* p = (MODE == 0) ? dsrc[db] + s0 : dsrc[db] + (int64_t)s0 * inner;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) pre[u][j] = (MODE == 0) ? p[j] : p[(int64_t)j * inner];
+      for (int j = 0; j < 8; ++j) pre[db][j] = (MODE == 0) ? p[j] : p[(int64_t)j * inner];
     }
     if (sthr) {
       // copy sr of the chunk window = table copy (vb + sr) & 7 from an aligned unit
@@ -548,41 +546,48 @@ __global__ __launch_bounds__(CT) void k_conv_split(const float* __restrict__ in,
       for (int pl = 0; pl < 3; ++pl) sp[pl] = sg[((int64_t)(pl * 8 + rr) * V) / 8 + e8];
     }
   };
-  auto sstore = [&](const float (&pre)[2][8], const u32x4 (&sp)[3], int s0, int buf)
-      __attribute__((always_inline)) {
-    // only the last chunk reaches past the band end (inputs there may be
-    // another slab's values or padding: zeroed)
-    float x[2][8];
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-#pragma unroll
-      for (int j = 0; j < 8; ++j) x[u][j] = pre[u][j];
-    if (s0 + KC - 1 > s_end) {
-      __asm__ volatile("" ::: "memory");  // a branch, not eight selects per group
-#pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const int nv = s_end - (s0 + 8 * gk[u]);
-#pragma unroll
-        for (int j = 0; j < 8; ++j) x[u][j] = (j <= nv) ? x[u][j] : 0.0f;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      u32x4 hv, mv, lv;
-      split3x8(x[u], hv, mv, lv);
-      const int d = gd[u];
-      const int slot = gk[u] ^ ((d >> 1) & 3);
-      dense[buf][0][d][slot] = hv;
-      dense[buf][1][d][slot] = mv;
-      dense[buf][2][d][slot] = lv;
-    }
+  auto sstore = [&](const u32x4 (&sp)[3], int buf) __attribute__((always_inline)) {
     if (sthr) {
 #pragma unroll
       for (int pl = 0; pl < 3; ++pl) strip[buf][pl][sr][smc] = sp[pl];
     }
   };
-  auto compute = [&](int ch, int buf) __attribute__((always_inline)) {
+  // Toeplitz fragments: block q of chunk ch + 1 equals block q - 2 of chunk ch
+  // (i - s is what counts and the chunk advances by 32 = two blocks), so a
+  // chunk reads blocks 0 and 1 into tvn and takes blocks 2 and 3 from the
+  // previous chunk's blocks 0 and 1 (tvo); the first chunk reads all four.
+  auto compute = [&](int ch, int buf, const float (&pre)[SX_NDB][8], u32x4 (&tvn)[2][3],
+                     u32x4 (&tvo)[2][3]) __attribute__((always_inline)) {
     const int sc0 = s_begin + ch * KC;
+    if (ch == 0) {
+#pragma unroll
+      for (int q = 2; q < 4; ++q)
+#pragma unroll
+        for (int pl = 0; pl < 3; ++pl)
+          tvo[q - 2][pl] = strip[buf][pl][cr][7 - 2 * q - (n >> 3) + g];
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int pl = 0; pl < 3; ++pl) tvn[q][pl] = strip[buf][pl][cr][7 - 2 * q - (n >> 3) + g];
+    // dense fragments: only the last chunk reaches past the band end (inputs
+    // there may be another slab's values or padding: zeroed)
+    float x[SX_NDB][8];
+#pragma unroll
+    for (int db = 0; db < SX_NDB; ++db)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) x[db][j] = pre[db][j];
+    if (sc0 + KC - 1 > s_end) {
+      __asm__ volatile("" ::: "memory");  // a branch, not eight selects per fragment
+      const int nv = s_end - (sc0 + 8 * g);
+#pragma unroll
+      for (int db = 0; db < SX_NDB; ++db)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x[db][j] = (j <= nv) ? x[db][j] : 0.0f;
+    }
+    u32x4 dv[SX_NDB][3];
+#pragma unroll
+    for (int db = 0; db < SX_NDB; ++db) split3x8(x[db], dv[db][0], dv[db][1], dv[db][2]);
     int live = 0;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
@@ -590,61 +595,41 @@ __global__ __launch_bounds__(CT) void k_conv_split(const float* __restrict__ in,
       if (!(sc0 + KC - 1 < ib + base - t_hi || sc0 > ib + 15 + base - t_lo)) live |= 1 << q;
     }
     live = __builtin_amdgcn_readfirstlane(live);
-    if (live != 0) {
-      const int n = lane & 15, g = lane >> 4;
-      u32x4 dv[SX_NDB][3];
 #pragma unroll
-      for (int db = 0; db < SX_NDB; ++db) {
-        const int d = dw + db * 16 + n;
-        const int slot = g ^ ((d >> 1) & 3);
+    for (int q = 0; q < NQ; ++q) {
+      if (live & (1 << q)) {
+        const u32x4 (&tq)[3] = (q < 2) ? tvn[q & 1] : tvo[q & 1];
 #pragma unroll
-        for (int pl = 0; pl < 3; ++pl) dv[db][pl] = dense[buf][pl][d][slot];
-      }
-      // every Toeplitz fragment is read before the first MFMA (one LDS
-      // round trip per chunk instead of one per block)
-      const int cr = 7 - (n & 7);
-      u32x4 tv[NQ][3];
+        for (int db = 0; db < SX_NDB; ++db) {
+          // the chunk's float32 sum starts from zero and is folded at once
+          const v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
+          const v4f c = (MODE == 0) ? mfma_split6(dv[db], tq, zero)
+                                    : mfma_split6(tq, dv[db], zero);
 #pragma unroll
-      for (int q = 0; q < NQ; ++q)
-#pragma unroll
-        for (int pl = 0; pl < 3; ++pl) tv[q][pl] = strip[buf][pl][cr][7 - 2 * q - (n >> 3) + g];
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        if (live & (1 << q)) {
-#pragma unroll
-          for (int db = 0; db < SX_NDB; ++db) {
-            // the chunk's float32 sum starts from zero and is folded at once
-            const v4f zero = {0.0f, 0.0f, 0.0f, 0.0f};
-            const v4f c = (MODE == 0) ? mfma_split6(dv[db], tv[q], zero)
-                                      : mfma_split6(tv[q], dv[db], zero);
-#pragma unroll
-            for (int j = 0; j < 4; ++j) tot[q][db][j] += (double)c[j];
-          }
+          for (int j = 0; j < 4; ++j) tot[q][db][j] += (double)c[j];
         }
       }
     }
   };
 
+  // chunk c's dense values live in register set A (even c) or B (odd c) from
+  // two chunks before; its Toeplitz window in LDS buffer c & 1
   if (nchunks > 0) {
     gload(preA, spA, s_begin);
-    sstore(preA, spA, s_begin, 0);
-    if (nchunks > 1) gload(preA, spA, s_begin + KC);
-    if (nchunks > 2) gload(preB, spB, s_begin + 2 * KC);
+    sstore(spA, 0);
+    if (nchunks > 1) gload(preB, spB, s_begin + KC);
   }
   __syncthreads();
+  u32x4 tvA[2][3], tvB[2][3];
   for (int ch = 0; ch < nchunks; ch += 2) {
-    compute(ch, 0);
-    if (ch + 1 < nchunks) {
-      sstore(preA, spA, s_begin + (ch + 1) * KC, 1);
-      if (ch + 3 < nchunks) gload(preA, spA, s_begin + (ch + 3) * KC);
-    }
+    if (ch + 1 < nchunks) sstore(spB, 1);
+    compute(ch, 0, preA, tvA, tvB);
+    if (ch + 2 < nchunks) gload(preA, spA, s_begin + (ch + 2) * KC);
     __syncthreads();
     if (ch + 1 < nchunks) {
-      compute(ch + 1, 1);
-      if (ch + 2 < nchunks) {
-        sstore(preB, spB, s_begin + (ch + 2) * KC, 0);
-        if (ch + 4 < nchunks) gload(preB, spB, s_begin + (ch + 4) * KC);
-      }
+      if (ch + 2 < nchunks) sstore(spA, 0);
+      compute(ch + 1, 1, preB, tvB, tvA);
+      if (ch + 3 < nchunks) gload(preB, spB, s_begin + (ch + 3) * KC);
       __syncthreads();
     }
   }

@@ -426,7 +426,7 @@ struct SpreadTab {
   double G[FLPR_MAX_DIM][SPREAD_MAX_TAPS];
 };
 
-// Register-accumulated spread.  Work item = (cell, split, row, channel), the
+// Register-accumulated spread.  Work item = (split, cell, row, channel), the
 // channel fastest so that the lanes sharing a point read the same point data.
 // An item accumulates its share (split s of nsplit) of the cell's points into
 // a footprint block held in float64 registers: the last F = d - FIX axes of
@@ -469,8 +469,10 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_cells(
     int rest = item / cn;
     const int row = rest % ROWS;
     rest /= ROWS;
-    const int sp = rest % nsplit;
-    const int lc = rest / nsplit;
+    // the split is the slowest index: neighbouring lanes flush neighbouring
+    // cells' footprints (the splits of one cell would add to the same addresses)
+    const int lc = rest % tc;
+    const int sp = rest / tc;
     const int p0 = key_start[kbase + lc], p1 = key_start[kbase + lc + 1];
     const int64_t len = p1 - p0;
     const int q0 = p0 + (int)((len * sp) / nsplit);
@@ -733,7 +735,7 @@ constexpr int ADJ_THREADS = 256;
 // Adjoint interpolation sums on the raw grid, per spread tile:
 //   mode 0: q[c][node]   += v_ic w_i(node)          (value channels)
 //   mode 1: W[e][rcell]  += w_i(qa) w_i(qb)          (Gram entries)
-// Work item = (spread cell, split, channel) with the channel fastest; the
+// Work item = (split, spread cell, channel) with the channel fastest; the
 // points of a spread cell touch at most three raw intervals per dimension, so
 // an item accumulates 4^d node (mode 0) or 3^d cell (mode 1) sums in
 // registers, then adds them into the tile's LDS window, which is written to
@@ -774,8 +776,8 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(
   for (int item = threadIdx.x; item < nitems; item += blockDim.x) {
     const int c = item % cn;
     const int rest = item / cn;
-    const int sp = rest % nsplit;
-    const int lc = rest / nsplit;
+    const int lc = rest % tc;   // the split slowest, as in k_spread_cells
+    const int sp = rest / tc;
     const int p0 = key_start[kbase + lc], p1 = key_start[kbase + lc + 1];
     const int64_t len = p1 - p0;
     const int q0 = p0 + (int)((len * sp) / nsplit);
