@@ -530,13 +530,9 @@ __global__ __launch_bounds__(CT) void k_conv_split(const float* __restrict__ in,
 
   float preA[SX_NDB][8], preB[SX_NDB][8];
   u32x4 spA[3], spB[3];
+  // the window units first: a wait for them then leaves the dense loads
+  // issued after them in flight
   auto gload = [&](float (&pre)[SX_NDB][8], u32x4 (&sp)[3], int s0) __attribute__((always_inline)) {
-#pragma unroll
-    for (int db = 0; db < SX_NDB; ++db) {
-      const float* p = (MODE == 0) ? dsrc[db] + s0 : dsrc[db] + (int64_t)s0 * inner;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) pre[db][j] = (MODE == 0) ? p[j] : p[(int64_t)j * inner];
-    }
     if (sthr) {
       // copy sr of the chunk window = table copy (vb + sr) & 7 from an aligned unit
       const int vb = nmax - 1 - i0 - base + s0;
@@ -544,6 +540,12 @@ __global__ __launch_bounds__(CT) void k_conv_split(const float* __restrict__ in,
       const int64_t e8 = ((vb + sr) >> 3) + smc;
 #pragma unroll
       for (int pl = 0; pl < 3; ++pl) sp[pl] = sg[((int64_t)(pl * 8 + rr) * V) / 8 + e8];
+    }
+#pragma unroll
+    for (int db = 0; db < SX_NDB; ++db) {
+      const float* p = (MODE == 0) ? dsrc[db] + s0 : dsrc[db] + (int64_t)s0 * inner;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) pre[db][j] = (MODE == 0) ? p[j] : p[(int64_t)j * inner];
     }
   };
   auto sstore = [&](const u32x4 (&sp)[3], int buf) __attribute__((always_inline)) {
@@ -612,8 +614,10 @@ __global__ __launch_bounds__(CT) void k_conv_split(const float* __restrict__ in,
     }
   };
 
-  // chunk c's dense values live in register set A (even c) or B (odd c) from
-  // two chunks before; its Toeplitz window in LDS buffer c & 1
+  // Chunk c: dense values in register set A (even c) or B (odd c), loaded
+  // after chunk c - 2; Toeplitz window in LDS buffer c & 1, its units loaded
+  // with the same set and stored one chunk later (after chunk c - 1), so that
+  // every load has a chunk of work to complete behind.
   if (nchunks > 0) {
     gload(preA, spA, s_begin);
     sstore(spA, 0);
@@ -622,14 +626,14 @@ __global__ __launch_bounds__(CT) void k_conv_split(const float* __restrict__ in,
   __syncthreads();
   u32x4 tvA[2][3], tvB[2][3];
   for (int ch = 0; ch < nchunks; ch += 2) {
-    if (ch + 1 < nchunks) sstore(spB, 1);
     compute(ch, 0, preA, tvA, tvB);
     if (ch + 2 < nchunks) gload(preA, spA, s_begin + (ch + 2) * KC);
+    if (ch + 1 < nchunks) sstore(spB, 1);
     __syncthreads();
     if (ch + 1 < nchunks) {
-      if (ch + 2 < nchunks) sstore(spA, 0);
       compute(ch + 1, 1, preB, tvB, tvA);
       if (ch + 3 < nchunks) gload(preB, spB, s_begin + (ch + 3) * KC);
+      if (ch + 2 < nchunks) sstore(spA, 0);
       __syncthreads();
     }
   }
