@@ -83,8 +83,8 @@ _GP = C.POINTER(FlprGeom)
 _SIGS = {
     "flpr_last_hip_error": [C.c_char_p, _I32],
     "flpr_xstats": [_P, _I64, _I32, _P, _I32, _P, _P, _P],
-    "flpr_points_prepare": [_P, _I64, _GP, _P, _P, _P],
-    "flpr_points_bin": [_P, _I64, _I32, _P, _P, _P, _P, _P],
+    "flpr_points_prepare": [_P, _I64, _GP, _P, _P, _P, _P],
+    "flpr_points_bin": [_P, _I64, _I32, _P, _P, _P, _P, _P, _P],
     "flpr_points_pack": [_GP, _P, _P, _I64, _P, _I32, _P, _I32, _P, _P, _P, _P, _P],
     "flpr_spread": [_GP, _P, _P, _P, _I64, _I32, _I32, _P, _P, _P, _P],
     "flpr_resample_work": [_GP, _GP, _I32, _I32, _P],

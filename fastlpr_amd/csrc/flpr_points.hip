@@ -211,7 +211,8 @@ constexpr int PREP_THREADS = 256;
 // knot mapping of knot_of.
 __global__ __launch_bounds__(PREP_THREADS) void k_prepare(const double* __restrict__ x, int64_t n,
                                                           FlprGeom g, int32_t* __restrict__ key,
-                                                          int32_t* __restrict__ key_count) {
+                                                          int32_t* __restrict__ key_count,
+                                                          int32_t* __restrict__ rank) {
   int tc = 1;
   for (int k = 0; k < g.d; ++k) tc *= g.tile[k];
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -229,7 +230,8 @@ __global__ __launch_bounds__(PREP_THREADS) void k_prepare(const double* __restri
     }
     const int kk = tid * tc + lid;
     key[i] = kk;
-    atomicAdd(&key_count[kk], 1);
+    if (rank) rank[i] = atomicAdd(&key_count[kk], 1);
+    else atomicAdd(&key_count[kk], 1);
   }
 }
 
@@ -300,16 +302,19 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const int32_t* __re
   if (threadIdx.x == 0) bsum[blockIdx.x] = 0;
 }
 
-// Scatter of point indices into key order (one atomic per point on its cell's
-// cursor; about n / cells points share a cursor).
+// Scatter of point indices into key order: at the rank the counting pass
+// returned, or with one atomic per point on its cell's cursor (about n / cells
+// points share a cursor).
 __global__ __launch_bounds__(256) void k_bin(const int32_t* __restrict__ key, int64_t n,
                                              const int32_t* __restrict__ start,
                                              int32_t* __restrict__ cursor,
+                                             const int32_t* __restrict__ rank,
                                              int32_t* __restrict__ order) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int k = key[i];
-    order[start[k] + atomicAdd(&cursor[k], 1)] = (int32_t)i;
+    const int r = rank ? rank[i] : atomicAdd(&cursor[k], 1);
+    order[start[k] + r] = (int32_t)i;
   }
 }
 
@@ -1510,18 +1515,19 @@ int flpr_xstats(const double* x, int64_t n, int32_t d, const double* y, int32_t 
 }
 
 int flpr_points_prepare(const double* x, int64_t n, const FlprGeom* g, int32_t* key,
-                        int32_t* key_count, void* stream) {
+                        int32_t* key_count, int32_t* rank, void* stream) {
   FLPR_CLEAR_ERROR();
   if (!x || !key || !key_count || n < 1 || !flpr_geom_ok(g)) return -1;
   if (flpr_nkeys(g) > ((int64_t)1 << 30)) return -1;
   hipLaunchKernelGGL(k_prepare, dim3(flpr_grid_for(n, PREP_THREADS, 4096)), dim3(PREP_THREADS), 0,
-                     as_stream(stream), x, n, *g, key, key_count);
+                     as_stream(stream), x, n, *g, key, key_count, rank);
   FLPR_CHECK_LAUNCH();
   return 0;
 }
 
 int flpr_points_bin(const int32_t* key, int64_t n, int32_t nkeys, const int32_t* key_count,
-                    int32_t* key_start, int32_t* key_cursor, int32_t* order, void* stream) {
+                    int32_t* key_start, int32_t* key_cursor, const int32_t* rank,
+                    int32_t* order, void* stream) {
   FLPR_CLEAR_ERROR();
   if (!key || !key_count || !key_start || !key_cursor || !order || n < 1 || nkeys < 1 ||
       n > INT32_MAX)
@@ -1540,7 +1546,7 @@ int flpr_points_bin(const int32_t* key, int64_t n, int32_t nkeys, const int32_t*
                      key_cursor, key_start);
   FLPR_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_bin, dim3(flpr_grid_for(n, 256, 4096)), dim3(256), 0, as_stream(stream),
-                     key, n, key_start, key_cursor, order);
+                     key, n, key_start, key_cursor, rank, order);
   FLPR_CHECK_LAUNCH();
   return 0;
 }

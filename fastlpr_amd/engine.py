@@ -749,9 +749,9 @@ def spread_fine(geo: Geometry, accuracy: int, x: torch.Tensor, n: int,
     cnt = torch.zeros((2 * nkeys,), dtype=i32, device=dev)
     key_start = torch.empty((nkeys + 1,), dtype=i32, device=dev)
     order_idx = torch.empty((n,), dtype=i32, device=dev)
-    call("flpr_points_prepare", ptr(x), n, gfp, ptr(key), ptr(cnt[:nkeys]), s)
+    call("flpr_points_prepare", ptr(x), n, gfp, ptr(key), ptr(cnt[:nkeys]), None, s)
     call("flpr_points_bin", ptr(key), n, nkeys, ptr(cnt[:nkeys]), ptr(key_start),
-         ptr(cnt[nkeys:]), ptr(order_idx), s)
+         ptr(cnt[nkeys:]), None, ptr(order_idx), s)
     del key
     nv = ny + K
     wab = torch.empty((2 * d * n,), dtype=f64, device=dev)
@@ -1015,9 +1015,11 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     key_count, key_cursor = cnt[:nkeys], cnt[nkeys:]
     key_start = torch.empty((nkeys + 1,), dtype=i32, device=dev)
     order_idx = torch.empty((n,), dtype=i32, device=dev)
-    call("flpr_points_prepare", ptr(x), n, gp, ptr(key), ptr(key_count), s)
+    rank = torch.empty((n,), dtype=i32, device=dev)
+    call("flpr_points_prepare", ptr(x), n, gp, ptr(key), ptr(key_count), ptr(rank), s)
     call("flpr_points_bin", ptr(key), n, nkeys, ptr(key_count), ptr(key_start), ptr(key_cursor),
-         ptr(order_idx), s)
+         ptr(rank), ptr(order_idx), s)
+    del rank
     wab = torch.empty((2 * d * n,), dtype=f64, device=dev)
     vals = torch.empty((max(nval, 1) * n,), dtype=f64, device=dev)
     rawi = torch.empty((d * n,), dtype=i32, device=dev)

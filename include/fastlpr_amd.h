@@ -77,16 +77,19 @@ int flpr_xstats(const double* x, int64_t n, int32_t d, const double* y, int32_t 
 /* Spread cell sort keys of every point, from the knot mapping of the
  * reference (convolution.py:169-199 + nufftn_type1.py:140-176, 520-543):
  *   key   (n)    int32     tile-major spread-cell index (tile, then cell in tile)
- *   key_count (nkeys = ntiles * prod(tile)) int32, zeroed by the caller. */
+ *   key_count (nkeys = ntiles * prod(tile)) int32, zeroed by the caller
+ *   rank  (n)    int32     the point's place among its key's points (may be NULL). */
 int flpr_points_prepare(const double* x, int64_t n, const FlprGeom* g, int32_t* key,
-                        int32_t* key_count, void* stream);
+                        int32_t* key_count, int32_t* rank, void* stream);
 
 /* Counting sort of the points by key.  key_start (nkeys+1) receives the
  * exclusive scan of key_count; key_cursor (nkeys) is scratch zeroed by the
  * caller (left zeroed only up to the scatter's own use); order (n) receives
- * point indices grouped by key. */
+ * point indices grouped by key.  With the ranks of flpr_points_prepare (rank
+ * non-NULL) the scatter needs no atomics. */
 int flpr_points_bin(const int32_t* key, int64_t n, int32_t nkeys, const int32_t* key_count,
-                    int32_t* key_start, int32_t* key_cursor, int32_t* order, void* stream);
+                    int32_t* key_start, int32_t* key_cursor, const int32_t* rank,
+                    int32_t* order, void* stream);
 
 /* Sorted point data for the grid kernels, point-major (point p's entries are
  * contiguous):
