@@ -153,6 +153,66 @@ __global__ __launch_bounds__(64) void k_select_scores(
 
 constexpr int SEL_LDS = 2048;  // bandwidths held in LDS for the selection
 
+// (value, index) minimum over the wave, the lower index on ties: the first
+// minimum of a sequential scan.
+__device__ inline void wave_argmin(double& v, int& i) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ov = __shfl_xor(v, off);
+    const int oi = __shfl_xor(i, off);
+    if (ov < v || (ov == v && oi < i)) {
+      v = ov;
+      i = oi;
+    }
+  }
+}
+
+// The selection of k_select_pick for dh <= 64 with one bandwidth per lane:
+// the same choices (first NaN or first minimum for idmin, first minimum over
+// the finite scores, the 1-SE rule preferring the larger sum(h), then the
+// first lowest score) from wave reductions instead of a serial walk.
+__device__ inline void select_pick_wave(int dh, const double* __restrict__ hsum_g, double dstd,
+                                        double* __restrict__ out, int32_t* __restrict__ sel_out) {
+  const int b = threadIdx.x;
+  const bool on = b < dh;
+  const double sc = on ? out[b] : 0.0;
+  const double er = on ? out[dh + b] : 0.0;
+  const double hs = on ? hsum_g[b] : 0.0;
+  const bool fin = on && isfinite(sc);
+  const uint64_t nan_mask = __ballot(on && sc != sc);
+  const int nvalid = __popcll(__ballot(fin));
+  // idmin: np.argmin, the first NaN when there is one
+  int idmin;
+  if (nan_mask) {
+    idmin = __ffsll((unsigned long long)nan_mask) - 1;
+  } else {
+    double v = on ? sc : INFINITY;
+    int i = on ? b : 1 << 30;
+    wave_argmin(v, i);
+    idmin = (i >= dh) ? 0 : i;   // all scores +inf: the first one
+  }
+  int sel = 0;
+  if (nvalid > 0) {
+    double v = fin ? sc : INFINITY;
+    int best = fin ? b : 1 << 30;
+    wave_argmin(v, best);
+    const double thr = __shfl(sc, best) + dstd * __shfl(er, best);
+    const bool cand = fin && sc <= thr;
+    double smax = cand ? hs : -INFINITY;
+    for (int off = 32; off > 0; off >>= 1) smax = fmax(smax, __shfl_xor(smax, off));
+    const bool c2 = cand && hs == smax;
+    double v2 = c2 ? sc : INFINITY;
+    int i2 = c2 ? b : 1 << 30;
+    wave_argmin(v2, i2);
+    sel = (i2 >= dh) ? best : i2;
+  }
+  if (b == 0) {
+    out[4 * dh] = (double)sel;
+    out[4 * dh + 1] = (double)idmin;
+    out[4 * dh + 2] = (double)nvalid;
+    if (sel_out) sel_out[0] = sel;
+  }
+}
+
 // The selection over the scores (one block; thread 0 walks the bandwidths in
 // LDS when they fit): out[4 dh] = sel, out[4 dh + 1] = idmin, out[4 dh + 2] =
 // number of finite scores.
@@ -162,6 +222,10 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select_pick(int dh,
                                                              double* __restrict__ out,
                                                              int32_t* __restrict__ sel_out) {
   __shared__ double lsc[SEL_LDS], ler[SEL_LDS], lhs[SEL_LDS];
+  if (dh <= 64) {
+    if (threadIdx.x < 64) select_pick_wave(dh, hsum_g, dstd, out, sel_out);
+    return;
+  }
   const bool in_lds = dh <= SEL_LDS;
   if (in_lds) {
     for (int b = threadIdx.x; b < dh; b += blockDim.x) {
