@@ -48,11 +48,26 @@ def _real_cols(a):
 @pytest.mark.parametrize("agg", ["first", "average", "sum"])
 @pytest.mark.parametrize("dstd", [0.0, 1.0])
 def test_select_matches_oracle(K, cplx, agg, dstd):
+    _check_select(K, cplx, agg, dstd, 9, None)
+
+
+@pytest.mark.parametrize("dh", [9, 64, 70])
+@pytest.mark.parametrize("nan_score", [None, 0, 4])
+def test_select_pick_sizes_and_nan_scores(dh, nan_score):
+    """The one-wave selection (dh <= 64) and the serial one (dh > 64), with a
+    NaN score first or inside (np.argmin's first NaN, the 1-SE rule over the
+    finite scores)."""
+    _check_select(3, False, "average", 1.0, dh, nan_score)
+
+
+def _check_select(K, cplx, agg, dstd, dh, nan_score):
     import torch
     from fastlpr_amd import _lib, engine
     _lib.load()
-    n, dh, ny_c = 400, 9, 1 if cplx else 3
+    n, ny_c = 400, 1 if cplx else 3
     p, phat, y, yh, h = _case(17 + K, n, K, dh, ny_c, cplx, nan_b=2)
+    if nan_score is not None:
+        yh[nan_score] = yh[nan_score] * np.nan
     dof, dse, pen_std, pen_mean, pph, pp = R.hutchinson(p, phat, n, cplx)
     scores, errs = [], []
     for b in range(dh):
@@ -90,7 +105,7 @@ def test_select_matches_oracle(K, cplx, agg, dstd):
     np.testing.assert_allclose(o[dh:2 * dh], errs, rtol=1e-10, atol=0)
     assert int(o[4 * dh]) == sel == int(seld.item())
     assert int(o[4 * dh + 1]) == int(np.argmin(scores))
-    assert int(o[4 * dh + 2]) == dh
+    assert int(o[4 * dh + 2]) == int(np.sum(np.isfinite(scores)))
 
 
 def test_select_rejects_bad_arguments():
