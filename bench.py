@@ -179,7 +179,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(the hot path shards by bandwidth, see DESIGN.md)", file=sys.stderr)
+              "(this build measures one GPU; the path would shard by bandwidth)",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
