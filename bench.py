@@ -179,7 +179,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(this build measures one GPU; the path would shard by bandwidth)",
+              "(the benchmark measures one GPU; the path would shard by bandwidth)",
               file=sys.stderr)
         sys.exit(2)
 
