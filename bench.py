@@ -20,8 +20,8 @@ exact three-term operand splits, float64 spread and solves), the plain form
 between them, a roofline entry for the dominant kernel (the float32 form's
 flpr_conv launches, measured with HIP events on the launch stream during the
 timed steps) and a CPU baseline (the oracle restatement, single thread, on a
-bounded sample: the mean pass with 1 and with 2 of the 30 bandwidths,
-extrapolated linearly to 30).
+bounded sample: the mean pass with 2 and with 4 of the 30 bandwidths after a
+warm-up call, extrapolated linearly to 30).
 """
 from __future__ import annotations
 
