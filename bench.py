@@ -211,6 +211,7 @@ def main():
     kt = engine.TIMER.collect()
     assert r2.metadata["dof_shared"]
     two_call_fits = pair_vs_calls = None
+    per_pass = {}
     if not args.no_two_calls:
         # the same pair as two independent calls
         step(cv, xd, yd, hl, "fp32")
@@ -223,6 +224,17 @@ def main():
         two_call_fits = 2.0 * args.steps / (tc1 - tc0)
         pair_vs_calls = max(float((r1.yhat - c1.yhat).abs().max() / c1.yhat.abs().max()),
                             float((r2.yhat - c2.yhat).abs().max() / c2.yhat.abs().max()))
+        # each pass on its own, as separate cv_fastlpr calls
+        opt = {"order": 2, "N": [256, 256], "precision": "fp32"}
+        r2y = (yd - c1.yhat) ** 2
+        for name, yy, oo in (("mean_pass", yd, opt),
+                             ("variance_pass", r2y, dict(opt, y_type_out="variance"))):
+            torch.cuda.synchronize()
+            tp0 = time.perf_counter()
+            for _ in range(args.steps):
+                cv(xd, yy, hl, oo)
+            torch.cuda.synchronize()
+            per_pass[name + "_fits_per_s"] = args.steps / (time.perf_counter() - tp0)
     if args.calibrate_copy:
         a = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device=dev).fill_(1.0)
         b = torch.empty_like(a)
@@ -298,6 +310,7 @@ def main():
     }
     line.update(extra)
     line["two_call_fits_per_s"] = two_call_fits
+    line.update(per_pass)
     line["pair_vs_two_calls_yhat_max_rel_diff"] = pair_vs_calls
     print(json.dumps(line), flush=True)
 
