@@ -54,3 +54,17 @@ def test_key_roundtrip_without_gpu():
         bits = struct.unpack("<Q", struct.pack("<d", v))[0]
         key = (~bits & (2**64 - 1)) if bits >> 63 else (bits | (1 << 63))
         assert lib.flpr_key_to_double(key) == v
+
+
+def test_conv_strip_length_without_gpu():
+    """flpr_conv_strip_len is host code: each of the eight shifted copies of a
+    tap row holds the reversed window positions nmax + 62 + 95 (the windows a
+    float32-form tile can read), rounded up to whole 16-byte units."""
+    from fastlpr_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lib = _lib.load()
+    assert lib.flpr_conv_strip_len(0) == -1
+    for nmax in (1, 40, 255, 256, 1024):
+        v = lib.flpr_conv_strip_len(nmax)
+        assert v % 8 == 0 and nmax + 157 <= v < nmax + 157 + 8

@@ -114,3 +114,16 @@ def test_select_rejects_bad_arguments():
     # nval must equal ny + nprobe; no device memory is touched on rejection
     rc = lib.flpr_select(None, 4, 5, 1, 3, None, None, None, 100, 0, 0, 0.0, None, None, None)
     assert rc == -1
+
+
+def test_conv_entry_points_reject_bad_arguments():
+    """flpr_conv_strips and flpr_conv validate before touching device memory."""
+    from fastlpr_amd import _lib
+    lib = _lib.load()
+    fake = 1 << 20   # never dereferenced: the checks fail first
+    # the tap rows must come in whole groups of npow
+    assert lib.flpr_conv_strips(fake, fake, 5, 3, 256, fake, None) == -1
+    assert lib.flpr_conv_strips(fake, fake, 6, 3, 0, fake, None) == -1
+    # nmax must be positive (the float32 form addresses its strip rows by it)
+    assert lib.flpr_conv(0, 0, fake, fake, fake, 0, fake, fake, 1, 1, 267, 256, 1, 0,
+                         None, 0, None) == -1
