@@ -741,7 +741,12 @@ constexpr int solve_group() { return 4; }
 // The first nout value channels are written to m_out; for channels with an
 // adjoint row in q, dot[h][c] += sum_node q_c(node) m_c(node) is reduced in
 // registers over the block's nodes and then once per block.
-template <typename T, int NT, bool PIV>
+// BUF (float32 moments whose per-bandwidth block is below 2 GiB): the
+// moments are read through one buffer resource per bandwidth, with the
+// per-thread part of each address (channel plane + node) in a 32-bit vector
+// offset and the moment's plane offset in a scalar one, so that the compiler
+// keeps no 64-bit base per (moment, channel).  Reads past the block return 0.
+template <typename T, int NT, bool PIV, bool BUF>
 __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(
     const T* __restrict__ mom, int64_t per_h, int64_t nodes, SolveTab tab,
     const uint64_t* __restrict__ key, double lam_scale, double eps_add,
@@ -753,6 +758,18 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(
   const double kmax = flpr_decode(*key);
   const int h = blockIdx.y;
   const T* b = mom + (int64_t)h * per_h;
+  // (unused, and removed, without BUF)
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<T*>(b), (short)0, BUF ? (int)(per_h * (int64_t)sizeof(T)) : 0, 0x00020000);
+  // moment element uni + thr of this bandwidth's block (uni uniform)
+  auto mload = [&](int64_t uni, int64_t thr) -> T {
+    if constexpr (BUF)
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(
+                                       rs, (int)(thr * (int64_t)sizeof(T)),
+                                       (int)(uni * (int64_t)sizeof(T)), 0));
+    else
+      return b[uni + thr];
+  };
   double dacc[MAXV];
 #pragma unroll
   for (int c = 0; c < MAXV; ++c) dacc[c] = 0.0;
@@ -772,16 +789,16 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(
         const bool on = g0 + j < nval;
         const int64_t c = g0 + j + 1;  // channel 0 is the ones channel
 #pragma unroll
-        for (int i = 0; i < NT; ++i) tnext[j][i] = on ? b[tab.t_off[i] + c * nodes + node] : T(0);
+        for (int i = 0; i < NT; ++i) tnext[j][i] = on ? mload(tab.t_off[i], c * nodes + node) : T(0);
         qnext[j] = (q && on) ? q[(int64_t)(g0 + j) * nodes + node] : 0.0;
       }
     };
     load_group(0);
     T t0v[NT];
 #pragma unroll
-    for (int i = 0; i < NT; ++i) t0v[i] = b[tab.t_off[i] + node];
+    for (int i = 0; i < NT; ++i) t0v[i] = mload(tab.t_off[i], node);
     if (NT == 1) {
-      s00 = (double)b[tab.s_off[0] + node];
+      s00 = (double)mload(tab.s_off[0], node);
       const double s = s00 + eps_add;                    // regression.py:127
       const double thr = (kmax + eps_add) * lam_scale;  // regression.py:217
       den = fmax(s, thr);
@@ -794,7 +811,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(
       for (int i = 0; i < NT; ++i)
 #pragma unroll
         for (int j = i; j < NT; ++j) {
-          const double v = (double)b[tab.s_off[p] + node];
+          const double v = (double)mload(tab.s_off[p], node);
           A[i][j] = v;
           A[j][i] = v;
           ++p;
@@ -949,12 +966,17 @@ void launch_solve(hipStream_t st, dim3 grd, bool piv, const T* mom, int64_t per_
                   int64_t nodes, const SolveTab& tab, const uint64_t* key, double lam_scale,
                   double eps_add, const double* shift, int nval, int nout, double* m_out,
                   const double* q, double* dot) {
+  const bool buf = sizeof(T) == 4 && per_h * (int64_t)sizeof(T) <= (int64_t)INT32_MAX;
   if (piv && NT > 1)
-    hipLaunchKernelGGL((k_solve<T, NT, true>), grd, dim3(SOLVE_THREADS), 0, st, mom, per_h,
-                       nodes, tab, key, lam_scale, eps_add, shift, nval, nout, m_out, q, dot);
+    hipLaunchKernelGGL((k_solve<T, NT, true, false>), grd, dim3(SOLVE_THREADS), 0, st, mom,
+                       per_h, nodes, tab, key, lam_scale, eps_add, shift, nval, nout, m_out, q, dot);
+  else if (buf)
+    hipLaunchKernelGGL((k_solve<T, NT, false, sizeof(T) == 4>), grd, dim3(SOLVE_THREADS), 0, st,
+                       mom, per_h, nodes, tab, key, lam_scale, eps_add, shift, nval, nout, m_out,
+                       q, dot);
   else
-    hipLaunchKernelGGL((k_solve<T, NT, false>), grd, dim3(SOLVE_THREADS), 0, st, mom, per_h,
-                       nodes, tab, key, lam_scale, eps_add, shift, nval, nout, m_out, q, dot);
+    hipLaunchKernelGGL((k_solve<T, NT, false, false>), grd, dim3(SOLVE_THREADS), 0, st, mom,
+                       per_h, nodes, tab, key, lam_scale, eps_add, shift, nval, nout, m_out, q, dot);
 }
 
 template <typename T>
