@@ -825,9 +825,9 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(
 #pragma unroll
         for (int k = 0; k < j; ++k) dsum -= A[j][k] * A[j][k];
         indefinite |= !(dsum > 0.0);
-        const double ljj = sqrt(dsum);
-        A[j][j] = ljj;
-        const double inv = 1.0 / ljj;
+        // the diagonal keeps 1 / L_jj: the triangular solves multiply by it
+        const double inv = 1.0 / sqrt(dsum);
+        A[j][j] = inv;
 #pragma unroll
         for (int i = j + 1; i < NT; ++i) {
           double s = A[i][j];
@@ -842,14 +842,14 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(
         double s = (i == 0) ? 1.0 : 0.0;
 #pragma unroll
         for (int k = 0; k < i; ++k) s -= A[i][k] * y[k];
-        y[i] = s / A[i][i];
+        y[i] = s * A[i][i];
       }
 #pragma unroll
       for (int i = NT - 1; i >= 0; --i) {
         double s = y[i];
 #pragma unroll
         for (int k = i + 1; k < NT; ++k) s -= A[k][i] * z[k];
-        z[i] = s / A[i][i];
+        z[i] = s * A[i][i];
       }
       if (PIV && indefinite) {
         // reload S + lam I; rows are exchanged with selects so that every
