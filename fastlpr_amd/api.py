@@ -187,6 +187,19 @@ def _check_cubic(N):
                              "requires at least  4 points per dimension.")
 
 
+MAX_VALUE_CHANNELS = 32
+
+
+def _check_channels(ny: int, K: int) -> None:
+    """The per-node solve reduces at most 32 value channels (response columns,
+    complex ones counted twice, plus the Hutchinson probes) in one launch; a
+    larger set is rejected before any device work."""
+    if ny + K > MAX_VALUE_CHANNELS:
+        raise NotImplementedError(
+            f"the GPU engine solves at most {MAX_VALUE_CHANNELS} value channels per fit: "
+            f"{ny} real response column(s) + num_dof_sample={K} probes = {ny + K}")
+
+
 def _grid_shape(opts: dict, n: int, dims: int) -> Tuple[int, ...]:
     """Grid nodes per dimension: options['N'] or ceil(n^(1/d)) (model.py:78-83, core.py:76-82)."""
     N = opts.get("N", None)
@@ -290,6 +303,7 @@ class GridInterpolant:
             pts = torch.as_tensor(np.atleast_2d(np.asarray(points, dtype=float)), device=dev)
         if pts.ndim == 1:
             pts = pts[:, None] if len(self._shape) == 1 else pts[None, :]
+        engine.check_points(self._geo, pts)
         pts = pts.contiguous()
         out, _ = engine.interp_eval(self._geo, pts, self._vals, self._vals.shape[0])
         if self._cplx:
@@ -480,14 +494,23 @@ def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
     variance = str(opts.get("y_type_out", "mean")) == "variance"
     fp64 = str(opts.get("precision", "fp32")).lower() == "fp64"
     N = _grid_shape(opts, n, dims)
+    y_is_complex = bool(y_arr.is_complex()) if isinstance(y_arr, torch.Tensor) \
+        else bool(np.iscomplexobj(y_arr))
+    # value channels of the solve, checked before any device work
+    if bool(opts.get("calc_dof", True)):
+        K_req = int(opts.get("num_dof_sample", 10))
+        rm = opts.get("random_matrix", None)
+        if rm is not None:
+            rshape = np.shape(rm.cpu() if isinstance(rm, torch.Tensor) else rm)
+            if len(rshape) == 2:
+                K_req = int(rshape[1])
+        _check_channels(dy * (2 if y_is_complex else 1), max(K_req, 0))
     dev = x.device if _is_dev(x) else (y.device if _is_dev(y) else _device())
     if not torch.cuda.is_available():
         _device()
 
     xd = (x_arr.to(dev) if isinstance(x_arr, torch.Tensor)
           else torch.as_tensor(x_arr, device=dev)).to(torch.float64).contiguous()
-    y_is_complex = bool(y_arr.is_complex()) if isinstance(y_arr, torch.Tensor) \
-        else bool(np.iscomplexobj(y_arr))
     yd = _y_channels(y_arr, dev)
     y_raw_orig = yd[:, 0].contiguous() if variance else None
     ny = yd.shape[1]

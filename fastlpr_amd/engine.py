@@ -1374,8 +1374,19 @@ def spline_coefficients(geo: Geometry, grids: torch.Tensor) -> torch.Tensor:
     return c.contiguous().reshape(G, -1)
 
 
+def check_points(geo: Geometry, x: torch.Tensor) -> None:
+    """Query points must be (n, d) for a d-dimensional grid: the interpolation
+    kernels read d coordinates per point, so any other shape would read past
+    the end of the buffer."""
+    d = len(geo.N)
+    if x.ndim != 2 or x.shape[1] != d:
+        raise ValueError(f"points must have shape (n, {d}) for a {d}-dimensional grid, "
+                         f"got {tuple(x.shape)}")
+
+
 def spline_eval(geo: Geometry, x: torch.Tensor, coef: torch.Tensor) -> torch.Tensor:
     """(n, G) values of the cubic splines with coefficients coef (G, nodes) at x (n, d)."""
+    check_points(geo, x)
     n = x.shape[0]
     G = coef.shape[0]
     out = torch.empty((n, G), dtype=torch.float64, device=x.device)
@@ -1437,6 +1448,7 @@ def fit_output(geo: Geometry, x: torch.Tensor, m_all: torch.Tensor, ncol: int,
 def interp_eval(geo: Geometry, x: torch.Tensor, m: torch.Tensor, ncol: int,
                 yvar: Optional[torch.Tensor] = None):
     """Multilinear interpolation of (ncol, nodes) float64 grid values at x (n, d)."""
+    check_points(geo, x)
     n = x.shape[0]
     out = torch.empty((n, ncol), dtype=torch.float64, device=x.device)
     es = torch.zeros((1,), dtype=torch.float64, device=x.device) if yvar is not None else None

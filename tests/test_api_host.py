@@ -157,3 +157,43 @@ def test_interval_nu_constants():
     assert abs(A._nu(3, 1) - 1 / (8 * math.pi ** 1.5)) < 1e-15
     with pytest.raises(ValueError):
         A._nu(4, 1)
+
+
+def test_value_channel_limit_rejected_before_device_work():
+    """The solve reduces at most 32 value channels (y columns + probes); a
+    larger set raises NotImplementedError before the GPU is touched (this
+    runs without one)."""
+    rs = np.random.default_rng(3)
+    x = rs.uniform(0, 1, (200, 1))
+    y = rs.standard_normal(200)
+    h = np.array([[0.1], [0.3]])
+    for K in (32, 40):
+        with pytest.raises(NotImplementedError, match="value channels"):
+            fastlpr_amd.cv_fastlpr(x, y, h, {"order": 1, "num_dof_sample": K})
+    # complex y counts twice: 2 + 31 > 32
+    with pytest.raises(NotImplementedError, match="value channels"):
+        fastlpr_amd.cv_fastlpr(x, (1 + 1j) * y, h, {"order": 1, "num_dof_sample": 31})
+    # a random_matrix gives the probe count by its columns
+    with pytest.raises(NotImplementedError, match="value channels"):
+        fastlpr_amd.cv_fastlpr(x, y, h, {"order": 1, "random_matrix": np.ones((200, 33))})
+    api._check_channels(1, 31)  # 32 channels: accepted
+
+
+def test_interpolant_rejects_wrong_point_columns():
+    """Query points with the wrong number of columns raise ValueError before
+    any kernel launch (the interpolation kernels read d coordinates per point)."""
+    from types import SimpleNamespace
+    import torch
+    from fastlpr_amd import engine
+    geo = SimpleNamespace(N=(8, 8))
+    with pytest.raises(ValueError, match=r"shape \(n, 2\)"):
+        engine.check_points(geo, torch.zeros((5, 1), dtype=torch.float64))
+    with pytest.raises(ValueError, match=r"shape \(n, 2\)"):
+        engine.check_points(geo, torch.zeros((5,), dtype=torch.float64))
+    engine.check_points(geo, torch.zeros((5, 2), dtype=torch.float64))
+    interp = api.GridInterpolant(geo, [np.arange(8.0)] * 2, torch.zeros((1, 64)), (8, 8),
+                                 False, 1, True, True)
+    with pytest.raises(ValueError, match=r"shape \(n, 2\)"):
+        interp(np.zeros((3, 1)))
+    with pytest.raises(ValueError, match=r"shape \(n, 2\)"):
+        interp(np.zeros((3, 3)))
