@@ -16,6 +16,7 @@ MAX_DIM = 3
 EXPORTS = (
     "flpr_last_hip_error",
     "flpr_xstats",
+    "flpr_xstats_noalloc",
     "flpr_points_prepare",
     "flpr_points_bin",
     "flpr_points_pack",
@@ -83,6 +84,7 @@ _GP = C.POINTER(FlprGeom)
 _SIGS = {
     "flpr_last_hip_error": [C.c_char_p, _I32],
     "flpr_xstats": [_P, _I64, _I32, _P, _I32, _P, _P, _P],
+    "flpr_xstats_noalloc": [_P, _I64, _I32, _P, _I32, _P, _P],
     "flpr_points_prepare": [_P, _I64, _GP, _P, _P, _P, _P],
     "flpr_points_bin": [_P, _I64, _I32, _P, _P, _P, _P, _P, _P],
     "flpr_points_pack": [_GP, _P, _P, _I64, _P, _I32, _P, _I32, _P, _P, _P, _P, _P],
@@ -142,7 +144,8 @@ def call(name: str, *args) -> None:
     lib = load()
     rc = getattr(lib, name)(*args)
     if rc != 0:
-        kinds = {-1: "invalid argument", -2: "HIP launch/runtime error", -3: "table too small"}
+        kinds = {-1: "invalid argument", -2: "HIP launch/runtime error",
+                 -3: "table or scratch too small"}
         detail = ""
         if rc == -2:
             buf = C.create_string_buffer(256)

@@ -29,6 +29,8 @@
 // is on the grid only.
 #include "flpr_common.h"
 
+#include <mutex>
+
 namespace {
 
 constexpr int XS_BLOCKS = 512;
@@ -1509,6 +1511,69 @@ int flpr_xstats(const double* x, int64_t n, int32_t d, const double* y, int32_t 
                      y, ny, partials);
   FLPR_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_xstats_final, dim3(1), dim3(XS_THREADS), 0, as_stream(stream), partials,
+                     XS_BLOCKS, d, ny, out);
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// flpr_xstats without allocation: the block partials live in a file-scope
+// device array instead of a caller-provided buffer.  Host code never passes
+// the array's name to a kernel (on the host it is only a stand-in for the
+// symbol): its device address is looked up once per device with
+// hipGetSymbolAddress and kept in g_scratch_ptr.
+// ---------------------------------------------------------------------------
+
+namespace {
+constexpr int XS_SCRATCH_DOUBLES = XS_BLOCKS * (5 * FLPR_MAX_DIM + 2 + MAXY);  // 196 KiB
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+}  // namespace
+
+static __device__ __align__(256) double scratch[XS_SCRATCH_DOUBLES];
+
+// Device address of this device's copy of scratch (every GPU holds its own
+// copy at its own address); looked up on the first call on each device.
+static int scratch_on_current_device(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    flpr_set_hip_error(e);
+    return -2;
+  }
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return -1;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      flpr_set_hip_error(e != hipSuccess ? e : hipErrorInvalidSymbol);
+      return -2;
+    }
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+int flpr_xstats_noalloc(const double* x, int64_t n, int32_t d, const double* y, int32_t ny,
+                        double* out, void* stream) {
+  FLPR_CLEAR_ERROR();
+  if ((d > 0 && !x) || !out || n < 1 || d < 0 || ny < 0 || (ny > 0 && !y)) return -1;
+  // the partials of d dimensions and ny columns must fit the scratch array
+  if (d > FLPR_MAX_DIM || ny > MAXY ||
+      (int64_t)XS_BLOCKS * (5 * d + 2 + ny) > (int64_t)XS_SCRATCH_DOUBLES)
+    return -3;
+  double* part = nullptr;
+  const int rc = scratch_on_current_device(&part);
+  if (rc != 0) return rc;
+  // k_xstats writes every partial slot before k_xstats_final reads it, so
+  // the contents left by an earlier call need no clearing
+  hipLaunchKernelGGL(k_xstats, dim3(XS_BLOCKS), dim3(XS_THREADS), 0, as_stream(stream), x, n, d,
+                     y, ny, part);
+  FLPR_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_xstats_final, dim3(1), dim3(XS_THREADS), 0, as_stream(stream), part,
                      XS_BLOCKS, d, ny, out);
   FLPR_CHECK_LAUNCH();
   return 0;

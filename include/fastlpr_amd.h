@@ -74,6 +74,20 @@ int flpr_last_hip_error(char* buf, int32_t len);
 int flpr_xstats(const double* x, int64_t n, int32_t d, const double* y, int32_t ny,
                 double* partials, double* out, void* stream);
 
+/* flpr_xstats without allocation.  No counterpart in the reference: this is
+ * for callers that cannot allocate device memory or pass a workspace (code
+ * under HIP graph capture, serving loops and harnesses that forbid
+ * hipMalloc).  Same inputs and out as flpr_xstats; the 512 block partials
+ * live in a library-owned static device array of 196 KiB per GPU
+ * (512 * (5*3 + 2 + 32) doubles), held until the process ends.  Returns -3
+ * when d > 3 or ny > 32 (the partials would not fit).  Allocates nothing
+ * (its first call on a device looks up that device's copy of the array).
+ * Every call on one GPU shares that array: calls that overlap in time on one
+ * GPU (different streams or threads) race and must be serialised by the
+ * caller. */
+int flpr_xstats_noalloc(const double* x, int64_t n, int32_t d, const double* y, int32_t ny,
+                        double* out, void* stream);
+
 /* Spread cell sort keys of every point, from the knot mapping of the
  * reference (convolution.py:169-199 + nufftn_type1.py:140-176, 520-543):
  *   key   (n)    int32     tile-major spread-cell index (tile, then cell in tile)
