@@ -3,7 +3,9 @@
 Public API (same names and semantics as the reference package ``fastlpr``):
 ``cv_fastlpr``, ``cv_fastkde``, ``get_hlist``, ``fastlpr_predict``,
 ``fastlpr_interval``, ``RegressionOutput``, ``KDEOutput``; plus
-``cv_fastlpr_pair`` (the heteroscedastic mean + variance fit in one call).
+``cv_fastlpr_pair`` (the heteroscedastic mean + variance fit in one call) and
+``cv_fastlpr_batch`` / ``BatchPlan`` (many independent problems in shared
+launches, optionally replayed from a captured HIP graph).
 """
 from .bandwidth import get_hlist
 from .structures import KDEOutput, RegressionOutput
@@ -17,8 +19,12 @@ def __getattr__(name):
                 "fastlpr_predict", "fastlpr_interval", "cv_fastlpr_pair"):
         from . import api
         return getattr(api, name)
+    if name in ("cv_fastlpr_batch", "BatchPlan"):
+        from . import batch
+        return getattr(batch, name)
     raise AttributeError(name)
 
 
 __all__ = ["cv_fastlpr", "cv_fastkde", "get_hlist", "RegressionOutput", "KDEOutput",
-           "clear_caches", "fastlpr_predict", "fastlpr_interval", "cv_fastlpr_pair"]
+           "clear_caches", "fastlpr_predict", "fastlpr_interval", "cv_fastlpr_pair",
+           "cv_fastlpr_batch", "BatchPlan"]

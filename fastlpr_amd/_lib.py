@@ -40,6 +40,21 @@ EXPORTS = (
     "flpr_select",
     "flpr_lcv",
     "flpr_spline_eval",
+    "flpr_fill_zero",
+    "flpr_batch_xstats",
+    "flpr_batch_points_prepare",
+    "flpr_batch_points_pack",
+    "flpr_batch_spread",
+    "flpr_batch_grid_center",
+    "flpr_batch_taps",
+    "flpr_batch_max_moment",
+    "flpr_batch_solve",
+    "flpr_batch_adjoint",
+    "flpr_batch_quadform",
+    "flpr_batch_select",
+    "flpr_batch_fit_output",
+    "flpr_batch_spline_coef",
+    "flpr_batch_point_sums",
 )
 
 
@@ -69,6 +84,12 @@ class FlprGeom(C.Structure):
         ("graw0", C.c_double * MAX_DIM),
         ("graw1", C.c_double * MAX_DIM),
     ]
+
+
+class FlprBatchItem(C.Structure):
+    """Mirror of ``FlprBatchItem`` in include/fastlpr_amd.h."""
+
+    _fields_ = [("g", FlprGeom), ("n", C.c_int64), ("pt", C.c_int64)]
 
 
 class FlprError(RuntimeError):
@@ -108,6 +129,27 @@ _SIGS = {
     "flpr_spline_eval": [_GP, _P, _I64, _P, _I32, _P, _P],
     "flpr_lcv": [_GP, _P, _I64, _P, _I32, _P, _P, _P, _P],
     "flpr_select": [_P, _I32, _I32, _I32, _I32, _P, _P, _P, _I64, _I32, _I32, _D, _P, _P, _P],
+    "flpr_fill_zero": [_P, _I64, _P],
+    "flpr_batch_xstats": [_P, _I32, _P, _I32, _P, _I32, _P, _P, _P],
+    "flpr_batch_points_prepare": [_GP, _P, _I32, _I64, _P, _P, _P, _P, _P],
+    "flpr_batch_points_pack": [_GP, _P, _I32, _I64, _P, _P, _P, _I32, _P, _I32, _P, _P, _P, _P,
+                               _P],
+    "flpr_batch_spread": [_GP, _P, _I32, _I64, _P, _P, _P, _I32, _I32, _P, _P, _I64, _P, _P],
+    "flpr_batch_grid_center": [_I32, _P, _I64, _I32, _I64, _P, _P, _P, _P],
+    "flpr_batch_taps": [_GP, _P, _I32, _P, _I32, _I32, _I32, _P, _P, _P],
+    "flpr_batch_max_moment": [_I32, _I32, _P, _I64, _I64, _I32, C.POINTER(_I64), _I32, _I64, _I32,
+                              _P, _P],
+    "flpr_batch_solve": [_I32, _I32, _P, _I64, _I64, _I32, _I64, _I32, _I32, C.POINTER(_I64),
+                         C.POINTER(_I64), _P, _D, _D, _P, _I32, _I32, _P, _I64, _P, _P, _I64, _P],
+    "flpr_batch_adjoint": [_GP, _P, _I32, _I64, _P, _P, _P, _P, _I32, _P, _P, _P, _P],
+    "flpr_batch_quadform": [_GP, _I32, _P, _I64, _I32, _I32, _P, _P, _I64, _P],
+    "flpr_batch_select": [_P, _I32, _P, _I64, _I32, _I32, _I32, _I32, _P, _P, _P, _I32, _I32, _D,
+                          _P, _I64, _P, _P],
+    "flpr_batch_fit_output": [_GP, _P, _I32, _I64, _P, _P, _I64, _I32, _P, _P, _I32, _P, _P, _P,
+                              _P, _P, _I64, _P],
+    "flpr_batch_spline_coef": [_GP, _I32, _P, _I64, _P, _I32, _P, _P, _P],
+    "flpr_batch_point_sums": [_GP, _P, _I32, _I64, _P, _P, _I32, _P, _I32, _P, _P, _I32, _I32,
+                              _I32, _P, _P, _I64, _P],
 }
 
 _lib = None

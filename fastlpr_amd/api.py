@@ -227,9 +227,12 @@ _PROBES: Dict[tuple, tuple] = {}
 def _probes(n: int, K: int, seed, random_matrix, dev):
     """Probe matrix p (n, K) = Generator(MT19937(seed)).standard_normal((n, K))
     (regression.py:1184-1194) and p'p per column, on the host and on the device.
-    The matrix depends only on (n, K, seed), not on the data, so it is generated
-    once per key and kept on the device; a caller-supplied random_matrix is used
-    as is.  Returns (p on the device, p'p on the host, p'p on the device)."""
+    The generator fills the matrix row by row from one stream, so the matrix
+    of n rows is the first n rows of the matrix of any larger row count with
+    the same K and seed: one matrix per (K, seed), at the largest n asked so
+    far, is kept on the device and every smaller n takes a prefix of it.  A
+    caller-supplied random_matrix is used as is.  Returns (p on the device,
+    p'p on the host, p'p on the device)."""
     if random_matrix is not None:
         p = np.asarray(random_matrix.cpu() if isinstance(random_matrix, torch.Tensor)
                        else random_matrix, dtype=float)
@@ -241,18 +244,36 @@ def _probes(n: int, K: int, seed, random_matrix, dev):
     key = (n, K, int(seed), str(dev))
     hit = _PROBES.get(key)
     if hit is None:
-        p = np.random.Generator(np.random.MT19937(seed)).standard_normal((n, K))
-        pp = np.sum(p * p, axis=0)
-        hit = (torch.as_tensor(p, device=dev).contiguous(), pp, torch.as_tensor(pp, device=dev))
-        if len(_PROBES) > 8:
+        base_h, base = _probe_matrix(n, K, seed, dev)
+        pp = np.sum(base_h[:n] * base_h[:n], axis=0)
+        hit = (base[:n], pp, torch.as_tensor(pp, device=dev))
+        if len(_PROBES) > 4096:
             _PROBES.clear()
         _PROBES[key] = hit
+    return hit
+
+
+_PROBE_BASE: Dict[tuple, tuple] = {}
+
+
+def _probe_matrix(n: int, K: int, seed, dev):
+    """The probe matrix of (K, seed) with at least n rows, on the host and on
+    the device (regenerated larger when n exceeds the rows held)."""
+    key = (K, int(seed), str(dev))
+    hit = _PROBE_BASE.get(key)
+    if hit is None or hit[0].shape[0] < n:
+        p = np.random.Generator(np.random.MT19937(seed)).standard_normal((n, K))
+        hit = (p, torch.as_tensor(p, device=dev).contiguous())
+        if len(_PROBE_BASE) > 4:
+            _PROBE_BASE.clear()
+        _PROBE_BASE[key] = hit
     return hit
 
 
 def clear_caches():
     """Drop the cached probe matrices, conv plans and host geometry / bandwidth tables."""
     _PROBES.clear()
+    _PROBE_BASE.clear()
     engine._PLANS.clear()
     engine._GEOMS.clear()
     engine._BTABS.clear()
@@ -320,28 +341,16 @@ class GridInterpolant:
 def _point_sums(fit, xd, yd, probes, ny: int, K: int, gcv_method: str, dof_method: str):
     """Probe dots and residual sums from the grid solutions interpolated at the
     points (the reference's direct route, regression.py:1256-1280 and
-    api.py:661-723), used when either interpolation is cubic.  Returns the fit
-    with its m_y narrowed to the y columns and its red buffer filled in the
-    layout flpr_select reads (y dots zero, residual sums in place of the
-    quadratic forms), and the matching all-zero sum of squares."""
+    api.py:661-723), used when either interpolation is cubic: the spline
+    coefficients and the sums at the points in two kernels
+    (engine.point_sums).  Returns the fit with its m_y narrowed to the y
+    columns and its red buffer filled in the layout flpr_select reads (y
+    dots zero, residual sums in place of the quadratic forms), and the
+    matching all-zero sum of squares."""
     dh, nval, nodes = fit.m_y.shape
-    n = xd.shape[0]
-    geo = fit.geo
-
-    def at_points(lo, hi, method):
-        grids = fit.m_y[:, lo:hi, :].reshape(-1, nodes).contiguous()
-        if method == "cubic":
-            out = engine.spline_eval(geo, xd, engine.spline_coefficients(geo, grids))
-        else:
-            out, _ = engine.interp_eval(geo, xd, grids, grids.shape[0])
-        return out.view(n, dh, hi - lo)
-
-    yh = at_points(0, ny, gcv_method)
-    rss = ((yd[:, None, :] - yh) ** 2).sum(dim=0)              # (dh, ny)
-    ph = at_points(ny, ny + K, dof_method)
-    dots = (ph * probes[:, None, :]).sum(dim=0)                 # (dh, K)
-    zeros = torch.zeros((dh, ny), dtype=torch.float64, device=xd.device)
-    red = torch.cat([torch.cat([zeros, dots], dim=1).reshape(-1), rss.reshape(-1)])
+    red = torch.zeros((dh * nval + dh * ny,), dtype=torch.float64, device=xd.device)
+    engine.point_sums(fit.geo, xd, yd, ny, probes, K, fit.m_y, dh, gcv_method == "cubic",
+                      dof_method == "cubic", red)
     fit.m_y = fit.m_y[:, :ny, :]
     fit.red = red
     return fit, torch.zeros((ny,), dtype=torch.float64, device=xd.device)

@@ -23,6 +23,16 @@ void flpr_set_hip_error(hipError_t e);
     }                                       \
   } while (0)
 
+// Batch view of one launch: the problem is blockIdx.z.  items (device
+// memory) holds the per-problem geometry and point range, or is null when
+// the launch serves one problem; s[] are the per-problem strides (elements)
+// of the kernel's per-problem pointers, in the order each kernel documents.
+// A launch for one problem passes a zeroed FlprBz with gridDim.z = 1.
+struct FlprBz {
+  const FlprBatchItem* items;
+  int64_t s[6];
+};
+
 static inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 static inline int flpr_grid_for(int64_t n, int threads, int cap) {
@@ -53,7 +63,7 @@ static inline int64_t flpr_nkeys(const FlprGeom* g) {
   return t;
 }
 
-static inline int64_t flpr_nodes(const FlprGeom* g) {
+static inline __host__ __device__ int64_t flpr_nodes(const FlprGeom* g) {
   int64_t t = 1;
   for (int k = 0; k < g->d; ++k) t *= g->n_grid[k];
   return t;

@@ -36,11 +36,31 @@ struct Mfma64 {
 
 constexpr int TAP_THREADS = 256;
 
+// Batch: items give the geometry and the point count (the norm); s[0] =
+// tap64 and tap32.
 __global__ __launch_bounds__(TAP_THREADS) void k_taps(FlprGeom g, const double* __restrict__ h,
                                                       int dh, int npow, int nmax, double norm,
                                                       double* __restrict__ tap64,
-                                                      float* __restrict__ tap32) {
+                                                      float* __restrict__ tap32, FlprBz bz) {
 #pragma clang fp contract(off)
+  // the per-axis fields in registers (a copied FlprGeom indexed by a runtime
+  // axis would live in scratch memory)
+  int ngr[FLPR_MAX_DIM];
+  double glo[FLPR_MAX_DIM], ghi[FLPR_MAX_DIM];
+  {
+    const FlprGeom* src = bz.items ? &bz.items[blockIdx.z].g : &g;
+#pragma unroll
+    for (int k = 0; k < FLPR_MAX_DIM; ++k) {
+      ngr[k] = src->n_grid[k];
+      glo[k] = src->xmin_s[k];
+      ghi[k] = src->xmax_s[k];
+    }
+  }
+  if (bz.items) {
+    norm = (double)bz.items[blockIdx.z].n;
+    tap64 += blockIdx.z * bz.s[0];
+    if (tap32) tap32 += blockIdx.z * bz.s[0];
+  }
   const int64_t total = (int64_t)dh * g.d * nmax;
   const double inv_sqrt_2pi = 1.0 / sqrt(2.0 * M_PI);
   for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
@@ -49,7 +69,7 @@ __global__ __launch_bounds__(TAP_THREADS) void k_taps(FlprGeom g, const double* 
     const int64_t r = idx / nmax;
     const int k = (int)(r % g.d);
     const int b = (int)(r / g.d);
-    const int N = g.n_grid[k];
+    const int N = k == 0 ? ngr[0] : (k == 1 ? ngr[1] : ngr[2]);
     const int64_t o = ((int64_t)(b * g.d + k) * npow) * nmax + t;
     if (t >= N) {
       for (int a = 0; a < npow; ++a) {
@@ -59,7 +79,8 @@ __global__ __launch_bounds__(TAP_THREADS) void k_taps(FlprGeom g, const double* 
       continue;
     }
     // numpy linspace(lo, hi, N): t*step + lo, last element exactly hi
-    const double lo = g.xmin_s[k], hi = g.xmax_s[k];
+    const double lo = k == 0 ? glo[0] : (k == 1 ? glo[1] : glo[2]);
+    const double hi = k == 0 ? ghi[0] : (k == 1 ? ghi[1] : ghi[2]);
     double xg = lo;
     if (N > 1) {
       const double step = (hi - lo) / (double)(N - 1);
@@ -665,11 +686,15 @@ struct OffTab {
 
 constexpr int RED_THREADS = 256;
 
+// Batch: s[0] = src; one key per problem.
 template <typename T>
 __global__ __launch_bounds__(RED_THREADS) void k_maxmom(const T* __restrict__ src, int64_t per_h,
                                                         OffTab offs, int nslab, int64_t nodes,
-                                                        int absval, uint64_t* __restrict__ key) {
+                                                        int absval, uint64_t* __restrict__ key,
+                                                        FlprBz bz) {
   __shared__ double sh[RED_THREADS / 64];
+  src += blockIdx.z * bz.s[0];
+  key += blockIdx.z;
   const int hs = blockIdx.y;  // h * nslab + slab
   const int h = hs / nslab, k = hs % nslab;
   const T* p = src + (int64_t)h * per_h + offs.v[k];
@@ -746,13 +771,24 @@ constexpr int solve_group() { return 4; }
 // per-thread part of each address (channel plane + node) in a 32-bit vector
 // offset and the moment's plane offset in a scalar one, so that the compiler
 // keeps no 64-bit base per (moment, channel).  Reads past the block return 0.
+// Batch (blockIdx.z = problem): s[0] = mom, s[1] = shift, s[2] = m_out,
+// s[3] = q, s[4] = dot; one key per problem.
 template <typename T, int NT, bool PIV, bool BUF>
 __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(
     const T* __restrict__ mom, int64_t per_h, int64_t nodes, SolveTab tab,
     const uint64_t* __restrict__ key, double lam_scale, double eps_add,
     const double* __restrict__ shift, int nval, int nout, double* __restrict__ m_out,
-    const double* __restrict__ q, double* __restrict__ dot) {
+    const double* __restrict__ q, double* __restrict__ dot, FlprBz bz) {
   __shared__ double red[SOLVE_THREADS / 64][MAXV];
+  {
+    const int64_t z = blockIdx.z;
+    mom += z * bz.s[0];
+    key += z;
+    if (shift) shift += z * bz.s[1];
+    if (m_out) m_out += z * bz.s[2];
+    if (q) q += z * bz.s[3];
+    if (dot) dot += z * bz.s[4];
+  }
   constexpr int CGR = solve_group<T>();
   static_assert(MAXV % CGR == 0, "channel groups");
   const double kmax = flpr_decode(*key);
@@ -965,18 +1001,18 @@ template <typename T, int NT>
 void launch_solve(hipStream_t st, dim3 grd, bool piv, const T* mom, int64_t per_h,
                   int64_t nodes, const SolveTab& tab, const uint64_t* key, double lam_scale,
                   double eps_add, const double* shift, int nval, int nout, double* m_out,
-                  const double* q, double* dot) {
+                  const double* q, double* dot, const FlprBz& bz) {
   const bool buf = sizeof(T) == 4 && per_h * (int64_t)sizeof(T) <= (int64_t)INT32_MAX;
   if (piv && NT > 1)
     hipLaunchKernelGGL((k_solve<T, NT, true, false>), grd, dim3(SOLVE_THREADS), 0, st, mom,
-                       per_h, nodes, tab, key, lam_scale, eps_add, shift, nval, nout, m_out, q, dot);
+                       per_h, nodes, tab, key, lam_scale, eps_add, shift, nval, nout, m_out, q, dot, bz);
   else if (buf)
     hipLaunchKernelGGL((k_solve<T, NT, false, sizeof(T) == 4>), grd, dim3(SOLVE_THREADS), 0, st,
                        mom, per_h, nodes, tab, key, lam_scale, eps_add, shift, nval, nout, m_out,
-                       q, dot);
+                       q, dot, bz);
   else
     hipLaunchKernelGGL((k_solve<T, NT, false, false>), grd, dim3(SOLVE_THREADS), 0, st, mom,
-                       per_h, nodes, tab, key, lam_scale, eps_add, shift, nval, nout, m_out, q, dot);
+                       per_h, nodes, tab, key, lam_scale, eps_add, shift, nval, nout, m_out, q, dot, bz);
 }
 
 template <typename T>
@@ -984,11 +1020,11 @@ int dispatch_solve(int nt, hipStream_t st, dim3 grd, bool piv, const T* mom, int
                    int64_t nodes,
                    const SolveTab& tab, const uint64_t* key, double lam_scale, double eps_add,
                    const double* shift, int nval, int nout, double* m_out, const double* q,
-                   double* dot) {
+                   double* dot, const FlprBz& bz) {
 #define FLPR_SOLVE_CASE(N)                                                                   \
   case N:                                                                                    \
     launch_solve<T, N>(st, grd, piv, mom, per_h, nodes, tab, key, lam_scale, eps_add, shift, \
-                       nval, nout, m_out, q, dot);                                           \
+                       nval, nout, m_out, q, dot, bz);                                       \
     return 0;
   switch (nt) {
     FLPR_SOLVE_CASE(1)
@@ -1015,7 +1051,7 @@ int flpr_taps(const FlprGeom* g, const double* h, int32_t dh, int32_t npow, int3
     if (g->n_grid[k] > nmax) return -3;
   const int64_t total = (int64_t)dh * g->d * nmax;
   hipLaunchKernelGGL(k_taps, dim3(flpr_grid_for(total, TAP_THREADS, 4096)), dim3(TAP_THREADS), 0,
-                     as_stream(stream), *g, h, dh, npow, nmax, norm, tap64, tap32);
+                     as_stream(stream), *g, h, dh, npow, nmax, norm, tap64, tap32, FlprBz{});
   FLPR_CHECK_LAUNCH();
   return 0;
 }
@@ -1103,10 +1139,10 @@ int flpr_max_moment(int32_t fp64, const void* src, int64_t per_h, int32_t dh, co
   dim3 grd(flpr_grid_for(nodes, RED_THREADS, per), (unsigned)(dh * nslab));
   if (fp64)
     hipLaunchKernelGGL((k_maxmom<double>), grd, dim3(RED_THREADS), 0, as_stream(stream),
-                       (const double*)src, per_h, t, nslab, nodes, absval, key);
+                       (const double*)src, per_h, t, nslab, nodes, absval, key, FlprBz{});
   else
     hipLaunchKernelGGL((k_maxmom<float>), grd, dim3(RED_THREADS), 0, as_stream(stream),
-                       (const float*)src, per_h, t, nslab, nodes, absval, key);
+                       (const float*)src, per_h, t, nslab, nodes, absval, key, FlprBz{});
   FLPR_CHECK_LAUNCH();
   return 0;
 }
@@ -1135,10 +1171,92 @@ int flpr_solve(int32_t fp64, const void* mom, int64_t per_h, int32_t dh, int64_t
   int rc;
   if (fp64)
     rc = dispatch_solve<double>(nt, st, grd, pivot != 0, (const double*)mom, per_h, nodes, tab, key,
-                                lam_scale, eps_add, shift, nval, nout, m_out, q, dot);
+                                lam_scale, eps_add, shift, nval, nout, m_out, q, dot, FlprBz{});
   else
     rc = dispatch_solve<float>(nt, st, grd, pivot != 0, (const float*)mom, per_h, nodes, tab, key,
-                               lam_scale, eps_add, shift, nval, nout, m_out, q, dot);
+                               lam_scale, eps_add, shift, nval, nout, m_out, q, dot, FlprBz{});
+  if (rc) return rc;
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+int flpr_batch_taps(const FlprGeom* g, const FlprBatchItem* items, int32_t P, const double* h,
+                    int32_t dh, int32_t npow, int32_t nmax, double* tap64, float* tap32,
+                    void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!flpr_geom_ok(g) || !items || P < 1 || P > 65535 || !h || !tap64 || dh < 1 || npow < 1 ||
+      nmax < 1)
+    return -1;
+  for (int k = 0; k < g->d; ++k)
+    if (g->n_grid[k] > nmax) return -3;
+  const int64_t total = (int64_t)dh * g->d * nmax;
+  FlprBz bz{};
+  bz.items = items;
+  bz.s[0] = total * npow;
+  hipLaunchKernelGGL(k_taps, dim3(flpr_grid_for(total, TAP_THREADS, 4096), 1, P),
+                     dim3(TAP_THREADS), 0, as_stream(stream), *g, h, dh, npow, nmax, 1.0, tap64,
+                     tap32, bz);
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+int flpr_batch_max_moment(int32_t fp64, int32_t P, const void* src, int64_t prob_stride,
+                          int64_t per_h, int32_t dh, const int64_t* offs, int32_t nslab,
+                          int64_t nodes, int32_t absval, uint64_t* key, void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (P < 1 || P > 65535 || !src || !offs || !key || dh < 1 || nslab < 1 ||
+      nslab > FLPR_MAX_MOMENTS || nodes < 1 || (int64_t)dh * nslab > 65535)
+    return -1;
+  OffTab t;
+  for (int k = 0; k < nslab; ++k) t.v[k] = offs[k];
+  const int per = 1024 / (dh * nslab) > 1 ? 1024 / (dh * nslab) : 1;
+  dim3 grd(flpr_grid_for(nodes, RED_THREADS, per), (unsigned)(dh * nslab), (unsigned)P);
+  FlprBz bz{};
+  bz.s[0] = prob_stride;
+  if (fp64)
+    hipLaunchKernelGGL((k_maxmom<double>), grd, dim3(RED_THREADS), 0, as_stream(stream),
+                       (const double*)src, per_h, t, nslab, nodes, absval, key, bz);
+  else
+    hipLaunchKernelGGL((k_maxmom<float>), grd, dim3(RED_THREADS), 0, as_stream(stream),
+                       (const float*)src, per_h, t, nslab, nodes, absval, key, bz);
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+int flpr_batch_solve(int32_t fp64, int32_t P, const void* mom, int64_t mom_stride, int64_t per_h,
+                     int32_t dh, int64_t nodes, int32_t order, int32_t nt, const int64_t* s_off,
+                     const int64_t* t_off, const uint64_t* key, double lam_scale, double eps_add,
+                     const double* shift, int32_t nval, int32_t nout, double* m_out,
+                     int64_t m_stride, const double* q, double* dot, int64_t dot_stride,
+                     void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (P < 1 || P > 65535 || !mom || !s_off || !t_off || !key || dh < 1 || dh > 65535 ||
+      nodes < 1 || nt < 1 || nt > FLPR_MAX_TERMS || nval < 0 || nval > MAXV || nout < 0 ||
+      nout > nval)
+    return -1;
+  if ((order == 0) != (nt == 1)) return -1;
+  if (nout > 0 && !m_out) return -1;
+  if (q && !dot) return -1;
+  SolveTab tab;
+  const int ns = nt * (nt + 1) / 2;
+  for (int i = 0; i < ns; ++i) tab.s_off[i] = s_off[i];
+  for (int i = 0; i < nt; ++i) tab.t_off[i] = t_off[i];
+  const int bx = flpr_grid_for(nodes, SOLVE_THREADS, 1024 / (dh > 64 ? 64 : dh) + 1);
+  dim3 grd((unsigned)bx, (unsigned)dh, (unsigned)P);
+  FlprBz bz{};
+  bz.s[0] = mom_stride;
+  bz.s[1] = nval + 1;
+  bz.s[2] = m_stride;
+  bz.s[3] = (int64_t)nval * nodes;
+  bz.s[4] = dot_stride;
+  hipStream_t st = as_stream(stream);
+  int rc;
+  if (fp64)
+    rc = dispatch_solve<double>(nt, st, grd, false, (const double*)mom, per_h, nodes, tab, key,
+                                lam_scale, eps_add, shift, nval, nout, m_out, q, dot, bz);
+  else
+    rc = dispatch_solve<float>(nt, st, grd, false, (const float*)mom, per_h, nodes, tab, key,
+                               lam_scale, eps_add, shift, nval, nout, m_out, q, dot, bz);
   if (rc) return rc;
   FLPR_CHECK_LAUNCH();
   return 0;

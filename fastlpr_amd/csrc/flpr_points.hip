@@ -70,10 +70,19 @@ __device__ inline double block_sum(double v, double* sh) {
 // y; sums of squares of the y columns.  Partials per block:
 //   [d][blocks][5] Welford, then [blocks][2] non-finite counts,
 //   then [ny][blocks] sums of squares.
+// Batch (blockIdx.z = problem): items give the point range; s[0] = partials.
 __global__ __launch_bounds__(XS_THREADS) void k_xstats(const double* __restrict__ x, int64_t n,
                                                        int d, const double* __restrict__ y,
-                                                       int ny, double* __restrict__ partials) {
+                                                       int ny, double* __restrict__ partials,
+                                                       FlprBz bz) {
   __shared__ Welford sh[XS_THREADS];
+  if (bz.items) {
+    const FlprBatchItem& it = bz.items[blockIdx.z];
+    if (x) x += it.pt * d;
+    if (y) y += it.pt * ny;
+    n = it.n;
+  }
+  partials += blockIdx.z * bz.s[0];
   __shared__ double red[XS_THREADS / 64];
   double bad_x = 0.0, bad_y = 0.0;
   for (int k = 0; k < d; ++k) {
@@ -126,10 +135,13 @@ __global__ __launch_bounds__(XS_THREADS) void k_xstats(const double* __restrict_
 }
 
 // out: [d][5] = min, max, mean, M2, count; then bad_x, bad_y; then ny sums of squares.
+// Batch: s[0] = partials, s[1] = out.
 __global__ __launch_bounds__(XS_THREADS) void k_xstats_final(const double* __restrict__ partials,
                                                              int nparts, int d, int ny,
-                                                             double* __restrict__ out) {
+                                                             double* __restrict__ out, FlprBz bz) {
   __shared__ Welford sh[XS_THREADS];
+  partials += blockIdx.z * bz.s[0];
+  out += blockIdx.z * bz.s[1];
   __shared__ double red[XS_THREADS / 64];
   for (int k = 0; k < d; ++k) {
     Welford w = {0.0, 0.0, 0.0, INFINITY, -INFINITY};
@@ -211,18 +223,35 @@ constexpr int PREP_THREADS = 256;
 
 // Spread cell and sort key (tile-major cell index) of every point, from the
 // knot mapping of knot_of.
+// Batch: items give the geometry and point range; problem z's keys are
+// offset by z * s[0] (its key count).
+// (D = g.d: the axis loops unroll, so a geometry copied from the batch table
+// stays in registers)
+template <int D>
 __global__ __launch_bounds__(PREP_THREADS) void k_prepare(const double* __restrict__ x, int64_t n,
                                                           FlprGeom g, int32_t* __restrict__ key,
                                                           int32_t* __restrict__ key_count,
-                                                          int32_t* __restrict__ rank) {
+                                                          int32_t* __restrict__ rank, FlprBz bz) {
+  int key0 = 0;
+  if (bz.items) {
+    const FlprBatchItem& it = bz.items[blockIdx.z];
+    g = it.g;
+    x += it.pt * D;
+    key += it.pt;
+    if (rank) rank += it.pt;
+    n = it.n;
+    key0 = (int)(blockIdx.z * bz.s[0]);
+  }
   int tc = 1;
-  for (int k = 0; k < g.d; ++k) tc *= g.tile[k];
+#pragma unroll
+  for (int k = 0; k < D; ++k) tc *= g.tile[k];
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     int tid = 0, lid = 0;
-    for (int k = 0; k < g.d; ++k) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
       int m;
-      knot_of(g, k, x[i * g.d + k], &m);
+      knot_of(g, k, x[i * D + k], &m);
       const int pm = (m > g.L[k] / 2) ? m - g.L[k] : m;
       // the data support maps into [1 + msp, G - 2 - msp]; the clamp only keeps
       // a malformed geometry from addressing outside the key table
@@ -230,7 +259,7 @@ __global__ __launch_bounds__(PREP_THREADS) void k_prepare(const double* __restri
       tid = tid * g.ntile[k] + s / g.tile[k];
       lid = lid * g.tile[k] + s % g.tile[k];
     }
-    const int kk = tid * tc + lid;
+    const int kk = key0 + tid * tc + lid;
     key[i] = kk;
     if (rank) rank[i] = atomicAdd(&key_count[kk], 1);
     else atomicAdd(&key_count[kk], 1);
@@ -351,6 +380,10 @@ __device__ inline void raw_locate(double v, double a, double b, int N, int* idx,
 //                   G[j] = exp(-hx^2 (j - MSP)^2 c), c = 1 / (4 tau)
 //   vals [n][nv]    y columns then probe columns
 //   rawi [n][d]     raw-grid interval, rawf [n][d] fraction in it
+// Batch: items give the geometry and the sorted range [pt, pt + n) (the
+// counting sort keeps every problem's points in its own range); x and vy are
+// indexed by batch row, pr by the row within the problem (one probe matrix
+// whose leading rows serve every problem).
 template <int D>
 __global__ __launch_bounds__(256) void k_pack(FlprGeom g, const double* __restrict__ x,
                                               const int32_t* __restrict__ order, int64_t n,
@@ -358,8 +391,22 @@ __global__ __launch_bounds__(256) void k_pack(FlprGeom g, const double* __restri
                                               const double* __restrict__ pr, int np,
                                               double* __restrict__ wab, double* __restrict__ vals,
                                               int32_t* __restrict__ rawi,
-                                              double* __restrict__ rawf) {
+                                              double* __restrict__ rawf, FlprBz bz) {
   const int nv = ny + np;
+  int64_t prow0 = 0;
+  if (bz.items) {
+    const FlprBatchItem& it = bz.items[blockIdx.z];
+    g = it.g;
+    n = it.n;
+    prow0 = it.pt;
+    order += it.pt;
+    if (wab) wab += it.pt * D * 2;
+    if (vals) vals += it.pt * nv;
+    if (rawi) {
+      rawi += it.pt * D;
+      rawf += it.pt * D;
+    }
+  }
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n && (wab || rawi);
        p += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = order[p];
@@ -391,7 +438,7 @@ __global__ __launch_bounds__(256) void k_pack(FlprGeom g, const double* __restri
     const int64_t p = e / nv;
     const int c = (int)(e - p * nv);
     const int64_t i = order[p];
-    vals[e] = (c < ny) ? vy[i * ny + c] : pr[i * np + (c - ny)];
+    vals[e] = (c < ny) ? vy[i * ny + c] : pr[(i - prow0) * np + (c - ny)];
   }
 }
 
@@ -424,6 +471,40 @@ __device__ inline void tap_weights(double A, double B, const double* G, double* 
   }
 }
 
+// Clears words of device memory with vector stores (a kernel node, unlike a
+// memset, replays correctly inside a captured HIP graph).
+__global__ __launch_bounds__(256) void k_fill_zero(uint32_t* __restrict__ p, int64_t words) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+// Points per work item the spread and adjoint aim for when a cell holds many
+// points (a cell's points are split over several items), and the number of
+// blocks below which channels are split over blocks.
+constexpr int ITEM_POINTS_1D = 8;
+constexpr int ITEM_POINTS = 16;
+constexpr int MIN_BLOCKS = 512;
+
+__host__ __device__ inline int item_splits(const FlprGeom* g, int64_t n) {
+  const int64_t nodes = flpr_nodes(g);
+  const int target = (g->d == 1) ? ITEM_POINTS_1D : ITEM_POINTS;
+  int64_t s = (n + nodes * target - 1) / (nodes * target);
+  if (s < 1) s = 1;
+  if (s > 64) s = 64;
+  return (int)s;
+}
+
+// Splits of a cell's points so that a block's work items cover its threads
+// (few channels per block, e.g. a single new y column): items of one split
+// number tile cells x rows x channels.
+__host__ __device__ inline int fill_splits(int nsplit, int64_t items_per_split, int threads) {
+  if (items_per_split < 1) return nsplit;
+  int64_t need = (threads + items_per_split - 1) / items_per_split;
+  if (need > 64) need = 64;
+  return (int)(nsplit > need ? nsplit : need);
+}
+
 constexpr int SPREAD_THREADS = 256;
 constexpr int SPREAD_MAX_TAPS = 2 * FLPR_MAX_MSP + 1;
 
@@ -443,12 +524,16 @@ struct SpreadTab {
 // (2 MSP + 1)^2 registers or fewer).  The block is added once into the tile's
 // LDS window, and the window is written to the tile's partial slot
 // (k_spread_gather sums the overlapping windows per cell, without atomics).
+// Batch: s[0] = key_start (the problem's key count), s[1] = part; a
+// problem's cell split follows its own point count (items).
 template <int D, int MSP>
 __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_cells(
     FlprGeom g, SpreadTab tab, const int32_t* __restrict__ key_start,
     const double* __restrict__ wab, const double* __restrict__ vals, int64_t n, int with_ones,
-    int C, int cg, int nsplit, double* __restrict__ part) {
+    int C, int cg, int nsplit, double* __restrict__ part, FlprBz bz) {
   extern __shared__ double lds[];
+  key_start += blockIdx.z * bz.s[0];
+  part += blockIdx.z * bz.s[1];
   constexpr int ntap = 2 * MSP + 1;
   constexpr int FIX = (D == 1) ? 0 : (D == 2 ? (MSP > 4 ? 1 : 0) : (MSP > 4 ? 2 : 1));
   constexpr int F = D - FIX;
@@ -470,6 +555,9 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_cells(
   for (int i = threadIdx.x; i < fcells * cn; i += blockDim.x) lds[i] = 0.0;
   __syncthreads();
   const int kbase = blockIdx.x * tc;
+  if (bz.items)
+    nsplit = fill_splits(item_splits(&g, bz.items[blockIdx.z].n),
+                         (int64_t)tc * ROWS * min(cg, C), SPREAD_THREADS);
   const int nitems = tc * nsplit * ROWS * cn;
   for (int item = threadIdx.x; item < nitems; item += blockDim.x) {
     const int c = item % cn;
@@ -575,12 +663,16 @@ __device__ inline int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a +
 
 // Sum of the overlapping tile windows for every spread-grid cell (channel =
 // blockIdx.y); also the per-channel grid sums used by the centring step.
+// Batch: s[0] = part, s[1] = grid, s[2] = sums.
 template <int D>
 __global__ __launch_bounds__(256) void k_spread_gather(FlprGeom g, const double* __restrict__ part,
                                                        int C, int64_t cells,
                                                        double* __restrict__ grid,
-                                                       double* __restrict__ sums) {
+                                                       double* __restrict__ sums, FlprBz bz) {
   __shared__ double red[4];
+  part += blockIdx.z * bz.s[0];
+  grid += blockIdx.z * bz.s[1];
+  sums += blockIdx.z * bz.s[2];
   const int c = blockIdx.y;
   const int msp = g.msp;
   int fp[3] = {1, 1, 1};
@@ -635,11 +727,16 @@ __global__ __launch_bounds__(256) void k_spread_gather(FlprGeom g, const double*
 // shift_c = sum F_c / sum F_0 (the kernel-weighted mean of the channel).  The
 // per-node solve adds shift_c times the ones-channel moments back, which is
 // exact by linearity; the float32 convolution then only carries y - shift.
+// Batch: s[0] = grid and grid32, s[1] = sums and shift.
 __global__ __launch_bounds__(256) void k_grid_center(double* __restrict__ grid, int64_t cells,
                                                      const double* __restrict__ sums,
                                                      double* __restrict__ shift,
-                                                     float* __restrict__ grid32) {
+                                                     float* __restrict__ grid32, FlprBz bz) {
   const int c = blockIdx.y;
+  grid += blockIdx.z * bz.s[0];
+  if (grid32) grid32 += blockIdx.z * bz.s[0];
+  sums += blockIdx.z * bz.s[1];
+  if (shift) shift += blockIdx.z * bz.s[1];
   const double s0 = sums[0];
   const double sh = (c == 0 || s0 == 0.0) ? 0.0 : sums[c] / s0;
   if (blockIdx.x == 0 && threadIdx.x == 0 && shift) shift[c] = sh;
@@ -747,13 +844,18 @@ constexpr int ADJ_THREADS = 256;
 // an item accumulates 4^d node (mode 0) or 3^d cell (mode 1) sums in
 // registers, then adds them into the tile's LDS window, which is written to
 // the tile's partial slot (k_adjoint_gather adds the overlapping windows).
+// Batch: s[0] = key_start (the problem's key count), s[1] = out, s[2] =
+// part; a problem's cell split follows its own point count (items).
 template <int D>
 __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(
     FlprGeom g, const int32_t* __restrict__ key_start, const int32_t* __restrict__ rawi,
     const double* __restrict__ rawf, const double* __restrict__ vals, int64_t n, int nch,
     int mode, int cg, int nsplit, double* __restrict__ out, int64_t out_stride,
-    double* __restrict__ part) {
+    double* __restrict__ part, FlprBz bz) {
   extern __shared__ double lds[];
+  key_start += blockIdx.z * bz.s[0];
+  out += blockIdx.z * bz.s[1];
+  part += blockIdx.z * bz.s[2];
   constexpr int NA = (D == 1) ? 4 : (D == 2 ? 16 : 64);   // 4^D node accumulators
   constexpr int NC = (D == 1) ? 3 : (D == 2 ? 9 : 27);    // 3^D cell accumulators
   const int c0 = blockIdx.y * cg;
@@ -772,6 +874,9 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(
   for (int i = threadIdx.x; i < wcells * cn; i += blockDim.x) lds[i] = 0.0;
   __syncthreads();
   const int kbase = blockIdx.x * tc;
+  if (bz.items)
+    nsplit = fill_splits(item_splits(&g, bz.items[blockIdx.z].n), (int64_t)tc * min(cg, nch),
+                         ADJ_THREADS);
   const int nitems = tc * nsplit * cn;
   // With an even number of nodes on every axis the raw nodes sit half a cell
   // off the spread cell centres, so a cell's points share one raw interval
@@ -971,11 +1076,14 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(
 
 // Adds the overlapping adjoint windows into out (nodes for mode 0, raw cells
 // for mode 1), one thread per (entry, channel = blockIdx.y), no atomics.
+// Batch: s[0] = part, s[1] = out.
 template <int D>
 __global__ __launch_bounds__(256) void k_adjoint_gather(FlprGeom g, const double* __restrict__ part,
                                                         int nch, int mode, int64_t count,
-                                                        double* __restrict__ out) {
+                                                        double* __restrict__ out, FlprBz bz) {
   const int c = blockIdx.y;
+  part += blockIdx.z * bz.s[0];
+  out += blockIdx.z * bz.s[1];
   int ext[3] = {1, 1, 1}, off[3] = {0, 0, 0}, lim[3] = {1, 1, 1};
   int wcells = 1;
 #pragma unroll
@@ -1025,13 +1133,17 @@ __global__ __launch_bounds__(256) void k_adjoint_gather(FlprGeom g, const double
 constexpr int QF_THREADS = 256;
 
 // quad[h][c] = sum_cell sum_{a,b} W_ab(cell) m(h,c,node_a) m(h,c,node_b)
+// Batch: s[0] = m, s[1] = gram, s[2] = quad.
 template <int D>
 __global__ __launch_bounds__(QF_THREADS) void k_quadform(FlprGeom g, const double* __restrict__ m,
                                                          int64_t nodes,
                                                          const double* __restrict__ gram,
                                                          int64_t rcells,
-                                                         double* __restrict__ quad) {
+                                                         double* __restrict__ quad, FlprBz bz) {
   __shared__ double red[QF_THREADS / 64];
+  m += blockIdx.z * bz.s[0];
+  gram += blockIdx.z * bz.s[1];
+  quad += blockIdx.z * bz.s[2];
   constexpr int NQ = 1 << D;
   const int hc = blockIdx.y;  // h * ncol + c
   const double* mm = m + (int64_t)hc * nodes;
@@ -1135,13 +1247,27 @@ __global__ __launch_bounds__(EVAL_THREADS) void k_interp_eval(FlprGeom g,
 // grid values exp(m) / d into grid, resid = yref - yhat.
 constexpr int FIT_BLOCKS = 1024;
 
+// Batch: items give the geometry and the point rows; s[0] = m_all, s[1] =
+// part; sel holds one index per problem.
 template <int D>
 __global__ __launch_bounds__(EVAL_THREADS) void k_fit_points(
     FlprGeom g, const double* __restrict__ x, int64_t n, const double* __restrict__ m_all,
     int ncol, int64_t nodes, const int32_t* __restrict__ sel, const double* __restrict__ yref,
     int variance, double* __restrict__ yhat, double* __restrict__ resid,
-    double* __restrict__ part) {
+    double* __restrict__ part, FlprBz bz) {
   __shared__ double sh[EVAL_THREADS / 64];
+  if (bz.items) {
+    const FlprBatchItem& it = bz.items[blockIdx.z];
+    g = it.g;
+    n = it.n;
+    x += it.pt * D;
+    if (yref) yref += it.pt * ncol;
+    yhat += it.pt * ncol;
+    if (resid) resid += it.pt * ncol;
+    m_all += blockIdx.z * bz.s[0];
+    if (sel) sel += blockIdx.z;
+    if (part) part += blockIdx.z * bz.s[1];
+  }
   const double* m = m_all + (sel ? (int64_t)sel[0] : 0) * ncol * nodes;
   double es = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -1166,12 +1292,26 @@ __global__ __launch_bounds__(EVAL_THREADS) void k_fit_points(
   }
 }
 
+// Batch: items give the point rows; s[0] = m_all, s[1] = part, s[2] = grid,
+// s[3] = fac_out; sel holds one index per problem.
 __global__ __launch_bounds__(EVAL_THREADS) void k_fit_finish(
     int64_t n, const double* __restrict__ m_all, int64_t nodes, const int32_t* __restrict__ sel,
     const double* __restrict__ yref, const double* __restrict__ part, int nparts,
     double* __restrict__ yhat, double* __restrict__ resid, double* __restrict__ grid,
-    double* __restrict__ fac_out) {
+    double* __restrict__ fac_out, FlprBz bz) {
   __shared__ double sh[EVAL_THREADS / 64];
+  if (bz.items) {
+    const FlprBatchItem& it = bz.items[blockIdx.z];
+    n = it.n;
+    yref += it.pt;
+    yhat += it.pt;
+    if (resid) resid += it.pt;
+    m_all += blockIdx.z * bz.s[0];
+    if (sel) sel += blockIdx.z;
+    part += blockIdx.z * bz.s[1];
+    grid += blockIdx.z * bz.s[2];
+    if (fac_out) fac_out += blockIdx.z * bz.s[3];
+  }
   __shared__ double fac_s;
   double p = 0.0;
   for (int b = threadIdx.x; b < nparts; b += blockDim.x) p += part[b];
@@ -1404,6 +1544,227 @@ __global__ __launch_bounds__(SPL_THREADS) void k_spline_eval(FlprGeom g,
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// GCV / DOF sums at the points for cubic (or mixed) interpolation
+// ---------------------------------------------------------------------------
+
+// One axis of the B-spline coefficient transform, batched over (problem,
+// grid) = blockIdx.z: out[(o Nk + i) inner + c] = sum_s A[i][s] in[(o Nk + s)
+// inner + c] with A the problem's inverse collocation matrix of that axis
+// (a + (z / ngrid) * a_stride, row-major Nk x Nk).  A block owns 64 rows i x
+// 64 columns j = (o, c); each thread 4 x 4 of them; A and in are staged
+// through LDS in chunks of 16 s.
+constexpr int AG_T = 64, AG_K = 16;
+
+__global__ __launch_bounds__(256) void k_axis_gemm(const double* __restrict__ a, int64_t a_stride,
+                                                   int ngrid, const double* __restrict__ in,
+                                                   double* __restrict__ out, int64_t nodes,
+                                                   int Nk, int64_t outer, int64_t inner) {
+  __shared__ double As[AG_K][AG_T + 1];
+  __shared__ double Bs[AG_K][AG_T + 1];
+  const int64_t z = blockIdx.z;
+  const double* A = a + (z / ngrid) * a_stride;
+  const double* src = in + z * nodes;
+  double* dst = out + z * nodes;
+  const int64_t ncol = outer * inner;
+  const int i0 = blockIdx.y * AG_T;
+  const int64_t j0 = (int64_t)blockIdx.x * AG_T;
+  const int t = threadIdx.x;
+  const int ti = t % 16, tj = t / 16;
+  double acc[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+  for (int s0 = 0; s0 < Nk; s0 += AG_K) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = t + 256 * q;
+      {
+        const int i = e / AG_K, sl = e % AG_K;
+        const int gi = i0 + i, gs = s0 + sl;
+        As[sl][i] = (gi < Nk && gs < Nk) ? A[(int64_t)gi * Nk + gs] : 0.0;
+      }
+      {
+        int sl, j;
+        if (inner == 1) {
+          j = e / AG_K;
+          sl = e % AG_K;
+        } else {
+          sl = e / AG_T;
+          j = e % AG_T;
+        }
+        const int64_t gj = j0 + j;
+        const int gs = s0 + sl;
+        double v = 0.0;
+        if (gj < ncol && gs < Nk) {
+          const int64_t o = gj / inner, c = gj - o * inner;
+          v = src[(o * Nk + gs) * inner + c];
+        }
+        Bs[sl][j] = v;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int sl = 0; sl < AG_K; ++sl) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) av[u] = As[sl][ti * 4 + u];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) bv[v] = Bs[sl][tj * 4 + v];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = fma(av[u], bv[v], acc[u][v]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int gi = i0 + ti * 4 + u;
+    if (gi >= Nk) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t gj = j0 + tj * 4 + v;
+      if (gj >= ncol) continue;
+      const int64_t o = gj / inner, c = gj - o * inner;
+      dst[(o * Nk + gi) * inner + c] = acc[u][v];
+    }
+  }
+}
+
+constexpr int PS_THREADS = 256;
+constexpr int PS_GROUP = 8;  // grids per block
+
+// Per-grid sums at the data points of one problem (blockIdx.z) for the grid
+// solutions m (dh, nval, nodes): value channels c < ny give the residual sums
+// sum_i (y_ic - yhat_ic)^2, the probe channels the Hutchinson dots sum_i
+// p_ik phat_ik (regression.py:1256-1280, api.py:661-723).  yhat / phat are
+// the cubic spline of the coefficients coef (gcv_cubic / dof_cubic) or the
+// multilinear interpolation of m.  blockIdx.y = group of PS_GROUP grids,
+// blockIdx.x = a share of the points; each block writes its PS_GROUP sums to
+// part, k_point_sums_final adds them in block order.
+template <int D>
+__global__ __launch_bounds__(PS_THREADS) void k_point_sums(
+    FlprGeom g, const double* __restrict__ x, int64_t n, const double* __restrict__ y, int ny,
+    const double* __restrict__ pr, int np, const double* __restrict__ m,
+    const double* __restrict__ coef, int64_t nodes, int dh, int gcv_cubic, int dof_cubic,
+    double* __restrict__ part, FlprBz bz) {
+  __shared__ double red[PS_THREADS / 64];
+  const int nval = ny + np;
+  const int ngrid = dh * nval;
+  int64_t pt = 0;
+  if (bz.items) {
+    const FlprBatchItem& it = bz.items[blockIdx.z];
+    g = it.g;
+    n = it.n;
+    pt = it.pt;
+  }
+  x += pt * D;
+  y += pt * ny;
+  m += blockIdx.z * bz.s[0];
+  coef += blockIdx.z * bz.s[0];
+  const int gq = blockIdx.y;
+  const int gl0 = gq * PS_GROUP;
+  const bool any_cubic = gcv_cubic || dof_cubic;
+  const bool any_linear = !gcv_cubic || !dof_cubic;
+  double acc[PS_GROUP];
+#pragma unroll
+  for (int q = 0; q < PS_GROUP; ++q) acc[q] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double bw[3][4];
+    int first[3] = {0, 0, 0};
+    if (any_cubic) {
+#pragma unroll
+      for (int k = 0; k < D; ++k)
+        first[k] = cubic_basis(x[i * D + k], g.graw0[k], g.graw1[k], g.n_grid[k], bw[k]);
+    }
+    CornersT<D> cr;
+    if (any_linear) cr = corners<D>(g, x + i * D);
+#pragma unroll
+    for (int q = 0; q < PS_GROUP; ++q) {
+      const int gi = gl0 + q;
+      if (gi >= ngrid) break;
+      const int c = gi % nval;
+      const bool cubic = (c < ny) ? gcv_cubic : dof_cubic;
+      double v = 0.0;
+      if (cubic) {
+        const double* cg = coef + (int64_t)gi * nodes;
+        if constexpr (D == 1) {
+#pragma unroll
+          for (int a = 0; a < 4; ++a) v += bw[0][a] * cg[first[0] + a];
+        } else if constexpr (D == 2) {
+          const int N1 = g.n_grid[1];
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            double r = 0.0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+              r += bw[1][b] * cg[(int64_t)(first[0] + a) * N1 + first[1] + b];
+            v += bw[0][a] * r;
+          }
+        } else {
+          const int N1 = g.n_grid[1], N2 = g.n_grid[2];
+#pragma unroll
+          for (int a = 0; a < 4; ++a) {
+            double r = 0.0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+              double qv = 0.0;
+              const int64_t row = ((int64_t)(first[0] + a) * N1 + first[1] + b) * N2 + first[2];
+#pragma unroll
+              for (int e = 0; e < 4; ++e) qv += bw[2][e] * cg[row + e];
+              r += bw[1][b] * qv;
+            }
+            v += bw[0][a] * r;
+          }
+        }
+      } else {
+        const double* mg = m + (int64_t)gi * nodes;
+#pragma unroll
+        for (int qq = 0; qq < (1 << D); ++qq) v += cr.w[qq] * mg[cr.node[qq]];
+      }
+      if (c < ny) {
+        const double r = y[i * ny + c] - v;
+        acc[q] += r * r;
+      } else {
+        acc[q] += pr[i * np + (c - ny)] * v;
+      }
+    }
+  }
+  double* dst = part + (((int64_t)blockIdx.z * gridDim.y + gq) * gridDim.x + blockIdx.x) * PS_GROUP;
+#pragma unroll
+  for (int q = 0; q < PS_GROUP; ++q) {
+    const double t = block_sum(acc[q], red);
+    if (threadIdx.x == 0) dst[q] = t;
+  }
+}
+
+// Adds the blocks' sums in block order and writes them where flpr_select
+// reads them: the probe dots at red[h nval + c], the residual sums at
+// red[dh nval + h ny + c] (the y dots stay zero: with residual sums in place
+// of the quadratic forms and zero sums of squares, select forms RSS itself).
+__global__ __launch_bounds__(256) void k_point_sums_final(const double* __restrict__ part,
+                                                          int nblk, int ngroups, int dh, int ny,
+                                                          int np, double* __restrict__ red,
+                                                          int64_t red_stride) {
+  const int nval = ny + np;
+  const int ngrid = dh * nval;
+  const int64_t z = blockIdx.z;
+  for (int gi = blockIdx.x * blockDim.x + threadIdx.x; gi < ngrid; gi += gridDim.x * blockDim.x) {
+    const int gq = gi / PS_GROUP, q = gi % PS_GROUP;
+    const double* src = part + ((z * ngroups + gq) * nblk) * PS_GROUP + q;
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += src[(int64_t)b * PS_GROUP];
+    const int h = gi / nval, c = gi % nval;
+    double* r = red + z * red_stride;
+    if (c < ny) r[(int64_t)dh * nval + h * ny + c] = s;
+    else r[(int64_t)h * nval + c] = s;
+  }
+}
+
 int lds_group(int64_t cells_per_channel, int channels, int cap, int64_t budget = 64 * 1024) {
   const int64_t per = cells_per_channel * (int64_t)sizeof(double);
   int cg = (int)(budget / per);
@@ -1416,11 +1777,11 @@ int lds_group(int64_t cells_per_channel, int channels, int cap, int64_t budget =
 template <int D>
 void launch_spread(int msp, dim3 grd, size_t lds, hipStream_t st, const FlprGeom& g,
                    const SpreadTab& tab, const int32_t* ks, const double* wab, const double* vals,
-                   int64_t n, int wo, int C, int cg, int nsplit, double* part) {
+                   int64_t n, int wo, int C, int cg, int nsplit, double* part, const FlprBz& bz) {
 #define FLPR_SPREAD_CASE(M)                                                                    \
   case M:                                                                                      \
     hipLaunchKernelGGL((k_spread_cells<D, M>), grd, dim3(SPREAD_THREADS), lds, st, g, tab, ks, \
-                       wab, vals, n, wo, C, cg, nsplit, part);                                 \
+                       wab, vals, n, wo, C, cg, nsplit, part, bz);                             \
     break;
   switch (msp) {
     FLPR_SPREAD_CASE(0)
@@ -1446,31 +1807,6 @@ void launch_spread(int msp, dim3 grd, size_t lds, hipStream_t st, const FlprGeom
     }
   }
 #undef FLPR_SPREAD_CASE
-}
-
-// Points per work item the spread and adjoint aim for when a cell holds many
-// points (a cell's points are split over several items), and the number of
-// blocks below which channels are split over blocks.
-constexpr int ITEM_POINTS_1D = 8;
-constexpr int ITEM_POINTS = 16;
-constexpr int MIN_BLOCKS = 512;
-
-int item_splits(const FlprGeom* g, int64_t n) {
-  const int64_t nodes = flpr_nodes(g);
-  const int target = (g->d == 1) ? ITEM_POINTS_1D : ITEM_POINTS;
-  int64_t s = (n + nodes * target - 1) / (nodes * target);
-  if (s < 1) s = 1;
-  if (s > 64) s = 64;
-  return (int)s;
-}
-
-// Splits of a cell's points so that a block's work items cover its threads
-// (few channels per block, e.g. a single new y column): items of one split
-// number tile cells x rows x channels.
-int fill_splits(int nsplit, int64_t items_per_split, int threads) {
-  if (items_per_split < 1) return nsplit;
-  const int64_t need = (threads + items_per_split - 1) / items_per_split;
-  return (int)std::max<int64_t>(nsplit, std::min<int64_t>(need, 64));
 }
 
 // Channels per block: all of them when the tiles alone fill the device, else
@@ -1508,10 +1844,10 @@ int flpr_xstats(const double* x, int64_t n, int32_t d, const double* y, int32_t 
       ny > MAXY || (ny > 0 && !y))
     return -1;
   hipLaunchKernelGGL(k_xstats, dim3(XS_BLOCKS), dim3(XS_THREADS), 0, as_stream(stream), x, n, d,
-                     y, ny, partials);
+                     y, ny, partials, FlprBz{});
   FLPR_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_xstats_final, dim3(1), dim3(XS_THREADS), 0, as_stream(stream), partials,
-                     XS_BLOCKS, d, ny, out);
+                     XS_BLOCKS, d, ny, out, FlprBz{});
   FLPR_CHECK_LAUNCH();
   return 0;
 }
@@ -1571,10 +1907,10 @@ int flpr_xstats_noalloc(const double* x, int64_t n, int32_t d, const double* y, 
   // k_xstats writes every partial slot before k_xstats_final reads it, so
   // the contents left by an earlier call need no clearing
   hipLaunchKernelGGL(k_xstats, dim3(XS_BLOCKS), dim3(XS_THREADS), 0, as_stream(stream), x, n, d,
-                     y, ny, part);
+                     y, ny, part, FlprBz{});
   FLPR_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_xstats_final, dim3(1), dim3(XS_THREADS), 0, as_stream(stream), part,
-                     XS_BLOCKS, d, ny, out);
+                     XS_BLOCKS, d, ny, out, FlprBz{});
   FLPR_CHECK_LAUNCH();
   return 0;
 }
@@ -1584,8 +1920,13 @@ int flpr_points_prepare(const double* x, int64_t n, const FlprGeom* g, int32_t* 
   FLPR_CLEAR_ERROR();
   if (!x || !key || !key_count || n < 1 || !flpr_geom_ok(g)) return -1;
   if (flpr_nkeys(g) > ((int64_t)1 << 30)) return -1;
-  hipLaunchKernelGGL(k_prepare, dim3(flpr_grid_for(n, PREP_THREADS, 4096)), dim3(PREP_THREADS), 0,
-                     as_stream(stream), x, n, *g, key, key_count, rank);
+  dim3 grd(flpr_grid_for(n, PREP_THREADS, 4096));
+  hipStream_t st = as_stream(stream);
+  switch (g->d) {
+    case 1: hipLaunchKernelGGL(k_prepare<1>, grd, dim3(PREP_THREADS), 0, st, x, n, *g, key, key_count, rank, FlprBz{}); break;
+    case 2: hipLaunchKernelGGL(k_prepare<2>, grd, dim3(PREP_THREADS), 0, st, x, n, *g, key, key_count, rank, FlprBz{}); break;
+    default: hipLaunchKernelGGL(k_prepare<3>, grd, dim3(PREP_THREADS), 0, st, x, n, *g, key, key_count, rank, FlprBz{}); break;
+  }
   FLPR_CHECK_LAUNCH();
   return 0;
 }
@@ -1627,9 +1968,9 @@ int flpr_points_pack(const FlprGeom* g, const double* x, const int32_t* order, i
   dim3 grd(flpr_grid_for(n, 256, 4096));
   hipStream_t st = as_stream(stream);
   switch (g->d) {
-    case 1: hipLaunchKernelGGL(k_pack<1>, grd, dim3(256), 0, st, *g, x, order, n, vals_y, ny, probes, nprobe, wab, vals, rawi, rawf); break;
-    case 2: hipLaunchKernelGGL(k_pack<2>, grd, dim3(256), 0, st, *g, x, order, n, vals_y, ny, probes, nprobe, wab, vals, rawi, rawf); break;
-    default: hipLaunchKernelGGL(k_pack<3>, grd, dim3(256), 0, st, *g, x, order, n, vals_y, ny, probes, nprobe, wab, vals, rawi, rawf); break;
+    case 1: hipLaunchKernelGGL(k_pack<1>, grd, dim3(256), 0, st, *g, x, order, n, vals_y, ny, probes, nprobe, wab, vals, rawi, rawf, FlprBz{}); break;
+    case 2: hipLaunchKernelGGL(k_pack<2>, grd, dim3(256), 0, st, *g, x, order, n, vals_y, ny, probes, nprobe, wab, vals, rawi, rawf, FlprBz{}); break;
+    default: hipLaunchKernelGGL(k_pack<3>, grd, dim3(256), 0, st, *g, x, order, n, vals_y, ny, probes, nprobe, wab, vals, rawi, rawf, FlprBz{}); break;
   }
   FLPR_CHECK_LAUNCH();
   return 0;
@@ -1675,16 +2016,16 @@ int flpr_spread(const FlprGeom* g, const int32_t* key_start, const double* wab,
   hipStream_t st = as_stream(stream);
   const int msp = g->msp;
   switch (g->d) {
-    case 1: launch_spread<1>(msp, grd, lds, st, *g, tab, key_start, wab, vals, n, wo, C, cg, nsplit, part); break;
-    case 2: launch_spread<2>(msp, grd, lds, st, *g, tab, key_start, wab, vals, n, wo, C, cg, nsplit, part); break;
-    default: launch_spread<3>(msp, grd, lds, st, *g, tab, key_start, wab, vals, n, wo, C, cg, nsplit, part); break;
+    case 1: launch_spread<1>(msp, grd, lds, st, *g, tab, key_start, wab, vals, n, wo, C, cg, nsplit, part, FlprBz{}); break;
+    case 2: launch_spread<2>(msp, grd, lds, st, *g, tab, key_start, wab, vals, n, wo, C, cg, nsplit, part, FlprBz{}); break;
+    default: launch_spread<3>(msp, grd, lds, st, *g, tab, key_start, wab, vals, n, wo, C, cg, nsplit, part, FlprBz{}); break;
   }
   FLPR_CHECK_LAUNCH();
   dim3 gg(flpr_grid_for(cells, 256, 256), (unsigned)C);
   switch (g->d) {
-    case 1: hipLaunchKernelGGL(k_spread_gather<1>, gg, dim3(256), 0, st, *g, part, C, cells, grid, sums); break;
-    case 2: hipLaunchKernelGGL(k_spread_gather<2>, gg, dim3(256), 0, st, *g, part, C, cells, grid, sums); break;
-    default: hipLaunchKernelGGL(k_spread_gather<3>, gg, dim3(256), 0, st, *g, part, C, cells, grid, sums); break;
+    case 1: hipLaunchKernelGGL(k_spread_gather<1>, gg, dim3(256), 0, st, *g, part, C, cells, grid, sums, FlprBz{}); break;
+    case 2: hipLaunchKernelGGL(k_spread_gather<2>, gg, dim3(256), 0, st, *g, part, C, cells, grid, sums, FlprBz{}); break;
+    default: hipLaunchKernelGGL(k_spread_gather<3>, gg, dim3(256), 0, st, *g, part, C, cells, grid, sums, FlprBz{}); break;
   }
   FLPR_CHECK_LAUNCH();
   return 0;
@@ -1725,7 +2066,7 @@ int flpr_grid_center(double* grid, int32_t C, int64_t cells, const double* sums,
   if (!grid || !sums || C < 1 || cells < 1) return -1;
   dim3 grd(flpr_grid_for(cells, 256, 256), (unsigned)C);
   hipLaunchKernelGGL(k_grid_center, grd, dim3(256), 0, as_stream(stream), grid, cells, sums, shift,
-                     grid32);
+                     grid32, FlprBz{});
   FLPR_CHECK_LAUNCH();
   return 0;
 }
@@ -1760,16 +2101,16 @@ int flpr_adjoint(const FlprGeom* g, const int32_t* key_start, const int32_t* raw
     if (lds > 64 * 1024) return -1;
     dim3 grd((unsigned)flpr_ntiles(g), (unsigned)((nch + cg - 1) / cg));
     switch (g->d) {
-      case 1: hipLaunchKernelGGL(k_adjoint<1>, grd, dim3(ADJ_THREADS), lds, st, *g, key_start, rawi, rawf, vals, n, nch, mode, cg, nsplit, out, ostride, part); break;
-      case 2: hipLaunchKernelGGL(k_adjoint<2>, grd, dim3(ADJ_THREADS), lds, st, *g, key_start, rawi, rawf, vals, n, nch, mode, cg, nsplit, out, ostride, part); break;
-      default: hipLaunchKernelGGL(k_adjoint<3>, grd, dim3(ADJ_THREADS), lds, st, *g, key_start, rawi, rawf, vals, n, nch, mode, cg, nsplit, out, ostride, part); break;
+      case 1: hipLaunchKernelGGL(k_adjoint<1>, grd, dim3(ADJ_THREADS), lds, st, *g, key_start, rawi, rawf, vals, n, nch, mode, cg, nsplit, out, ostride, part, FlprBz{}); break;
+      case 2: hipLaunchKernelGGL(k_adjoint<2>, grd, dim3(ADJ_THREADS), lds, st, *g, key_start, rawi, rawf, vals, n, nch, mode, cg, nsplit, out, ostride, part, FlprBz{}); break;
+      default: hipLaunchKernelGGL(k_adjoint<3>, grd, dim3(ADJ_THREADS), lds, st, *g, key_start, rawi, rawf, vals, n, nch, mode, cg, nsplit, out, ostride, part, FlprBz{}); break;
     }
     FLPR_CHECK_LAUNCH();
     dim3 gg(flpr_grid_for(ostride, 256, 256), (unsigned)nch);
     switch (g->d) {
-      case 1: hipLaunchKernelGGL(k_adjoint_gather<1>, gg, dim3(256), 0, st, *g, part, nch, mode, ostride, out); break;
-      case 2: hipLaunchKernelGGL(k_adjoint_gather<2>, gg, dim3(256), 0, st, *g, part, nch, mode, ostride, out); break;
-      default: hipLaunchKernelGGL(k_adjoint_gather<3>, gg, dim3(256), 0, st, *g, part, nch, mode, ostride, out); break;
+      case 1: hipLaunchKernelGGL(k_adjoint_gather<1>, gg, dim3(256), 0, st, *g, part, nch, mode, ostride, out, FlprBz{}); break;
+      case 2: hipLaunchKernelGGL(k_adjoint_gather<2>, gg, dim3(256), 0, st, *g, part, nch, mode, ostride, out, FlprBz{}); break;
+      default: hipLaunchKernelGGL(k_adjoint_gather<3>, gg, dim3(256), 0, st, *g, part, nch, mode, ostride, out, FlprBz{}); break;
     }
     FLPR_CHECK_LAUNCH();
   }
@@ -1786,9 +2127,9 @@ int flpr_quadform(const FlprGeom* g, const double* m, int32_t ncol, int32_t dh,
   dim3 grd(flpr_grid_for(rcells, QF_THREADS, 64), (unsigned)(dh * ncol));
   hipStream_t st = as_stream(stream);
   switch (g->d) {
-    case 1: hipLaunchKernelGGL(k_quadform<1>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad); break;
-    case 2: hipLaunchKernelGGL(k_quadform<2>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad); break;
-    default: hipLaunchKernelGGL(k_quadform<3>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad); break;
+    case 1: hipLaunchKernelGGL(k_quadform<1>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad, FlprBz{}); break;
+    case 2: hipLaunchKernelGGL(k_quadform<2>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad, FlprBz{}); break;
+    default: hipLaunchKernelGGL(k_quadform<3>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad, FlprBz{}); break;
   }
   FLPR_CHECK_LAUNCH();
   return 0;
@@ -1824,14 +2165,14 @@ int flpr_fit_output(const FlprGeom* g, const double* x, int64_t n, const double*
   hipStream_t st = as_stream(stream);
   const int var = variance ? 1 : 0;
   switch (g->d) {
-    case 1: hipLaunchKernelGGL(k_fit_points<1>, grd, dim3(EVAL_THREADS), 0, st, *g, x, n, m_all, ncol, nodes, sel, yref, var, yhat, resid, part); break;
-    case 2: hipLaunchKernelGGL(k_fit_points<2>, grd, dim3(EVAL_THREADS), 0, st, *g, x, n, m_all, ncol, nodes, sel, yref, var, yhat, resid, part); break;
-    default: hipLaunchKernelGGL(k_fit_points<3>, grd, dim3(EVAL_THREADS), 0, st, *g, x, n, m_all, ncol, nodes, sel, yref, var, yhat, resid, part); break;
+    case 1: hipLaunchKernelGGL(k_fit_points<1>, grd, dim3(EVAL_THREADS), 0, st, *g, x, n, m_all, ncol, nodes, sel, yref, var, yhat, resid, part, FlprBz{}); break;
+    case 2: hipLaunchKernelGGL(k_fit_points<2>, grd, dim3(EVAL_THREADS), 0, st, *g, x, n, m_all, ncol, nodes, sel, yref, var, yhat, resid, part, FlprBz{}); break;
+    default: hipLaunchKernelGGL(k_fit_points<3>, grd, dim3(EVAL_THREADS), 0, st, *g, x, n, m_all, ncol, nodes, sel, yref, var, yhat, resid, part, FlprBz{}); break;
   }
   FLPR_CHECK_LAUNCH();
   if (variance) {
     hipLaunchKernelGGL(k_fit_finish, grd, dim3(EVAL_THREADS), 0, st, n, m_all, nodes, sel, yref,
-                       part, (int)grd.x, yhat, resid, grid, fac);
+                       part, (int)grd.x, yhat, resid, grid, fac, FlprBz{});
     FLPR_CHECK_LAUNCH();
   }
   return 0;
@@ -1871,6 +2212,347 @@ int flpr_spline_eval(const FlprGeom* g, const double* x, int64_t n, const double
     case 2: hipLaunchKernelGGL(k_spline_eval<2>, grd, dim3(SPL_THREADS), 0, st, *g, x, n, coef, ngrid, nodes, out); break;
     default: hipLaunchKernelGGL(k_spline_eval<3>, grd, dim3(SPL_THREADS), 0, st, *g, x, n, coef, ngrid, nodes, out); break;
   }
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// batches of independent problems: the single-problem kernels with the
+// problem on blockIdx.z
+// ---------------------------------------------------------------------------
+
+static bool batch_ok(const FlprGeom* g, const FlprBatchItem* items, int32_t P, int64_t nmax) {
+  return flpr_geom_ok(g) && items && P >= 1 && P <= 65535 && nmax >= 1;
+}
+
+int flpr_fill_zero(void* p, int64_t bytes, void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!p || bytes < 0 || (bytes & 3)) return -1;
+  if (bytes == 0) return 0;
+  const int64_t words = bytes / 4;
+  hipLaunchKernelGGL(k_fill_zero, dim3(flpr_grid_for(words, 256, 4096)), dim3(256), 0,
+                     as_stream(stream), (uint32_t*)p, words);
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+int flpr_batch_xstats(const FlprBatchItem* items, int32_t P, const double* x, int32_t d,
+                      const double* y, int32_t ny, double* partials, double* out, void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!items || P < 1 || P > 65535 || (d > 0 && !x) || !partials || !out || d < 0 ||
+      d > FLPR_MAX_DIM || ny < 0 || ny > MAXY || (ny > 0 && !y) || d + ny < 1)
+    return -1;
+  const int per = 5 * d + 2 + ny;
+  FlprBz bz{};
+  bz.items = items;
+  bz.s[0] = (int64_t)XS_BLOCKS * per;
+  hipLaunchKernelGGL(k_xstats, dim3(XS_BLOCKS, 1, P), dim3(XS_THREADS), 0, as_stream(stream), x,
+                     (int64_t)0, d, y, ny, partials, bz);
+  FLPR_CHECK_LAUNCH();
+  FlprBz bf{};
+  bf.s[0] = (int64_t)XS_BLOCKS * per;
+  bf.s[1] = per;
+  hipLaunchKernelGGL(k_xstats_final, dim3(1, 1, P), dim3(XS_THREADS), 0, as_stream(stream),
+                     partials, XS_BLOCKS, d, ny, out, bf);
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+int flpr_batch_points_prepare(const FlprGeom* g, const FlprBatchItem* items, int32_t P,
+                              int64_t nmax, const double* x, int32_t* key, int32_t* key_count,
+                              int32_t* rank, void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!batch_ok(g, items, P, nmax) || !x || !key || !key_count) return -1;
+  if (flpr_nkeys(g) * P > ((int64_t)1 << 30)) return -1;
+  FlprBz bz{};
+  bz.items = items;
+  bz.s[0] = flpr_nkeys(g);
+  dim3 grd(flpr_grid_for(nmax, PREP_THREADS, 4096), 1, P);
+  hipStream_t st = as_stream(stream);
+  switch (g->d) {
+    case 1: hipLaunchKernelGGL(k_prepare<1>, grd, dim3(PREP_THREADS), 0, st, x, nmax, *g, key, key_count, rank, bz); break;
+    case 2: hipLaunchKernelGGL(k_prepare<2>, grd, dim3(PREP_THREADS), 0, st, x, nmax, *g, key, key_count, rank, bz); break;
+    default: hipLaunchKernelGGL(k_prepare<3>, grd, dim3(PREP_THREADS), 0, st, x, nmax, *g, key, key_count, rank, bz); break;
+  }
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+int flpr_batch_points_pack(const FlprGeom* g, const FlprBatchItem* items, int32_t P, int64_t nmax,
+                           const double* x, const int32_t* order, const double* vals_y, int32_t ny,
+                           const double* probes, int32_t nprobe, double* wab, double* vals,
+                           int32_t* rawi, double* rawf, void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!batch_ok(g, items, P, nmax) || !x || !order || (!rawi != !rawf) || ny < 0 ||
+      nprobe < 0 || ((ny + nprobe) > 0 && !vals) || (ny > 0 && !vals_y) ||
+      (nprobe > 0 && !probes))
+    return -1;
+  FlprBz bz{};
+  bz.items = items;
+  dim3 grd(flpr_grid_for(nmax * std::max(1, ny + nprobe), 256, 4096), 1, P);
+  hipStream_t st = as_stream(stream);
+  switch (g->d) {
+    case 1: hipLaunchKernelGGL(k_pack<1>, grd, dim3(256), 0, st, *g, x, order, nmax, vals_y, ny, probes, nprobe, wab, vals, rawi, rawf, bz); break;
+    case 2: hipLaunchKernelGGL(k_pack<2>, grd, dim3(256), 0, st, *g, x, order, nmax, vals_y, ny, probes, nprobe, wab, vals, rawi, rawf, bz); break;
+    default: hipLaunchKernelGGL(k_pack<3>, grd, dim3(256), 0, st, *g, x, order, nmax, vals_y, ny, probes, nprobe, wab, vals, rawi, rawf, bz); break;
+  }
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+int flpr_batch_spread(const FlprGeom* g, const FlprBatchItem* items, int32_t P, int64_t nmax,
+                      const int32_t* key_start, const double* wab, const double* vals,
+                      int32_t with_ones, int32_t nval, double* part, double* grid,
+                      int64_t grid_stride, double* sums, void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!batch_ok(g, items, P, nmax) || !key_start || !wab || !part || !grid || !sums ||
+      nval < 0 || (nval > 0 && !vals))
+    return -1;
+  if (g->msp > (g->d == 3 ? 6 : FLPR_MAX_MSP)) return -1;
+  const int wo = with_ones ? 1 : 0;
+  const int C = wo + nval;
+  if (C < 1 || C > FLPR_MAX_CHANNELS) return -1;
+  int64_t fcells = 1, cells = 1;
+  for (int k = 0; k < g->d; ++k) {
+    fcells *= g->tile[k] + 2 * g->msp;
+    cells *= g->G[k];
+  }
+  if (grid_stride < C * cells) return -1;
+  const int64_t ntiles = flpr_ntiles(g);
+  // the channel grouping of one problem: every problem of the batch is
+  // spread exactly as a single call would spread it
+  const int cg = channel_group(ntiles, fcells, C);
+  const size_t lds = (size_t)fcells * cg * sizeof(double);
+  if (lds > 64 * 1024) return -1;
+  SpreadTab tab;
+  for (int k = 0; k < FLPR_MAX_DIM; ++k)
+    for (int j = 0; j < SPREAD_MAX_TAPS; ++j) {
+      const double c2 = (k < g->d) ? g->hx[k] * g->hx[k] * g->inv4tau[k] : 0.0;
+      const double o = (double)(j - g->msp);
+      tab.G[k][j] = exp(-c2 * o * o);
+    }
+  FlprBz bz{};
+  bz.items = items;
+  bz.s[0] = flpr_nkeys(g);
+  bz.s[1] = ntiles * C * fcells;
+  dim3 grd((unsigned)ntiles, (unsigned)((C + cg - 1) / cg), (unsigned)P);
+  hipStream_t st = as_stream(stream);
+  const int msp = g->msp;
+  switch (g->d) {
+    case 1: launch_spread<1>(msp, grd, lds, st, *g, tab, key_start, wab, vals, nmax, wo, C, cg, 1, part, bz); break;
+    case 2: launch_spread<2>(msp, grd, lds, st, *g, tab, key_start, wab, vals, nmax, wo, C, cg, 1, part, bz); break;
+    default: launch_spread<3>(msp, grd, lds, st, *g, tab, key_start, wab, vals, nmax, wo, C, cg, 1, part, bz); break;
+  }
+  FLPR_CHECK_LAUNCH();
+  FlprBz bg{};
+  bg.s[0] = ntiles * C * fcells;
+  bg.s[1] = grid_stride;
+  bg.s[2] = C;
+  dim3 gg(flpr_grid_for(cells, 256, 256), (unsigned)C, (unsigned)P);
+  switch (g->d) {
+    case 1: hipLaunchKernelGGL(k_spread_gather<1>, gg, dim3(256), 0, st, *g, part, C, cells, grid, sums, bg); break;
+    case 2: hipLaunchKernelGGL(k_spread_gather<2>, gg, dim3(256), 0, st, *g, part, C, cells, grid, sums, bg); break;
+    default: hipLaunchKernelGGL(k_spread_gather<3>, gg, dim3(256), 0, st, *g, part, C, cells, grid, sums, bg); break;
+  }
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+int flpr_batch_grid_center(int32_t P, double* grid, int64_t grid_stride, int32_t C, int64_t cells,
+                           const double* sums, double* shift, float* grid32, void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (P < 1 || P > 65535 || !grid || !sums || C < 1 || cells < 1 || grid_stride < C * cells)
+    return -1;
+  FlprBz bz{};
+  bz.s[0] = grid_stride;
+  bz.s[1] = C;
+  dim3 grd(flpr_grid_for(cells, 256, 256), (unsigned)C, (unsigned)P);
+  hipLaunchKernelGGL(k_grid_center, grd, dim3(256), 0, as_stream(stream), grid, cells, sums, shift,
+                     grid32, bz);
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+int flpr_batch_adjoint(const FlprGeom* g, const FlprBatchItem* items, int32_t P, int64_t nmax,
+                       const int32_t* key_start, const int32_t* rawi, const double* rawf,
+                       const double* vals, int32_t nval, double* part, double* q, double* gram,
+                       void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!batch_ok(g, items, P, nmax) || !key_start || !rawi || !rawf || !part || nval < 0 ||
+      (nval > 0 && (!vals || !q)))
+    return -1;
+  int64_t wcells = 1, nodes = 1, rcells = 1;
+  for (int k = 0; k < g->d; ++k) {
+    wcells *= g->tile[k] + 5;
+    nodes *= g->n_grid[k];
+    rcells *= g->n_grid[k] - 1;
+  }
+  const int nq = 1 << g->d;
+  const int nw = nq * (nq + 1) / 2;
+  const int64_t ntiles = flpr_ntiles(g);
+  const int64_t part_stride = ntiles * std::max(nval, nw) * wcells;
+  hipStream_t st = as_stream(stream);
+  for (int mode = 0; mode < 2; ++mode) {
+    const int nch = (mode == 0) ? nval : (gram ? nw : 0);
+    if (nch == 0) continue;
+    double* out = (mode == 0) ? q : gram;
+    const int64_t ostride = (mode == 0) ? nodes : rcells;
+    const int cg = channel_group(ntiles, wcells, nch);
+    const size_t lds = (size_t)wcells * cg * sizeof(double);
+    if (lds > 64 * 1024) return -1;
+    FlprBz bz{};
+    bz.items = items;
+    bz.s[0] = flpr_nkeys(g);
+    bz.s[1] = (int64_t)nch * ostride;
+    bz.s[2] = part_stride;
+    dim3 grd((unsigned)ntiles, (unsigned)((nch + cg - 1) / cg), (unsigned)P);
+    switch (g->d) {
+      case 1: hipLaunchKernelGGL(k_adjoint<1>, grd, dim3(ADJ_THREADS), lds, st, *g, key_start, rawi, rawf, vals, nmax, nch, mode, cg, 1, out, ostride, part, bz); break;
+      case 2: hipLaunchKernelGGL(k_adjoint<2>, grd, dim3(ADJ_THREADS), lds, st, *g, key_start, rawi, rawf, vals, nmax, nch, mode, cg, 1, out, ostride, part, bz); break;
+      default: hipLaunchKernelGGL(k_adjoint<3>, grd, dim3(ADJ_THREADS), lds, st, *g, key_start, rawi, rawf, vals, nmax, nch, mode, cg, 1, out, ostride, part, bz); break;
+    }
+    FLPR_CHECK_LAUNCH();
+    FlprBz bg{};
+    bg.s[0] = part_stride;
+    bg.s[1] = (int64_t)nch * ostride;
+    dim3 gg(flpr_grid_for(ostride, 256, 256), (unsigned)nch, (unsigned)P);
+    switch (g->d) {
+      case 1: hipLaunchKernelGGL(k_adjoint_gather<1>, gg, dim3(256), 0, st, *g, part, nch, mode, ostride, out, bg); break;
+      case 2: hipLaunchKernelGGL(k_adjoint_gather<2>, gg, dim3(256), 0, st, *g, part, nch, mode, ostride, out, bg); break;
+      default: hipLaunchKernelGGL(k_adjoint_gather<3>, gg, dim3(256), 0, st, *g, part, nch, mode, ostride, out, bg); break;
+    }
+    FLPR_CHECK_LAUNCH();
+  }
+  return 0;
+}
+
+int flpr_batch_quadform(const FlprGeom* g, int32_t P, const double* m, int64_t m_stride,
+                        int32_t ncol, int32_t dh, const double* gram, double* quad,
+                        int64_t quad_stride, void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!flpr_geom_ok(g) || P < 1 || P > 65535 || !m || !gram || !quad || ncol < 1 || dh < 1)
+    return -1;
+  const int64_t nodes = flpr_nodes(g);
+  int64_t rcells = 1;
+  for (int k = 0; k < g->d; ++k) rcells *= g->n_grid[k] - 1;
+  const int nq = 1 << g->d;
+  FlprBz bz{};
+  bz.s[0] = m_stride;
+  bz.s[1] = (int64_t)(nq * (nq + 1) / 2) * rcells;
+  bz.s[2] = quad_stride;
+  dim3 grd(flpr_grid_for(rcells, QF_THREADS, 64), (unsigned)(dh * ncol), (unsigned)P);
+  hipStream_t st = as_stream(stream);
+  switch (g->d) {
+    case 1: hipLaunchKernelGGL(k_quadform<1>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad, bz); break;
+    case 2: hipLaunchKernelGGL(k_quadform<2>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad, bz); break;
+    default: hipLaunchKernelGGL(k_quadform<3>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad, bz); break;
+  }
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+int flpr_batch_fit_output(const FlprGeom* g, const FlprBatchItem* items, int32_t P, int64_t nmax,
+                          const double* x, const double* m_all, int64_t m_stride, int32_t ncol,
+                          const int32_t* sel, const double* yref, int32_t variance, double* yhat,
+                          double* resid, double* grid, double* part, double* fac,
+                          int64_t fac_stride, void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!batch_ok(g, items, P, nmax) || !x || !m_all || !yhat || ncol < 1 || (resid && !yref) ||
+      (variance && (ncol != 1 || !yref || !grid || !part)))
+    return -1;
+  const unsigned gx = (unsigned)flpr_grid_for(nmax, EVAL_THREADS, FIT_BLOCKS);
+  const int64_t nodes = flpr_nodes(g);
+  hipStream_t st = as_stream(stream);
+  const int var = variance ? 1 : 0;
+  FlprBz bz{};
+  bz.items = items;
+  bz.s[0] = m_stride;
+  bz.s[1] = gx;
+  dim3 grd(gx, 1, (unsigned)P);
+  switch (g->d) {
+    case 1: hipLaunchKernelGGL(k_fit_points<1>, grd, dim3(EVAL_THREADS), 0, st, *g, x, nmax, m_all, ncol, nodes, sel, yref, var, yhat, resid, part, bz); break;
+    case 2: hipLaunchKernelGGL(k_fit_points<2>, grd, dim3(EVAL_THREADS), 0, st, *g, x, nmax, m_all, ncol, nodes, sel, yref, var, yhat, resid, part, bz); break;
+    default: hipLaunchKernelGGL(k_fit_points<3>, grd, dim3(EVAL_THREADS), 0, st, *g, x, nmax, m_all, ncol, nodes, sel, yref, var, yhat, resid, part, bz); break;
+  }
+  FLPR_CHECK_LAUNCH();
+  if (variance) {
+    FlprBz bf{};
+    bf.items = items;
+    bf.s[0] = m_stride;
+    bf.s[1] = gx;
+    bf.s[2] = nodes;
+    bf.s[3] = fac_stride;
+    hipLaunchKernelGGL(k_fit_finish, grd, dim3(EVAL_THREADS), 0, st, nmax, m_all, nodes, sel, yref,
+                       part, (int)gx, yhat, resid, grid, fac, bf);
+    FLPR_CHECK_LAUNCH();
+  }
+  return 0;
+}
+
+int flpr_batch_spline_coef(const FlprGeom* g, int32_t P, const double* inv, int64_t inv_stride,
+                           const double* m, int32_t ngrid, double* work, double* coef,
+                           void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!flpr_geom_ok(g) || P < 1 || !inv || !m || !work || !coef || ngrid < 1 ||
+      (int64_t)P * ngrid > 65535)
+    return -1;
+  const int d = g->d;
+  int64_t nodes = 1, aoff[FLPR_MAX_DIM] = {0, 0, 0}, acc = 0;
+  for (int k = 0; k < d; ++k) {
+    nodes *= g->n_grid[k];
+    aoff[k] = acc;
+    acc += (int64_t)g->n_grid[k] * g->n_grid[k];
+  }
+  if (inv_stride < acc) return -1;
+  hipStream_t st = as_stream(stream);
+  // d passes that end in coef: m -> (work | coef) -> ... -> coef
+  const double* src = m;
+  for (int k = 0; k < d; ++k) {
+    double* dst = ((d - 1 - k) % 2 == 0) ? coef : work;
+    int64_t outer = 1, inner = 1;
+    for (int j = 0; j < k; ++j) outer *= g->n_grid[j];
+    for (int j = k + 1; j < d; ++j) inner *= g->n_grid[j];
+    const int Nk = g->n_grid[k];
+    dim3 grd((unsigned)((outer * inner + AG_T - 1) / AG_T), (unsigned)((Nk + AG_T - 1) / AG_T),
+             (unsigned)(P * ngrid));
+    hipLaunchKernelGGL(k_axis_gemm, grd, dim3(256), 0, st, inv + aoff[k], inv_stride, ngrid, src,
+                       dst, nodes, Nk, outer, inner);
+    FLPR_CHECK_LAUNCH();
+    src = dst;
+  }
+  return 0;
+}
+
+int flpr_batch_point_sums(const FlprGeom* g, const FlprBatchItem* items, int32_t P, int64_t nmax,
+                          const double* x, const double* y, int32_t ny, const double* probes,
+                          int32_t nprobe, const double* m, const double* coef, int32_t dh,
+                          int32_t gcv_cubic, int32_t dof_cubic, double* part, double* red,
+                          int64_t red_stride, void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!batch_ok(g, items, P, nmax) || !x || !y || ny < 1 || nprobe < 1 || !probes || !m ||
+      ((gcv_cubic || dof_cubic) && !coef) || dh < 1 || !part || !red)
+    return -1;
+  for (int k = 0; k < g->d; ++k)
+    if ((gcv_cubic || dof_cubic) && g->n_grid[k] < 4) return -1;
+  const int nval = ny + nprobe;
+  const int ngroups = (dh * nval + PS_GROUP - 1) / PS_GROUP;
+  if (ngroups > 65535) return -1;
+  const int nblk = flpr_grid_for(nmax, PS_THREADS, 64);
+  const int64_t nodes = flpr_nodes(g);
+  FlprBz bz{};
+  bz.items = items;
+  bz.s[0] = (int64_t)dh * nval * nodes;
+  dim3 grd((unsigned)nblk, (unsigned)ngroups, (unsigned)P);
+  hipStream_t st = as_stream(stream);
+  const int gc = gcv_cubic ? 1 : 0, dc = dof_cubic ? 1 : 0;
+  switch (g->d) {
+    case 1: hipLaunchKernelGGL(k_point_sums<1>, grd, dim3(PS_THREADS), 0, st, *g, x, nmax, y, ny, probes, nprobe, m, coef, nodes, dh, gc, dc, part, bz); break;
+    case 2: hipLaunchKernelGGL(k_point_sums<2>, grd, dim3(PS_THREADS), 0, st, *g, x, nmax, y, ny, probes, nprobe, m, coef, nodes, dh, gc, dc, part, bz); break;
+    default: hipLaunchKernelGGL(k_point_sums<3>, grd, dim3(PS_THREADS), 0, st, *g, x, nmax, y, ny, probes, nprobe, m, coef, nodes, dh, gc, dc, part, bz); break;
+  }
+  FLPR_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_point_sums_final, dim3((unsigned)((dh * nval + 255) / 256), 1, (unsigned)P),
+                     dim3(256), 0, st, part, nblk, ngroups, dh, ny, nprobe, red, red_stride);
   FLPR_CHECK_LAUNCH();
   return 0;
 }

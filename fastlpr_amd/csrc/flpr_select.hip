@@ -62,11 +62,18 @@ __device__ inline double np_sum(F f, int n) {
 //   red     (dh, nval) dots [y columns, probes] then (dh, ny) quadratic forms
 //   ysq     (ny) sum_i y_ic^2; pp (K) p'p per probe
 //   out     scores, errs, dof, dof_stderr (dh each)
+// Batch (blockIdx.z = problem): items give the point count; s[0] = red,
+// s[1] = ysq, s[2] = pp, s[3] = out.
 __global__ __launch_bounds__(64) void k_select_scores(
     const double* __restrict__ red, int dh, int nval, int ny, int K,
     const double* __restrict__ ysq, const double* __restrict__ pp, double n, int cplx, int agg,
-    double* __restrict__ out) {
+    double* __restrict__ out, FlprBz bz) {
 #pragma clang fp contract(off)
+  if (bz.items) n = (double)bz.items[blockIdx.z].n;
+  red += blockIdx.z * bz.s[0];
+  ysq += blockIdx.z * bz.s[1];
+  pp += blockIdx.z * bz.s[2];
+  out += blockIdx.z * bz.s[3];
   __shared__ double tr[FLPR_MAX_CHANNELS], pdv[FLPR_MAX_CHANNELS], penv[FLPR_MAX_CHANNELS];
   const int b = blockIdx.x;
   const double* dot = red + (int64_t)b * nval;
@@ -216,12 +223,16 @@ __device__ inline void select_pick_wave(int dh, const double* __restrict__ hsum_
 // The selection over the scores (one block; thread 0 walks the bandwidths in
 // LDS when they fit): out[4 dh] = sel, out[4 dh + 1] = idmin, out[4 dh + 2] =
 // number of finite scores.
+// Batch: s[3] = out; one selected index per problem.
 __global__ __launch_bounds__(SEL_THREADS) void k_select_pick(int dh,
                                                              const double* __restrict__ hsum_g,
                                                              double dstd,
                                                              double* __restrict__ out,
-                                                             int32_t* __restrict__ sel_out) {
+                                                             int32_t* __restrict__ sel_out,
+                                                             FlprBz bz) {
   __shared__ double lsc[SEL_LDS], ler[SEL_LDS], lhs[SEL_LDS];
+  out += blockIdx.z * bz.s[3];
+  if (sel_out) sel_out += blockIdx.z;
   if (dh <= 64) {
     if (threadIdx.x < 64) select_pick_wave(dh, hsum_g, dstd, out, sel_out);
     return;
@@ -289,10 +300,36 @@ int flpr_select(const double* red, int32_t dh, int32_t nval, int32_t ny, int32_t
       agg < 0 || agg > 2 || (cplx && (ny % 2)))
     return -1;
   hipLaunchKernelGGL(k_select_scores, dim3((unsigned)dh), dim3(64), 0, as_stream(stream), red, dh,
-                     nval, ny, nprobe, ysq, pp, (double)n, cplx, agg, out);
+                     nval, ny, nprobe, ysq, pp, (double)n, cplx, agg, out, FlprBz{});
   FLPR_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_select_pick, dim3(1), dim3(SEL_THREADS), 0, as_stream(stream), dh, hsum,
-                     dstd, out, sel);
+                     dstd, out, sel, FlprBz{});
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+int flpr_batch_select(const FlprBatchItem* items, int32_t P, const double* red,
+                      int64_t red_stride, int32_t dh, int32_t nval, int32_t ny, int32_t nprobe,
+                      const double* ysq, const double* pp, const double* hsum, int32_t cplx,
+                      int32_t agg, double dstd, double* out, int64_t out_stride, int32_t* sel,
+                      void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!items || P < 1 || P > 65535 || !red || !ysq || !pp || !hsum || !out || dh < 1 || ny < 1 ||
+      nprobe < 1 || nprobe > FLPR_MAX_CHANNELS || ny > FLPR_MAX_CHANNELS || nval != ny + nprobe ||
+      agg < 0 || agg > 2 || (cplx && (ny % 2)) || out_stride < 4 * dh + 3)
+    return -1;
+  FlprBz bz{};
+  bz.items = items;
+  bz.s[0] = red_stride;
+  bz.s[1] = ny;
+  bz.s[2] = nprobe;
+  bz.s[3] = out_stride;
+  hipLaunchKernelGGL(k_select_scores, dim3((unsigned)dh, 1, (unsigned)P), dim3(64), 0,
+                     as_stream(stream), red, dh, nval, ny, nprobe, ysq, pp, 1.0, cplx, agg, out,
+                     bz);
+  FLPR_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_select_pick, dim3(1, 1, (unsigned)P), dim3(SEL_THREADS), 0,
+                     as_stream(stream), dh, hsum, dstd, out, sel, bz);
   FLPR_CHECK_LAUNCH();
   return 0;
 }

@@ -1331,27 +1331,41 @@ def _cubic_basis(t: np.ndarray, v: np.ndarray, N: int):
     return basis, l - 3
 
 
-_CUBIC: Dict[tuple, List[torch.Tensor]] = {}
+_CUBIC: Dict[tuple, torch.Tensor] = {}
+_CUBIC_HOST: Dict[tuple, np.ndarray] = {}
 
 
-def cubic_tables(geo: Geometry, dev) -> List[torch.Tensor]:
-    """Per axis, the inverse of the not-a-knot collocation matrix A[i, j] =
-    B_j(x_i) on the raw axis (float64, on the device): coefficients = A^-1 values."""
+def cubic_inverse(ax: np.ndarray) -> np.ndarray:
+    """Inverse of the not-a-knot collocation matrix A[i, j] = B_j(x_i) of one
+    raw axis (float64, N x N): coefficients = A^-1 values.  Memoised on the
+    axis' end points and length."""
+    key = (float(ax[0]), float(ax[-1]), len(ax))
+    hit = _CUBIC_HOST.get(key)
+    if hit is None:
+        N = len(ax)
+        if N < 4:
+            raise ValueError("cubic interpolation needs at least 4 grid nodes per dimension")
+        t = _nak_knots(ax)
+        basis, first = _cubic_basis(t, ax, N)
+        A = np.zeros((N, N))
+        for r in range(4):
+            A[np.arange(N), first + r] += basis[:, r]
+        hit = np.linalg.inv(A)
+        if len(_CUBIC_HOST) >= 4096:
+            _CUBIC_HOST.clear()
+        _CUBIC_HOST[key] = hit
+    return hit
+
+
+def cubic_tables(geo: Geometry, dev) -> torch.Tensor:
+    """The inverse collocation matrices of every axis, concatenated (A_0^-1,
+    A_1^-1, ... row-major), on the device."""
     key = (tuple(float(a[0]) for a in geo.axes_raw), tuple(float(a[-1]) for a in geo.axes_raw),
            tuple(geo.N), str(dev))
     hit = _CUBIC.get(key)
     if hit is None:
-        hit = []
-        for ax in geo.axes_raw:
-            N = len(ax)
-            if N < 4:
-                raise ValueError("cubic interpolation needs at least 4 grid nodes per dimension")
-            t = _nak_knots(ax)
-            basis, first = _cubic_basis(t, ax, N)
-            A = np.zeros((N, N))
-            for r in range(4):
-                A[np.arange(N), first + r] += basis[:, r]
-            hit.append(torch.tensor(np.linalg.inv(A), dtype=torch.float64, device=dev))
+        host = np.concatenate([cubic_inverse(ax).reshape(-1) for ax in geo.axes_raw])
+        hit = upload(host, torch.float64, dev)
         if len(_CUBIC) >= _MEMO_MAX:
             _CUBIC.clear()
         _CUBIC[key] = hit
@@ -1360,18 +1374,45 @@ def cubic_tables(geo: Geometry, dev) -> List[torch.Tensor]:
 
 def spline_coefficients(geo: Geometry, grids: torch.Tensor) -> torch.Tensor:
     """B-spline coefficients of grids (G, nodes) float64: A_k^-1 applied along
-    every axis (float64 GEMMs)."""
+    every axis (flpr_batch_spline_coef, one problem)."""
     inv = cubic_tables(geo, grids.device)
-    N = tuple(geo.N)
-    G = grids.shape[0]
-    c = grids.reshape((G,) + N)
-    d = len(N)
-    for k in range(d):
-        # move axis k last, contract with A_k^-1, move back
-        c = torch.movedim(c, 1 + k, -1).contiguous()
-        c = torch.matmul(c, inv[k].t().contiguous())
-        c = torch.movedim(c, -1, 1 + k)
-    return c.contiguous().reshape(G, -1)
+    G, nodes = grids.shape
+    grids = grids.contiguous()
+    coef = torch.empty_like(grids)
+    work = torch.empty_like(grids) if len(geo.N) > 1 else coef
+    call("flpr_batch_spline_coef", _lib.C.byref(geo.geom), 1, ptr(inv), inv.numel(), ptr(grids),
+         G, ptr(work), ptr(coef), stream_handle())
+    return coef
+
+
+def one_item(geo: Geometry, n: int, dev) -> torch.Tensor:
+    """A one-problem FlprBatchItem table on the device (rows [0, n))."""
+    it = _lib.FlprBatchItem()
+    it.g = geo.geom
+    it.n = int(n)
+    it.pt = 0
+    return upload(np.frombuffer(bytes(it), dtype=np.uint8), torch.uint8, dev)
+
+
+def point_sums(geo: Geometry, x: torch.Tensor, y: torch.Tensor, ny: int, probes: torch.Tensor,
+               K: int, m: torch.Tensor, dh: int, gcv_cubic: bool, dof_cubic: bool,
+               red: torch.Tensor) -> None:
+    """Residual sums and probe dots at the points for grid solutions m (dh,
+    ny + K, nodes) (flpr_batch_point_sums, one problem), written into red in
+    flpr_select's layout; cubic parts use the spline of m."""
+    check_points(geo, x)
+    n = x.shape[0]
+    nodes = m.shape[-1]
+    coef = None
+    if gcv_cubic or dof_cubic:
+        coef = spline_coefficients(geo, m.reshape(-1, nodes))
+    nval = ny + K
+    ngroups = -(-(dh * nval) // 8)
+    nblk = max(1, min(64, -(-n // 256)))
+    part = torch.empty((ngroups * nblk * 8,), dtype=torch.float64, device=x.device)
+    call("flpr_batch_point_sums", _lib.C.byref(geo.geom), ptr(one_item(geo, n, x.device)), 1, n,
+         ptr(x), ptr(y.contiguous()), ny, ptr(probes), K, ptr(m.contiguous()), ptr(coef), dh,
+         int(gcv_cubic), int(dof_cubic), ptr(part), ptr(red), red.numel(), stream_handle())
 
 
 def check_points(geo: Geometry, x: torch.Tensor) -> None:

@@ -60,6 +60,18 @@ typedef struct FlprGeom {
   double graw1[FLPR_MAX_DIM];
 } FlprGeom;
 
+/* One problem of a batch (the flpr_batch_* entry points).  The problems of
+ * one batch share the grid structure of the geometry passed beside the
+ * table (d, msp, n_grid, L, E, G, tile, ntile); each has its own knot
+ * mapping and raw grid (the rest of its FlprGeom) and its own points.  Point
+ * arrays of a batch (x, y, the sorted point data) hold the problems' rows one
+ * after the other: problem p owns rows [pt, pt + n). */
+typedef struct FlprBatchItem {
+  FlprGeom g;
+  int64_t n;   /* points of the problem */
+  int64_t pt;  /* its first row in the batch's point arrays */
+} FlprBatchItem;
+
 /* Last HIP error code seen by the calling thread inside the library (0 if
  * none); its message is copied into buf (len bytes, NUL-terminated). */
 int flpr_last_hip_error(char* buf, int32_t len);
@@ -337,6 +349,133 @@ int flpr_select(const double* red, int32_t dh, int32_t nval, int32_t ny, int32_t
                 const double* ysq, const double* pp, const double* hsum, int64_t n,
                 int32_t cplx, int32_t agg, double dstd, double* out, int32_t* sel,
                 void* stream);
+
+/* ------------------------------------------------------------------------
+ * Batches of independent problems (the reference fits one problem per
+ * cv_fastlpr call, api.py:167-832; these run the same stages for P problems
+ * of one grid structure in one launch each, problem = blockIdx.z).  items is
+ * a DEVICE array of P FlprBatchItem; g (host) is the shared grid structure
+ * (items[p].g agrees with it on d, msp, n_grid, L, E, G, tile, ntile); nmax
+ * is the largest n of the batch.  Per-problem buffers lie at the stated
+ * strides (elements); every problem computes exactly what the single-problem
+ * entry point computes for it.
+ * ------------------------------------------------------------------------ */
+
+/* Clears bytes (a multiple of 4) of device memory with a kernel: a node that
+ * a captured HIP graph replays correctly, where a memset node may not. */
+int flpr_fill_zero(void* p, int64_t bytes, void* stream);
+
+/* flpr_xstats per problem: partials P * 512 * (5d + 2 + ny), out P * (5d + 2
+ * + ny) doubles; x (rows, d) and y (rows, ny) hold the batch's rows (d = 0:
+ * the y part only, x may be NULL). */
+int flpr_batch_xstats(const FlprBatchItem* items, int32_t P, const double* x, int32_t d,
+                      const double* y, int32_t ny, double* partials, double* out, void* stream);
+
+/* flpr_points_prepare per problem with its own knot mapping; problem p's keys
+ * are offset by p * nkeys(g), so one flpr_points_bin over all rows and P *
+ * nkeys keys sorts every problem into its own row range [pt, pt + n). */
+int flpr_batch_points_prepare(const FlprGeom* g, const FlprBatchItem* items, int32_t P,
+                              int64_t nmax, const double* x, int32_t* key, int32_t* key_count,
+                              int32_t* rank, void* stream);
+
+/* flpr_points_pack per problem (its sorted rows); probes (nmax rows) are read
+ * at the row within the problem: one probe matrix whose leading rows serve
+ * every problem (the reference's MT19937 matrix of n rows is the leading n
+ * rows of any longer one, regression.py:1193-1194). */
+int flpr_batch_points_pack(const FlprGeom* g, const FlprBatchItem* items, int32_t P, int64_t nmax,
+                           const double* x, const int32_t* order, const double* vals_y, int32_t ny,
+                           const double* probes, int32_t nprobe, double* wab, double* vals,
+                           int32_t* rawi, double* rawf, void* stream);
+
+/* flpr_spread per problem: part P * (ntiles * C * window cells) doubles, the
+ * grid of problem p at grid + p * grid_stride (C * cells used), sums P * C
+ * (zeroed by the caller). */
+int flpr_batch_spread(const FlprGeom* g, const FlprBatchItem* items, int32_t P, int64_t nmax,
+                      const int32_t* key_start, const double* wab, const double* vals,
+                      int32_t with_ones, int32_t nval, double* part, double* grid,
+                      int64_t grid_stride, double* sums, void* stream);
+
+/* flpr_grid_center per problem (grid and grid32 at grid_stride, sums and
+ * shift at C). */
+int flpr_batch_grid_center(int32_t P, double* grid, int64_t grid_stride, int32_t C, int64_t cells,
+                           const double* sums, double* shift, float* grid32, void* stream);
+
+/* flpr_taps per problem (its geometry; norm = its point count): tap tables at
+ * a stride of dh * d * npow * nmax. */
+int flpr_batch_taps(const FlprGeom* g, const FlprBatchItem* items, int32_t P, const double* h,
+                    int32_t dh, int32_t npow, int32_t nmax, double* tap64, float* tap32,
+                    void* stream);
+
+/* flpr_max_moment per problem: slabs at src + p * prob_stride, key[p]. */
+int flpr_batch_max_moment(int32_t fp64, int32_t P, const void* src, int64_t prob_stride,
+                          int64_t per_h, int32_t dh, const int64_t* offs, int32_t nslab,
+                          int64_t nodes, int32_t absval, uint64_t* key, void* stream);
+
+/* flpr_solve per problem (no pivoting): moments at mom_stride, key[p], shift
+ * at nval + 1, m_out at m_stride, q at nval * nodes, dot at dot_stride. */
+int flpr_batch_solve(int32_t fp64, int32_t P, const void* mom, int64_t mom_stride, int64_t per_h,
+                     int32_t dh, int64_t nodes, int32_t order, int32_t nt, const int64_t* s_off,
+                     const int64_t* t_off, const uint64_t* key, double lam_scale, double eps_add,
+                     const double* shift, int32_t nval, int32_t nout, double* m_out,
+                     int64_t m_stride, const double* q, double* dot, int64_t dot_stride,
+                     void* stream);
+
+/* flpr_adjoint per problem: part P * (ntiles * max(nval, nw) * window
+ * cells), q at nval * nodes, gram at nw * rcells (both zeroed by the
+ * caller). */
+int flpr_batch_adjoint(const FlprGeom* g, const FlprBatchItem* items, int32_t P, int64_t nmax,
+                       const int32_t* key_start, const int32_t* rawi, const double* rawf,
+                       const double* vals, int32_t nval, double* part, double* q, double* gram,
+                       void* stream);
+
+/* flpr_quadform per problem: m at m_stride, gram at nw * rcells, quad at
+ * quad_stride (zeroed by the caller). */
+int flpr_batch_quadform(const FlprGeom* g, int32_t P, const double* m, int64_t m_stride,
+                        int32_t ncol, int32_t dh, const double* gram, double* quad,
+                        int64_t quad_stride, void* stream);
+
+/* flpr_select per problem (n from items): red at red_stride, ysq at ny, pp at
+ * nprobe, out at out_stride (>= 4 dh + 3), sel[p]. */
+int flpr_batch_select(const FlprBatchItem* items, int32_t P, const double* red,
+                      int64_t red_stride, int32_t dh, int32_t nval, int32_t ny, int32_t nprobe,
+                      const double* ysq, const double* pp, const double* hsum, int32_t cplx,
+                      int32_t agg, double dstd, double* out, int64_t out_stride, int32_t* sel,
+                      void* stream);
+
+/* flpr_fit_output per problem: m_all at m_stride, sel[p], yref / yhat / resid
+ * at the problem's rows, grid at nodes, part P * 1024, fac at fac_stride. */
+int flpr_batch_fit_output(const FlprGeom* g, const FlprBatchItem* items, int32_t P, int64_t nmax,
+                          const double* x, const double* m_all, int64_t m_stride, int32_t ncol,
+                          const int32_t* sel, const double* yref, int32_t variance, double* yhat,
+                          double* resid, double* grid, double* part, double* fac,
+                          int64_t fac_stride, void* stream);
+
+/* B-spline coefficients for the cubic GCV / DOF interpolation
+ * (RegularGridInterpolator 'cubic', api.py:588-594, 651-657): for every
+ * problem p and grid of m (P * ngrid grids of prod(N) values, row-major on
+ * the raw grid), coef = A_0^-1 x_0 A_1^-1 x_1 ... m, the inverse not-a-knot
+ * collocation matrix of every axis applied along that axis.  inv holds the
+ * problem's matrices (A_0^-1 then A_1^-1 ..., N_k x N_k row-major) at
+ * inv_stride; work is a scratch of the size of m (unused for d = 1). */
+int flpr_batch_spline_coef(const FlprGeom* g, int32_t P, const double* inv, int64_t inv_stride,
+                           const double* m, int32_t ngrid, double* work, double* coef,
+                           void* stream);
+
+/* Residual sums and Hutchinson dots at the data points for the grid
+ * solutions m (P x (dh, ny + nprobe, nodes)): yhat of the y channels and phat
+ * of the probe channels are the cubic spline of coef (gcv_cubic, dof_cubic)
+ * or the multilinear interpolation of m; writes sum_i (y_ic - yhat_ic)^2 at
+ * red[dh (ny + nprobe) + h ny + c] and sum_i p_ik phat_ik at red[h (ny +
+ * nprobe) + ny + k] (problem p at red + p red_stride; the y dots are left
+ * as they are), summed in a fixed order (regression.py:1256-1280,
+ * api.py:661-723).  y (rows, ny) at the problems' rows, probes (nmax rows)
+ * at the row within the problem.  part holds P * ceil(dh (ny + nprobe) / 8)
+ * * 8 * min(64, ceil(nmax / 256)) doubles. */
+int flpr_batch_point_sums(const FlprGeom* g, const FlprBatchItem* items, int32_t P, int64_t nmax,
+                          const double* x, const double* y, int32_t ny, const double* probes,
+                          int32_t nprobe, const double* m, const double* coef, int32_t dh,
+                          int32_t gcv_cubic, int32_t dof_cubic, double* part, double* red,
+                          int64_t red_stride, void* stream);
 
 #ifdef __cplusplus
 }
