@@ -70,6 +70,8 @@ class FlprGeom(C.Structure):
         ("G", C.c_int32 * MAX_DIM),
         ("tile", C.c_int32 * MAX_DIM),
         ("ntile", C.c_int32 * MAX_DIM),
+        ("item_points", C.c_int32),
+        ("reserved_", C.c_int32),
         ("hx", C.c_double * MAX_DIM),
         ("inv4tau", C.c_double * MAX_DIM),
         ("mean", C.c_double * MAX_DIM),

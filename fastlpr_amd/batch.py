@@ -48,7 +48,7 @@ from typing import Dict, List, Optional, Sequence
 import numpy as np
 import torch
 
-from . import _lib, api, engine
+from . import _lib, api, engine, tune
 from ._lib import FlprBatchItem, call, i64_array
 from .structures import RegressionOutput
 
@@ -94,6 +94,7 @@ class _Problem:
     groups: Optional[list] = None
     band_np: Optional[np.ndarray] = None
     batched: bool = True
+    acc_n: int = 4
 
 
 @dataclass
@@ -123,7 +124,11 @@ class _Class:
 class BatchPlan:
     """The x side of a batch of cv_fastlpr problems (module docstring)."""
 
-    def __init__(self, xs: Sequence, h=None, options=None):
+    def __init__(self, xs: Sequence, h=None, options=None, tune: bool = False):
+        """tune: measure the launch shapes of every class on first use
+        (fastlpr_amd.tune); choices already made, in this process or in the
+        file FASTLPR_AMD_TUNE_FILE names, are applied either way."""
+        self.tune = bool(tune)
         P = len(xs)
         if P == 0:
             raise ValueError("cv_fastlpr_batch needs at least one problem")
@@ -232,6 +237,7 @@ class BatchPlan:
                 continue
             bw = p.h_given if p.h_given is not None else api._prepare_bandwidths(
                 None, dims, o, np.column_stack((0.1 * p.stats.std, 1.0 * p.stats.std)))
+            p.acc_n = acc_n
             geo, gkey = engine.geometry_for(p.stats, p.N, acc_n)
             if any(geo.geom.G[k] > geo.geom.L[k] for k in range(dims)):
                 p.batched = False   # the torus fold of small grids: one call
@@ -261,8 +267,29 @@ class BatchPlan:
                 c = _Class(sub, dims, order, N, ny, cplx, ny // 2 if cplx else ny, K, variance,
                            fp64, agg, dstd, reg, sub[0].hf, sub[0].groups, sub[0].geo, cubic)
                 self._build_class(c)
+                choice = (tune.tune_class(self, c, self._retile) if self.tune
+                          else tune.lookup(c))
+                if choice is not None and choice != tune.DEFAULT[c.d]:
+                    self._retile(c, *choice)
+                c.tabs["launch_shape"] = choice or tune.DEFAULT[c.d]
                 classes.append(c)
         return classes
+
+    def _retile(self, c: "_Class", tile: int, item_points: int) -> None:
+        """Give class c's problems geometries with another spread tile edge
+        and points per work item (the autotuner's knobs)."""
+        for p in c.probs:
+            p.geo, _ = engine.geometry_for(p.stats, p.N, p.acc_n, tile=tile,
+                                           item_points=item_points)
+        c.geo0 = c.probs[0].geo
+        P = len(c.probs)
+        items = (FlprBatchItem * P)()
+        for j, p in enumerate(c.probs):
+            items[j].g = p.geo.geom
+            items[j].n = p.n
+            items[j].pt = int(c.tabs["pts"][j])
+        c.bufs["items"] = torch.as_tensor(np.frombuffer(items, dtype=np.uint8).copy(),
+                                          device=self.dev)
 
     def _split(self, ps: List[_Problem], ny: int, K: int):
         """Cut a class into launch sets that respect the moment-slab budget and

@@ -266,6 +266,73 @@ __global__ __launch_bounds__(PREP_THREADS) void k_prepare(const double* __restri
   }
 }
 
+// k_prepare for key tables that fit in LDS (nkeys <= PREP_LDS_KEYS): a
+// block counts its points per key in LDS (the LDS atomic's return value is
+// the point's rank within the block), then takes one global atomic per
+// (block, key) for the key's base and adds it to its points' ranks.  With
+// few keys (1-D grids: about a thousand cells on a few dozen cache lines)
+// this replaces one contended global atomic per point by one per block and
+// key.  A block owns a contiguous run of points.
+constexpr int PREP_LDS_KEYS = 8192;
+
+template <int D>
+__global__ __launch_bounds__(PREP_THREADS) void k_prepare_lds(const double* __restrict__ x,
+                                                              int64_t n, FlprGeom g,
+                                                              int32_t* __restrict__ key,
+                                                              int32_t* __restrict__ key_count,
+                                                              int32_t* __restrict__ rank,
+                                                              int nkeys, FlprBz bz) {
+  extern __shared__ int hist[];
+  int key0 = 0;
+  if (bz.items) {
+    const FlprBatchItem& it = bz.items[blockIdx.z];
+    g = it.g;
+    x += it.pt * D;
+    key += it.pt;
+    rank += it.pt;
+    n = it.n;
+    key0 = (int)(blockIdx.z * bz.s[0]);
+  }
+  for (int k = threadIdx.x; k < nkeys; k += blockDim.x) hist[k] = 0;
+  __syncthreads();
+  const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = (int64_t)blockIdx.x * per;
+  const int64_t i1 = min(n, i0 + per);
+  int tc = 1;
+#pragma unroll
+  for (int k = 0; k < D; ++k) tc *= g.tile[k];
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    int tid = 0, lid = 0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      int m;
+      knot_of(g, k, x[i * D + k], &m);
+      const int pm = (m > g.L[k] / 2) ? m - g.L[k] : m;
+      const int s = min(max(pm + g.E[k], g.msp), g.G[k] - 1 - g.msp);
+      tid = tid * g.ntile[k] + s / g.tile[k];
+      lid = lid * g.tile[k] + s % g.tile[k];
+    }
+    const int kl = tid * tc + lid;
+    key[i] = key0 + kl;
+    rank[i] = atomicAdd(&hist[kl], 1);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nkeys; k += blockDim.x) {
+    const int c = hist[k];
+    if (c > 0) hist[k] = atomicAdd(&key_count[key0 + k], c);
+  }
+  __syncthreads();
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) rank[i] += hist[key[i] - key0];
+}
+
+// blocks of k_prepare_lds: about 16 points per key and block, at most 4096
+static inline int prep_lds_blocks(int64_t n, int64_t nkeys) {
+  int64_t b = n / (16 * nkeys);
+  if (b < 1) b = 1;
+  if (b > 4096) b = 4096;
+  return (int)b;
+}
+
 constexpr int SCAN_THREADS = 1024;
 
 // Exclusive scan in three launches: block sums, a scan of the block sums, and a
@@ -488,7 +555,8 @@ constexpr int MIN_BLOCKS = 512;
 
 __host__ __device__ inline int item_splits(const FlprGeom* g, int64_t n) {
   const int64_t nodes = flpr_nodes(g);
-  const int target = (g->d == 1) ? ITEM_POINTS_1D : ITEM_POINTS;
+  const int target = g->item_points > 0 ? g->item_points
+                                        : ((g->d == 1) ? ITEM_POINTS_1D : ITEM_POINTS);
   int64_t s = (n + nodes * target - 1) / (nodes * target);
   if (s < 1) s = 1;
   if (s > 64) s = 64;
@@ -1920,8 +1988,20 @@ int flpr_points_prepare(const double* x, int64_t n, const FlprGeom* g, int32_t* 
   FLPR_CLEAR_ERROR();
   if (!x || !key || !key_count || n < 1 || !flpr_geom_ok(g)) return -1;
   if (flpr_nkeys(g) > ((int64_t)1 << 30)) return -1;
-  dim3 grd(flpr_grid_for(n, PREP_THREADS, 4096));
   hipStream_t st = as_stream(stream);
+  const int64_t nk = flpr_nkeys(g);
+  if (rank && nk <= PREP_LDS_KEYS) {
+    dim3 gl(prep_lds_blocks(n, nk));
+    const size_t lds = (size_t)nk * sizeof(int);
+    switch (g->d) {
+      case 1: hipLaunchKernelGGL(k_prepare_lds<1>, gl, dim3(PREP_THREADS), lds, st, x, n, *g, key, key_count, rank, (int)nk, FlprBz{}); break;
+      case 2: hipLaunchKernelGGL(k_prepare_lds<2>, gl, dim3(PREP_THREADS), lds, st, x, n, *g, key, key_count, rank, (int)nk, FlprBz{}); break;
+      default: hipLaunchKernelGGL(k_prepare_lds<3>, gl, dim3(PREP_THREADS), lds, st, x, n, *g, key, key_count, rank, (int)nk, FlprBz{}); break;
+    }
+    FLPR_CHECK_LAUNCH();
+    return 0;
+  }
+  dim3 grd(flpr_grid_for(n, PREP_THREADS, 4096));
   switch (g->d) {
     case 1: hipLaunchKernelGGL(k_prepare<1>, grd, dim3(PREP_THREADS), 0, st, x, n, *g, key, key_count, rank, FlprBz{}); break;
     case 2: hipLaunchKernelGGL(k_prepare<2>, grd, dim3(PREP_THREADS), 0, st, x, n, *g, key, key_count, rank, FlprBz{}); break;
@@ -2267,8 +2347,20 @@ int flpr_batch_points_prepare(const FlprGeom* g, const FlprBatchItem* items, int
   FlprBz bz{};
   bz.items = items;
   bz.s[0] = flpr_nkeys(g);
-  dim3 grd(flpr_grid_for(nmax, PREP_THREADS, 4096), 1, P);
   hipStream_t st = as_stream(stream);
+  const int64_t nk = flpr_nkeys(g);
+  if (rank && nk <= PREP_LDS_KEYS) {
+    dim3 gl(prep_lds_blocks(nmax, nk), 1, P);
+    const size_t lds = (size_t)nk * sizeof(int);
+    switch (g->d) {
+      case 1: hipLaunchKernelGGL(k_prepare_lds<1>, gl, dim3(PREP_THREADS), lds, st, x, nmax, *g, key, key_count, rank, (int)nk, bz); break;
+      case 2: hipLaunchKernelGGL(k_prepare_lds<2>, gl, dim3(PREP_THREADS), lds, st, x, nmax, *g, key, key_count, rank, (int)nk, bz); break;
+      default: hipLaunchKernelGGL(k_prepare_lds<3>, gl, dim3(PREP_THREADS), lds, st, x, nmax, *g, key, key_count, rank, (int)nk, bz); break;
+    }
+    FLPR_CHECK_LAUNCH();
+    return 0;
+  }
+  dim3 grd(flpr_grid_for(nmax, PREP_THREADS, 4096), 1, P);
   switch (g->d) {
     case 1: hipLaunchKernelGGL(k_prepare<1>, grd, dim3(PREP_THREADS), 0, st, x, nmax, *g, key, key_count, rank, bz); break;
     case 2: hipLaunchKernelGGL(k_prepare<2>, grd, dim3(PREP_THREADS), 0, st, x, nmax, *g, key, key_count, rank, bz); break;

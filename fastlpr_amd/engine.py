@@ -602,7 +602,8 @@ class Geometry:
 
 def build_geometry(st: Stats, N: Tuple[int, ...], accuracy: int, xrange=None,
                    flag_power2: bool = True, kde: bool = False,
-                   margin: Optional[int] = None) -> Geometry:
+                   margin: Optional[int] = None, tile: Optional[int] = None,
+                   item_points: int = 0) -> Geometry:
     """Knot-mapping constants exactly as the reference computes them in float64
     (model.py:58-72, convolution.py:169-199, nufftn_type1.py:140-176,
     kernel.py:97-156); the extreme points give the min / max of every
@@ -611,7 +612,10 @@ def build_geometry(st: Stats, N: Tuple[int, ...], accuracy: int, xrange=None,
     margin: cells of spread grid kept beyond the nodes on each side (at least
     the footprint half-width); two geometries with the same margin share one
     grid, so channels spread at different accuracies can be convolved
-    together."""
+    together.  tile (spread tile edge in cells, default TILE[d]) and
+    item_points (points per spread / adjoint work item, 0 = the library
+    default) change how the kernels cut the work, not what they compute
+    beyond the order of float64 additions."""
     d = len(N)
     if accuracy > 6:
         raise ValueError("the spread grid geometry takes accuracy <= 6; accuracy > 6 spreads "
@@ -650,13 +654,13 @@ def build_geometry(st: Stats, N: Tuple[int, ...], accuracy: int, xrange=None,
         tau = (math.pi * msp) / (L_k * L_k * 1 * (1 - 0.5))
         E = n_k // 2 + 1 + margin
         G = 2 * E + 1
-        tile = TILE[d]
+        tile_k = TILE[d] if tile is None else int(tile)
         g.n_grid[k] = n_k
         g.L[k] = L_k
         g.E[k] = E
         g.G[k] = G
-        g.tile[k] = tile
-        g.ntile[k] = -(-G // tile)
+        g.tile[k] = tile_k
+        g.ntile[k] = -(-G // tile_k)
         g.hx[k] = 2 * np.pi / L_k
         # accuracy 0 bins every point to its nearest cell (nufftn_type1.py:442-516):
         # footprint weight exp(0) = 1
@@ -686,6 +690,7 @@ def build_geometry(st: Stats, N: Tuple[int, ...], accuracy: int, xrange=None,
         centre[k], xmin_s[k], xmax_s[k] = c, lo, hi
         axes_raw.append(ax_r)
         axes_scaled.append(ax_s)
+    g.item_points = int(item_points)
     return Geometry(g, tuple(N), L, msp, xmin_s, xmax_s, axes_raw, axes_scaled,
                     st.mean.copy(), st.std.copy(), centre, margin)
 
@@ -889,17 +894,19 @@ def _memo_put(table: dict, key, value):
 
 
 def geometry_for(st: Stats, N, accuracy: int, xrange=None, flag_power2: bool = True,
-                 kde: bool = False, margin: Optional[int] = None):
+                 kde: bool = False, margin: Optional[int] = None,
+                 tile: Optional[int] = None, item_points: int = 0):
     """build_geometry, memoised on the exact statistics and grid options.
     Returns (geometry, key)."""
     xr = None if xrange is None else tuple(tuple(float(v) for v in r) for r in xrange)
     mg = int(accuracy) if margin is None else max(int(accuracy), int(margin))
     key = (st.xmin.tobytes(), st.xmax.tobytes(), st.mean.tobytes(), st.std.tobytes(),
-           tuple(int(v) for v in N), int(accuracy), xr, bool(flag_power2), bool(kde), mg)
+           tuple(int(v) for v in N), int(accuracy), xr, bool(flag_power2), bool(kde), mg,
+           tile, int(item_points))
     geo = _GEOMS.get(key)
     if geo is None:
         geo = _memo_put(_GEOMS, key, build_geometry(st, N, accuracy, xrange, flag_power2, kde,
-                                                    mg))
+                                                    mg, tile, item_points))
     return geo, key
 
 

@@ -45,6 +45,10 @@ typedef struct FlprGeom {
   int32_t G[FLPR_MAX_DIM];        /* 2E+1 */
   int32_t tile[FLPR_MAX_DIM];     /* spread tile edge in cells */
   int32_t ntile[FLPR_MAX_DIM];    /* tiles per dimension = ceil(G / tile) */
+  int32_t item_points;            /* points per spread / adjoint work item when a cell
+                                     holds many (its points are split over items);
+                                     0 = the library default (8 in 1-D, 16 otherwise) */
+  int32_t reserved_;              /* 0 */
   double hx[FLPR_MAX_DIM];        /* 2 pi / L */
   double inv4tau[FLPR_MAX_DIM];   /* 1 / (4 tau) */
   double mean[FLPR_MAX_DIM];      /* z-score mean (ddof=1 std below) */

@@ -34,9 +34,9 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_geometry_struct_matches_header_layout():
-    """FlprGeom field order/size in ctypes equals the C struct (20 int32 then 13x3 doubles)."""
+    """FlprGeom field order/size in ctypes equals the C struct (22 int32 then 13x3 doubles)."""
     from fastlpr_amd._lib import FlprGeom
-    assert ctypes.sizeof(FlprGeom) == 20 * 4 + 13 * 3 * 8
+    assert ctypes.sizeof(FlprGeom) == 22 * 4 + 13 * 3 * 8
     text = open(HEADER).read()
     body = text[text.index("typedef struct FlprGeom"):text.index("} FlprGeom;")]
     fields = re.findall(r"(?:int32_t|double)\s+(\w+)\s*[\[;]", body)

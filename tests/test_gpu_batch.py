@@ -144,3 +144,24 @@ def test_batch_graph_replay(config4):
             got2 = plan.fit(by2)
             for a, b in zip(got2, eager2):
                 _compare(a, b, 1e-9)
+
+
+def test_batch_autotuned_launch_shapes(config4):
+    """The autotuner measures tile / points-per-item candidates per class and
+    keeps the fastest; the tuned plan computes the same fits (the knobs only
+    reorder float64 additions)."""
+    import fastlpr_amd as fl
+    from fastlpr_amd import tune
+    xs, ys, hs, opts = config4
+    sel = list(range(3, P_ALL, 24))
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        plan = fl.BatchPlan([xs[i] for i in sel], [hs[i] for i in sel], [opts[i] for i in sel],
+                            tune=True)
+        got = plan.fit([ys[i] for i in sel])
+        classes = [c for v in plan._kindsets.values() for c in v]
+        assert classes and all(c.tabs.get("tune_ms") or tune.lookup(c) for c in classes)
+        for c in classes:
+            assert c.tabs["launch_shape"] in tune.CANDIDATES[c.d] + [tune.DEFAULT[c.d]]
+        for i, r in zip(sel, got):
+            _compare(r, fl.cv_fastlpr(xs[i], ys[i], hs[i], opts[i]), 1e-11)
