@@ -21,7 +21,12 @@ between them, a roofline entry for the dominant kernel (the float32 form's
 flpr_conv launches, measured with HIP events on the launch stream during the
 timed steps) and a CPU baseline (the oracle restatement, single thread, on a
 bounded sample: the mean pass with 2 and with 4 of the 30 bandwidths after a
-warm-up call, extrapolated linearly to 30).
+warm-up call, extrapolated linearly to 30, and the same on all the host's
+cores).  Further legs, outside the headline value: the spread's roofline
+entry (HIP events on the spread launches of the timed steps), a cold call
+(caches cleared: probe generation and every plan rebuilt), and the batched
+512-problem workload (config 4) as problems per second, batched against one
+cv_fastlpr call per problem and against its captured-graph replay.
 """
 from __future__ import annotations
 
@@ -43,6 +48,8 @@ WORKLOAD = {"workload": "cv_fastlpr 2-D order-2 heteroscedastic pair (mean + var
 PEAK_MFMA_FP32 = 157.3e12
 PEAK_MFMA_FP64 = 78.6e12
 PEAK_MFMA_BF16 = 2.5e15
+PEAK_FP64_VALU = 78.6e12          # MI355X float64 vector FMA peak
+PEAK_HBM = 8.0e12
 # the float32 form's convolution computes each float32 product as six bf16
 # MFMA products (hi*hi, hi*mid, mid*hi, hi*lo, lo*hi, mid*mid)
 SPLIT_PRODUCTS = 6
@@ -79,17 +86,20 @@ def step_pair(pair, xd, yd, hl, prec):
     return pair(xd, yd, hl, {"order": 2, "N": [256, 256], "precision": prec})
 
 
-def cpu_baseline(x, y, hl):
+def cpu_baseline(x, y, hl, threads: int = 1):
     """Oracle (NumPy restatement of the reference dataflow, float32 'ref' mode),
-    one thread, mean pass: one untimed call with 1 bandwidth (FFT plans, probe
-    matrix), then timed calls with 2 and 4 of the 30 bandwidths; the per-
-    bandwidth slope extrapolates to 30.  About 20 s of CPU work."""
+    mean pass: one untimed call with 1 bandwidth (FFT plans, probe matrix),
+    then timed calls with 2 and 4 of the 30 bandwidths; the per-bandwidth
+    slope extrapolates to 30.  threads: BLAS / OpenMP threads and scipy.fft
+    workers (1 = the reference's effective threading; the host's core count
+    = the best this restatement does on all cores).  About 20 s of CPU work."""
+    import scipy.fft as sfft
     from threadpoolctl import threadpool_limits
     from oracle import restate as R
     xs, ys = x.astype(np.float64), y.astype(np.float64)
     opt = {"order": 2, "N": [256, 256]}
     times = {}
-    with threadpool_limits(1):
+    with threadpool_limits(threads), sfft.set_workers(threads):
         R.cv_fastlpr(xs, ys, hl[14:15], opt, mode="ref")
         for k in (2, 4):
             sub = hl[13:13 + k]
@@ -98,12 +108,97 @@ def cpu_baseline(x, y, hl):
             times[k] = time.perf_counter() - t
     slope = max(times[4] - times[2], 0.0) / 2.0
     per_fit = times[2] + (hl.shape[0] - 2) * slope
-    return {"value": 1.0 / per_fit, "unit": "fits/s", "cores": 1, "kind": "port",
+    return {"value": 1.0 / per_fit, "unit": "fits/s", "cores": threads, "kind": "port",
+            "host_cores": len(os.sched_getaffinity(0)),
             "sample": (f"oracle restatement (reference dataflow, float32 mode) of the mean pass "
-                       f"on the same 1e6 points and 256x256 grid, single thread, with 2 and 4 "
+                       f"on the same 1e6 points and 256x256 grid, {threads} thread(s) (BLAS and "
+                       f"scipy.fft workers; the spread is single-threaded NumPy), with 2 and 4 "
                        f"of the 30 bandwidths ({times[2]:.1f} s, {times[4]:.1f} s) after one "
                        f"untimed warm-up call; extrapolated linearly to 30 bandwidths: "
                        f"{per_fit:.1f} s per fit")}
+
+
+def config4_problem(i: int):
+    """Problem i of the batched uneven workload: default_rng(1000 + i) draws
+    n = round(10^U(3, 6)) and the order integers(0, 3); i < 256 is 1-D on 1024
+    nodes, else 2-D on 128 x 128; 20 bandwidths log-spaced in [0.02, 0.5];
+    every tenth problem has complex y = (a + ib) f(x), a, b ~ N(0, 1);
+    f = sin(2 pi x1) (+ cos(2 pi x2) in 2-D) + 0.1 N(0, 1)."""
+    rng = np.random.default_rng(1000 + i)
+    n = int(round(10 ** rng.uniform(3, 6)))
+    order = int(rng.integers(0, 3))
+    d = 1 if i < 256 else 2
+    x = rng.uniform(0, 1, (n, d))
+    f = np.sin(2 * np.pi * x[:, 0]) + (np.cos(2 * np.pi * x[:, 1]) if d == 2 else 0.0)
+    f = f + 0.1 * rng.standard_normal(n)
+    if i % 10 == 0:
+        a, b = rng.standard_normal(2)
+        f = (a + 1j * b) * f
+    hh = np.logspace(np.log10(0.02), np.log10(0.5), 20)
+    return x, f, np.column_stack([hh] * d), {"order": order, "N": 1024 if d == 1 else [128, 128]}
+
+
+def config4_leg(steps: int):
+    """512 uneven problems: one cv_fastlpr call each (looped) against the
+    batched BatchPlan (autotuned launch shapes; the x side and the tuning are
+    set up before the timed region, as for a plan reused over many response
+    sets) and its captured-graph replay.  Inputs resident on the device."""
+    import torch
+    import fastlpr_amd as fl
+    P = 512
+    xs, ys, hs, os_ = [], [], [], []
+    for i in range(P):
+        x, f, h, o = config4_problem(i)
+        xs.append(torch.as_tensor(x, device="cuda"))
+        ys.append(torch.as_tensor(f, device="cuda"))
+        hs.append(h)
+        os_.append(o)
+    rows = int(sum(x.shape[0] for x in xs))
+
+    def looped():
+        return [fl.cv_fastlpr(xs[i], ys[i], hs[i], os_[i]) for i in range(P)]
+
+    looped()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    ref = looped()
+    torch.cuda.synchronize()
+    t_loop = time.perf_counter() - t
+    t = time.perf_counter()
+    plan = fl.BatchPlan(xs, hs, os_, tune=True)
+    plan.fit(ys)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t
+    best = float("inf")
+    for _ in range(max(steps, 1)):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        got = plan.fit(ys)
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t)
+    diff = 0.0
+    same_sel = True
+    for a, b in zip(got, ref):
+        ya, yb = a.yhat, b.yhat
+        diff = max(diff, float((ya - yb).abs().max() / yb.abs().max()))
+        same_sel = same_sel and a.gcv_yhat["id1se"] == b.gcv_yhat["id1se"]
+    plan.capture(ys)
+    gbest = float("inf")
+    for _ in range(max(steps, 1)):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        plan.fit(ys)
+        torch.cuda.synchronize()
+        gbest = min(gbest, time.perf_counter() - t)
+    classes = [c for v in plan._kindsets.values() for c in v]
+    return {"workload": "512 uneven problems (config 4), one result per problem as cv_fastlpr",
+            "points": rows, "classes": len(classes),
+            "batched_problems_per_s": P / best, "batched_ms": best * 1e3,
+            "graph_replay_problems_per_s": P / gbest, "graph_replay_ms": gbest * 1e3,
+            "looped_problems_per_s": P / t_loop, "looped_ms": t_loop * 1e3,
+            "plan_setup_and_tuning_ms": t_setup * 1e3,
+            "batched_vs_looped_yhat_max_rel_diff": diff, "same_selected_h": same_sel,
+            "launch_shapes": sorted({str(c.tabs.get("launch_shape")) for c in classes})}
 
 
 def pmc_summary(dbs):
@@ -162,6 +257,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the float64 plain form")
     ap.add_argument("--no-two-calls", action="store_true",
                     help="skip timing the pair as two separate cv_fastlpr calls")
+    ap.add_argument("--no-config4", action="store_true",
+                    help="skip the batched 512-problem leg")
+    ap.add_argument("--no-cold", action="store_true",
+                    help="skip the cold-cache call")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, run a plain 256 MiB streaming kernel (negation) "
                          "three times: the known-size reference for FETCH_SIZE / WRITE_SIZE")
@@ -248,6 +347,20 @@ def main():
           f"(two calls: {two_call_fits} fits/s, yhat difference {pair_vs_calls})",
           file=sys.stderr, flush=True)
     conv = kt.get("flpr_conv", [])
+    spr = kt.get("flpr_spread", [])
+    spread_entry = None
+    if spr:
+        sp_ms = float(np.mean([c[0] for c in spr]))
+        sp_flop = float(np.mean([c[1][0] for c in spr]))
+        sp_bytes = float(np.mean([c[1][1] for c in spr]))
+        sp_ach = sp_flop / (sp_ms * 1e-3)
+        spread_entry = {"bound": "valu-fp64", "kernel": "flpr_spread (k_spread_cells + gather)",
+                        "achieved": sp_ach / 1e12, "peak": PEAK_FP64_VALU / 1e12,
+                        "unit": "TFLOP/s", "frac": sp_ach / PEAK_FP64_VALU,
+                        "hbm_gb_s": sp_bytes / (sp_ms * 1e-3) / 1e9,
+                        "hbm_frac": sp_bytes / (sp_ms * 1e-3) / PEAK_HBM,
+                        "avg_launch_ms": sp_ms, "flop_per_launch": sp_flop,
+                        "bytes_per_launch": sp_bytes, "launches_timed": len(spr)}
     split_peak = PEAK_MFMA_BF16 / SPLIT_PRODUCTS / 1e12
     conv_ms = float(np.mean([c[0] for c in conv])) if conv else float("nan")
     conv_flop = float(np.mean([c[1] for c in conv])) if conv else float("nan")
@@ -276,11 +389,45 @@ def main():
         print(f"[bench] plain fp64: {plain_fits:.1f} fits/s; chosen-vs-plain yhat err "
               f"{e1:.2e} / {e2:.2e}", file=sys.stderr, flush=True)
 
-    cpu = None
+    cold = None
+    if not args.no_cold:
+        # a first call at this n: the probe matrix is generated on the host
+        # (NumPy MT19937, as the reference does on every call) and every plan,
+        # geometry and table is rebuilt
+        fastlpr_amd.clear_caches()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        step_pair(pair, xd, yd, hl, "fp32")
+        torch.cuda.synchronize()
+        t_pair = time.perf_counter() - t
+        fastlpr_amd.clear_caches()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        cv(xd, yd, hl, {"order": 2, "N": [256, 256], "precision": "fp32"})
+        torch.cuda.synchronize()
+        t_one = time.perf_counter() - t
+        cold = {"pair_ms": t_pair * 1e3, "fits_per_s_pair": 2.0 / t_pair,
+                "single_call_ms": t_one * 1e3, "fits_per_s_single_call": 1.0 / t_one,
+                "note": "clear_caches() before each call: probe generation on the host "
+                        "(1e6 x 10 normals) and every plan rebuilt"}
+        print(f"[bench] cold call: pair {t_pair*1e3:.1f} ms, single {t_one*1e3:.1f} ms",
+              file=sys.stderr, flush=True)
+    cfg4 = None
+    if not args.no_config4:
+        print("[bench] config 4: 512 uneven problems, batched vs looped ...", file=sys.stderr,
+              flush=True)
+        cfg4 = config4_leg(3)
+        print(f"[bench] config 4: batched {cfg4['batched_problems_per_s']:.0f} problems/s, "
+              f"looped {cfg4['looped_problems_per_s']:.0f}", file=sys.stderr, flush=True)
+    cpu = cpu_all = None
     if not args.no_cpu:
         print("[bench] cpu baseline (oracle, 1 thread, bounded sample) ...", file=sys.stderr,
               flush=True)
         cpu = cpu_baseline(x, y, hl)
+        ncores = len(os.sched_getaffinity(0))
+        print(f"[bench] cpu baseline (oracle, {ncores} threads, bounded sample) ...",
+              file=sys.stderr, flush=True)
+        cpu_all = cpu_baseline(x, y, hl, threads=ncores)
 
     line = {
         "metric": "cv_fastlpr fits/sec, 2D N=1e6 order2 30 h",
@@ -307,6 +454,10 @@ def main():
                      "flop_per_launch": conv_flop, "launches_timed": len(conv),
                      "share_of_step_time": conv_share},
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
+        "roofline_spread": spread_entry,
+        "cold_call": cold,
+        "config4_batched": cfg4,
     }
     line.update(extra)
     line["two_call_fits_per_s"] = two_call_fits

@@ -202,6 +202,16 @@ class Pass:
     jobs_host: np.ndarray = None
 
 
+def spread_work(n: int, d: int, msp: int, C: int, cells: int) -> Tuple[float, float]:
+    """Algorithmic work of one flpr_spread call: (flops, bytes).  flops: one
+    multiply-add per (point, footprint cell, channel), 2 n (2 msp + 1)^d C;
+    bytes: the sorted point data read once (2 d weights and C - 1 values, or
+    C values without the ones channel, 8 bytes each) and the grid written
+    once (C cells of 8 bytes)."""
+    taps = (2 * msp + 1) ** d
+    return (2.0 * n * taps * C, 8.0 * n * (2 * d + C) + 8.0 * C * cells)
+
+
 def conv_flops(ps: "Pass", bands: np.ndarray) -> float:
     """Algorithmic flops of one convolution pass: 2 per (output element, tap
     inside the band whose input cell lies on the grid)."""
@@ -1049,8 +1059,9 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
         # every channel through the oversampled grid (acc_y == acc_s here)
         spread_fine(geo, acc_s, x, n, vals_y, ny, y_pairs, probes, K, True, grid64, sums, 0, s)
     else:
-        call("flpr_spread", gp, ptr(key_start), ptr(wab), ptr(vals), n, 1, nval, ptr(part),
-             ptr(grid64), ptr(sums), s)
+        TIMER.timed("flpr_spread", spread_work(n, d, geo.msp, C, cells), call, "flpr_spread", gp,
+                    ptr(key_start), ptr(wab), ptr(vals), n, 1, nval, ptr(part), ptr(grid64),
+                    ptr(sums), s)
     if split_acc and not fine_s and not fine_y:
         # the y columns again, with the footprint and width of their own
         # accuracy, over the channels the first spread wrote
@@ -1244,8 +1255,9 @@ def run_grid_y(sh: dict, x: torch.Tensor, vals_y: torch.Tensor, ny: int,
                  None, 0, None, ptr(vals), None, None, s)
         fcells = int(np.prod([g.tile[k] + 2 * acc_y for k in range(d)]))
         part = torch.empty((ntiles * ny * fcells,), dtype=f64, device=dev)
-        call("flpr_spread", gy, ptr(sh["key_start"]), ptr(wab), ptr(vals), n, 0, ny, ptr(part),
-             grid64.data_ptr() + cells * 8, sums.data_ptr() + 8, s)
+        TIMER.timed("flpr_spread", spread_work(n, d, acc_y, ny, cells), call, "flpr_spread", gy,
+                    ptr(sh["key_start"]), ptr(wab), ptr(vals), n, 0, ny, ptr(part),
+                    grid64.data_ptr() + cells * 8, sums.data_ptr() + 8, s)
         del part
         periodize(geo, grid64, 1, ny)
     shift = torch.empty((C2,), dtype=f64, device=dev)
