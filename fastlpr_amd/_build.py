@@ -7,7 +7,7 @@ import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SOURCES = ["csrc/flpr_points.hip", "csrc/flpr_conv.hip", "csrc/flpr_select.hip",
-           "csrc/flpr_resample.hip"]
+           "csrc/flpr_resample.hip", "csrc/flpr_mlp.hip"]
 OUT = os.path.join(HERE, "libfastlpr_amd.so")
 
 FLAGS = [

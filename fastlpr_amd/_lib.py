@@ -33,6 +33,7 @@ EXPORTS = (
     "flpr_max_moment",
     "flpr_key_to_double",
     "flpr_solve",
+    "flpr_solve_refined",
     "flpr_adjoint",
     "flpr_quadform",
     "flpr_interp_eval",
@@ -55,6 +56,13 @@ EXPORTS = (
     "flpr_batch_fit_output",
     "flpr_batch_spline_coef",
     "flpr_batch_point_sums",
+    "flpr_mlp_param_count",
+    "flpr_mlp_gemm",
+    "flpr_mlp_forward",
+    "flpr_mlp_backward",
+    "flpr_mlp_loss",
+    "flpr_mlp_adam",
+    "flpr_bracket_features",
 )
 
 
@@ -71,7 +79,7 @@ class FlprGeom(C.Structure):
         ("tile", C.c_int32 * MAX_DIM),
         ("ntile", C.c_int32 * MAX_DIM),
         ("item_points", C.c_int32),
-        ("reserved_", C.c_int32),
+        ("spread_f32", C.c_int32),
         ("hx", C.c_double * MAX_DIM),
         ("inv4tau", C.c_double * MAX_DIM),
         ("mean", C.c_double * MAX_DIM),
@@ -124,6 +132,8 @@ _SIGS = {
     "flpr_max_moment": [_I32, _P, _I64, _I32, C.POINTER(_I64), _I32, _I64, _I32, _P, _P],
     "flpr_solve": [_I32, _P, _I64, _I32, _I64, _I32, _I32, _I32, C.POINTER(_I64), C.POINTER(_I64), _P,
                    _D, _D, _P, _I32, _I32, _P, _P, _P, _P],
+    "flpr_solve_refined": [_P, _I64, _I32, _I64, _I32, _I32, C.POINTER(_I64), C.POINTER(_I64), _P,
+                           _D, _D, _P, _I32, _I32, _P, _P, _P, _P],
     "flpr_adjoint": [_GP, _P, _P, _P, _P, _I64, _I32, _P, _P, _P, _P],
     "flpr_quadform": [_GP, _P, _I32, _I32, _P, _P, _P],
     "flpr_interp_eval": [_GP, _P, _I64, _P, _I32, _P, _P, _P, _P],
@@ -152,6 +162,14 @@ _SIGS = {
     "flpr_batch_spline_coef": [_GP, _I32, _P, _I64, _P, _I32, _P, _P, _P],
     "flpr_batch_point_sums": [_GP, _P, _I32, _I64, _P, _P, _I32, _P, _I32, _P, _P, _I32, _I32,
                               _I32, _P, _P, _I64, _P],
+    "flpr_mlp_param_count": [],
+    "flpr_mlp_gemm": [_I32, _I32, _I32, _P, _I32, _I32, _P, _I32, _I32, _P, _I32, _P, _P, _I32, _P],
+    "flpr_mlp_forward": [_P, _I32, _P, _P, _P, _P, _P, _P, _P],
+    "flpr_mlp_backward": [_P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "flpr_mlp_loss": [_I32, _P, _P, C.c_float, C.c_float, _P, _P, _P],
+    "flpr_mlp_adam": [_I32, _P, _P, _P, _P, C.c_float, C.c_float, C.c_float, C.c_float, _I32,
+                      C.c_float, _P],
+    "flpr_bracket_features": [_I32, _P, _I64, _I32, _I32, _P, _P, _P, _P, _P, _P, _P],
 }
 
 _lib = None

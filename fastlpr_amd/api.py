@@ -9,9 +9,17 @@ Hutchinson probes are the reference's), and the O(dh) GCV / LCV selection.
 Inputs may be NumPy arrays (results come back as NumPy) or torch tensors on a
 GPU (results stay on the device as torch tensors, nothing crosses PCIe).
 
-Extra option (no counterpart in the reference):
+Extra options (no counterpart in the reference):
   ``precision``: "fp32" (default; float32 MFMA convolution with float64
-  spread and float64 solves) or "fp64" (float64 end to end).
+  spread and float64 solves), "fp64" (float64 end to end) or "mixed" (the
+  precision form: float32 spread with compensated summation, float32
+  convolution, float32 solves with one float64 refinement step and a
+  float64 fallback above a condition estimate of 1e6).
+  ``bandwidth_bracket``: "full" (default, every bandwidth) or "learned"
+  (2-D real single-column mean fits: a trained MLP proposes a bracket of
+  five consecutive bandwidths around the GCV minimum, only those are solved,
+  with the regularisation of the whole list; when the chosen one sits on an
+  inner edge of the bracket the full sweep runs instead; see ``bracket``).
 """
 from __future__ import annotations
 
@@ -378,7 +386,8 @@ def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
     options as the reference: order, regularization, num_dof_sample, dof_seed,
     random_matrix, y_type_out ('mean' | 'variance'), calc_dof, gcv_selection,
     dstd, accuracy, N, xrange, flag_power2, dof_interp_method,
-    gcv_interp_method, interp_method; plus ``precision`` ('fp32' | 'fp64').
+    gcv_interp_method, interp_method; plus ``precision`` ('fp32' | 'fp64') and
+    ``bandwidth_bracket`` ('full' | 'learned').
     """
     finish, _ = _fit(x, y, h, options)
     return finish()
@@ -501,7 +510,11 @@ def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
     # range needs the x statistics
     bw_given = _prepare_bandwidths(h, dims, opts, None) if h is not None else None
     variance = str(opts.get("y_type_out", "mean")) == "variance"
-    fp64 = str(opts.get("precision", "fp32")).lower() == "fp64"
+    prec = str(opts.get("precision", "fp32")).lower()
+    if prec not in ("fp32", "fp64", "mixed"):
+        raise ValueError("options['precision'] must be 'fp32', 'fp64' or 'mixed'")
+    fp64 = prec == "fp64"
+    mixed = prec == "mixed"
     N = _grid_shape(opts, n, dims)
     y_is_complex = bool(y_arr.is_complex()) if isinstance(y_arr, torch.Tensor) \
         else bool(np.iscomplexobj(y_arr))
@@ -557,13 +570,14 @@ def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
 
     share = {"key": None, "probe_dots": None}
     reuse = False
+    binfo = None
     if not calc_dof:
         method = opts.get("interp_method", "linear")
         if method != "linear":
             raise NotImplementedError("only linear interpolation is implemented on the GPU")
         fit = engine.run_grid(xd, yd, ny, None, 0, bandwidths, order, N, int(opts["accuracy"]),
                               fp64, reg, xrange, flag_power2, stats=stats, with_ones_eps=False,
-                              y_pairs=y_is_complex)
+                              y_pairs=y_is_complex, mixed=mixed)
         sel = 0
         sel_dev = sel_out = None
     else:
@@ -592,42 +606,63 @@ def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
         # regression.py:86-87, 756-758); only y takes the option's (api.py:632-642)
         acc_n = max(6 - int(np.ceil(np.log10(n))), 4)
         key = (xd.data_ptr(), xd._version, xd.shape, np.asarray(bandwidths).tobytes(), order,
-               tuple(N), acc_n, int(opts["accuracy"]), fp64, reg, repr(xrange), flag_power2, K,
+               tuple(N), acc_n, int(opts["accuracy"]), prec, reg, repr(xrange), flag_power2, K,
                probes.data_ptr(), y_is_complex, ny)
         reuse = (prior is not None and not cubic and prior["key"] == key)
         xshare = {} if (keep_x and not cubic) else None
-        if reuse and prior.get("xshare"):
-            fit = engine.run_grid_y(prior["xshare"], xd, yd, ny, y_is_complex)
-        else:
-            fit = engine.run_grid(xd, yd, ny, None if reuse else probes, 0 if reuse else K,
-                                  bandwidths, order, N, acc_n,
-                                  fp64, reg, xrange, flag_power2, want_rss=True, stats=stats,
-                                  readback=False, keep_all=cubic,
-                                  accuracy_y=int(opts["accuracy"]), y_pairs=y_is_complex,
-                                  share_out=xshare)
-        if xshare:
-            share["xshare"] = xshare
-        dh = fit.h.shape[0]
-        if reuse:
-            # y dots, the first pass's probe dots, then the quadratic forms
-            red = torch.empty((dh * (ny + K) + dh * ny,), dtype=torch.float64, device=dev)
-            dots = red[:dh * (ny + K)].view(dh, ny + K)
-            dots[:, :ny] = fit.red[:dh * ny].view(dh, ny)
-            dots[:, ny:] = prior["probe_dots"]
-            red[dh * (ny + K):] = fit.red[dh * ny:]
-            fit.red = red
-        elif not cubic:
-            share["key"] = key
-            share["probe_dots"] = fit.red[:dh * (ny + K)].view(dh, ny + K)[:, ny:].clone()
-        ysq_dev = stats.ysq_dev
-        if cubic:
-            fit, ysq_dev = _point_sums(fit, xd, yd, probes, ny, K, gcv_method, dof_method)
         agg = str(opts.get("gcv_selection", "first"))
-        # Hutchinson DOF, GCV and the 1-SE choice stay on the device; the
-        # interpolation at the chosen bandwidth is queued behind them and the
-        # host reads everything back once, at the end
-        sel_out, sel_dev = engine.select(fit, ysq_dev, pp_dev, n, ny, K, y_is_complex,
-                                         agg, float(opts.get("dstd", 0.0)))
+        bracket_mode = str(opts.get("bandwidth_bracket", "full")).lower()
+        if bracket_mode not in ("full", "learned"):
+            raise ValueError("options['bandwidth_bracket'] must be 'full' or 'learned'")
+        use_bracket = (bracket_mode == "learned" and dims == 2 and not cubic and not reuse
+                       and prior is None and not keep_x and not y_is_complex and ny == 1
+                       and not variance and xrange is None)
+
+        def sweep(run_h, select_h=None):
+            """run_grid over the bandwidths run_h, then the device selection."""
+            if reuse and prior.get("xshare"):
+                f = engine.run_grid_y(prior["xshare"], xd, yd, ny, y_is_complex)
+            else:
+                f = engine.run_grid(xd, yd, ny, None if reuse else probes, 0 if reuse else K,
+                                    run_h, order, N, acc_n,
+                                    fp64, reg, xrange, flag_power2, want_rss=True, stats=stats,
+                                    readback=False, keep_all=cubic,
+                                    accuracy_y=int(opts["accuracy"]), y_pairs=y_is_complex,
+                                    share_out=xshare, mixed=mixed, select_h=select_h)
+            if xshare:
+                share["xshare"] = xshare
+            dh_ = f.h.shape[0]
+            if reuse:
+                # y dots, the first pass's probe dots, then the quadratic forms
+                red = torch.empty((dh_ * (ny + K) + dh_ * ny,), dtype=torch.float64, device=dev)
+                dots = red[:dh_ * (ny + K)].view(dh_, ny + K)
+                dots[:, :ny] = f.red[:dh_ * ny].view(dh_, ny)
+                dots[:, ny:] = prior["probe_dots"]
+                red[dh_ * (ny + K):] = f.red[dh_ * ny:]
+                f.red = red
+            elif not cubic:
+                share["key"] = key
+                share["probe_dots"] = f.red[:dh_ * (ny + K)].view(dh_, ny + K)[:, ny:].clone()
+            ysq = stats.ysq_dev
+            if cubic:
+                f, ysq = _point_sums(f, xd, yd, probes, ny, K, gcv_method, dof_method)
+            # Hutchinson DOF, GCV and the 1-SE choice stay on the device; the
+            # interpolation at the chosen bandwidth is queued behind them and
+            # the host reads everything back once, at the end
+            so, sd = engine.select(f, ysq, pp_dev, n, ny, K, y_is_complex, agg,
+                                   float(opts.get("dstd", 0.0)))
+            return f, so, sd
+
+        binfo = None
+        if use_bracket:
+            from . import bracket
+            binfo = {"n": n}
+            fit, sel_out, sel_dev = sweep(bandwidths, bracket.selector(bracket.model(),
+                                                                       stats.ysq_dev, binfo))
+            if "indices" not in binfo:
+                binfo = None
+        else:
+            fit, sel_out, sel_dev = sweep(bandwidths)
     if fit.warn:
         warnings.warn(fit.warn, stacklevel=2)
 
@@ -656,9 +691,20 @@ def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
         return t.cpu().numpy() if as_numpy else t
 
     def finish():
-        return _finish(fit, sel_out, fac if variance else None, bad_y_dev, yd_in, dh, opts,
-                       order, geo, yhat, resid, grid_var, N, y_is_complex, dy, as_numpy,
-                       reuse, conv)
+        r = _finish(fit, sel_out, fac if variance else None, bad_y_dev, yd_in, dh, opts,
+                    order, geo, yhat, resid, grid_var, N, y_is_complex, dy, as_numpy,
+                    reuse, conv)
+        if binfo is not None:
+            # the minimum on an inner edge of the bracket may lie outside it:
+            # then the full sweep decides
+            s_b = int(r.gcv_yhat["id1se"])
+            idx = binfo["indices"]
+            if (s_b == 0 and idx[0] > 0) or (s_b == len(idx) - 1 and idx[-1] < binfo["n_full"] - 1):
+                r = cv_fastlpr(x, y, h, dict(opts, bandwidth_bracket="full"))
+                binfo["fallback"] = True
+            r.metadata["bracket"] = {k: binfo[k] for k in ("indices", "center", "width",
+                                                           "fallback", "hlist")}
+        return r
 
     return finish, share
 
@@ -754,7 +800,11 @@ def cv_fastkde(x, h: Optional[Sequence[Sequence[float]]] = None,
         raise NotImplementedError("only linear interpolation is implemented on the GPU")
     _check_accuracy(int(opts["accuracy"]), dims)
     bw_given = _prepare_bandwidths(h, dims, opts, None) if h is not None else None
-    fp64 = str(opts.get("precision", "fp32")).lower() == "fp64"
+    prec = str(opts.get("precision", "fp32")).lower()
+    if prec not in ("fp32", "fp64", "mixed"):
+        raise ValueError("options['precision'] must be 'fp32', 'fp64' or 'mixed'")
+    fp64 = prec == "fp64"
+    mixed = prec == "mixed"
     N = _grid_shape(opts, n, dims)
     dev = x.device if _is_dev(x) else _device()
     xd = (x_arr.to(dev) if isinstance(x_arr, torch.Tensor)

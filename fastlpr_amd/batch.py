@@ -229,6 +229,8 @@ class BatchPlan:
             if any(cubic) and min(p.N) < 4:
                 methods_ok = False    # the single call raises cubic's error
             p.batched = (bool(o.get("calc_dof", True)) and methods_ok
+                         and str(o.get("precision", "fp32")).lower() != "mixed"
+                         and str(o.get("bandwidth_bracket", "full")).lower() == "full"
                          and acc == acc_n and acc_n <= 6 and o.get("xrange") is None
                          and o.get("random_matrix") is None and K >= 1 and ny + K <= 32
                          and bool(o.get("flag_power2", True))

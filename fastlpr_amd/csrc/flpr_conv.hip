@@ -773,7 +773,15 @@ constexpr int solve_group() { return 4; }
 // keeps no 64-bit base per (moment, channel).  Reads past the block return 0.
 // Batch (blockIdx.z = problem): s[0] = mom, s[1] = shift, s[2] = m_out,
 // s[3] = q, s[4] = dot; one key per problem.
-template <typename T, int NT, bool PIV, bool BUF>
+// REF (the precision form, float32 moments): S + lam I is factored in
+// float32 and z = (S + lam I)^-1 e1 takes one refinement step with the
+// residual e1 - (S + lam I) z formed in float64; a node whose float32
+// Cholesky meets a non-positive pivot, or whose pivot ratio max/min (a cheap
+// lower estimate of the condition number) exceeds REF_COND, is solved in
+// float64 as without REF.
+constexpr float REF_COND = 1e6f;
+
+template <typename T, int NT, bool PIV, bool BUF, bool REF = false>
 __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(
     const T* __restrict__ mom, int64_t per_h, int64_t nodes, SolveTab tab,
     const uint64_t* __restrict__ key, double lam_scale, double eps_add,
@@ -841,6 +849,89 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(
       z[0] = 1.0;
     } else {
       const double lam = kmax * lam_scale;  // regression.py:243-267
+      bool done = false;
+      if constexpr (REF) {
+        float F[NT][NT];
+        int q = 0;
+#pragma unroll
+        for (int i = 0; i < NT; ++i)
+#pragma unroll
+          for (int j = i; j < NT; ++j) {
+            const float v = (float)mload(tab.s_off[q], node);
+            F[i][j] = v;
+            F[j][i] = v;
+            ++q;
+          }
+        const float lamf = (float)lam;
+#pragma unroll
+        for (int i = 0; i < NT; ++i) F[i][i] += lamf;
+        float dmin = INFINITY, dmax = 0.0f;
+        bool indef = false;
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          float dsum = F[j][j];
+#pragma unroll
+          for (int k = 0; k < j; ++k) dsum -= F[j][k] * F[j][k];
+          indef |= !(dsum > 0.0f);
+          dmin = fminf(dmin, dsum);
+          dmax = fmaxf(dmax, dsum);
+          const float inv = 1.0f / sqrtf(dsum);
+          F[j][j] = inv;
+#pragma unroll
+          for (int i = j + 1; i < NT; ++i) {
+            float sm = F[i][j];
+#pragma unroll
+            for (int k = 0; k < j; ++k) sm -= F[i][k] * F[j][k];
+            F[i][j] = sm * inv;
+          }
+        }
+        if (!indef && dmax <= REF_COND * dmin) {
+          // (L L')^-1 b in float32
+          auto chol_solve = [&](const float (&b)[NT], float (&x)[NT]) {
+            float yv[NT];
+#pragma unroll
+            for (int i = 0; i < NT; ++i) {
+              float sm = b[i];
+#pragma unroll
+              for (int k = 0; k < i; ++k) sm -= F[i][k] * yv[k];
+              yv[i] = sm * F[i][i];
+            }
+#pragma unroll
+            for (int i = NT - 1; i >= 0; --i) {
+              float sm = yv[i];
+#pragma unroll
+              for (int k = i + 1; k < NT; ++k) sm -= F[k][i] * x[k];
+              x[i] = sm * F[i][i];
+            }
+          };
+          float e[NT], zf[NT], dz[NT];
+#pragma unroll
+          for (int i = 0; i < NT; ++i) e[i] = (i == 0) ? 1.0f : 0.0f;
+          chol_solve(e, zf);
+          // residual in float64 from the moments
+          double r[NT];
+#pragma unroll
+          for (int i = 0; i < NT; ++i) r[i] = (i == 0) ? 1.0 : 0.0;
+          q = 0;
+#pragma unroll
+          for (int i = 0; i < NT; ++i)
+#pragma unroll
+            for (int j = i; j < NT; ++j) {
+              const double v = (double)mload(tab.s_off[q], node) + (i == j ? lam : 0.0);
+              r[i] -= v * (double)zf[j];
+              if (i != j) r[j] -= v * (double)zf[i];
+              ++q;
+            }
+          float rf[NT];
+#pragma unroll
+          for (int i = 0; i < NT; ++i) rf[i] = (float)r[i];
+          chol_solve(rf, dz);
+#pragma unroll
+          for (int i = 0; i < NT; ++i) z[i] = (double)zf[i] + (double)dz[i];
+          done = true;
+        }
+      }
+      if (!done) {
       double A[NT][NT];
       int p = 0;
 #pragma unroll
@@ -944,6 +1035,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(
           z[i] = s / A[i][i];
         }
       }
+      }  // !done
       // z . T_0 (ones channel) for the centring correction
 #pragma unroll
       for (int i = 0; i < NT; ++i) zt0 += z[i] * (double)t0v[i];
@@ -1001,8 +1093,19 @@ template <typename T, int NT>
 void launch_solve(hipStream_t st, dim3 grd, bool piv, const T* mom, int64_t per_h,
                   int64_t nodes, const SolveTab& tab, const uint64_t* key, double lam_scale,
                   double eps_add, const double* shift, int nval, int nout, double* m_out,
-                  const double* q, double* dot, const FlprBz& bz) {
+                  const double* q, double* dot, const FlprBz& bz, bool ref = false) {
   const bool buf = sizeof(T) == 4 && per_h * (int64_t)sizeof(T) <= (int64_t)INT32_MAX;
+  if (ref && sizeof(T) == 4 && NT > 1) {
+    if (buf)
+      hipLaunchKernelGGL((k_solve<T, NT, false, sizeof(T) == 4, true>), grd, dim3(SOLVE_THREADS),
+                         0, st, mom, per_h, nodes, tab, key, lam_scale, eps_add, shift, nval,
+                         nout, m_out, q, dot, bz);
+    else
+      hipLaunchKernelGGL((k_solve<T, NT, false, false, true>), grd, dim3(SOLVE_THREADS), 0, st,
+                         mom, per_h, nodes, tab, key, lam_scale, eps_add, shift, nval, nout,
+                         m_out, q, dot, bz);
+    return;
+  }
   if (piv && NT > 1)
     hipLaunchKernelGGL((k_solve<T, NT, true, false>), grd, dim3(SOLVE_THREADS), 0, st, mom,
                        per_h, nodes, tab, key, lam_scale, eps_add, shift, nval, nout, m_out, q, dot, bz);
@@ -1020,11 +1123,11 @@ int dispatch_solve(int nt, hipStream_t st, dim3 grd, bool piv, const T* mom, int
                    int64_t nodes,
                    const SolveTab& tab, const uint64_t* key, double lam_scale, double eps_add,
                    const double* shift, int nval, int nout, double* m_out, const double* q,
-                   double* dot, const FlprBz& bz) {
+                   double* dot, const FlprBz& bz, bool ref = false) {
 #define FLPR_SOLVE_CASE(N)                                                                   \
   case N:                                                                                    \
     launch_solve<T, N>(st, grd, piv, mom, per_h, nodes, tab, key, lam_scale, eps_add, shift, \
-                       nval, nout, m_out, q, dot, bz);                                       \
+                       nval, nout, m_out, q, dot, bz, ref);                                  \
     return 0;
   switch (nt) {
     FLPR_SOLVE_CASE(1)
@@ -1175,6 +1278,32 @@ int flpr_solve(int32_t fp64, const void* mom, int64_t per_h, int32_t dh, int64_t
   else
     rc = dispatch_solve<float>(nt, st, grd, pivot != 0, (const float*)mom, per_h, nodes, tab, key,
                                lam_scale, eps_add, shift, nval, nout, m_out, q, dot, FlprBz{});
+  if (rc) return rc;
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
+int flpr_solve_refined(const void* mom, int64_t per_h, int32_t dh, int64_t nodes,
+                       int32_t order, int32_t nt, const int64_t* s_off, const int64_t* t_off,
+                       const uint64_t* key, double lam_scale, double eps_add,
+                       const double* shift, int32_t nval, int32_t nout, double* m_out,
+                       const double* q, double* dot, void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!mom || !s_off || !t_off || !key || dh < 1 || dh > 65535 || nodes < 1 || nt < 1 ||
+      nt > FLPR_MAX_TERMS || nval < 0 || nval > MAXV || nout < 0 || nout > nval)
+    return -1;
+  if ((order == 0) != (nt == 1)) return -1;
+  if (nout > 0 && !m_out) return -1;
+  if (q && !dot) return -1;
+  SolveTab tab;
+  const int ns = nt * (nt + 1) / 2;
+  for (int i = 0; i < ns; ++i) tab.s_off[i] = s_off[i];
+  for (int i = 0; i < nt; ++i) tab.t_off[i] = t_off[i];
+  const int bx = flpr_grid_for(nodes, SOLVE_THREADS, 1024 / (dh > 64 ? 64 : dh) + 1);
+  dim3 grd((unsigned)bx, (unsigned)dh);
+  const int rc = dispatch_solve<float>(nt, as_stream(stream), grd, false, (const float*)mom, per_h,
+                                       nodes, tab, key, lam_scale, eps_add, shift, nval, nout,
+                                       m_out, q, dot, FlprBz{}, true);
   if (rc) return rc;
   FLPR_CHECK_LAUNCH();
   return 0;

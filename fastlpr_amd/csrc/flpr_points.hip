@@ -528,12 +528,12 @@ __device__ inline void local_coords(const FlprGeom& g, int lc, int* l) {
 }
 
 // Footprint weights along one axis: w[j] = A B^j G[j], j = 0 .. 2 MSP.
-template <int MSP>
-__device__ inline void tap_weights(double A, double B, const double* G, double* w) {
-  double bp = A;
+template <int MSP, typename T>
+__device__ inline void tap_weights(T A, T B, const double* G, T* w) {
+  T bp = A;
 #pragma unroll
   for (int j = 0; j <= 2 * MSP; ++j) {
-    w[j] = bp * G[j];
+    w[j] = bp * (T)G[j];
     bp *= B;
   }
 }
@@ -592,14 +592,20 @@ struct SpreadTab {
 // (2 MSP + 1)^2 registers or fewer).  The block is added once into the tile's
 // LDS window, and the window is written to the tile's partial slot
 // (k_spread_gather sums the overlapping windows per cell, without atomics).
+// A = float (the precision form's spread, g.spread_f32): the weights and the
+// footprint block are float32 and each accumulator carries a Kahan
+// compensation term; the LDS window and the partial slots are float32 (the
+// gather adds the windows in float64).
 // Batch: s[0] = key_start (the problem's key count), s[1] = part; a
 // problem's cell split follows its own point count (items).
-template <int D, int MSP>
+template <int D, int MSP, typename A>
 __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_cells(
     FlprGeom g, SpreadTab tab, const int32_t* __restrict__ key_start,
     const double* __restrict__ wab, const double* __restrict__ vals, int64_t n, int with_ones,
-    int C, int cg, int nsplit, double* __restrict__ part, FlprBz bz) {
-  extern __shared__ double lds[];
+    int C, int cg, int nsplit, A* __restrict__ part, FlprBz bz) {
+  extern __shared__ char lds_raw[];
+  A* lds = reinterpret_cast<A*>(lds_raw);
+  constexpr bool KAHAN = sizeof(A) == 4;
   key_start += blockIdx.z * bz.s[0];
   part += blockIdx.z * bz.s[1];
   constexpr int ntap = 2 * MSP + 1;
@@ -620,7 +626,7 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_cells(
     fcells *= fp[k];
     tc *= g.tile[k];
   }
-  for (int i = threadIdx.x; i < fcells * cn; i += blockDim.x) lds[i] = 0.0;
+  for (int i = threadIdx.x; i < fcells * cn; i += blockDim.x) lds[i] = A(0);
   __syncthreads();
   const int kbase = blockIdx.x * tc;
   if (bz.items)
@@ -652,9 +658,24 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_cells(
       r[0] = row / ntap;
       r[1] = row % ntap;
     }
-    double acc[NACC];
+    A acc[NACC];
+    A cmp[KAHAN ? NACC : 1];
 #pragma unroll
-    for (int a = 0; a < NACC; ++a) acc[a] = 0.0;
+    for (int a = 0; a < NACC; ++a) acc[a] = A(0);
+#pragma unroll
+    for (int a = 0; a < (KAHAN ? NACC : 1); ++a) cmp[a] = A(0);
+    // acc[a] += v, compensated in float32 (Kahan): cmp holds minus the
+    // low-order part lost so far
+    auto add = [&](int a, A v) {
+      if constexpr (KAHAN) {
+        const A yv = v - cmp[a];
+        const A t = acc[a] + yv;
+        cmp[a] = (t - acc[a]) - yv;
+        acc[a] = t;
+      } else {
+        acc[a] += v;
+      }
+    };
     // software pipeline, two points deep: the data of points p + 1 and p + 2
     // is in flight while point p's footprint is accumulated
     double nxt[2][2 * D];
@@ -673,32 +694,33 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_cells(
         cur[k] = nxt[0][k];
         nxt[0][k] = nxt[1][k];
       }
-      double s = nxv[0];
+      A s = (A)nxv[0];
       nxv[0] = nxv[1];
       if (p + 2 < q1) {
 #pragma unroll
         for (int k = 0; k < 2 * D; ++k) nxt[1][k] = wab[(int64_t)(p + 2) * (2 * D) + k];
         nxv[1] = ones ? 1.0 : vp[(int64_t)(p + 2) * nv];
       }
-      double w[D][ntap];
+      A w[D][ntap];
 #pragma unroll
-      for (int k = 0; k < D; ++k) tap_weights<MSP>(cur[2 * k], cur[2 * k + 1], tab.G[k], w[k]);
+      for (int k = 0; k < D; ++k)
+        tap_weights<MSP, A>((A)cur[2 * k], (A)cur[2 * k + 1], tab.G[k], w[k]);
 #pragma unroll
       for (int k = 0; k < FIX; ++k) {
-        double sel = 0.0;
+        A sel = A(0);
 #pragma unroll
         for (int j = 0; j < ntap; ++j) sel = (j == r[k]) ? w[k][j] : sel;
         s *= sel;
       }
       if constexpr (F == 1) {
 #pragma unroll
-        for (int j = 0; j < ntap; ++j) acc[j] += s * w[D - 1][j];
+        for (int j = 0; j < ntap; ++j) add(j, s * w[D - 1][j]);
       } else {
 #pragma unroll
         for (int a = 0; a < ntap; ++a) {
-          const double sa = s * w[D - 2][a];
+          const A sa = s * w[D - 2][a];
 #pragma unroll
-          for (int b = 0; b < ntap; ++b) acc[a * ntap + b] += sa * w[D - 1][b];
+          for (int b = 0; b < ntap; ++b) add(a * ntap + b, sa * w[D - 1][b]);
         }
       }
     }
@@ -706,20 +728,25 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_cells(
     int base = 0;
 #pragma unroll
     for (int k = 0; k < D; ++k) base = base * fp[k] + pos[k] + (k < FIX ? r[k] : 0);
+    // the compensated sum is acc - cmp
+    auto total = [&](int a) -> A {
+      if constexpr (KAHAN) return acc[a] - cmp[a];
+      else return acc[a];
+    };
     if constexpr (F == 1) {
 #pragma unroll
-      for (int j = 0; j < ntap; ++j) atomicAdd(&lds[(base + j) * cn + c], acc[j]);
+      for (int j = 0; j < ntap; ++j) atomicAdd(&lds[(base + j) * cn + c], total(j));
     } else {
       const int sa = fp[D - 1];
 #pragma unroll
       for (int a = 0; a < ntap; ++a)
 #pragma unroll
         for (int b = 0; b < ntap; ++b)
-          atomicAdd(&lds[(base + a * sa + b) * cn + c], acc[a * ntap + b]);
+          atomicAdd(&lds[(base + a * sa + b) * cn + c], total(a * ntap + b));
     }
   }
   __syncthreads();
-  double* dst = part + ((int64_t)blockIdx.x * C + c0) * fcells;
+  A* dst = part + ((int64_t)blockIdx.x * C + c0) * fcells;
   for (int e = threadIdx.x; e < fcells * cn; e += blockDim.x) {
     const int c = e / fcells, cell = e % fcells;
     dst[(int64_t)c * fcells + cell] = lds[cell * cn + c];
@@ -732,8 +759,8 @@ __device__ inline int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a +
 // Sum of the overlapping tile windows for every spread-grid cell (channel =
 // blockIdx.y); also the per-channel grid sums used by the centring step.
 // Batch: s[0] = part, s[1] = grid, s[2] = sums.
-template <int D>
-__global__ __launch_bounds__(256) void k_spread_gather(FlprGeom g, const double* __restrict__ part,
+template <int D, typename A>
+__global__ __launch_bounds__(256) void k_spread_gather(FlprGeom g, const A* __restrict__ part,
                                                        int C, int64_t cells,
                                                        double* __restrict__ grid,
                                                        double* __restrict__ sums, FlprBz bz) {
@@ -766,13 +793,13 @@ __global__ __launch_bounds__(256) void k_spread_gather(FlprGeom g, const double*
     double v = 0.0;
     if constexpr (D == 1) {
       for (int t0 = tlo[0]; t0 <= thi[0]; ++t0)
-        v += part[((int64_t)t0 * C + c) * fcells + (sc[0] - (t0 * g.tile[0] - msp))];
+        v += (double)part[((int64_t)t0 * C + c) * fcells + (sc[0] - (t0 * g.tile[0] - msp))];
     } else if constexpr (D == 2) {
       for (int t0 = tlo[0]; t0 <= thi[0]; ++t0)
         for (int t1 = tlo[1]; t1 <= thi[1]; ++t1) {
           const int64_t tile = (int64_t)t0 * g.ntile[1] + t1;
           const int l0 = sc[0] - (t0 * g.tile[0] - msp), l1 = sc[1] - (t1 * g.tile[1] - msp);
-          v += part[(tile * C + c) * fcells + l0 * fp[1] + l1];
+          v += (double)part[(tile * C + c) * fcells + l0 * fp[1] + l1];
         }
     } else {
       for (int t0 = tlo[0]; t0 <= thi[0]; ++t0)
@@ -781,7 +808,7 @@ __global__ __launch_bounds__(256) void k_spread_gather(FlprGeom g, const double*
             const int64_t tile = ((int64_t)t0 * g.ntile[1] + t1) * g.ntile[2] + t2;
             const int l0 = sc[0] - (t0 * g.tile[0] - msp), l1 = sc[1] - (t1 * g.tile[1] - msp),
                       l2 = sc[2] - (t2 * g.tile[2] - msp);
-            v += part[(tile * C + c) * fcells + (l0 * fp[1] + l1) * fp[2] + l2];
+            v += (double)part[(tile * C + c) * fcells + (l0 * fp[1] + l1) * fp[2] + l2];
           }
     }
     grid[(int64_t)c * cells + lin] = v;
@@ -1842,14 +1869,15 @@ int lds_group(int64_t cells_per_channel, int channels, int cap, int64_t budget =
   return cg;
 }
 
-template <int D>
-void launch_spread(int msp, dim3 grd, size_t lds, hipStream_t st, const FlprGeom& g,
-                   const SpreadTab& tab, const int32_t* ks, const double* wab, const double* vals,
-                   int64_t n, int wo, int C, int cg, int nsplit, double* part, const FlprBz& bz) {
+template <int D, typename A>
+void launch_spread_t(int msp, dim3 grd, size_t lds, hipStream_t st, const FlprGeom& g,
+                     const SpreadTab& tab, const int32_t* ks, const double* wab,
+                     const double* vals, int64_t n, int wo, int C, int cg, int nsplit, A* part,
+                     const FlprBz& bz) {
 #define FLPR_SPREAD_CASE(M)                                                                    \
   case M:                                                                                      \
-    hipLaunchKernelGGL((k_spread_cells<D, M>), grd, dim3(SPREAD_THREADS), lds, st, g, tab, ks, \
-                       wab, vals, n, wo, C, cg, nsplit, part, bz);                             \
+    hipLaunchKernelGGL((k_spread_cells<D, M, A>), grd, dim3(SPREAD_THREADS), lds, st, g, tab,  \
+                       ks, wab, vals, n, wo, C, cg, nsplit, part, bz);                         \
     break;
   switch (msp) {
     FLPR_SPREAD_CASE(0)
@@ -1875,6 +1903,31 @@ void launch_spread(int msp, dim3 grd, size_t lds, hipStream_t st, const FlprGeom
     }
   }
 #undef FLPR_SPREAD_CASE
+}
+
+// The spread of g (float32 compensated accumulation when g.spread_f32; the
+// partial windows are then float32, half the bytes of part).
+template <int D>
+void launch_spread(int msp, dim3 grd, size_t lds, hipStream_t st, const FlprGeom& g,
+                   const SpreadTab& tab, const int32_t* ks, const double* wab, const double* vals,
+                   int64_t n, int wo, int C, int cg, int nsplit, double* part, const FlprBz& bz) {
+  if (g.spread_f32)
+    launch_spread_t<D, float>(msp, grd, lds / 2, st, g, tab, ks, wab, vals, n, wo, C, cg, nsplit,
+                              reinterpret_cast<float*>(part), bz);
+  else
+    launch_spread_t<D, double>(msp, grd, lds, st, g, tab, ks, wab, vals, n, wo, C, cg, nsplit,
+                               part, bz);
+}
+
+template <int D>
+void launch_gather(dim3 gg, hipStream_t st, const FlprGeom& g, const double* part, int C,
+                   int64_t cells, double* grid, double* sums, const FlprBz& bz) {
+  if (g.spread_f32)
+    hipLaunchKernelGGL((k_spread_gather<D, float>), gg, dim3(256), 0, st, g,
+                       reinterpret_cast<const float*>(part), C, cells, grid, sums, bz);
+  else
+    hipLaunchKernelGGL((k_spread_gather<D, double>), gg, dim3(256), 0, st, g, part, C, cells,
+                       grid, sums, bz);
 }
 
 // Channels per block: all of them when the tiles alone fill the device, else
@@ -2103,9 +2156,9 @@ int flpr_spread(const FlprGeom* g, const int32_t* key_start, const double* wab,
   FLPR_CHECK_LAUNCH();
   dim3 gg(flpr_grid_for(cells, 256, 256), (unsigned)C);
   switch (g->d) {
-    case 1: hipLaunchKernelGGL(k_spread_gather<1>, gg, dim3(256), 0, st, *g, part, C, cells, grid, sums, FlprBz{}); break;
-    case 2: hipLaunchKernelGGL(k_spread_gather<2>, gg, dim3(256), 0, st, *g, part, C, cells, grid, sums, FlprBz{}); break;
-    default: hipLaunchKernelGGL(k_spread_gather<3>, gg, dim3(256), 0, st, *g, part, C, cells, grid, sums, FlprBz{}); break;
+    case 1: launch_gather<1>(gg, st, *g, part, C, cells, grid, sums, FlprBz{}); break;
+    case 2: launch_gather<2>(gg, st, *g, part, C, cells, grid, sums, FlprBz{}); break;
+    default: launch_gather<3>(gg, st, *g, part, C, cells, grid, sums, FlprBz{}); break;
   }
   FLPR_CHECK_LAUNCH();
   return 0;
@@ -2442,9 +2495,9 @@ int flpr_batch_spread(const FlprGeom* g, const FlprBatchItem* items, int32_t P, 
   bg.s[2] = C;
   dim3 gg(flpr_grid_for(cells, 256, 256), (unsigned)C, (unsigned)P);
   switch (g->d) {
-    case 1: hipLaunchKernelGGL(k_spread_gather<1>, gg, dim3(256), 0, st, *g, part, C, cells, grid, sums, bg); break;
-    case 2: hipLaunchKernelGGL(k_spread_gather<2>, gg, dim3(256), 0, st, *g, part, C, cells, grid, sums, bg); break;
-    default: hipLaunchKernelGGL(k_spread_gather<3>, gg, dim3(256), 0, st, *g, part, C, cells, grid, sums, bg); break;
+    case 1: launch_gather<1>(gg, st, *g, part, C, cells, grid, sums, bg); break;
+    case 2: launch_gather<2>(gg, st, *g, part, C, cells, grid, sums, bg); break;
+    default: launch_gather<3>(gg, st, *g, part, C, cells, grid, sums, bg); break;
   }
   FLPR_CHECK_LAUNCH();
   return 0;

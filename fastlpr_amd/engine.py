@@ -443,22 +443,20 @@ def conv_schedule(ps: Pass, fp64: bool, bands: np.ndarray):
     sb = np.maximum(i0[None, :] + ps.base - bnd[:, 1:2], 0)
     se = np.minimum(i0[None, :] + CONV_LT - 1 + ps.base - bnd[:, 0:1], ps.K - 1)
     tw = np.where(se >= sb, (se - sb) // kc + 2, 1)                  # (nj, ntt)
-    ids, works, jobs_of = [], [], []
-    for j in range(nj):
-        if ps.mode == 0:
-            nrow = -(-int(ro[j]) // CONV_LD[bool(fp64)])
-            tt, td = np.divmod(np.arange(nrow * ntt), nrow)
-            loc = td + ntd * tt
-        else:
-            o, rem = np.divmod(np.arange(int(tiles[j])), ntt * ntd)
-            tt, td = np.divmod(rem, ntd)
-            loc = td + ntd * (tt + ntt * o)
-        ids.append(loc)
-        works.append(tw[j, tt])
-        jobs_of.append(np.full(len(loc), j))
-    loc = np.concatenate(ids)
-    wk = np.concatenate(works).astype(np.float64)
-    jb = np.concatenate(jobs_of)
+    # every tile of every job, vectorised over jobs: tile k of job j is local
+    # index k in [0, tiles[j])
+    cnts = tiles.astype(np.int64)
+    jb = np.repeat(np.arange(nj), cnts)
+    k = np.arange(int(cnts.sum()), dtype=np.int64) - np.repeat(np.cumsum(cnts) - cnts, cnts)
+    if ps.mode == 0:
+        nrow = np.repeat(-(-ro // CONV_LD[bool(fp64)]), cnts)
+        tt, td = np.divmod(k, nrow)
+        loc = td + ntd * tt
+    else:
+        o, rem = np.divmod(k, ntt * ntd)
+        tt, td = np.divmod(rem, ntd)
+        loc = td + ntd * (tt + ntt * o)
+    wk = tw[jb, tt].astype(np.float64)
     assert len(loc) == total_tiles
     if ps.mode == 0:
         o = np.argsort(-wk, kind="stable")
@@ -613,7 +611,7 @@ class Geometry:
 def build_geometry(st: Stats, N: Tuple[int, ...], accuracy: int, xrange=None,
                    flag_power2: bool = True, kde: bool = False,
                    margin: Optional[int] = None, tile: Optional[int] = None,
-                   item_points: int = 0) -> Geometry:
+                   item_points: int = 0, spread_f32: bool = False) -> Geometry:
     """Knot-mapping constants exactly as the reference computes them in float64
     (model.py:58-72, convolution.py:169-199, nufftn_type1.py:140-176,
     kernel.py:97-156); the extreme points give the min / max of every
@@ -701,6 +699,7 @@ def build_geometry(st: Stats, N: Tuple[int, ...], accuracy: int, xrange=None,
         axes_raw.append(ax_r)
         axes_scaled.append(ax_s)
     g.item_points = int(item_points)
+    g.spread_f32 = int(bool(spread_f32))
     return Geometry(g, tuple(N), L, msp, xmin_s, xmax_s, axes_raw, axes_scaled,
                     st.mean.copy(), st.std.copy(), centre, margin)
 
@@ -905,18 +904,18 @@ def _memo_put(table: dict, key, value):
 
 def geometry_for(st: Stats, N, accuracy: int, xrange=None, flag_power2: bool = True,
                  kde: bool = False, margin: Optional[int] = None,
-                 tile: Optional[int] = None, item_points: int = 0):
+                 tile: Optional[int] = None, item_points: int = 0, spread_f32: bool = False):
     """build_geometry, memoised on the exact statistics and grid options.
     Returns (geometry, key)."""
     xr = None if xrange is None else tuple(tuple(float(v) for v in r) for r in xrange)
     mg = int(accuracy) if margin is None else max(int(accuracy), int(margin))
     key = (st.xmin.tobytes(), st.xmax.tobytes(), st.mean.tobytes(), st.std.tobytes(),
            tuple(int(v) for v in N), int(accuracy), xr, bool(flag_power2), bool(kde), mg,
-           tile, int(item_points))
+           tile, int(item_points), bool(spread_f32))
     geo = _GEOMS.get(key)
     if geo is None:
         geo = _memo_put(_GEOMS, key, build_geometry(st, N, accuracy, xrange, flag_power2, kde,
-                                                    mg, tile, item_points))
+                                                    mg, tile, item_points, spread_f32))
     return geo, key
 
 
@@ -951,6 +950,7 @@ class DeviceFit:
     plan: Plan
     warn: Optional[str]
     red: Optional[torch.Tensor] = None   # device (dh, nval) dots then (dh, ny) quad forms
+    key: Optional[torch.Tensor] = None   # global regularisation key (key_only)
 
 
 def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
@@ -961,7 +961,9 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
              debug: Optional[dict] = None, readback: bool = True,
              moments_only: bool = False, kde: bool = False,
              keep_all: bool = False, accuracy_y: Optional[int] = None,
-             y_pairs: bool = False, share_out: Optional[dict] = None) -> DeviceFit:
+             y_pairs: bool = False, share_out: Optional[dict] = None,
+             key_only: bool = False, key_in: Optional[torch.Tensor] = None,
+             mixed: bool = False, select_h=None) -> DeviceFit:
     """Spread, convolve and solve for all bandwidths, plus the probe dot products and
     the residual sums.
 
@@ -986,6 +988,19 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     Accuracies above 6 spread onto the oversampled grid and deconvolve
     (spread_fine); y_pairs says that the y columns are (re, im) pairs of
     complex responses, which that deconvolution must keep together.
+    key_only: with moments_only, also reduce the global regularisation key
+    over every bandwidth of h (DeviceFit.key, a device uint64) and stop;
+    select_h: a callable (geo, grid64, cells, sums, h) -> indices or None,
+    called once the spread grid is complete and before it is centred (so
+    channels 0 and 1 are the raw sums of the weights and of y): with
+    indices, only those bandwidths of the filtered list h are convolved
+    with every channel and solved, and the regularisation key is reduced
+    over the whole list from the ones channel alone (the learned bracket).
+    mixed: the precision form (float32 spread with compensated summation,
+    float32 solves with one float64 refinement step, flpr_solve_refined);
+    fp64 must be False.
+    key_in: use this key instead of reducing one over h (a fit on a subset of
+    bandwidths whose lambda must be the full list's, regression.py:243-267).
     share_out: when a dict is given (linear interpolation paths only), it
     receives the x-side state of the fit (sorted points, the ones channel,
     the moment slabs, lambda, the interpolation Gram matrices) for
@@ -1004,7 +1019,9 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     else:
         margin = max(acc_s, acc_y) if split_acc else None
     # with accuracy > 6 the ones channel is not spread onto this grid: msp 0
-    geo, gkey = geometry_for(st, N, 0 if fine_s else acc_s, xrange, flag_power2, kde, margin)
+    geo, gkey = geometry_for(st, N, 0 if fine_s else acc_s, xrange, flag_power2, kde, margin,
+                             spread_f32=mixed and not fp64)
+    refine = mixed and not fp64 and not fine_s
     nval = ny + K
     C = 1 + nval
     g = geo.geom
@@ -1065,7 +1082,8 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     if split_acc and not fine_s and not fine_y:
         # the y columns again, with the footprint and width of their own
         # accuracy, over the channels the first spread wrote
-        geo_y, _ = geometry_for(st, N, int(accuracy_y), xrange, flag_power2, kde, margin)
+        geo_y, _ = geometry_for(st, N, int(accuracy_y), xrange, flag_power2, kde, margin,
+                                spread_f32=mixed and not fp64)
         gy = _lib.C.byref(geo_y.geom)
         wab_y = torch.empty((2 * d * n,), dtype=f64, device=dev)
         vals_yp = torch.empty((ny * n,), dtype=f64, device=dev)
@@ -1080,6 +1098,14 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     if fine_y and not fine_s:
         spread_fine(geo, acc_y, x, n, vals_y, ny, y_pairs, None, 0, False, grid64, sums, 1, s)
     del part
+    if select_h is not None:
+        idx = select_h(geo, grid64, cells, sums, h)
+        if idx is not None:
+            regularisation_key(geo, gkey, h, order, n, fp64, grid64, cells, key_max, s)
+            key_in = key_max
+            h, _w, groups, band_np = bandwidth_tables(geo, gkey, h[np.asarray(idx)], n, order,
+                                                      fp64)
+            dh = h.shape[0]
     any32 = any(not gf for (_a, _b, gf) in groups)
     shift = torch.empty((C,), dtype=f64, device=dev)
     grid32 = torch.empty((C * cells + pad,), dtype=torch.float32, device=dev) if any32 else None
@@ -1125,7 +1151,7 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
                 dst = stage0 if ps.out_stage == 0 else inter[ps.out_stage]
                 launch_conv(ps, gf, ptr(src), ptr(dst), tap_ptr, nmax, band_ptr, bands_g, s)
         del inter
-        if not moments_only:
+        if key_in is None and (not moments_only or key_only):
             if order == 0:
                 offs, absval = [plan.s_off[0]], 0
             else:
@@ -1134,9 +1160,13 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
                  i64_array(offs), len(offs), plan.nodes, absval, ptr(key_max), s)
         stage0s.append((a, b, gf, plan, stage0))
 
+    if key_in is not None:
+        key_max = key_in
     nodes = plan0.nodes
     per_h0 = plan0.stage_per_h[0]
     off0 = plan0.s_off[0]
+    if key_only:
+        return DeviceFit(h, None, None, None, None, geo, plan0, warn, None, key=key_max)
     if moments_only:
         s0_parts = [st0.view(b - a, per_h0)[:, off0:off0 + nodes].to(f64)
                     for (a, b, _gf, _p, st0) in stage0s]
@@ -1159,11 +1189,16 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     nyc = max(nout, 1)
     m_y = torch.empty((dh * nyc * nodes,), dtype=f64, device=dev)
     for (a, b, gf, plan, stage0) in stage0s:
-        call("flpr_solve", int(gf), ptr(stage0), per_h0, b - a, nodes, order, plan.nt,
-             int(fine_s),
-             i64_array(plan.s_off), i64_array(plan.t_off), ptr(key_max), lam_scale, eps_add,
-             ptr(shift), nval, nout, (m_y.data_ptr() + a * nyc * nodes * 8) if nout > 0 else None,
-             ptr(q), (dot.data_ptr() + a * nval * 8) if dot is not None else None, s)
+        m_ptr = (m_y.data_ptr() + a * nyc * nodes * 8) if nout > 0 else None
+        d_ptr = (dot.data_ptr() + a * nval * 8) if dot is not None else None
+        if refine and not gf:
+            call("flpr_solve_refined", ptr(stage0), per_h0, b - a, nodes, order, plan.nt,
+                 i64_array(plan.s_off), i64_array(plan.t_off), ptr(key_max), lam_scale, eps_add,
+                 ptr(shift), nval, nout, m_ptr, ptr(q), d_ptr, s)
+        else:
+            call("flpr_solve", int(gf), ptr(stage0), per_h0, b - a, nodes, order, plan.nt,
+                 int(fine_s), i64_array(plan.s_off), i64_array(plan.t_off), ptr(key_max),
+                 lam_scale, eps_add, ptr(shift), nval, nout, m_ptr, ptr(q), d_ptr, s)
     quad = None
     if nq_out:
         quad = red[dh * nval:]
@@ -1187,7 +1222,7 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
             bands=bands, band_np=band_np,
             lam_scale=lam_scale, eps_add=eps_add, fine_s=fine_s, acc_s=acc_s, acc_y=acc_y,
             split_acc=split_acc, margin=margin, st=st, xrange=xrange, flag_power2=flag_power2,
-            kde=kde, N=tuple(N), plan0=plan0, any32=any32)
+            kde=kde, N=tuple(N), plan0=plan0, any32=any32, mixed=mixed, refine=refine)
     if not readback:
         return DeviceFit(h, m_y.view(dh, nyc, nodes), None, None, s0_all, geo, plan0, warn, red)
     red_h = None if red is None else red.cpu().numpy()
@@ -1201,6 +1236,62 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
         rss_h = ysq[None, :] - 2.0 * dot_h[:, :ny] + quad_h
     probe_dot = None if (dot_h is None or K == 0) else dot_h[:, ny:]
     return DeviceFit(h, m_y.view(dh, nyc, nodes), rss_h, probe_dot, s0_all, geo, plan0, warn, red)
+
+
+def regularisation_key(geo: Geometry, gkey, h: np.ndarray, order: int, n: int, fp64: bool,
+                       grid64: torch.Tensor, cells: int, key_max: torch.Tensor, s) -> None:
+    """The global regularisation key of the bandwidth list h (regression.py:
+    217, 243-267: the largest |S_kk| over nodes, diagonal moments and every
+    bandwidth, or the largest S for order 0) from channel 0 (the ones channel)
+    of a spread grid alone: a one-channel convolution of every bandwidth.
+    Reduces into key_max (zeroed by the caller)."""
+    d = len(geo.N)
+    dev = grid64.device
+    f64, i32 = torch.float64, torch.int32
+    hf, _w, groups, band_np = bandwidth_tables(geo, gkey, np.asarray(h, dtype=float), n, order,
+                                               fp64)
+    dh = hf.shape[0]
+    pad = conv_pad(geo)
+    any32 = any(not gf for (_a, _b, gf) in groups)
+    g32 = None
+    if any32:
+        g32 = torch.empty((cells + pad,), dtype=torch.float32, device=dev)
+        g32[:cells].copy_(grid64[:cells])
+    plan0 = get_plan(d, order, geo.N, geo.L, geo.margin, 1, groups[0][1] - groups[0][0],
+                     groups[0][2], dev)
+    npow, nmax = plan0.npow, plan0.nmax
+    per_tap = d * npow * nmax
+    h_dev = device_table(hf, f64, dev)
+    tap64 = torch.empty((dh * per_tap,), dtype=f64, device=dev)
+    tap32 = torch.empty((dh * per_tap,), dtype=torch.float32, device=dev) if any32 else None
+    call("flpr_taps", _lib.C.byref(geo.geom), ptr(h_dev), dh, npow, nmax, float(n), ptr(tap64),
+         ptr(tap32), s)
+    bands = device_table(band_np, i32, dev)
+    strips, strip_row = (split_strips(tap32, bands, dh, d, npow, nmax, s) if any32
+                         else (None, 0))
+    for (a, b, gf) in groups:
+        plan = get_plan(d, order, geo.N, geo.L, geo.margin, 1, b - a, gf, dev)
+        fdt = f64 if gf else torch.float32
+        src_grid = grid64 if gf else g32
+        tap_ptr = (tap64.data_ptr() + a * per_tap * 8) if gf else \
+            (strips.data_ptr() + a * strip_row * 2)
+        band_ptr = bands.data_ptr() + a * d * 2 * 4
+        bands_g = band_np[a * d * 2:b * d * 2]
+        stage0 = torch.empty(((b - a) * plan.stage_per_h[0] + pad,), dtype=fdt, device=dev)
+        inter = {k: torch.empty((plan.chunk * plan.stage_per_h[k] + pad,), dtype=fdt, device=dev)
+                 for k in range(1, d)}
+        for ci in range(len(plan.chunks)):
+            for ps in plan.pass_tables[ci]:
+                src = src_grid if ps.in_stage == d else inter[ps.in_stage]
+                dst = stage0 if ps.out_stage == 0 else inter[ps.out_stage]
+                launch_conv(ps, gf, ptr(src), ptr(dst), tap_ptr, nmax, band_ptr, bands_g, s)
+        del inter
+        if order == 0:
+            offs, absval = [plan.s_off[0]], 0
+        else:
+            offs, absval = plan.diag_off, 1
+        call("flpr_max_moment", int(gf), ptr(stage0), plan.stage_per_h[0], b - a,
+             i64_array(offs), len(offs), plan.nodes, absval, ptr(key_max), s)
 
 
 def run_grid_y(sh: dict, x: torch.Tensor, vals_y: torch.Tensor, ny: int,
@@ -1244,7 +1335,7 @@ def run_grid_y(sh: dict, x: torch.Tensor, vals_y: torch.Tensor, ny: int,
     else:
         if sh["split_acc"]:
             geo_y, _ = geometry_for(sh["st"], sh["N"], acc_y, sh["xrange"], sh["flag_power2"],
-                                    sh["kde"], sh["margin"])
+                                    sh["kde"], sh["margin"], spread_f32=sh["mixed"])
             gy = _lib.C.byref(geo_y.geom)
             wab = torch.empty((2 * d * n,), dtype=f64, device=dev)
             call("flpr_points_pack", gy, ptr(x), ptr(sh["order_idx"]), n, ptr(vals_y), ny,
@@ -1297,10 +1388,16 @@ def run_grid_y(sh: dict, x: torch.Tensor, vals_y: torch.Tensor, ny: int,
     del apart
     m_y = torch.empty((dh * ny * nodes,), dtype=f64, device=dev)
     for (a, b, gf, plan, stage0) in sh["stage0s"]:
-        call("flpr_solve", int(gf), ptr(stage0), per_h0, b - a, nodes, order, plan.nt,
-             int(sh["fine_s"]), i64_array(plan.s_off), i64_array(plan.t_off),
-             ptr(sh["key_max"]), sh["lam_scale"], sh["eps_add"], ptr(shift), ny, ny,
-             m_y.data_ptr() + a * ny * nodes * 8, ptr(q), red.data_ptr() + a * ny * 8, s)
+        if sh["refine"] and not gf:
+            call("flpr_solve_refined", ptr(stage0), per_h0, b - a, nodes, order, plan.nt,
+                 i64_array(plan.s_off), i64_array(plan.t_off), ptr(sh["key_max"]),
+                 sh["lam_scale"], sh["eps_add"], ptr(shift), ny, ny,
+                 m_y.data_ptr() + a * ny * nodes * 8, ptr(q), red.data_ptr() + a * ny * 8, s)
+        else:
+            call("flpr_solve", int(gf), ptr(stage0), per_h0, b - a, nodes, order, plan.nt,
+                 int(sh["fine_s"]), i64_array(plan.s_off), i64_array(plan.t_off),
+                 ptr(sh["key_max"]), sh["lam_scale"], sh["eps_add"], ptr(shift), ny, ny,
+                 m_y.data_ptr() + a * ny * nodes * 8, ptr(q), red.data_ptr() + a * ny * 8, s)
     call("flpr_quadform", gp, ptr(m_y), ny, dh, ptr(sh["gram"]), red.data_ptr() + dh * ny * 8, s)
     return DeviceFit(h, m_y.view(dh, ny, nodes), None, None, sh["s0_all"], geo, plan0,
                      sh["warn"], red)
@@ -1410,7 +1507,7 @@ def one_item(geo: Geometry, n: int, dev) -> torch.Tensor:
     it.g = geo.geom
     it.n = int(n)
     it.pt = 0
-    return upload(np.frombuffer(bytes(it), dtype=np.uint8), torch.uint8, dev)
+    return upload(np.frombuffer(bytes(it), dtype=np.uint8).copy(), torch.uint8, dev)
 
 
 def point_sums(geo: Geometry, x: torch.Tensor, y: torch.Tensor, ny: int, probes: torch.Tensor,

@@ -48,7 +48,9 @@ typedef struct FlprGeom {
   int32_t item_points;            /* points per spread / adjoint work item when a cell
                                      holds many (its points are split over items);
                                      0 = the library default (8 in 1-D, 16 otherwise) */
-  int32_t reserved_;              /* 0 */
+  int32_t spread_f32;             /* 1: flpr_spread accumulates the footprints in float32
+                                     with compensated (Kahan) summation and float32
+                                     partial windows (the precision form); 0: float64 */
   double hx[FLPR_MAX_DIM];        /* 2 pi / L */
   double inv4tau[FLPR_MAX_DIM];   /* 1 / (4 tau) */
   double mean[FLPR_MAX_DIM];      /* z-score mean (ddof=1 std below) */
@@ -273,6 +275,19 @@ int flpr_solve(int32_t fp64, const void* mom, int64_t per_h, int32_t dh, int64_t
                const double* shift, int32_t nval, int32_t nout, double* m_out, const double* q,
                double* dot, void* stream);
 
+/* flpr_solve for float32 moments in the precision form (no pivoting): for
+ * order >= 1 each node's S + lam I is factored in float32 (Cholesky) and z =
+ * (S + lam I)^-1 e1 takes one refinement step with the residual formed in
+ * float64; nodes whose float32 factorisation meets a non-positive pivot or
+ * whose pivot ratio max/min (a cheap lower estimate of the condition
+ * number) exceeds 1e6 are solved in float64.  The rest as flpr_solve
+ * (regression.py:162-277). */
+int flpr_solve_refined(const void* mom, int64_t per_h, int32_t dh, int64_t nodes,
+                       int32_t order, int32_t nt, const int64_t* s_off, const int64_t* t_off,
+                       const uint64_t* key, double lam_scale, double eps_add,
+                       const double* shift, int32_t nval, int32_t nout, double* m_out,
+                       const double* q, double* dot, void* stream);
+
 /* Adjoint interpolation sums on the raw grid, from the sorted point data:
  *   q (nval, nodes)        q[c][node] = sum_i vals[i][c] w_i(node)
  *   gram (nw, rcells)      per raw cell the Gram entries sum_i w_i(a) w_i(b) of
@@ -480,6 +495,63 @@ int flpr_batch_point_sums(const FlprGeom* g, const FlprBatchItem* items, int32_t
                           int32_t nprobe, const double* m, const double* coef, int32_t dh,
                           int32_t gcv_cubic, int32_t dof_cubic, double* part, double* red,
                           int64_t red_stride, void* stream);
+
+/* ------------------------------------------------------------------------
+ * The learned bandwidth bracket.  No counterpart in the reference, which
+ * sweeps every bandwidth (api.py:664-740): a small MLP proposes where the
+ * GCV minimum lies so that only a few bandwidths need full solves; the full
+ * sweep remains the fallback.  The MLP is 64 -> 128 -> 128 -> 2 with GELU
+ * (0.5 z (1 + erf(z / sqrt 2))); parameters packed as W1 (128 x 64), b1
+ * (128), W2 (128 x 128), b2, W3 (2 x 128), b3 (2), float32.  GEMM operands
+ * are rounded to bfloat16, products accumulated in float32 (bf16 MFMA).
+ * ------------------------------------------------------------------------ */
+int flpr_mlp_param_count(void);
+
+/* C (M x N, ldc) = op(A) op(B) with op(A)[i][k] = A[i lda + k] (ta = 0) or
+ * A[k lda + i], op(B)[k][j] = B[k ldb + j] (tb = 0) or B[j ldb + k];
+ * mode 0: C = acc (+ bias[j] if bias); mode 1: aux = acc + bias[j], C =
+ * gelu(aux); mode 2: C = acc * gelu'(aux) (aux read, ldc layout). */
+int flpr_mlp_gemm(int32_t M, int32_t N, int32_t K, const float* A, int32_t lda, int32_t ta,
+                  const float* B, int32_t ldb, int32_t tb, const float* bias, int32_t mode,
+                  float* aux, float* C, int32_t ldc, void* stream);
+
+/* Forward pass of M rows X (M x 64): pre-activations Z1, Z2 and activations
+ * H1, H2 (M x 128 each) for the backward pass, outputs O (M x 2). */
+int flpr_mlp_forward(const float* params, int32_t M, const float* X, float* Z1, float* H1,
+                     float* Z2, float* H2, float* O, void* stream);
+
+/* Gradient of sum(O * dO) with respect to the packed parameters (grads, the
+ * parameter layout), from a forward pass's X, Z1, H1, Z2, H2; work holds
+ * 2 * M * 128 floats. */
+int flpr_mlp_backward(const float* params, int32_t M, const float* X, const float* Z1,
+                      const float* H1, const float* Z2, const float* H2, const float* dO,
+                      float* grads, float* work, void* stream);
+
+/* loss[0] = sum_i (w0 (O_i0 - T_i0)^2 + w1 (O_i1 - T_i1)^2) / M and its
+ * gradient dO (M x 2). */
+int flpr_mlp_loss(int32_t M, const float* O, const float* T, float w0, float w1, float* dO,
+                  float* loss, void* stream);
+
+/* One Adam step (bias-corrected, step >= 1, L2 weight decay added to the
+ * gradient) on n float32 parameters with moments m, v. */
+int flpr_mlp_adam(int32_t n, float* params, const float* grads, float* m, float* v, float lr,
+                  float beta1, float beta2, float eps, int32_t step, float weight_decay,
+                  void* stream);
+
+/* The 64 bracket features of P spread grids (problem p at grid + p
+ * grid_stride: channels F0 = sum w and F1 = sum w y of G0 x G1 cells each,
+ * before any centring), with ysq[p] = sum_i y_i^2 and npts[p] points:
+ * 0..55 log weighted mean squared increments of m = F1 / F0 over 14 scales
+ * (1 .. 48 cells) along (1,0), (0,1), (1,1), (1,-1), with weight
+ * F0(c) F0(c') / (F0(c) + F0(c')) where both exceed floor_[p]; 56 log10
+ * npts; 57 log variance of y; 58 log local variance (ysq / n minus the
+ * F0-weighted mean of m^2); 59 log mean F0; 60 log squared coefficient of
+ * variation of F0; 61 share of cells below the floor; 62 log(variance /
+ * local variance); 63 log10 dx[p].  part holds P * 57 * 64 * 8 doubles; feat
+ * (P x 64) float32. */
+int flpr_bracket_features(int32_t P, const double* grid, int64_t grid_stride, int32_t G0,
+                          int32_t G1, const double* floor_, const double* npts, const double* ysq,
+                          const double* dx, double* part, float* feat, void* stream);
 
 #ifdef __cplusplus
 }

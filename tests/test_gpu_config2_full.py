@@ -7,7 +7,9 @@ the mean pass, then the variance pass on (y - yhat)^2.  Both precision forms
 are checked against oracle.restate.cv_fastlpr(mode="f64") (the same discrete
 estimator in float64, FFT route):
 
-* yhat, normwise max |a - b| / max |b|: <= 1e-8 (fp64 form), <= 1e-5 (fp32);
+* yhat, normwise max |a - b| / max |b|: <= 1e-8 (fp64 form), <= 1e-5 (fp32,
+  and the precision form "mixed": float32 spread with compensated summation,
+  float32 solves with one float64 refinement step);
 * the selected grid at 4096 nodes drawn with default_rng(1): same bounds;
 * the GCV curve and the Hutchinson DOF over all 30 bandwidths: same bounds;
 * the selected index (1-SE) equal.
@@ -28,7 +30,7 @@ pytestmark = [pytest.mark.gpu, pytest.mark.timeout(900)]
 
 N_PTS = 1_000_000
 GRID = [256, 256]
-TOL = {"fp64": 1e-8, "fp32": 1e-5}
+TOL = {"fp64": 1e-8, "fp32": 1e-5, "mixed": 1e-5}
 
 
 def _inputs():
@@ -87,14 +89,14 @@ def _check(r, o, prec, what):
     assert e_dof <= tol, e_dof
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
 def test_mean_pass_full_size(oracle_mean, dev_x, prec):
     yd = torch.as_tensor(Y, device="cuda")
     r = fastlpr_amd.cv_fastlpr(dev_x, yd, HL, dict(OPT, precision=prec))
     _check(r, oracle_mean, prec, "mean pass")
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
 def test_variance_pass_full_size(oracle_var, var_input, dev_x, prec):
     yd = torch.as_tensor(var_input, device="cuda")
     r = fastlpr_amd.cv_fastlpr(dev_x, yd, HL, dict(OPT, precision=prec, y_type_out="variance"))
