@@ -23,10 +23,16 @@ timed steps) and a CPU baseline (the oracle restatement, single thread, on a
 bounded sample: the mean pass with 2 and with 4 of the 30 bandwidths after a
 warm-up call, extrapolated linearly to 30, and the same on all the host's
 cores).  Further legs, outside the headline value: the spread's roofline
-entry (HIP events on the spread launches of the timed steps), a cold call
-(caches cleared: probe generation and every plan rebuilt), and the batched
-512-problem workload (config 4) as problems per second, batched against one
-cv_fastlpr call per problem and against its captured-graph replay.
+entry (HIP events on the spread launches of the timed steps), the precision
+form "mixed" (float32 spread with compensated summation, float32 solves with
+one float64 refinement step) as two calls with its yhat difference from the
+float64 form, the learned bandwidth bracket (mean pass with
+bandwidth_bracket="learned" against the full sweep, and whether both select
+the same bandwidth), a cold call (caches cleared: probe generation and every
+plan rebuilt), and the batched 512-problem workload as problems per second,
+batched against one cv_fastlpr call per problem and against its
+captured-graph replay.  The default of 50 timed steps (about 0.1 s) keeps a
+short host-side disturbance from dominating the launch-bound step time.
 """
 from __future__ import annotations
 
@@ -118,7 +124,7 @@ def cpu_baseline(x, y, hl, threads: int = 1):
                        f"{per_fit:.1f} s per fit")}
 
 
-def config4_problem(i: int):
+def uneven_problem(i: int):
     """Problem i of the batched uneven workload: default_rng(1000 + i) draws
     n = round(10^U(3, 6)) and the order integers(0, 3); i < 256 is 1-D on 1024
     nodes, else 2-D on 128 x 128; 20 bandwidths log-spaced in [0.02, 0.5];
@@ -138,7 +144,7 @@ def config4_problem(i: int):
     return x, f, np.column_stack([hh] * d), {"order": order, "N": 1024 if d == 1 else [128, 128]}
 
 
-def config4_leg(steps: int):
+def uneven_batch_leg(steps: int):
     """512 uneven problems: one cv_fastlpr call each (looped) against the
     batched BatchPlan (autotuned launch shapes; the x side and the tuning are
     set up before the timed region, as for a plan reused over many response
@@ -148,7 +154,7 @@ def config4_leg(steps: int):
     P = 512
     xs, ys, hs, os_ = [], [], [], []
     for i in range(P):
-        x, f, h, o = config4_problem(i)
+        x, f, h, o = uneven_problem(i)
         xs.append(torch.as_tensor(x, device="cuda"))
         ys.append(torch.as_tensor(f, device="cuda"))
         hs.append(h)
@@ -191,7 +197,8 @@ def config4_leg(steps: int):
         torch.cuda.synchronize()
         gbest = min(gbest, time.perf_counter() - t)
     classes = [c for v in plan._kindsets.values() for c in v]
-    return {"workload": "512 uneven problems (config 4), one result per problem as cv_fastlpr",
+    return {"workload": "512 uneven problems (1-D and 2-D, 1e3-1e6 points), "
+                        "one result per problem as cv_fastlpr",
             "points": rows, "classes": len(classes),
             "batched_problems_per_s": P / best, "batched_ms": best * 1e3,
             "graph_replay_problems_per_s": P / gbest, "graph_replay_ms": gbest * 1e3,
@@ -251,13 +258,13 @@ def read_traffic():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the float64 plain form")
     ap.add_argument("--no-two-calls", action="store_true",
                     help="skip timing the pair as two separate cv_fastlpr calls")
-    ap.add_argument("--no-config4", action="store_true",
+    ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched 512-problem leg")
     ap.add_argument("--no-cold", action="store_true",
                     help="skip the cold-cache call")
@@ -388,6 +395,42 @@ def main():
                                       "plain_mean_pass": int(p1.gcv_yhat["id1se"])}}
         print(f"[bench] plain fp64: {plain_fits:.1f} fits/s; chosen-vs-plain yhat err "
               f"{e1:.2e} / {e2:.2e}", file=sys.stderr, flush=True)
+        # the precision form (float32 spread with compensated summation,
+        # float32 solves refined once in float64)
+        step_pair(pair, xd, yd, hl, "mixed")
+        torch.cuda.synchronize()
+        tm0 = time.perf_counter()
+        for _ in range(ksteps):
+            q1, q2 = step_pair(pair, xd, yd, hl, "mixed")
+        torch.cuda.synchronize()
+        mixed_fits = 2.0 * ksteps / (time.perf_counter() - tm0)
+        extra["mixed_precision_fits_per_s"] = mixed_fits
+        extra["mixed_vs_plain_yhat_max_rel_err"] = {
+            "mean_pass": float((q1.yhat - p1.yhat).abs().max() / p1.yhat.abs().max()),
+            "variance_pass": float((q2.yhat - p2.yhat).abs().max() / p2.yhat.abs().max())}
+        print(f"[bench] precision form (mixed): {mixed_fits:.1f} fits/s", file=sys.stderr,
+              flush=True)
+        # the learned bandwidth bracket on the mean pass (drop-in cv_fastlpr)
+        ob = {"order": 2, "N": [256, 256], "precision": "fp32"}
+        res = {}
+        for mode in ("full", "learned"):
+            rb = cv(xd, yd, hl, dict(ob, bandwidth_bracket=mode))
+            torch.cuda.synchronize()
+            tb0 = time.perf_counter()
+            for _ in range(ksteps):
+                rb = cv(xd, yd, hl, dict(ob, bandwidth_bracket=mode))
+            torch.cuda.synchronize()
+            res[mode] = (ksteps / (time.perf_counter() - tb0), rb)
+        binfo = res["learned"][1].metadata["bracket"]
+        hsel = res["learned"][1].h
+        extra["learned_bracket"] = {
+            "fits_per_s": res["learned"][0], "full_sweep_fits_per_s": res["full"][0],
+            "bracket_indices": [int(i) for i in binfo["indices"]],
+            "fallback": bool(binfo["fallback"]),
+            "same_selected_h": bool(np.allclose(hsel, res["full"][1].h)),
+            "call": "cv_fastlpr mean pass, options bandwidth_bracket='learned' vs 'full'"}
+        print(f"[bench] learned bracket: {res['learned'][0]:.1f} fits/s vs full sweep "
+              f"{res['full'][0]:.1f}", file=sys.stderr, flush=True)
 
     cold = None
     if not args.no_cold:
@@ -412,13 +455,13 @@ def main():
                         "(1e6 x 10 normals) and every plan rebuilt"}
         print(f"[bench] cold call: pair {t_pair*1e3:.1f} ms, single {t_one*1e3:.1f} ms",
               file=sys.stderr, flush=True)
-    cfg4 = None
-    if not args.no_config4:
-        print("[bench] config 4: 512 uneven problems, batched vs looped ...", file=sys.stderr,
+    b512 = None
+    if not args.no_batch:
+        print("[bench] batch: 512 uneven problems, batched vs looped ...", file=sys.stderr,
               flush=True)
-        cfg4 = config4_leg(3)
-        print(f"[bench] config 4: batched {cfg4['batched_problems_per_s']:.0f} problems/s, "
-              f"looped {cfg4['looped_problems_per_s']:.0f}", file=sys.stderr, flush=True)
+        b512 = uneven_batch_leg(3)
+        print(f"[bench] batch: batched {b512['batched_problems_per_s']:.0f} problems/s, "
+              f"looped {b512['looped_problems_per_s']:.0f}", file=sys.stderr, flush=True)
     cpu = cpu_all = None
     if not args.no_cpu:
         print("[bench] cpu baseline (oracle, 1 thread, bounded sample) ...", file=sys.stderr,
@@ -457,7 +500,7 @@ def main():
         "cpu_baseline_all_cores": cpu_all,
         "roofline_spread": spread_entry,
         "cold_call": cold,
-        "config4_batched": cfg4,
+        "uneven_512_batched": b512,
     }
     line.update(extra)
     line["two_call_fits_per_s"] = two_call_fits

@@ -411,8 +411,8 @@ def conv_schedule(ps: Pass, fp64: bool, bands: np.ndarray):
     and go to one XCD (their input is then fetched into one L2), groups placed
     heaviest first on the least loaded XCD (a group heavier than an XCD's
     share is cut into contiguous pieces), the XCD lists interleaved so that
-    block b runs on XCD b mod 8.  (Measured on config 2: ROW 0.273 -> 0.266 ms,
-    COL 0.636 -> 0.544 ms.)  Memoised per pass and band table.  Returns
+    block b runs on XCD b mod 8.  (Measured on the 2-D order-2 workload with
+    1e6 points: ROW 0.273 -> 0.266 ms, COL 0.636 -> 0.544 ms.)  Memoised per pass and band table.  Returns
     (device int32 table, number of blocks), or (None, 0) for launches too
     small to gain from it."""
     key = (id(ps), bool(fp64), bands.tobytes())

@@ -83,7 +83,7 @@ def make(name: str):
         x = rng.uniform(0, 1, (n, 3))
         y = np.sum((x - 0.5) ** 2, axis=1) + 0.1 * rng.standard_normal(n)
         return x, y, _iso(_logspace_h(5, 0.1, 0.6), 3), {"order": 1, "N": [12, 12, 12]}
-    if name == "cfg1_1d_o1_n1e5":
+    if name == "large1d_o1_n1e5":
         # 1D local linear, N=1e5, x~U(0,1), y=sin(2 pi x)+0.1 N(0,1), M=1024, 20 h in [0.01, 1], seed 0
         rng = np.random.default_rng(0)
         n = 100000
@@ -125,7 +125,7 @@ LPR_SMALL = [
 LPR_CUBIC = ["lpr2d_o1_cubic", "lpr3d_o1_cubic"]
 LPR_ACC = ["acc0_1d", "acc2_1d", "acc0_2d", "acc3_2d"]
 LPR_ACC_HI = ["acc7_1d", "acc12_1d", "acc7_2d", "acc9_2d", "acc7_2d_nodof"]
-LPR_LARGE = ["cfg1_1d_o1_n1e5"]
+LPR_LARGE = ["large1d_o1_n1e5"]
 IVL = ["ivl1d_o1", "ivl1d_o2", "ivl2d_o1"]
 
 

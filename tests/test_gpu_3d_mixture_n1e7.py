@@ -1,4 +1,4 @@
-"""Config 3 at full size: 3-D local linear regression and KDE with LCV on 1e7
+"""The 3-D mixture workload at full size: 3-D local linear regression and KDE with LCV on 1e7
 points from a 4-component Gaussian mixture in [0,1]^3, 128^3 grid, 16
 bandwidths in [0.01, 0.3].  The float64 oracle cannot run at this size, so the
 float32 form is checked against the float64 form (which equals the float64
@@ -30,7 +30,7 @@ def _normwise(a, b):
     return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
 
-def test_config3_full_size():
+def test_3d_mixture_n1e7_full_size():
     import torch
     import fastlpr_amd as fl
     n = 10_000_000

@@ -1,14 +1,15 @@
 """Batched problems (cv_fastlpr_batch / BatchPlan) against one cv_fastlpr
 call per problem.
 
-The workload is config 4's 512 uneven problems (problem i from
+The workload is the 512 uneven problems (problem i from
 default_rng(1000 + i): n = round(10^U(3, 6)), order integers(0, 3), 1-D on
 1024 nodes for i < 256 else 2-D on 128 x 128, 20 bandwidths in [0.02, 0.5],
-complex y for every tenth problem; the recipe of test_gpu_config4).  Every
-problem's batched result must equal its single call: the float64 form to
-1e-12 (normwise yhat, the GCV curve, the DOF) with the same selected index;
-the float32 form to 1e-9 (its float32 moments round a spread grid whose
-float64 sums may differ in the last bit between runs).  A sample is also
+complex y for every tenth problem; the recipe of
+test_gpu_uneven_problems).  Every problem's batched result must equal its
+single call: the float64 form to 1e-12 (normwise yhat, the GCV curve, the
+DOF) with the same selected index; the float32 form to 1e-9 (its float32
+moments round a spread grid whose float64 sums may differ in the last bit
+between runs).  A sample is also
 checked against the float64 oracle.  The graph test captures the batched
 fit and replays it several times, with eager work between the replays,
 against the eager result.
@@ -20,7 +21,7 @@ import pytest
 import torch
 
 from oracle import restate as R
-from test_gpu_config4 import problem
+from test_gpu_uneven_problems import problem
 
 pytestmark = pytest.mark.gpu
 
@@ -35,7 +36,7 @@ def _rel(a, b):
 
 
 @pytest.fixture(scope="module")
-def config4():
+def uneven512():
     xs, ys, hs, opts = [], [], [], []
     for i in range(P_ALL):
         x, y, h, o = problem(i)
@@ -55,9 +56,9 @@ def _compare(rb, rs, tol):
 
 
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-9)])
-def test_batch_equals_single_calls(config4, prec, tol):
+def test_batch_equals_single_calls(uneven512, prec, tol):
     import fastlpr_amd as fl
-    xs, ys, hs, opts = config4
+    xs, ys, hs, opts = uneven512
     o2 = [dict(o, precision=prec) for o in opts]
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
@@ -69,9 +70,9 @@ def test_batch_equals_single_calls(config4, prec, tol):
     print(f"{prec}: worst batched-vs-single difference {worst:.2e}")
 
 
-def test_batch_sample_matches_oracle(config4):
+def test_batch_sample_matches_oracle(uneven512):
     import fastlpr_amd as fl
-    xs, ys, hs, opts = config4
+    xs, ys, hs, opts = uneven512
     sample = [5, 10, 19, 130, 140, 256, 261, 320]
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")
@@ -120,12 +121,12 @@ def test_batch_mixed_kinds_and_variance():
                 assert abs(rb[i].d - r1.d) <= 1e-12 * abs(r1.d)
 
 
-def test_batch_graph_replay(config4):
+def test_batch_graph_replay(uneven512):
     """The captured batch replays to the eager result several times, with the
     program's own eager work (another batch fit) between the replays, and a
     new response set fed through the same graph."""
     import fastlpr_amd as fl
-    xs, ys, hs, opts = config4
+    xs, ys, hs, opts = uneven512
     sel = list(range(0, P_ALL, 16))
     bx = [xs[i] for i in sel]
     by = [ys[i] for i in sel]
@@ -146,13 +147,13 @@ def test_batch_graph_replay(config4):
                 _compare(a, b, 1e-9)
 
 
-def test_batch_autotuned_launch_shapes(config4):
+def test_batch_autotuned_launch_shapes(uneven512):
     """The autotuner measures tile / points-per-item candidates per class and
     keeps the fastest; the tuned plan computes the same fits (the knobs only
     reorder float64 additions)."""
     import fastlpr_amd as fl
     from fastlpr_amd import tune
-    xs, ys, hs, opts = config4
+    xs, ys, hs, opts = uneven512
     sel = list(range(3, P_ALL, 24))
     with warnings.catch_warnings():
         warnings.simplefilter("ignore")

@@ -165,9 +165,9 @@ def test_gpu_matches_reference_golden(fl, name):
     assert _rel(r.s0, g["s0"]) < 1e-5
 
 
-def test_gpu_matches_reference_config1_full_size(fl):
-    g = np.load(os.path.join(GOLDEN, "cfg1_1d_o1_n1e5.npz"), allow_pickle=False)
-    x, y, h, opt = recipes.make("cfg1_1d_o1_n1e5")
+def test_gpu_matches_reference_1d_o1_n1e5_full_size(fl):
+    g = np.load(os.path.join(GOLDEN, "large1d_o1_n1e5.npz"), allow_pickle=False)
+    x, y, h, opt = recipes.make("large1d_o1_n1e5")
     r = fl.cv_fastlpr(x, y, h, dict(opt))
     assert r.gcv_yhat["id1se"] == int(g["id1se"])
     idx = g["yhat_idx"]

@@ -3,7 +3,7 @@ summation, float32 convolution, float32 per-node solves with one float64
 refinement step and a float64 fallback above a condition estimate of 1e6)
 against the float64 oracle: yhat and the GCV curve within 1e-5 (normwise),
 the same selected bandwidth.  The full-size config-2 case is in
-test_gpu_config2_full."""
+test_gpu_2d_o2_n1e6_full."""
 import warnings
 
 import numpy as np

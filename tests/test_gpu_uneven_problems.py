@@ -1,4 +1,4 @@
-"""Config 4 (batched uneven problems) problem by problem: problem i draws with
+"""The 512 batched uneven problems, problem by problem: problem i draws with
 default_rng(1000 + i) its size N = round(10^U(3, 6)), then its order
 integers(0, 3); i < 256 is 1-D on 1024 nodes, otherwise 2-D on 128 x 128; 20
 bandwidths log-spaced in [0.02, 0.5]; every tenth problem has complex
@@ -47,7 +47,7 @@ def _rel(a, b):
 
 
 @pytest.mark.parametrize("i", SAMPLE)
-def test_config4_problem_matches_oracle(i):
+def test_uneven_problem_matches_oracle(i):
     import fastlpr_amd as fl
     x, y, h, opt = problem(i)
     with warnings.catch_warnings():

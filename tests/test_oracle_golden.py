@@ -53,9 +53,9 @@ def test_oracle_lpr_matches_reference(name):
         assert abs(f.d - float(g["d"])) / abs(float(g["d"])) < 2e-5
 
 
-def test_oracle_config1_full_size_matches_reference():
-    g = _load("cfg1_1d_o1_n1e5")
-    x, y, h, opt = recipes.make("cfg1_1d_o1_n1e5")
+def test_oracle_1d_o1_n1e5_full_size_matches_reference():
+    g = _load("large1d_o1_n1e5")
+    x, y, h, opt = recipes.make("large1d_o1_n1e5")
     import hashlib
     assert hashlib.sha256(np.ascontiguousarray(x).tobytes()).hexdigest() == str(g["x_hash"])
     f = R.cv_fastlpr(x, y, h, dict(opt), mode="ref")

@@ -261,8 +261,8 @@ def test_plan_stage_layout_counts():
 
 @pytest.mark.parametrize("fp64", [False, True])
 def test_conv_block_schedule_covers_every_tile_once(fp64):
-    """conv_schedule (config 2: 2-D order 2, 256 x 256 nodes, 30 bandwidths,
-    12 channels): every (job, tile) of both passes appears exactly once, each
+    """conv_schedule (the 2-D order-2 workload: 256 x 256 nodes, 30
+    bandwidths, 12 channels): every (job, tile) of both passes appears exactly once, each
     tile decodes inside its job (flpr_conv's tile numbering), idle entries are
     marked -1, and XCD b mod 8 receives a balanced share of the work."""
     from fastlpr_amd import _lib
