@@ -39,6 +39,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -122,6 +123,70 @@ def cpu_baseline(x, y, hl, threads: int = 1):
                        f"of the 30 bandwidths ({times[2]:.1f} s, {times[4]:.1f} s) after one "
                        f"untimed warm-up call; extrapolated linearly to 30 bandwidths: "
                        f"{per_fit:.1f} s per fit")}
+
+
+def mixture_3d(n: int, seed: int = 0):
+    """The 3-D workload: n points from a 4-component Gaussian mixture in
+    [0,1]^3 (means U(0.2, 0.8), isotropic sd U(0.05, 0.15), equal weights,
+    points outside the cube drawn again), y = ||x - 0.5||^2 + 0.1 N(0,1)."""
+    rng = np.random.default_rng(seed)
+    means = rng.uniform(0.2, 0.8, (4, 3))
+    sd = rng.uniform(0.05, 0.15, 4)
+    comp = rng.integers(0, 4, n)
+    x = means[comp] + sd[comp, None] * rng.standard_normal((n, 3))
+    bad = np.any((x < 0) | (x > 1), axis=1)
+    while bad.any():
+        k = int(bad.sum())
+        c = rng.integers(0, 4, k)
+        x[bad] = means[c] + sd[c, None] * rng.standard_normal((k, 3))
+        bad = np.any((x < 0) | (x > 1), axis=1)
+    y = np.sum((x - 0.5) ** 2, axis=1) + 0.1 * rng.standard_normal(n)
+    return x, y
+
+
+def mixture_3d_leg(steps: int):
+    """3-D local linear regression (GCV) and KDE (LCV) on 1e7 mixture points,
+    128^3 grid, 16 bandwidths in [0.01, 0.3], float32 form: one untimed call
+    of each, then `steps` timed calls of each.  Returns the JSON record."""
+    import torch
+    import fastlpr_amd as fl
+    n = 10_000_000
+    x, y = mixture_3d(n)
+    hh = fl.get_hlist(16, [0.01, 0.3])[:, 0]
+    hl = np.column_stack([hh, hh, hh])
+    dev = torch.device("cuda", 0)
+    xd, yd = torch.as_tensor(x, device=dev), torch.as_tensor(y, device=dev)
+    opt = {"order": 1, "N": [128, 128, 128]}
+    out = {"workload": "3-D local linear (GCV) + KDE (LCV), 1e7 Gaussian-mixture points, "
+                       "128^3 grid, 16 bandwidths in [0.01, 0.3]",
+           "n_points": n, "grid": [128, 128, 128], "bandwidths": 16, "form": "float32",
+           "steps": steps}
+    for name, fn in (("regression", lambda: fl.cv_fastlpr(xd, yd, hl, dict(opt))),
+                     ("kde", lambda: fl.cv_fastkde(xd, hl, {"N": [128, 128, 128]}))):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) * 1e3 / steps
+        out[name + "_ms_per_fit"] = ms
+        out[name + "_fits_per_s"] = 1e3 / ms
+        print(f"[bench] 3-D mixture {name}: {ms:.1f} ms per fit", file=sys.stderr, flush=True)
+    return out
+
+
+def calibration_copy():
+    """A plain streaming kernel of known size, three times: torch.neg over
+    256 MiB of float32 (256 MiB read, 256 MiB written), the reference for the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    a = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    del a, b
 
 
 def uneven_problem(i: int):
@@ -208,28 +273,136 @@ def uneven_batch_leg(steps: int):
             "launch_shapes": sorted({str(c.tabs.get("launch_shape")) for c in classes})}
 
 
-def pmc_summary(dbs):
-    """Per-kernel averages of the counters in rocprofv3 --pmc databases (rocpd
-    format), and the HBM traffic of flpr_conv per launch scaled by the
-    calibration copy (a plain 256 MiB device copy, --calibrate-copy)."""
+N_CU, N_SIMD, N_XCD = 256, 1024, 8     # MI355X: 8 XCDs of 32 CUs, 4 SIMDs per CU
+
+
+def _kname(name: str) -> str:
+    return name.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "")
+
+
+def _trace_summary(db, f, kernels=None):
+    """Kernel-trace database: per kernel, calls, total / mean / min / max
+    duration, share of the traced kernel time, and the launch resources."""
+    rows = db.execute("select name, duration, vgpr_count, accum_vgpr_count, lds_size, "
+                      "workgroup_x * workgroup_y * workgroup_z, grid_x * grid_y * grid_z "
+                      "from kernels").fetchall()
+    agg = {}
+    for name, dur, vg, ag, lds, wg, grid in rows:
+        a = agg.setdefault(_kname(name), [0, 0.0, float("inf"), 0.0, vg, ag, lds, wg, grid])
+        a[0] += 1
+        a[1] += dur
+        a[2] = min(a[2], dur)
+        a[3] = max(a[3], dur)
+    total = sum(a[1] for a in agg.values()) or 1.0
+    lines = [f"# kernel trace {f}: {len(rows)} dispatches, {total / 1e6:.3f} ms kernel time",
+             f"{'kernel':60s} {'calls':>6s} {'total_us':>10s} {'avg_us':>9s} {'min_us':>9s} "
+             f"{'max_us':>9s} {'share':>6s}  vgpr agpr  lds  wg  grid"]
+    for k, a in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        if kernels and not re.search(kernels, k):
+            continue
+        lines.append(f"{k[:60]:60s} {a[0]:6d} {a[1] / 1e3:10.1f} {a[1] / a[0] / 1e3:9.2f} "
+                     f"{a[2] / 1e3:9.2f} {a[3] / 1e3:9.2f} {100 * a[1] / total:5.1f}%  "
+                     f"{a[4]:4d} {a[5]:4d} {a[6]:5d} {a[7]:4d} {a[8]}")
+    return lines
+
+
+def _derived(c, dur_ns):
+    """Per-dispatch metrics from the mean counters of one kernel, following
+    rocprofiler-sdk's gfx950 expressions (MfmaUtil, VALUBusy,
+    OccupancyPercent, LDSBankConflict), which divide by the GRBM_GUI_ACTIVE of
+    one XCD; the database holds it summed over the XCDs when GRBM per
+    nanosecond exceeds the clock, and is then divided by their number."""
+    out = []
+    g = c.get("GRBM_GUI_ACTIVE")
+    if not g:
+        return out
+    per_ns = g / dur_ns
+    cyc = g / N_XCD if per_ns > 4.0 else g
+    out.append(f"GRBM_GUI_ACTIVE/ns {per_ns:.2f} -> {cyc:.0f} cycles per dispatch "
+               f"({cyc / dur_ns:.2f} GHz)")
+    if "SQ_VALU_MFMA_BUSY_CYCLES" in c:
+        out.append(f"MFMA busy {100 * c['SQ_VALU_MFMA_BUSY_CYCLES'] / (cyc * N_SIMD):.1f}%")
+    if "SQ_ACTIVE_INST_VALU" in c:
+        out.append(f"VALU busy {100 * c['SQ_ACTIVE_INST_VALU'] / N_CU / cyc:.1f}%")
+    if "SQ_WAVE_CYCLES" in c:
+        occ = 400 * c["SQ_WAVE_CYCLES"] / cyc / N_CU / 32
+        out.append(f"occupancy {occ:.1f}% of 32 wave slots per CU "
+                   f"({occ * 8 / 100:.2f} waves per SIMD)")
+        if "SQ_WAIT_ANY" in c:
+            out.append(f"waiting {100 * c['SQ_WAIT_ANY'] / c['SQ_WAVE_CYCLES']:.1f}% of wave cycles")
+    if "SQ_LDS_BANK_CONFLICT" in c:
+        out.append(f"LDS bank conflict {100 * c['SQ_LDS_BANK_CONFLICT'] / N_CU / cyc:.2f}% "
+                   f"of CU cycles")
+    if "SQ_LDS_BANK_CONFLICT" in c and c.get("SQ_LDS_IDX_ACTIVE"):
+        out.append(f"conflict / LDS active {100 * c['SQ_LDS_BANK_CONFLICT'] / c['SQ_LDS_IDX_ACTIVE']:.1f}%")
+    if "SQ_INSTS_VALU_MFMA_MOPS_BF16" in c:
+        fl = c["SQ_INSTS_VALU_MFMA_MOPS_BF16"] * 512.0
+        out.append(f"bf16 MFMA {fl:.3e} flop per dispatch = {fl / dur_ns / 1e3:.1f} TFLOP/s")
+    waves = c.get("SQ_WAVES")
+    if waves:
+        mix = [f"{n[9:].lower()} {c[n] / waves:.1f}" for n in
+               ("SQ_INSTS_VALU", "SQ_INSTS_MFMA", "SQ_INSTS_LDS", "SQ_INSTS_SALU", "SQ_INSTS_VMEM")
+               if n in c]
+        if mix:
+            out.append("instructions per wave: " + ", ".join(mix))
+    return out
+
+
+def pmc_summary(dbs, kernels=None, brief=False):
+    """Summaries of rocprofv3 databases (rocpd format).  A kernel-trace
+    database gives the per-kernel time table; a --pmc database gives, per
+    kernel, the mean of each counter per dispatch, the mean duration and the
+    metrics derived from them, and, for FETCH_SIZE / WRITE_SIZE, the HBM
+    traffic of flpr_conv per launch scaled by the calibration copy (a plain
+    256 MiB device copy, --calibrate-copy), also per kernel as HBM bytes
+    and GB/s.  Databases of several --pmc passes over the same program
+    merge per kernel.  kernels: a regular expression; only the kernels whose
+    name it matches are listed (the calibration copy is always used).  brief:
+    the derived metrics without the raw counter lines."""
     import sqlite3
-    tot, cnt = {}, {}
+    tot, cnt, dur, res = {}, {}, {}, {}
+    lines = []
     for f in dbs:
         db = sqlite3.connect(f)
-        rows = db.execute("select kernel_name, counter_name, value, dispatch_id "
-                          "from counters_collection").fetchall()
-        for name, cname, val, did in rows:
-            k = name.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "")
+        rows = db.execute("select kernel_name, counter_name, value, dispatch_id, duration, "
+                          "vgpr_count, accum_vgpr_count, lds_block_size, workgroup_size, "
+                          "grid_size from counters_collection").fetchall()
+        if not rows:
+            lines += _trace_summary(db, f, kernels)
+            continue
+        for name, cname, val, did, d, vg, ag, lds, wg, grid in rows:
+            k = _kname(name)
             tot.setdefault(k, {}).setdefault(cname, 0.0)
             tot[k][cname] += val
             cnt.setdefault(k, {}).setdefault(cname, set()).add((f, did))
+            dur.setdefault(k, {})[(f, did)] = d
+            res[k] = (vg, ag, lds, wg, grid)
     avg = {k: {c: v / len(cnt[k][c]) for c, v in d.items()} for k, d in tot.items()}
-    lines = []
-    for k in sorted(avg):
-        for c, v in sorted(avg[k].items()):
-            lines.append(f"{k[:70]:70s} {c:12s} {v:14.1f} KiB/dispatch  "
-                         f"({len(cnt[k][c])} dispatches)")
     copy = [v for k, v in avg.items() if "neg_kernel" in k]
+    scale = {}
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        cal = max((d.get(c, 0.0) for d in copy), default=0.0)
+        if cal > 0:
+            scale[c] = (COPY_BYTES / 1024.0) / cal
+    for k in sorted(avg):
+        if kernels and not re.search(kernels, k):
+            continue
+        dmean = sum(dur[k].values()) / len(dur[k])
+        vg, ag, lds, wg, grid = res[k]
+        lines.append(f"== {k[:90]}  ({len(dur[k])} dispatches, mean {dmean / 1e3:.2f} us, "
+                     f"vgpr {vg} agpr {ag} lds {lds} wg {wg} grid {grid})")
+        for c, v in sorted(avg[k].items()):
+            if brief:
+                break
+            unit = " KiB" if c in ("FETCH_SIZE", "WRITE_SIZE") else ""
+            lines.append(f"   {c:30s} {v:16.1f}{unit} per dispatch ({len(cnt[k][c])} dispatches)")
+        lines += ["   " + t for t in _derived(avg[k], dmean)]
+        if len(scale) == 2 and "FETCH_SIZE" in avg[k] and "WRITE_SIZE" in avg[k]:
+            byt = (avg[k]["FETCH_SIZE"] * scale["FETCH_SIZE"] +
+                   avg[k]["WRITE_SIZE"] * scale["WRITE_SIZE"]) * 1024.0
+            lines.append(f"   HBM {byt / 1e6:.1f} MB per dispatch (FETCH x {scale['FETCH_SIZE']:.3f}"
+                         f" + WRITE x {scale['WRITE_SIZE']:.3f}) = {byt / dmean:.0f} GB/s = "
+                         f"{byt / dmean / (PEAK_HBM / 1e9):.3f} of 8 TB/s")
     out = {}
     for c in ("FETCH_SIZE", "WRITE_SIZE"):
         cal = max((d.get(c, 0.0) for d in copy), default=0.0)
@@ -271,11 +444,19 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, run a plain 256 MiB streaming kernel (negation) "
                          "three times: the known-size reference for FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--mixture-3d", type=int, default=0, metavar="K",
+                    help="instead of the headline, time K fits each of the 3-D regression "
+                         "and KDE on 1e7 mixture points (128^3 grid, 16 bandwidths) and "
+                         "print that record")
     ap.add_argument("--pmc-summary", nargs="+", metavar="DB",
-                    help="summarise rocprofv3 --pmc databases (no GPU) and exit")
+                    help="summarise rocprofv3 databases (kernel trace or --pmc; no GPU) and exit")
+    ap.add_argument("--pmc-kernels", metavar="REGEX",
+                    help="with --pmc-summary: list only the kernels whose name matches")
+    ap.add_argument("--pmc-brief", action="store_true",
+                    help="with --pmc-summary: derived metrics only, no raw counter lines")
     args = ap.parse_args()
     if args.pmc_summary:
-        lines, out = pmc_summary(args.pmc_summary)
+        lines, out = pmc_summary(args.pmc_summary, args.pmc_kernels, args.pmc_brief)
         print("\n".join(lines))
         if "FETCH_SIZE" in out and "WRITE_SIZE" in out:
             (f, ff), (w, fw) = out["FETCH_SIZE"], out["WRITE_SIZE"]
@@ -288,6 +469,12 @@ def main():
               "(the benchmark measures one GPU; the path would shard by bandwidth)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.mixture_3d > 0:
+        print(json.dumps(mixture_3d_leg(args.mixture_3d)), flush=True)
+        if args.calibrate_copy:
+            calibration_copy()
+        return
 
     import torch
     import fastlpr_amd
@@ -342,12 +529,7 @@ def main():
             torch.cuda.synchronize()
             per_pass[name + "_fits_per_s"] = args.steps / (time.perf_counter() - tp0)
     if args.calibrate_copy:
-        a = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device=dev).fill_(1.0)
-        b = torch.empty_like(a)
-        for _ in range(3):
-            torch.neg(a, out=b)  # a plain streaming kernel: 256 MiB in, 256 MiB out
-        torch.cuda.synchronize()
-        del a, b
+        calibration_copy()
     ms_step = (t1 - t0) * 1e3 / args.steps
     fits_s = 2.0 * args.steps / (t1 - t0)
     print(f"[bench] chosen: {ms_step:.2f} ms/step, {fits_s:.1f} fits/s "
