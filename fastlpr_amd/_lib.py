@@ -21,6 +21,7 @@ EXPORTS = (
     "flpr_points_bin",
     "flpr_points_pack",
     "flpr_spread",
+    "flpr_spread_wide",
     "flpr_resample_work",
     "flpr_resample",
     "flpr_grid_center",
@@ -120,6 +121,7 @@ _SIGS = {
     "flpr_points_bin": [_P, _I64, _I32, _P, _P, _P, _P, _P, _P],
     "flpr_points_pack": [_GP, _P, _P, _I64, _P, _I32, _P, _I32, _P, _P, _P, _P, _P],
     "flpr_spread": [_GP, _P, _P, _P, _I64, _I32, _I32, _P, _P, _P, _P],
+    "flpr_spread_wide": [_GP, _P, _P, _P, _I64, _I32, _I32, _P, _P, _P],
     "flpr_resample_work": [_GP, _GP, _I32, _I32, _P],
     "flpr_grid_periodize": [_GP, _P, _I32, _P, _P],
     "flpr_conv_tiles": [_I32, _I32, _I32, _I32, _I32, _I32, _P, _P],

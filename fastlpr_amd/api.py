@@ -164,14 +164,11 @@ def _check_finite(a, what: str, fn: str):
 
 def _check_accuracy(acc: int, dims: int) -> None:
     """NUFFT accuracies the engine runs: 0 (nearest-cell binning) to 6 on the
-    L-periodic grid, 7..12 on the oversampled grid in 1-D and 2-D."""
+    L-periodic grid, 7..12 on the oversampled grid."""
     if acc < 0:
         raise ValueError("options['accuracy'] must be a non-negative integer.")
     if acc > 12:
         raise NotImplementedError("accuracy > 12 is not supported on the GPU")
-    if acc > 6 and dims == 3:
-        raise NotImplementedError("accuracy > 6 in 3-D is not supported on the GPU (the "
-                                  "oversampled footprint does not fit the spread tile)")
 
 
 def _check_axes(stats, xrange=None):

@@ -754,6 +754,84 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_cells(
 }
 
 
+// Three-dimensional spread with footprints wider than the LDS window admits
+// (the oversampled grid of accuracies 7..12: up to 25 cells per axis, a
+// window of (tile + 2 MSP)^3 doubles per channel would exceed the LDS).  Work
+// item = (cell, row, channel) as in k_spread_cells with the first two axes
+// fixed (row = offsets r0, r1 of the footprint): the item accumulates its
+// cell's points into one footprint row of 2 MSP + 1 float64 registers and
+// adds the row straight into the grid with float64 atomics (grid zeroed by
+// the caller).  The weights are those of tap_weights (A B^j G[j], the same
+// products in the same order); the order of the atomics varies between runs,
+// so the grid varies in its last bits.
+template <int MSP>
+__global__ __launch_bounds__(SPREAD_THREADS) void k_spread_wide3(
+    FlprGeom g, SpreadTab tab, const int32_t* __restrict__ key_start,
+    const double* __restrict__ wab, const double* __restrict__ vals, int with_ones, int C,
+    double* __restrict__ grid, int64_t cells) {
+  constexpr int ntap = 2 * MSP + 1;
+  constexpr int ROWS = ntap * ntap;
+  const int nv = C - with_ones;
+  int t[3] = {0, 0, 0};
+  tile_coords<3>(g, blockIdx.x, t);
+  const int tc = g.tile[0] * g.tile[1] * g.tile[2];
+  const int kbase = blockIdx.x * tc;
+  const int64_t nitems = (int64_t)tc * ROWS * C;
+  for (int64_t item = threadIdx.x; item < nitems; item += blockDim.x) {
+    const int c = (int)(item % C);
+    const int64_t rest = item / C;
+    const int row = (int)(rest % ROWS);
+    const int lc = (int)(rest / ROWS);
+    const int p0 = key_start[kbase + lc], p1 = key_start[kbase + lc + 1];
+    if (p0 == p1) continue;
+    const bool ones = c < with_ones;
+    const double* vp = vals + (ones ? 0 : c - with_ones);
+    int pos[3] = {0, 0, 0};
+    local_coords<3>(g, lc, pos);
+    const int r0 = row / ntap, r1 = row % ntap;
+    // grid cell of footprint offsets (r0, r1, j): tile origin - MSP + position + offset
+    const int s0 = t[0] * g.tile[0] - MSP + pos[0] + r0;
+    const int s1 = t[1] * g.tile[1] - MSP + pos[1] + r1;
+    if (s0 < 0 || s0 >= g.G[0] || s1 < 0 || s1 >= g.G[1]) continue;
+    double acc[ntap];
+#pragma unroll
+    for (int j = 0; j < ntap; ++j) acc[j] = 0.0;
+    for (int p = p0; p < p1; ++p) {
+      const double* wp = wab + (int64_t)p * 6;
+      double b0 = wp[0], b1 = wp[2];
+      for (int j = 0; j < r0; ++j) b0 *= wp[1];
+      for (int j = 0; j < r1; ++j) b1 *= wp[3];
+      const double s = (ones ? 1.0 : vp[(int64_t)p * nv]) * (b0 * tab.G[0][r0]) *
+                       (b1 * tab.G[1][r1]);
+      double w2[ntap];
+      tap_weights<MSP, double>(wp[4], wp[5], tab.G[2], w2);
+#pragma unroll
+      for (int j = 0; j < ntap; ++j) acc[j] += s * w2[j];
+    }
+    const int s2b = t[2] * g.tile[2] - MSP + pos[2];
+    double* gp = grid + (int64_t)c * cells + ((int64_t)s0 * g.G[1] + s1) * g.G[2];
+#pragma unroll
+    for (int j = 0; j < ntap; ++j) {
+      const int s2 = s2b + j;
+      if (s2 >= 0 && s2 < g.G[2] && acc[j] != 0.0) atomicAdd(&gp[s2], acc[j]);
+    }
+  }
+}
+
+// Per-channel totals of a grid (channel = blockIdx.y) into sums (zeroed by
+// the caller).
+__global__ __launch_bounds__(256) void k_grid_sums(const double* __restrict__ grid, int64_t cells,
+                                                   double* __restrict__ sums) {
+  __shared__ double red[4];
+  const int c = blockIdx.y;
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cells;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += grid[(int64_t)c * cells + i];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) atomicAdd(&sums[c], s);
+}
+
 __device__ inline int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
 
 // Sum of the overlapping tile windows for every spread-grid cell (channel =
@@ -2109,6 +2187,57 @@ int flpr_points_pack(const FlprGeom* g, const double* x, const int32_t* order, i
   return 0;
 }
 
+// G[k][j] = exp(-hx_k^2 (j - msp)^2 / (4 tau_k)) of a geometry.
+static SpreadTab spread_tab(const FlprGeom& g) {
+  SpreadTab tab;
+  for (int k = 0; k < FLPR_MAX_DIM; ++k)
+    for (int j = 0; j < SPREAD_MAX_TAPS; ++j) {
+      const double c2 = (k < g.d) ? g.hx[k] * g.hx[k] * g.inv4tau[k] : 0.0;
+      const double o = (double)(j - g.msp);
+      tab.G[k][j] = exp(-c2 * o * o);
+    }
+  return tab;
+}
+
+int flpr_spread_wide(const FlprGeom* g, const int32_t* key_start, const double* wab,
+                     const double* vals, int64_t n, int32_t with_ones, int32_t nval,
+                     double* grid, double* sums, void* stream) {
+  FLPR_CLEAR_ERROR();
+  if (!flpr_geom_ok(g) || !key_start || !wab || !grid || !sums || n < 1 || nval < 0 ||
+      (nval > 0 && !vals))
+    return -1;
+  if (g->d != 3 || g->msp < 7 || g->msp > FLPR_MAX_MSP) return -1;
+  const int wo = with_ones ? 1 : 0;
+  const int C = wo + nval;
+  if (C < 1 || C > FLPR_MAX_CHANNELS) return -1;
+  int64_t cells = 1;
+  for (int k = 0; k < 3; ++k) cells *= g->G[k];
+  const SpreadTab tab = spread_tab(*g);
+  const int64_t ntiles = flpr_ntiles(g);
+  if (ntiles > INT32_MAX) return -1;
+  hipStream_t st = as_stream(stream);
+#define FLPR_WIDE_CASE(M)                                                                     \
+  case M:                                                                                     \
+    hipLaunchKernelGGL((k_spread_wide3<M>), dim3((unsigned)ntiles), dim3(SPREAD_THREADS), 0, \
+                       st, *g, tab, key_start, wab, vals, wo, C, grid, cells);                \
+    break;
+  switch (g->msp) {
+    FLPR_WIDE_CASE(7)
+    FLPR_WIDE_CASE(8)
+    FLPR_WIDE_CASE(9)
+    FLPR_WIDE_CASE(10)
+    FLPR_WIDE_CASE(11)
+    default:
+      FLPR_WIDE_CASE(12)
+  }
+#undef FLPR_WIDE_CASE
+  FLPR_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_grid_sums, dim3(flpr_grid_for(cells, 256, 256), (unsigned)C), dim3(256), 0,
+                     st, grid, cells, sums);
+  FLPR_CHECK_LAUNCH();
+  return 0;
+}
+
 int flpr_spread(const FlprGeom* g, const int32_t* key_start, const double* wab,
                 const double* vals, int64_t n, int32_t with_ones, int32_t nval, double* part,
                 double* grid, double* sums, void* stream) {
@@ -2131,13 +2260,7 @@ int flpr_spread(const FlprGeom* g, const int32_t* key_start, const double* wab,
   const int cg = channel_group(ntiles, fcells, C);
   const size_t lds = (size_t)fcells * cg * sizeof(double);
   if (lds > 64 * 1024) return -1;
-  SpreadTab tab;
-  for (int k = 0; k < FLPR_MAX_DIM; ++k)
-    for (int j = 0; j < SPREAD_MAX_TAPS; ++j) {
-      const double c2 = (k < g->d) ? g->hx[k] * g->hx[k] * g->inv4tau[k] : 0.0;
-      const double o = (double)(j - g->msp);
-      tab.G[k][j] = exp(-c2 * o * o);
-    }
+  const SpreadTab tab = spread_tab(*g);
   int64_t tc = 1;
   for (int k = 0; k < g->d; ++k) tc *= g->tile[k];
   // footprint rows per item (k_spread_cells' ROWS)

@@ -722,9 +722,6 @@ def fine_geometry(geo: Geometry, accuracy: int) -> FlprGeom:
     d = len(geo.N)
     if acc > MAX_FINE_ACCURACY:
         raise NotImplementedError(f"accuracy > {MAX_FINE_ACCURACY} is not supported on the GPU")
-    if d == 3:
-        raise NotImplementedError("accuracy > 6 in 3-D is not supported on the GPU (the "
-                                  "oversampled footprint does not fit the spread tile)")
     R = 2 if acc <= 11 else 3
     f = FlprGeom.from_buffer_copy(geo.geom)
     f.msp = acc
@@ -775,13 +772,21 @@ def spread_fine(geo: Geometry, accuracy: int, x: torch.Tensor, n: int,
     wo = 1 if with_ones else 0
     Cf = wo + nv
     cells_f = int(np.prod([gf.G[k] for k in range(d)]))
-    fcells = int(np.prod([gf.tile[k] + 2 * gf.msp for k in range(d)]))
-    part = torch.empty((ntiles * Cf * fcells,), dtype=f64, device=dev)
-    fine = torch.empty((Cf * cells_f,), dtype=f64, device=dev)
     sums_f = torch.zeros((Cf,), dtype=f64, device=dev)
-    call("flpr_spread", gfp, ptr(key_start), ptr(wab), ptr(vals), n, wo, nv, ptr(part),
-         ptr(fine), ptr(sums_f), s)
-    del part, wab, vals
+    if d == 3:
+        # the 3-D oversampled footprint (up to 25^3 cells) exceeds the spread's
+        # LDS window: rows added into the zeroed grid (flpr_spread_wide)
+        fine = torch.zeros((Cf * cells_f,), dtype=f64, device=dev)
+        call("flpr_spread_wide", gfp, ptr(key_start), ptr(wab), ptr(vals), n, wo, nv, ptr(fine),
+             ptr(sums_f), s)
+    else:
+        fcells = int(np.prod([gf.tile[k] + 2 * gf.msp for k in range(d)]))
+        part = torch.empty((ntiles * Cf * fcells,), dtype=f64, device=dev)
+        fine = torch.empty((Cf * cells_f,), dtype=f64, device=dev)
+        call("flpr_spread", gfp, ptr(key_start), ptr(wab), ptr(vals), n, wo, nv, ptr(part),
+             ptr(fine), ptr(sums_f), s)
+        del part
+    del wab, vals
     cells = int(np.prod([geo.geom.G[k] for k in range(d)]))
     groups = []
     if wo:
@@ -1013,6 +1018,11 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     acc_s = int(accuracy)
     acc_y = acc_s if (accuracy_y is None or ny == 0) else int(accuracy_y)
     fine_s, fine_y = acc_s > 6, acc_y > 6
+    if d == 3 and (fine_s or fine_y):
+        # the oversampled grid is deconvolved along three axes: the float32
+        # convolution of that grid cancels to 6.6e-5 of yhat (12^3 nodes, order
+        # 1, accuracy 9), so these fits convolve in float64 in every form
+        fp64, mixed = True, False
     split_acc = acc_y != acc_s
     if fine_s or fine_y:
         margin = max(fine_margin(N), min(acc_s, 6), min(acc_y, 6))

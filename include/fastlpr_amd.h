@@ -152,6 +152,17 @@ int flpr_spread(const FlprGeom* g, const int32_t* key_start, const double* wab,
                 const double* vals, int64_t n, int32_t with_ones, int32_t nval, double* part,
                 double* grid, double* sums, void* stream);
 
+/* flpr_spread for three-dimensional footprints wider than its LDS window
+ * (the oversampled grid of accuracy 7..12, nufftn_type1.py:520-636 with
+ * kernel.py:97-156): d == 3 and 7 <= msp <= 12 only.  Same inputs; no
+ * partial windows: every (cell, footprint row, channel) adds its row into
+ * grid with float64 atomics, so grid (C, G0, G1, G2) and sums (C) must be
+ * zeroed by the caller, and the sums of the atomics vary between runs in
+ * their last bits. */
+int flpr_spread_wide(const FlprGeom* g, const int32_t* key_start, const double* wab,
+                     const double* vals, int64_t n, int32_t with_ones, int32_t nval,
+                     double* grid, double* sums, void* stream);
+
 /* Accuracy > 6 (nufftn_type1.py:693-770, kernel.py:97-156,
  * convolution.py:235-236): the nch channels of fine_grid, spread with the
  * oversampled geometry `fine` (L = R * coarse L, R = 2 or 3, msp = accuracy,

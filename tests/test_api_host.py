@@ -74,7 +74,7 @@ def test_cv_fastlpr_validation_errors(bad):
 
 
 @pytest.mark.parametrize("acc,dims,exc", [(-1, 1, ValueError), (13, 1, NotImplementedError),
-                                          (7, 3, NotImplementedError)])
+                                          (13, 3, NotImplementedError)])
 def test_unsupported_accuracy_raises_before_device_work(acc, dims, exc):
     x = np.random.default_rng(0).uniform(size=(60, dims))
     with pytest.raises(exc):

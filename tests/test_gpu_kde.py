@@ -71,6 +71,28 @@ def test_kde_fp32_within_1e5_of_fp64_oracle(fl, name):
     assert _rel(r.lcv["lcv_m"], o.lcv["lcv_m"]) < 1e-5
 
 
+@pytest.mark.parametrize("acc", [8, 11])
+def test_kde_3d_oversampled_accuracy_equals_oracle(fl, acc):
+    """3-D KDE at accuracy > 6 (the oversampled spread of flpr_spread_wide and
+    the deconvolving resample) against the float64 oracle: the float64 form to
+    1e-8, the float32 form to 1e-5, with the same LCV selection."""
+    rng = np.random.default_rng(20 + acc)
+    x = np.clip(0.5 + 0.15 * rng.standard_normal((2500, 3)), 0, 1)
+    # no bandwidth too small for the grid (the reference's cv_fastkde indexes
+    # its densities with the unfiltered list, api.py:997, 1074-1097)
+    h = np.column_stack([np.logspace(np.log10(0.1), np.log10(0.4), 5)] * 3)
+    opt = {"N": [12, 10, 12], "accuracy": acc}
+    o = _oracle(x, h, opt)
+    r64 = fl.cv_fastkde(x, h, dict(opt, precision="fp64"))
+    assert r64.lcv["id1se"] == o.lcv["id1se"]
+    assert _rel(r64.fhat, o.fhat) < 1e-8
+    assert _rel(r64.lcv["lcv_m"], o.lcv["lcv_m"]) < 1e-8
+    r32 = fl.cv_fastkde(x, h, dict(opt))
+    assert r32.lcv["id1se"] == o.lcv["id1se"]
+    assert _rel(r32.fhat, o.fhat) < 1e-5
+    assert _rel(r32.lcv["lcv_m"], o.lcv["lcv_m"]) < 1e-5
+
+
 def test_kde_large_2d_properties(fl):
     """1e6 points: the density integrates to one on the grid, is non-negative,
     the interpolant reproduces the grid at the nodes, and device inputs give

@@ -93,6 +93,14 @@ def _cases():
     y3 = np.sum((x3 - 0.5) ** 2, axis=1) + 0.1 * rng.standard_normal(n3)
     h3 = np.column_stack([np.logspace(-1, np.log10(0.5), 4)] * 3)
     out["3d_o1_m14"] = (x3, y3, h3, {"order": 1, "N": [14, 14, 14]})
+    # accuracy > 6 in 3-D: the oversampled spread (flpr_spread_wide), y alone
+    # (the ones channel and the probes stay at the sample-size accuracy) and,
+    # without DOF, every channel; R = 3 at accuracy 12
+    out["3d_o1_acc7"] = (x3, y3, h3, {"order": 1, "N": [12, 12, 12], "accuracy": 7})
+    out["3d_o1_acc9_nodof"] = (x3, y3, h3, {"order": 1, "N": [12, 12, 12], "accuracy": 9,
+                                            "calc_dof": False})
+    out["3d_o0_acc12_nodof"] = (x3, y3, h3, {"order": 0, "N": [12, 12, 12], "accuracy": 12,
+                                             "calc_dof": False})
     return out
 
 
