@@ -68,13 +68,27 @@ class KernelTimer:
 
     Disabled by default.  When enabled, every timed call records an event pair
     around the launch; ``collect()`` synchronises and returns per-name lists of
-    milliseconds together with the algorithmic work attached to each launch."""
+    milliseconds together with the algorithmic work attached to each launch.
+    ``start(pool)`` creates ``pool`` event pairs up front (each recorded once
+    so that the HIP event exists), so a timed launch only records two events
+    and creates none; past the pool, events are created as needed."""
 
     def __init__(self):
         self.enabled = False
         self._pending = []
+        self._pool = []
 
-    def start(self):
+    def start(self, pool: int = 0):
+        self._pool = []
+        for _ in range(pool):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            b.record()
+            self._pool.append((a, b))
+        if pool:
+            torch.cuda.synchronize()
+        self._pool.reverse()
         self.enabled = True
         self._pending = []
 
@@ -86,8 +100,11 @@ class KernelTimer:
         (evaluated in collect(), outside the timed region)."""
         if not self.enabled:
             return fn(*args)
-        a = torch.cuda.Event(enable_timing=True)
-        b = torch.cuda.Event(enable_timing=True)
+        if self._pool:
+            a, b = self._pool.pop()
+        else:
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
         a.record()
         out = fn(*args)
         b.record()
@@ -101,6 +118,7 @@ class KernelTimer:
             w = work() if callable(work) else work
             res.setdefault(name, []).append((a.elapsed_time(b), w))
         self._pending = []
+        self._pool = []
         return res
 
 
