@@ -25,14 +25,32 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False) -> str:
+    """Compile every source to an object in parallel (one hipcc process per
+    file), then link the shared library."""
     srcs = [os.path.join(HERE, s) for s in SOURCES]
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    cflags = [f for f in FLAGS if f != "-shared"] + ["-c"]
+    procs = []
+    for src in srcs:
+        obj = os.path.join(objdir, os.path.basename(src) + ".o")
+        procs.append((src, obj, subprocess.Popen([hipcc()] + cflags + ["-o", obj, src],
+                                                 stdout=subprocess.PIPE,
+                                                 stderr=subprocess.PIPE, text=True)))
+    errs = []
+    for src, obj, p in procs:
+        _out, err = p.communicate()
+        if p.returncode != 0:
+            errs.append(f"{os.path.basename(src)} ({p.returncode}):\n{err[-4000:]}")
+        elif verbose and err:
+            print(err)
+    if errs:
+        raise RuntimeError("hipcc failed: " + "\n".join(errs))
     tmp = OUT + ".tmp"
-    cmd = [hipcc()] + FLAGS + ["-o", tmp] + srcs
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + [o for _s, o, _p in procs]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stderr[-4000:]}")
-    if verbose and res.stderr:
-        print(res.stderr)
+        raise RuntimeError(f"link failed ({res.returncode}):\n{res.stderr[-4000:]}")
     os.replace(tmp, OUT)
     return OUT
 
