@@ -149,3 +149,81 @@ def lpr_direct(xmod: np.ndarray, y: np.ndarray, axes_scaled: List[np.ndarray], h
             B = np.stack([T[tuple(p)][node] for p in powers])
             out[node + (ih,)] = gauss_jordan_first(A, B)
     return out
+
+
+# ----------------------------------------------------------------------------
+# The direct local polynomial fit (no gridding, no FFT)
+# ----------------------------------------------------------------------------
+#
+# The reference's estimator at a grid node z is e1' (S + lam I)^-1 T with
+#   S_ij(z) = sum_i K(u_i) u_i^(p_i + p_j),  T_i(z) = sum_i K(u_i) u_i^p_i y_i,
+#   u_i = (xs_i - zs) / h,  K(u) = exp(-|u|^2 / 2) / (sqrt(2 pi) prod h),
+# xs the standardised points and zs the node on the standardised grid
+# (model.py:58-72, core.py:64-103, convolution.py:412-512), computed there
+# through Gaussian gridding and FFTs.  The kernel is sampled on the centred
+# grid axis (convolution.py:412-512) and zero-padded (convolution.py:747-794),
+# so it is cut off where |xs_i - zs| exceeds half the grid's extent along an
+# axis; the sums below apply the same cut.  They equal the reference's
+# moments where its gridding is exact: accuracy > 6 (the oversampled,
+# deconvolved NUFFT, nufftn_type1.py:693-770) with bandwidths of a few grid
+# cells (the FFT product convolves the sampled kernel, whose trigonometric
+# interpolant is the Gaussian only when the Gaussian is resolved by the grid
+# and its cut-off tail is negligible).  At accuracy <= 6 the reference
+# spreads without deconvolution (nufftn_type1.py:565-636), which smooths the
+# data by the gridding Gaussian (about sqrt(Msp / pi) cells wide) as well.
+
+def standardised(x: np.ndarray) -> np.ndarray:
+    """z-score with ddof = 1, then centred on the midpoint of min / max
+    (model.py:58-72)."""
+    mean = x.mean(axis=0)
+    std = x.std(axis=0, ddof=1)
+    std = np.where(std == 0, 1.0, std)
+    xs = (x - mean) / std
+    return xs - (xs.min(axis=0) + xs.max(axis=0)) / 2
+
+
+def direct_moments(xs: np.ndarray, y: np.ndarray, zs: np.ndarray, h: np.ndarray,
+                   powers: Sequence[Tuple[int, ...]], cut=None, chunk: int = 64):
+    """S (m, nt, nt) and T (m, nt) at the m standardised nodes zs from the
+    point sums, float64; h (d,); cut (d,) or None: points farther than
+    cut[k] from the node along axis k get no weight."""
+    n, d = xs.shape
+    nt = len(powers)
+    m = zs.shape[0]
+    S = np.zeros((m, nt, nt))
+    T = np.zeros((m, nt))
+    P = np.array(powers)                                  # (nt, d)
+    norm = 1.0 / (math.sqrt(2 * math.pi) * float(np.prod(h)))
+    for a in range(0, m, chunk):
+        z = zs[a:a + chunk]
+        diff = xs[None, :, :] - z[:, None, :]                        # (c, n, d)
+        u = diff / h[None, None, :]
+        K = np.exp(-0.5 * np.sum(u * u, axis=2)) * norm              # (c, n)
+        if cut is not None:
+            K = K * np.all(np.abs(diff) <= np.asarray(cut)[None, None, :], axis=2)
+        mono = np.prod(u[:, :, None, :] ** P[None, None, :, :], axis=3)   # (c, n, nt)
+        Km = K[:, :, None] * mono
+        S[a:a + chunk] = np.einsum("cni,cnj->cij", Km, mono)
+        T[a:a + chunk] = np.einsum("cni,n->ci", Km, y)
+    return S, T
+
+
+def direct_lpr_nodes(x: np.ndarray, y: np.ndarray, N: Sequence[int], nodes: np.ndarray,
+                     h: np.ndarray, powers: Sequence[Tuple[int, ...]], lam: float) -> np.ndarray:
+    """The direct local polynomial fit e1' (S + lam I)^-1 T at grid nodes
+    (index tuples, (m, d)) of the N-node grid over the data's range,
+    bandwidth h (d,), with the kernel cut at half the grid's extent.  lam is
+    the ridge (the reference's 1e-6 max |S_kk| over nodes and bandwidths,
+    regression.py:217, 243-267, supplied by the caller)."""
+    n, d = x.shape
+    xs = standardised(x)
+    lo, hi = xs.min(axis=0), xs.max(axis=0)
+    axes_s = [np.linspace(lo[k], hi[k], int(N[k])) for k in range(d)]
+    # the kernel grid's offsets span [-(hi - lo) / 2, (hi - lo) / 2]
+    cut = (hi - lo) / 2 * (1 + 1e-12)
+    zs = np.stack([axes_s[k][nodes[:, k]] for k in range(d)], axis=1)
+    S, T = direct_moments(xs, y, zs, np.asarray(h, float), powers, cut)
+    out = np.empty(len(zs))
+    for j in range(len(zs)):
+        out[j] = gauss_jordan_first(S[j] + lam * np.eye(len(powers)), T[j][:, None])[0]
+    return out
