@@ -26,8 +26,8 @@ warm-up call, extrapolated linearly to 30, and the same on all the host's
 cores).  Further legs, outside the headline value: the spread's roofline
 entry (HIP events on the spread launches of that second pass), the precision
 form "mixed" (float32 spread with compensated summation, float32 solves with
-one float64 refinement step) as two calls with its yhat difference from the
-float64 form, the learned bandwidth bracket (mean pass with
+one float64 refinement step) as one pair call with its yhat difference from
+the float64 form, the learned bandwidth bracket (mean pass with
 bandwidth_bracket="learned" against the full sweep, and whether both select
 the same bandwidth), a cold call (caches cleared: probe generation and every
 plan rebuilt), and the batched 512-problem workload as problems per second,
