@@ -754,15 +754,15 @@ def _items(ps, geos):
 def _zeroed(dev, specs, stream):
     """Zero-filled buffers from one allocation, cleared by flpr_fill_zero (a
     kernel, so that a captured graph clears them on every replay)."""
-    offs, total = [], 0
+    offs, sizes, total = [], [], 0
     for (cnt, dt) in specs:
         offs.append(total)
-        nb = int(cnt) * torch.empty((), dtype=dt).element_size()
+        nb = int(cnt) * dt.itemsize
+        sizes.append(nb)
         total += (nb + 255) // 256 * 256
     arena = torch.empty((max(total, 256),), dtype=torch.uint8, device=dev)
     call("flpr_fill_zero", arena.data_ptr(), arena.numel(), stream)
     out = []
-    for (cnt, dt), o in zip(specs, offs):
-        nb = int(cnt) * torch.empty((), dtype=dt).element_size()
+    for (cnt, dt), o, nb in zip(specs, offs, sizes):
         out.append(arena[o:o + nb].view(dt) if cnt else None)
     return out

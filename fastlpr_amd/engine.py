@@ -133,19 +133,16 @@ def zeroed(dev, *specs):
     """Several zero-filled device buffers from one allocation and one fill:
     specs are (count, dtype) pairs (count 0 gives None); every buffer starts
     on a 256-byte boundary."""
-    offs, total = [], 0
+    offs, sizes, total = [], [], 0
     for (cnt, dt) in specs:
         offs.append(total)
-        nb = int(cnt) * torch.empty((), dtype=dt).element_size()
+        nb = int(cnt) * dt.itemsize
+        sizes.append(nb)
         total += (nb + 255) // 256 * 256
     arena = torch.zeros((max(total, 256),), dtype=torch.uint8, device=dev)
     out = []
-    for (cnt, dt), o in zip(specs, offs):
-        if int(cnt) == 0:
-            out.append(None)
-            continue
-        nb = int(cnt) * torch.empty((), dtype=dt).element_size()
-        out.append(arena[o:o + nb].view(dt))
+    for (cnt, dt), o, nb in zip(specs, offs, sizes):
+        out.append(arena[o:o + nb].view(dt) if nb else None)
     return out
 
 
@@ -577,12 +574,24 @@ class Stats:
 def x_stats(x: torch.Tensor, y: Optional[torch.Tensor] = None) -> Stats:
     """Per-dimension min, max, mean and std(ddof=1) of x (n, d) float64 on the device,
     counts of non-finite values in x and y, and sum_i y_ic^2 per column of y."""
+    return x_stats_read(x_stats_launch(x, y))
+
+
+def x_stats_launch(x: torch.Tensor, y: Optional[torch.Tensor] = None):
+    """Queues the statistics pass of x_stats; x_stats_read waits for it, so
+    host work placed between the two overlaps the device."""
     n, d = x.shape
     ny = 0 if y is None else y.shape[1]
-    dev = x.device
-    part = torch.empty(512 * (5 * d + 2 + ny), dtype=torch.float64, device=dev)
-    out = torch.empty(5 * d + 2 + ny, dtype=torch.float64, device=dev)
+    m = 5 * d + 2 + ny
+    buf = torch.empty(513 * m, dtype=torch.float64, device=x.device)
+    part, out = buf[:512 * m], buf[512 * m:]
     call("flpr_xstats", ptr(x), n, d, ptr(y), ny, ptr(part), ptr(out), stream_handle())
+    return out, d
+
+
+def x_stats_read(pending) -> Stats:
+    """The Stats of a pass queued by x_stats_launch (one host read)."""
+    out, d = pending
     o = out.cpu().numpy()
     w = o[:5 * d].reshape(d, 5)
     cnt = w[:, 4]
@@ -1073,11 +1082,11 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
         dev, (2 * nkeys, i32), (C, f64), (1, torch.int64),
         (dh * nval + nq_out if need_q else 0, f64), (nval * nodes_all if need_q else 0, f64),
         (nw * rcells if (need_q and want_rss) else 0, f64))
-    key = torch.empty((n,), dtype=i32, device=dev)
+    # cell keys, their starts, the sorted order and the ranks: one allocation
+    ints = torch.empty((3 * n + nkeys + 1,), dtype=i32, device=dev)
+    key, order_idx, rank = ints[:n], ints[n:2 * n], ints[2 * n:3 * n]
+    key_start = ints[3 * n:]
     key_count, key_cursor = cnt[:nkeys], cnt[nkeys:]
-    key_start = torch.empty((nkeys + 1,), dtype=i32, device=dev)
-    order_idx = torch.empty((n,), dtype=i32, device=dev)
-    rank = torch.empty((n,), dtype=i32, device=dev)
     call("flpr_points_prepare", ptr(x), n, gp, ptr(key), ptr(key_count), ptr(rank), s)
     call("flpr_points_bin", ptr(key), n, nkeys, ptr(key_count), ptr(key_start), ptr(key_cursor),
          ptr(rank), ptr(order_idx), s)
