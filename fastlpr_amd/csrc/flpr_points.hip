@@ -1255,16 +1255,13 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adjoint(
 // WALK_LANES consecutive lanes take one item, each walking every
 // WALK_LANES-th of the cell's points in sorted order and summing their 2^D
 // corner products v_ic w_i(corner) (MODE 0) or nw corner-weight products
-// (MODE 1) in registers.  The lanes' sums are added with shuffles and the
-// first lane adds each to its node / raw cell with one float64 atomic; a
-// point outside the main interval adds its own products.  No LDS windows,
-// partials or gather pass; out must be zeroed.  On 1e6 points and 256^2 nodes:
-// Gram entries 16 us against 57 for k_adjoint + gather, one value channel 15
-// against 24, eleven value channels 107 against 79 (their 2.9e6 atomics cost
-// more than the LDS windows), so flpr_adjoint walks up to WALK_MAX_CHANNELS
-// value channels and gives more to k_adjoint.
+// (MODE 1) in registers; the lanes' sums are added with shuffles.  MODE 0
+// stores the item's 2^D corner sums to part[item][corner] (zeros for a cell
+// without points), which k_adjoint_walk_gather adds into the nodes in a fixed
+// order; MODE 1 adds each entry to its raw cell with one float64 atomic (a
+// raw cell has one main spread cell).  A point outside the main interval
+// adds its own products to out with atomics.  out must be zeroed.
 constexpr int WALK_LANES = 4;
-constexpr int WALK_MAX_CHANNELS = 4;
 
 template <int D, int MODE>
 __global__ __launch_bounds__(256) void k_adjoint_walk(FlprGeom g,
@@ -1273,7 +1270,8 @@ __global__ __launch_bounds__(256) void k_adjoint_walk(FlprGeom g,
                                                       const double* __restrict__ rawf,
                                                       const double* __restrict__ vals, int nch,
                                                       int64_t nkeys, double* __restrict__ out,
-                                                      int64_t out_stride) {
+                                                      int64_t out_stride,
+                                                      double* __restrict__ part) {
   constexpr int NQ2 = 1 << D;
   constexpr int NACC = (MODE == 0) ? NQ2 : NQ2 * (NQ2 + 1) / 2;
   int tc = 1;
@@ -1289,7 +1287,7 @@ __global__ __launch_bounds__(256) void k_adjoint_walk(FlprGeom g,
     const int c = (int)(it % per);
     const int64_t key = it / per;
     const int p0 = key_start[key], p1 = key_start[key + 1];
-    if (p0 == p1) continue;
+    if (MODE == 1 && p0 == p1) continue;
     int t[3] = {0, 0, 0}, l[3] = {0, 0, 0}, im[3] = {0, 0, 0};
     tile_coords<D>(g, (int)(key / tc), t);
     local_coords<D>(g, (int)(key % tc), l);
@@ -1354,22 +1352,69 @@ __global__ __launch_bounds__(256) void k_adjoint_walk(FlprGeom g,
 #pragma unroll
       for (int e = 0; e < NACC; ++e) acc[e] += __shfl_xor(acc[e], off);
     }
-    if (sub != 0 || !any_main) continue;
     if constexpr (MODE == 0) {
+      // corner q of the item in lane q of the group (NQ2 <= 8 lanes: a
+      // 3-D item writes its upper four corners from the same lanes)
 #pragma unroll
-      for (int q = 0; q < NQ2; ++q) {
-        int64_t node = 0;
-#pragma unroll
-        for (int k = 0; k < D; ++k) node = node * g.n_grid[k] + im[k] + ((q >> (D - 1 - k)) & 1);
-        atomicAdd(&out[(int64_t)c * out_stride + node], acc[q]);
-      }
+      for (int q = 0; q < NQ2; ++q)
+        if (sub == (q % WALK_LANES)) part[it * NQ2 + q] = acc[q];
     } else {
+      if (sub != 0 || !any_main) continue;
       int64_t rc = 0;
 #pragma unroll
       for (int k = 0; k < D; ++k) rc = rc * (g.n_grid[k] - 1) + im[k];
 #pragma unroll
       for (int e = 0; e < NACC; ++e) atomicAdd(&out[(int64_t)e * out_stride + rc], acc[e]);
     }
+  }
+}
+
+// q[c][node] += the corner sums of the (up to 2^D) cells around the node,
+// from k_adjoint_walk's part, in a fixed order; one thread per (node,
+// channel = blockIdx.y).
+template <int D>
+__global__ __launch_bounds__(256) void k_adjoint_walk_gather(FlprGeom g,
+                                                             const double* __restrict__ part,
+                                                             int nch, int64_t nodes,
+                                                             double* __restrict__ q) {
+  constexpr int NQ2 = 1 << D;
+  const int c = blockIdx.y;
+  int tc = 1, tstride[3] = {1, 1, 1}, lstride[3] = {1, 1, 1};
+#pragma unroll
+  for (int k = D - 1; k >= 0; --k) {
+    if (k < D - 1) {
+      tstride[k] = tstride[k + 1] * g.ntile[k + 1];
+      lstride[k] = lstride[k + 1] * g.tile[k + 1];
+    }
+    tc *= g.tile[k];
+  }
+  for (int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; node < nodes;
+       node += (int64_t)gridDim.x * blockDim.x) {
+    int r[3] = {0, 0, 0};
+    int64_t rem = node;
+#pragma unroll
+    for (int k = D - 1; k >= 0; --k) {
+      r[k] = (int)(rem % g.n_grid[k]);
+      rem /= g.n_grid[k];
+    }
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ2; ++q) {
+      // the interval im = r - bit along each axis holds this node as its corner q
+      bool ok = true;
+      int64_t key_t = 0, key_l = 0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        const int bit = (q >> (D - 1 - k)) & 1;
+        const int im = r[k] - bit;
+        ok = ok && im >= 0 && im <= g.n_grid[k] - 2;
+        const int sidx = im - (g.n_grid[k] - 1) / 2 + g.E[k];
+        key_t += (int64_t)(sidx / g.tile[k]) * tstride[k];
+        key_l += (int64_t)(sidx % g.tile[k]) * lstride[k];
+      }
+      if (ok) v += part[((key_t * tc + key_l) * nch + c) * NQ2 + q];
+    }
+    q[(int64_t)c * nodes + node] += v;
   }
 }
 
@@ -2470,7 +2515,7 @@ int flpr_adjoint(const FlprGeom* g, const int32_t* key_start, const int32_t* raw
   const int nw = nq * (nq + 1) / 2;
   const int nsplit0 = item_splits(g, n);
   hipStream_t st = as_stream(stream);
-  // even node counts: the Gram entries and a few value channels by k_adjoint_walk
+  // even node counts: both sums by k_adjoint_walk
   bool aligned = true;
   for (int k = 0; k < g->d; ++k) aligned = aligned && !(g->n_grid[k] & 1);
   if (aligned) {
@@ -2482,21 +2527,31 @@ int flpr_adjoint(const FlprGeom* g, const int32_t* key_start, const int32_t* raw
       dim3 grd(flpr_grid_for(nkeys * (nch_) * WALK_LANES, 256, 16384));                        \
       switch (g->d) {                                                                           \
         case 1: hipLaunchKernelGGL((k_adjoint_walk<1, M>), grd, dim3(256), 0, st, *g, key_start, \
-                                   rawi, rawf, vals, nval, nkeys, out_, stride_); break;        \
+                                   rawi, rawf, vals, nval, nkeys, out_, stride_, part); break;  \
         case 2: hipLaunchKernelGGL((k_adjoint_walk<2, M>), grd, dim3(256), 0, st, *g, key_start, \
-                                   rawi, rawf, vals, nval, nkeys, out_, stride_); break;        \
+                                   rawi, rawf, vals, nval, nkeys, out_, stride_, part); break;  \
         default: hipLaunchKernelGGL((k_adjoint_walk<3, M>), grd, dim3(256), 0, st, *g,           \
-                                    key_start, rawi, rawf, vals, nval, nkeys, out_, stride_);   \
+                                    key_start, rawi, rawf, vals, nval, nkeys, out_, stride_,    \
+                                    part);                                                      \
           break;                                                                                \
       }                                                                                         \
       FLPR_CHECK_LAUNCH();                                                                      \
     } while (0)
-    if (nval > 0 && nval <= WALK_MAX_CHANNELS) FLPR_WALK(0, q, nodes, nval);
     if (gram) FLPR_WALK(1, gram, rcells, 1);
+    if (nval > 0) {
+      FLPR_WALK(0, q, nodes, nval);
+      dim3 gg(flpr_grid_for(nodes, 256, 1024), (unsigned)nval);
+      switch (g->d) {
+        case 1: hipLaunchKernelGGL(k_adjoint_walk_gather<1>, gg, dim3(256), 0, st, *g, part, nval, nodes, q); break;
+        case 2: hipLaunchKernelGGL(k_adjoint_walk_gather<2>, gg, dim3(256), 0, st, *g, part, nval, nodes, q); break;
+        default: hipLaunchKernelGGL(k_adjoint_walk_gather<3>, gg, dim3(256), 0, st, *g, part, nval, nodes, q); break;
+      }
+      FLPR_CHECK_LAUNCH();
+    }
 #undef FLPR_WALK
-    if (nval <= WALK_MAX_CHANNELS) return 0;
+    return 0;
   }
-  for (int mode = 0; mode < (aligned ? 1 : 2); ++mode) {
+  for (int mode = 0; mode < 2; ++mode) {
     const int nch = (mode == 0) ? nval : (gram ? nw : 0);
     if (nch == 0) continue;
     double* out = (mode == 0) ? q : gram;

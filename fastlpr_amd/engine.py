@@ -985,6 +985,14 @@ class DeviceFit:
     key: Optional[torch.Tensor] = None   # global regularisation key (key_only)
 
 
+def adjoint_workspace(g, d: int, ntiles: int, nch: int) -> int:
+    """Doubles of flpr_adjoint's workspace: per tile and channel, its LDS
+    window (tile + 5 cells per axis) or the 2^d corner sums of its cells."""
+    wcells = int(np.prod([g.tile[k] + 5 for k in range(d)]))
+    corners = int(np.prod([g.tile[k] for k in range(d)])) << d
+    return ntiles * nch * max(wcells, corners)
+
+
 def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
              probes: Optional[torch.Tensor], K: int, h: np.ndarray, order: int,
              N: Tuple[int, ...], accuracy: int, fp64: bool, regularization: float,
@@ -1217,8 +1225,8 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     # quadratic forms (dh, ny); q, gram: adjoint sums (all zeroed above)
     dot = red[:dh * nval] if need_q else None
     if need_q:
-        wcells = int(np.prod([g.tile[k] + 5 for k in range(d)]))
-        apart = torch.empty((ntiles * max(nval, nw) * wcells,), dtype=f64, device=dev)
+        apart = torch.empty((adjoint_workspace(g, d, ntiles, max(nval, nw)),), dtype=f64,
+                            device=dev)
         call("flpr_adjoint", gp, ptr(key_start), ptr(rawi), ptr(rawf), ptr(vals), n, nval,
              ptr(apart), ptr(q), ptr(gram), s)
         del apart
@@ -1418,8 +1426,7 @@ def run_grid_y(sh: dict, x: torch.Tensor, vals_y: torch.Tensor, ny: int,
     nodes = plan0.nodes
     per_h0 = plan0.stage_per_h[0]
     nw = (1 << d) * ((1 << d) + 1) // 2
-    wcells = int(np.prod([g.tile[k] + 5 for k in range(d)]))
-    apart = torch.empty((ntiles * max(ny, nw) * wcells,), dtype=f64, device=dev)
+    apart = torch.empty((adjoint_workspace(g, d, ntiles, max(ny, nw)),), dtype=f64, device=dev)
     call("flpr_adjoint", gp, ptr(sh["key_start"]), ptr(sh["rawi"]), ptr(sh["rawf"]), ptr(vals), n,
          ny, ptr(apart), ptr(q), None, s)
     del apart

@@ -308,8 +308,8 @@ int flpr_solve_refined(const void* mom, int64_t per_h, int32_t dh, int64_t nodes
  *   sum_i v_i yhat_i = sum_node q m   and   sum_i yhat_i^2 = sum_cell m' W m,
  * which turn the Hutchinson dots p' phat and the residual sums of
  * regression.py:1256-1280 / api.py:686-723 into grid-sized work per bandwidth.
- * part is workspace of ntiles * max(nval, nw) * prod(tile + 5) doubles; q and
- * gram must be zeroed by the caller. */
+ * part is workspace of ntiles * max(nval, nw) * max(prod(tile + 5),
+ * 2^d prod(tile)) doubles; q and gram must be zeroed by the caller. */
 int flpr_adjoint(const FlprGeom* g, const int32_t* key_start, const int32_t* rawi,
                  const double* rawf, const double* vals, int64_t n, int32_t nval, double* part,
                  double* q, double* gram, void* stream);
