@@ -823,7 +823,7 @@ def cv_fastkde(x, h: Optional[Sequence[Sequence[float]]] = None,
     dh = hl.shape[0]
     # normalise so that sum(f) dx = 1 on the raw grid, clip at zero (api.py:1062-1096)
     dx_grid = float(np.prod([ax[1] - ax[0] for ax in geo.axes_raw]))
-    s = fit.s0_all
+    s = fit.s0_all.all()
     sdx = s.sum(dim=1, keepdim=True) * dx_grid
     dens = torch.where(sdx > 0, s / sdx, s).clamp_min(0.0)
     k0h = K0_GAUSS / np.prod(hl, axis=1)

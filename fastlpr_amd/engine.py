@@ -977,12 +977,37 @@ class DeviceFit:
     m_y: torch.Tensor                # (dh, ny, nodes) float64
     rss: Optional[np.ndarray]        # (dh, ncol_out)
     dot: Optional[np.ndarray]        # (dh, K)
-    s0_all: torch.Tensor             # (dh, nodes) S_00 (float64)
+    s0_all: "S00Rows"                # S_00 per bandwidth (rows on demand)
     geo: Geometry
     plan: Plan
     warn: Optional[str]
     red: Optional[torch.Tensor] = None   # device (dh, nval) dots then (dh, ny) quad forms
     key: Optional[torch.Tensor] = None   # global regularisation key (key_only)
+
+
+class S00Rows:
+    """S_00 per bandwidth, (dh, nodes) float64, read from the stage-0 slabs of
+    the precision groups: a row (the selected bandwidth) is converted when it
+    is indexed, all rows by all() (the density path)."""
+
+    def __init__(self, stage0s, per_h0: int, off0: int, nodes: int, eps_add: float):
+        self._parts = [(a, b, st0) for (a, b, _gf, _p, st0) in stage0s]
+        self._per_h0, self._off0, self._nodes, self._eps = per_h0, off0, nodes, eps_add
+
+    def _rows(self, st0, n):
+        return st0.view(n, self._per_h0)[:, self._off0:self._off0 + self._nodes]
+
+    def __getitem__(self, i: int) -> torch.Tensor:
+        for (a, b, st0) in self._parts:
+            if a <= i < b:
+                r = self._rows(st0, b - a)[i - a].to(torch.float64)
+                return r + self._eps if self._eps else r
+        raise IndexError(i)
+
+    def all(self) -> torch.Tensor:
+        rows = [self._rows(st0, b - a).to(torch.float64) for (a, b, st0) in self._parts]
+        out = rows[0] if len(rows) == 1 else torch.cat(rows, dim=0)
+        return out + self._eps if self._eps else out
 
 
 def adjoint_workspace(g, d: int, ntiles: int, nch: int) -> int:
@@ -1213,11 +1238,7 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     if key_only:
         return DeviceFit(h, None, None, None, None, geo, plan0, warn, None, key=key_max)
     if moments_only:
-        s0_parts = [st0.view(b - a, per_h0)[:, off0:off0 + nodes].to(f64)
-                    for (a, b, _gf, _p, st0) in stage0s]
-        s0_all = s0_parts[0] if len(s0_parts) == 1 else torch.cat(s0_parts, dim=0)
-        if order == 0:
-            s0_all = s0_all + eps_add
+        s0_all = S00Rows(stage0s, per_h0, off0, nodes, eps_add if order == 0 else 0.0)
         return DeviceFit(h, None, None, None, s0_all, geo, plan0, warn, None)
 
     # --- adjoint sums, solve (+ dots), residual quadratic form
@@ -1248,11 +1269,7 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     if nq_out:
         quad = red[dh * nval:]
         call("flpr_quadform", gp, ptr(m_y), ny, dh, ptr(gram), ptr(quad), s)
-    s0_parts = [st0.view(b - a, per_h0)[:, off0:off0 + nodes].to(f64)
-                for (a, b, _gf, _p, st0) in stage0s]
-    s0_all = s0_parts[0] if len(s0_parts) == 1 else torch.cat(s0_parts, dim=0)
-    if order == 0:
-        s0_all = s0_all + eps_add
+    s0_all = S00Rows(stage0s, per_h0, off0, nodes, eps_add if order == 0 else 0.0)
     if debug is not None:
         debug.update(grid64=grid64, stage0=stage0s[0][4], stage0s=stage0s, key=key_max, q=q,
                      dot=dot, m_y=m_y, plan=plan0, geo=geo, h=h, order_idx=order_idx,
