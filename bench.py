@@ -34,10 +34,17 @@ plan rebuilt), and the batched 512-problem workload as problems per second,
 batched against one cv_fastlpr call per problem and against its
 captured-graph replay.  The default of 50 timed steps (about 0.1 s) keeps a
 short host-side disturbance from dominating the launch-bound step time.
+After the warm-up the process's garbage is collected and the surviving
+objects are frozen (gc.freeze), outside every timed region: a full
+collection over the objects of the imports and of the first calls takes
+about 35 ms on the bench host, and would otherwise land in one timed pass or
+another at random (+0.7 ms per step in a pass of 50).  A long-running caller
+holds those objects in the oldest generation the same way.
 """
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import re
@@ -494,6 +501,8 @@ def main():
     for _ in range(max(args.warmup, 1)):
         step_pair(pair, xd, yd, hl, "fp32")
     torch.cuda.synchronize()
+    gc.collect()
+    gc.freeze()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         r1, r2 = step_pair(pair, xd, yd, hl, "fp32")

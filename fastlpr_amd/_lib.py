@@ -130,7 +130,7 @@ _SIGS = {
     "flpr_grid_center": [_P, _I32, _I64, _P, _P, _P, _P],
     "flpr_taps": [_GP, _P, _I32, _I32, _I32, _D, _P, _P, _P],
     "flpr_conv": [_I32, _I32, _P, _P, _P, _I32, _P, _P, _I32, _I32, _I32, _I32, _I32, _I32,
-                  _P, _I32, _P],
+                  _I32, _P, _I32, _P],
     "flpr_max_moment": [_I32, _P, _I64, _I32, C.POINTER(_I64), _I32, _I64, _I32, _P, _P],
     "flpr_solve": [_I32, _P, _I64, _I32, _I64, _I32, _I32, _I32, C.POINTER(_I64), C.POINTER(_I64), _P,
                    _D, _D, _P, _I32, _I32, _P, _P, _P, _P],

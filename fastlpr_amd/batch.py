@@ -367,7 +367,8 @@ class BatchPlan:
                     allj.append(t)
                 jb = np.concatenate(allj)
                 bp = engine.Pass(ps.axis, ps.mode, ps.K, ps.nout, ps.inner, ps.base, ps.max_ro,
-                                 torch.as_tensor(jb, device=dev), ps.in_stage, ps.out_stage, jb)
+                                 torch.as_tensor(jb, device=dev), ps.in_stage, ps.out_stage, jb,
+                                 ps.max_np)
                 bands_g = band_all[a * d * 2:]
                 engine.conv_schedule(bp, gf, bands_g)   # builds and keeps the block table
                 passes.append((bp, bands_g))

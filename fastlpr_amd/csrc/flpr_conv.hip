@@ -389,6 +389,7 @@ constexpr int SX_COPY = 88;         // elements of one shifted window copy in LD
 constexpr int SX_UNITS = SX_COPY / 8;
 constexpr int SX_PITCH = 12;        // LDS units per copy: with 12 the fragment reads of the
                                     // eight copies measure conflict-free on gfx950 (11: 1.25x)
+constexpr int SX_MAX_POWERS = 3;   // tap rows one job of the float32 form may apply
 constexpr int SX_TOP_PAD = 62;      // reversed table index v = (nmax + 62) - t
 
 __host__ __device__ inline int64_t split_strip_len(int nmax) {
@@ -489,7 +490,11 @@ __device__ inline v4f mfma_split6(const u32x4 (&x)[3], const u32x4 (&y)[3], v4f 
 // 8 K values 8 g .. 8 g + 7 of dense index 16 db + n, two chunks ahead) and
 // splits them there; only the Toeplitz windows, which all four waves read,
 // go through LDS (double-buffered, one barrier per chunk).
-template <int MODE>
+// Powers: a job applies np <= NP consecutive tap rows (the polynomial powers
+// of one axis) to the same input, writing power p's output at
+// out_off + p * out_stride; every dense fragment is loaded and split once for
+// all of them, and each power keeps its own accumulators and LDS window.
+template <int MODE, int NP>
 __global__ __launch_bounds__(CT) void k_conv_split(const float* __restrict__ in,
                                                    float* __restrict__ out,
                                                    const uint16_t* __restrict__ strips, int nmax,
@@ -498,7 +503,7 @@ __global__ __launch_bounds__(CT) void k_conv_split(const float* __restrict__ in,
                                                    int nout, int inner, int base, int ntd, int ntt,
                                                    const int32_t* __restrict__ blocks) {
   constexpr int KC = SX_KC;
-  __shared__ u32x4 strip[2][3][8][SX_PITCH];
+  __shared__ u32x4 strip[2][NP][3][8][SX_PITCH];
   int job = blockIdx.y, bx = blockIdx.x;
   if (blocks) {
     job = blocks[2 * blockIdx.x];
@@ -508,6 +513,8 @@ __global__ __launch_bounds__(CT) void k_conv_split(const float* __restrict__ in,
   const int64_t* jb = jobs + 8 * (int64_t)job;
   const int64_t in_off = jb[0], out_off = jb[1], tap_off = jb[2];
   const int band = (int)jb[3], ro = (int)jb[4];
+  const int jnp = min(max((int)jb[5], 1), NP);
+  const int64_t ostride = jb[6];
   const int t_lo = bands[2 * band], t_hi = bands[2 * band + 1];
   const int td = bx % ntd;
   bx /= ntd;
@@ -523,7 +530,8 @@ __global__ __launch_bounds__(CT) void k_conv_split(const float* __restrict__ in,
   const float* src = in + in_off + (MODE == 0 ? 0 : (int64_t)o * K * inner);
   float* dst = out + out_off + (MODE == 0 ? 0 : (int64_t)o * nout * inner);
   const int64_t V = split_strip_len(nmax);
-  const u32x4* sg = reinterpret_cast<const u32x4*>(strips) + (tap_off / nmax) * 3 * 8 * (V / 8);
+  const int64_t rowu = 3 * 8 * (V / 8);  // 16-byte units per table row
+  const u32x4* sg = reinterpret_cast<const u32x4*>(strips) + (tap_off / nmax) * rowu;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int dw = wave * (SX_NDB * 16);
@@ -539,30 +547,51 @@ __global__ __launch_bounds__(CT) void k_conv_split(const float* __restrict__ in,
     const int dd = d0 + dw + 16 * db + n;
     dsrc[db] = (MODE == 0) ? src + (int64_t)dd * K + 8 * g : src + (int64_t)(8 * g) * inner + dd;
   }
-  // strip staging: threads < 8 * SX_UNITS own (copy r, unit mc)
-  const bool sthr = tid < 8 * SX_UNITS;
-  const int sr = tid / SX_UNITS, smc = tid % SX_UNITS;
+  // window staging: unit (power p, plane pl, copy r, unit mc) of a chunk's
+  // windows, NP * 3 * 8 * SX_UNITS of them, thread tid owning units tid + CT u.
+  // Copy r of the window of chunk c is table copy (vb + r) & 7 from unit
+  // ((vb + r) >> 3) + mc, vb = nmax - 1 - i0 - base + s_begin + KC c; KC is a
+  // multiple of 8, so the copy is the same for every chunk and the unit
+  // advances by KC / 8 per chunk.
+  constexpr int SITEMS = NP * 3 * 8 * SX_UNITS;
+  constexpr int SPT = (SITEMS + CT - 1) / CT;
+  int soff[SPT], sdst[SPT];
+  {
+    const int vb0 = nmax - 1 - i0 - base + s_begin;
+#pragma unroll
+    for (int u = 0; u < SPT; ++u) {
+      const int idx = tid + CT * u;
+      const int p = idx / (3 * 8 * SX_UNITS);
+      const int rem = idx - p * (3 * 8 * SX_UNITS);
+      const int pl = rem / (8 * SX_UNITS);
+      const int r = (rem - pl * 8 * SX_UNITS) / SX_UNITS;
+      const int mc = rem - pl * 8 * SX_UNITS - r * SX_UNITS;
+      const bool on = idx < SITEMS && p < jnp;
+      soff[u] = on ? (int)(p * rowu + ((int64_t)(pl * 8 + ((vb0 + r) & 7)) * V) / 8 +
+                           ((vb0 + r) >> 3) + mc)
+                   : 0;
+      sdst[u] = on ? ((p * 3 + pl) * 8 + r) * SX_PITCH + mc : -1;
+    }
+  }
 
   // float32 MFMA accumulators over the whole K range of the tile
-  v4f acc[NQ][SX_NDB];
+  v4f acc[NP][NQ][SX_NDB];
 #pragma unroll
-  for (int q = 0; q < NQ; ++q)
+  for (int p = 0; p < NP; ++p)
 #pragma unroll
-    for (int db = 0; db < SX_NDB; ++db) acc[q][db] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int db = 0; db < SX_NDB; ++db) acc[p][q][db] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
 
   float preA[SX_NDB][8], preB[SX_NDB][8];
-  u32x4 spA[3], spB[3];
+  u32x4 spA[SPT], spB[SPT];
   // the window units first: a wait for them then leaves the dense loads
   // issued after them in flight
-  auto gload = [&](float (&pre)[SX_NDB][8], u32x4 (&sp)[3], int s0) __attribute__((always_inline)) {
-    if (sthr) {
-      // copy sr of the chunk window = table copy (vb + sr) & 7 from an aligned unit
-      const int vb = nmax - 1 - i0 - base + s0;
-      const int rr = (vb + sr) & 7;
-      const int64_t e8 = ((vb + sr) >> 3) + smc;
+  auto gload = [&](float (&pre)[SX_NDB][8], u32x4 (&sp)[SPT], int c) __attribute__((always_inline)) {
 #pragma unroll
-      for (int pl = 0; pl < 3; ++pl) sp[pl] = sg[((int64_t)(pl * 8 + rr) * V) / 8 + e8];
-    }
+    for (int u = 0; u < SPT; ++u)
+      if (sdst[u] >= 0) sp[u] = sg[soff[u] + (KC / 8) * c];
+    const int s0 = s_begin + c * KC;
 #pragma unroll
     for (int db = 0; db < SX_NDB; ++db) {
       const float* p = (MODE == 0) ? dsrc[db] + s0 : dsrc[db] + (int64_t)s0 * inner;
@@ -570,30 +599,35 @@ __global__ __launch_bounds__(CT) void k_conv_split(const float* __restrict__ in,
       for (int j = 0; j < 8; ++j) pre[db][j] = (MODE == 0) ? p[j] : p[(int64_t)j * inner];
     }
   };
-  auto sstore = [&](const u32x4 (&sp)[3], int buf) __attribute__((always_inline)) {
-    if (sthr) {
+  auto sstore = [&](const u32x4 (&sp)[SPT], int buf) __attribute__((always_inline)) {
+    u32x4* sb = &strip[buf][0][0][0][0];
 #pragma unroll
-      for (int pl = 0; pl < 3; ++pl) strip[buf][pl][sr][smc] = sp[pl];
-    }
+    for (int u = 0; u < SPT; ++u)
+      if (sdst[u] >= 0) sb[sdst[u]] = sp[u];
   };
   // Toeplitz fragments: block q of chunk ch + 1 equals block q - 2 of chunk ch
-  // (i - s is what counts and the chunk advances by 32 = two blocks), so a
-  // chunk reads blocks 0 and 1 into tvn and takes blocks 2 and 3 from the
-  // previous chunk's blocks 0 and 1 (tvo); the first chunk reads all four.
+  // (i - s is what counts and the chunk advances by 32 = two blocks).  With
+  // one power, a chunk reads blocks 0 and 1 into tvn and takes blocks 2 and 3
+  // from the previous chunk's blocks 0 and 1 (tvo); the first chunk reads all
+  // four.  With several powers every fragment is read from LDS where it is
+  // used (holding two blocks per power would cost 24 registers a power).
   auto compute = [&](int ch, int buf, const float (&pre)[SX_NDB][8], u32x4 (&tvn)[2][3],
                      u32x4 (&tvo)[2][3]) __attribute__((always_inline)) {
     const int sc0 = s_begin + ch * KC;
-    if (ch == 0) {
+    if constexpr (NP == 1) {
+      if (ch == 0) {
 #pragma unroll
-      for (int q = 2; q < 4; ++q)
+        for (int q = 2; q < 4; ++q)
+#pragma unroll
+          for (int pl = 0; pl < 3; ++pl)
+            tvo[q - 2][pl] = strip[buf][0][pl][cr][7 - 2 * q - (n >> 3) + g];
+      }
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
 #pragma unroll
         for (int pl = 0; pl < 3; ++pl)
-          tvo[q - 2][pl] = strip[buf][pl][cr][7 - 2 * q - (n >> 3) + g];
+          tvn[q][pl] = strip[buf][0][pl][cr][7 - 2 * q - (n >> 3) + g];
     }
-#pragma unroll
-    for (int q = 0; q < 2; ++q)
-#pragma unroll
-      for (int pl = 0; pl < 3; ++pl) tvn[q][pl] = strip[buf][pl][cr][7 - 2 * q - (n >> 3) + g];
     // dense fragments: only the last chunk reaches past the band end (inputs
     // there may be another slab's values or padding: zeroed)
     float x[SX_NDB][8];
@@ -619,61 +653,87 @@ __global__ __launch_bounds__(CT) void k_conv_split(const float* __restrict__ in,
       if (!(sc0 + KC - 1 < ib + base - t_hi || sc0 > ib + 15 + base - t_lo)) live |= 1 << q;
     }
     live = __builtin_amdgcn_readfirstlane(live);
+    if constexpr (NP == 1) {
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      if (live & (1 << q)) {
-        const u32x4 (&tq)[3] = (q < 2) ? tvn[q & 1] : tvo[q & 1];
+      for (int q = 0; q < NQ; ++q) {
+        if (live & (1 << q)) {
+          const u32x4 (&tq)[3] = (q < 2) ? tvn[q & 1] : tvo[q & 1];
 #pragma unroll
-        for (int db = 0; db < SX_NDB; ++db)
-          acc[q][db] = (MODE == 0) ? mfma_split6(dv[db], tq, acc[q][db])
-                                   : mfma_split6(tq, dv[db], acc[q][db]);
+          for (int db = 0; db < SX_NDB; ++db)
+            acc[0][q][db] = (MODE == 0) ? mfma_split6(dv[db], tq, acc[0][q][db])
+                                        : mfma_split6(tq, dv[db], acc[0][q][db]);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        if (p < jnp) {
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            if (live & (1 << q)) {
+              u32x4 tq[3];
+#pragma unroll
+              for (int pl = 0; pl < 3; ++pl)
+                tq[pl] = strip[buf][p][pl][cr][7 - 2 * q - (n >> 3) + g];
+#pragma unroll
+              for (int db = 0; db < SX_NDB; ++db)
+                acc[p][q][db] = (MODE == 0) ? mfma_split6(dv[db], tq, acc[p][q][db])
+                                            : mfma_split6(tq, dv[db], acc[p][q][db]);
+            }
+          }
+        }
       }
     }
   };
 
   // Chunk c: dense values in register set A (even c) or B (odd c), loaded
-  // after chunk c - 2; Toeplitz window in LDS buffer c & 1, its units loaded
-  // with the same set and stored one chunk later (after chunk c - 1), so that
-  // every load has a chunk of work to complete behind.
+  // after chunk c - 2; Toeplitz windows in LDS buffer c & 1, their units
+  // loaded with the same set and stored one chunk later (after chunk c - 1),
+  // so that every load has a chunk of work to complete behind.
   if (nchunks > 0) {
-    gload(preA, spA, s_begin);
+    gload(preA, spA, 0);
     sstore(spA, 0);
-    if (nchunks > 1) gload(preB, spB, s_begin + KC);
+    if (nchunks > 1) gload(preB, spB, 1);
   }
   __syncthreads();
   u32x4 tvA[2][3], tvB[2][3];
   for (int ch = 0; ch < nchunks; ch += 2) {
     compute(ch, 0, preA, tvA, tvB);
-    if (ch + 2 < nchunks) gload(preA, spA, s_begin + (ch + 2) * KC);
+    if (ch + 2 < nchunks) gload(preA, spA, ch + 2);
     if (ch + 1 < nchunks) sstore(spB, 1);
     __syncthreads();
     if (ch + 1 < nchunks) {
       compute(ch + 1, 1, preB, tvB, tvA);
-      if (ch + 3 < nchunks) gload(preB, spB, s_begin + (ch + 3) * KC);
+      if (ch + 3 < nchunks) gload(preB, spB, ch + 3);
       if (ch + 2 < nchunks) sstore(spA, 0);
       __syncthreads();
     }
   }
 #pragma unroll
-  for (int q = 0; q < NQ; ++q)
+  for (int p = 0; p < NP; ++p) {
+    if (p >= jnp) break;
+    float* dp = dst + p * ostride;
 #pragma unroll
-    for (int db = 0; db < SX_NDB; ++db)
+    for (int q = 0; q < NQ; ++q)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int rr = 4 * (lane >> 4) + j, cc = lane & 15;
-        int dd, ii;
-        if (MODE == 0) {
-          dd = d0 + dw + db * 16 + rr;
-          ii = i0 + q * 16 + cc;
-        } else {
-          ii = i0 + q * 16 + rr;
-          dd = d0 + dw + db * 16 + cc;
+      for (int db = 0; db < SX_NDB; ++db)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int rr = 4 * (lane >> 4) + j, cc = lane & 15;
+          int dd, ii;
+          if (MODE == 0) {
+            dd = d0 + dw + db * 16 + rr;
+            ii = i0 + q * 16 + cc;
+          } else {
+            ii = i0 + q * 16 + rr;
+            dd = d0 + dw + db * 16 + cc;
+          }
+          if (dd < ndense && ii < nout) {
+            if (MODE == 0) dp[(int64_t)dd * nout + ii] = acc[p][q][db][j];
+            else dp[(int64_t)ii * inner + dd] = acc[p][q][db][j];
+          }
         }
-        if (dd < ndense && ii < nout) {
-          if (MODE == 0) dst[(int64_t)dd * nout + ii] = acc[q][db][j];
-          else dst[(int64_t)ii * inner + dd] = acc[q][db][j];
-        }
-      }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1193,11 +1253,12 @@ int flpr_conv_tiles(int32_t mode, int32_t fp64, int32_t rows_or_outer, int32_t m
 int flpr_conv(int32_t mode, int32_t fp64, const void* in, void* out, const void* tap,
               int32_t nmax, const int64_t* jobs, const int32_t* bands, int32_t nj,
               int32_t max_rows_or_outer, int32_t K, int32_t nout, int32_t inner, int32_t base,
-              const int32_t* blocks, int32_t nblocks, void* stream) {
+              int32_t max_powers, const int32_t* blocks, int32_t nblocks, void* stream) {
   FLPR_CLEAR_ERROR();
   if (!in || !out || !tap || !jobs || !bands || nj < 1 || nj > 65535 || max_rows_or_outer < 1 || K < 1 ||
       nout < 1 || (mode != 0 && mode != 1) || (mode == 1 && inner < 1) ||
-      (blocks && nblocks < 1) || nmax < 1)
+      (blocks && nblocks < 1) || nmax < 1 || max_powers < 1 ||
+      max_powers > (fp64 ? 1 : SX_MAX_POWERS))
     return -1;
   const int ld = fp64 ? LD : SX_LD;
   const int ntd = (mode == 0) ? (max_rows_or_outer + ld - 1) / ld : (inner + ld - 1) / ld;
@@ -1216,14 +1277,20 @@ int flpr_conv(int32_t mode, int32_t fp64, const void* in, void* out, const void*
                          (double*)out, (const double*)tap, jobs, bands, K, nout, inner, base, ntd, ntt,
                          blocks);
   } else {
-    if (mode == 0)
-      hipLaunchKernelGGL((k_conv_split<0>), grd, dim3(CT), 0, st, (const float*)in, (float*)out,
-                         (const uint16_t*)tap, nmax, jobs, bands, K, nout, inner, base, ntd, ntt,
-                         blocks);
-    else
-      hipLaunchKernelGGL((k_conv_split<1>), grd, dim3(CT), 0, st, (const float*)in, (float*)out,
-                         (const uint16_t*)tap, nmax, jobs, bands, K, nout, inner, base, ntd, ntt,
-                         blocks);
+#define FLPR_SPLIT_LAUNCH(M, P)                                                                 \
+  hipLaunchKernelGGL((k_conv_split<M, P>), grd, dim3(CT), 0, st, (const float*)in, (float*)out, \
+                     (const uint16_t*)tap, nmax, jobs, bands, K, nout, inner, base, ntd, ntt,   \
+                     blocks)
+    if (mode == 0) {
+      if (max_powers == 1) FLPR_SPLIT_LAUNCH(0, 1);
+      else if (max_powers == 2) FLPR_SPLIT_LAUNCH(0, 2);
+      else FLPR_SPLIT_LAUNCH(0, 3);
+    } else {
+      if (max_powers == 1) FLPR_SPLIT_LAUNCH(1, 1);
+      else if (max_powers == 2) FLPR_SPLIT_LAUNCH(1, 2);
+      else FLPR_SPLIT_LAUNCH(1, 3);
+    }
+#undef FLPR_SPLIT_LAUNCH
   }
   FLPR_CHECK_LAUNCH();
   return 0;

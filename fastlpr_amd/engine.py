@@ -215,6 +215,7 @@ class Pass:
     in_stage: int
     out_stage: int
     jobs_host: np.ndarray = None
+    max_np: int = 1      # most tap rows (powers) of one job: flpr_conv's max_powers
 
 
 def spread_work(n: int, d: int, msp: int, C: int, cells: int) -> Tuple[float, float]:
@@ -243,8 +244,34 @@ def conv_flops(ps: "Pass", bands: np.ndarray) -> float:
             t1 = np.minimum(hi, i + ps.base)
             per = float(np.sum(np.maximum(t1 - t0 + 1, 0)))
             cache[band] = per
-        total += 2.0 * per * ro * (ps.inner if ps.mode == 1 else 1)
+        total += 2.0 * per * ro * (ps.inner if ps.mode == 1 else 1) * max(1, int(rec[5]))
     return total
+
+
+CONV_NP = 3   # tap rows (powers) one job of the float32 form applies to its input
+
+
+def power_runs(entries, limit: int):
+    """Jobs of one input slab: entries (a, out_off, ro) in increasing power a,
+    cut into runs of consecutive powers with the same ro and a constant,
+    positive out_off step, at most limit long (flpr_conv's multi-power jobs:
+    the input is loaded and split once for every power of a run).  Returns
+    [(a0, out_off0, ro, np, out_stride)]."""
+    runs = []
+    i = 0
+    while i < len(entries):
+        a0, o0, r0 = entries[i]
+        n, stride = 1, 0
+        while n < limit and i + n < len(entries):
+            a1, o1, r1 = entries[i + n]
+            step = o1 - entries[i + n - 1][1]
+            if a1 != a0 + n or r1 != r0 or step <= 0 or (n > 1 and step != stride):
+                break
+            stride = step
+            n += 1
+        runs.append((a0, o0, r0, n, stride))
+        i += n
+    return runs
 
 
 class Plan:
@@ -324,27 +351,29 @@ class Plan:
             pad_before = (L - N) // 2 + 1
             base = (qout0 - pad_before) + self.E[axis]
             jobs = []
-            max_ro = 1
+            max_ro, max_np = 1, 1
             in_per_h = self.stage_per_h[k]
             out_per_h = self.stage_per_h[k - 1]
+            limit = 1 if self.fp64 else CONV_NP
             for hl in range(h1 - h0):
                 hg = h0 + hl
+                # stage d (spread grid) has no bandwidth axis; stage 0 is
+                # indexed by the global bandwidth; other stages by chunk-local h
+                in_h = 0 if k == d else hl
+                out_h = hg if k - 1 == 0 else hl
                 for beta, (ioff, _icb) in self.stage_slabs[k].items():
+                    ents = []
                     for a in range(2 * self.order - sum(beta) + 1):
-                        nb = (a,) + beta
-                        ooff, ocb = self.stage_slabs[k - 1][nb]
-                        ro = ocb * prefix
-                        max_ro = max(max_ro, ro)
-                        # stage d (spread grid) has no bandwidth axis; stage 0 is
-                        # indexed by the global bandwidth; other stages by chunk-local h
-                        in_h = 0 if k == d else hl
-                        out_h = hg if k - 1 == 0 else hl
-                        tap_off = ((hg * d + axis) * self.npow + a) * self.nmax
-                        jobs.append([in_h * in_per_h + ioff, out_h * out_per_h + ooff, tap_off,
-                                     hg * d + axis, ro, 0, 0, 0])
+                        ooff, ocb = self.stage_slabs[k - 1][(a,) + beta]
+                        ents.append((a, out_h * out_per_h + ooff, ocb * prefix))
+                    for (a0, ooff, ro, npw, ostride) in power_runs(ents, limit):
+                        max_ro, max_np = max(max_ro, ro), max(max_np, npw)
+                        tap_off = ((hg * d + axis) * self.npow + a0) * self.nmax
+                        jobs.append([in_h * in_per_h + ioff, ooff, tap_off, hg * d + axis, ro,
+                                     npw, ostride, 0])
             jh = np.asarray(jobs, dtype=np.int64)
             jt = torch.tensor(jh, device=self.device)
-            out.append(Pass(axis, mode, K, nout, inner, base, max_ro, jt, k, k - 1, jh))
+            out.append(Pass(axis, mode, K, nout, inner, base, max_ro, jt, k, k - 1, jh, max_np))
         return out
 
     def y_pass_tables(self, ny: int):
@@ -383,22 +412,29 @@ class Plan:
                 base = (-(-(L - N) // 2) - ((L - N) // 2 + 1)) + self.E[axis]
                 ro = ny * prefix
                 jobs = []
+                max_np = 1
+                limit = 1 if self.fp64 else CONV_NP
                 for hl in range(h1 - h0):
                     hg = h0 + hl
                     for beta, ioff in yslabs[k].items():
+                        in_off = 0 if k == d else hl * yper[k] + ioff
+                        ents = []
                         for a in range(self.order - sum(beta) + 1):
                             nb = (a,) + beta
-                            in_off = 0 if k == d else hl * yper[k] + ioff
                             if k - 1 == 0:
                                 out_off = hg * self.stage_per_h[0] + self.stage_slabs[0][nb][0] \
                                     + nodes
                             else:
                                 out_off = hl * yper[k - 1] + yslabs[k - 1][nb]
-                            tap_off = ((hg * d + axis) * self.npow + a) * self.nmax
-                            jobs.append([in_off, out_off, tap_off, hg * d + axis, ro, 0, 0, 0])
+                            ents.append((a, out_off, ro))
+                        for (a0, out_off, _ro, npw, ostride) in power_runs(ents, limit):
+                            max_np = max(max_np, npw)
+                            tap_off = ((hg * d + axis) * self.npow + a0) * self.nmax
+                            jobs.append([in_off, out_off, tap_off, hg * d + axis, ro, npw,
+                                         ostride, 0])
                 jh = np.asarray(jobs, dtype=np.int64)
                 jt = torch.tensor(jh, device=self.device)
-                out.append(Pass(axis, mode, K, nout, inner, base, ro, jt, k, k - 1, jh))
+                out.append(Pass(axis, mode, K, nout, inner, base, ro, jt, k, k - 1, jh, max_np))
             tables.append(out)
         memo[ny] = (yper, tables)
         return memo[ny]
@@ -412,7 +448,8 @@ XCDS = 8
 CONV_LT = 64                       # flpr_conv's Toeplitz tile extent
 CONV_KC = {False: 32, True: 16}    # its K chunk, float32 / float64
 CONV_LD = {False: 128, True: 64}   # its dense tile extent, float32 / float64
-SCHED_MIN_TILES = 4096             # smaller launches keep the plain (tiles, jobs) grid
+SCHED_MIN_TILES = 4096             # launches with fewer (tile, power) pairs keep the
+                                   # plain (tiles, jobs) grid
 _SCHED: Dict[tuple, tuple] = {}
 
 
@@ -448,7 +485,7 @@ def conv_schedule(ps: Pass, fp64: bool, bands: np.ndarray):
         tiles[ro == r] = int(cnt.value)
     ntd = int(stride.value)
     total_tiles = int(tiles.sum())
-    if total_tiles < SCHED_MIN_TILES:
+    if int((tiles * np.maximum(jh[:, 5], 1)).sum()) < SCHED_MIN_TILES:
         _SCHED[key] = (ps, None, 0)
         return None, 0
     # work of output tile tt of job j: K chunks between the band's first and
@@ -471,7 +508,7 @@ def conv_schedule(ps: Pass, fp64: bool, bands: np.ndarray):
         o, rem = np.divmod(k, ntt * ntd)
         tt, td = np.divmod(rem, ntd)
         loc = td + ntd * (tt + ntt * o)
-    wk = tw[jb, tt].astype(np.float64)
+    wk = tw[jb, tt].astype(np.float64) * np.maximum(jh[jb, 5], 1)
     assert len(loc) == total_tiles
     if ps.mode == 0:
         o = np.argsort(-wk, kind="stable")
@@ -542,8 +579,8 @@ def launch_conv(ps: Pass, gf: bool, src: int, dst: int, tap_ptr: int, nmax: int,
                 (lambda ps=ps, bg=bands_g: conv_flops(ps, bg)),
                 call, "flpr_conv", ps.mode, int(gf), src, dst, tap_ptr, nmax, ptr(ps.jobs),
                 band_ptr,
-                int(ps.jobs.shape[0]), ps.max_ro, ps.K, ps.nout, ps.inner, ps.base, ptr(table),
-                nblocks, s)
+                int(ps.jobs.shape[0]), ps.max_ro, ps.K, ps.nout, ps.inner, ps.base, ps.max_np,
+                ptr(table), nblocks, s)
 
 
 def get_plan(d, order, N, L, msp, C, dh, fp64, device) -> Plan:

@@ -222,22 +222,26 @@ int flpr_conv_strips(const float* tap32, const int32_t* bands, int32_t ntap, int
  *   mode 0 (ROW, contiguous axis):  out[r][i] = sum_s in[r][s] * tap[i + base - s]
  *   mode 1 (COL, strided axis):     out[o][i][c] = sum_s tap[i + base - s] * in[o][s][c]
  * Job j (device array of nj records of 8 int64: in_off, out_off, tap_off,
- * band, rows_or_outer, 0, 0, 0) reads from in + in_off and writes
- * out + out_off; taps outside [bands[2*band], bands[2*band+1]] (device int32
- * array) are treated as zero.  K = input axis length, nout = output axis
+ * band, rows_or_outer, powers, out_stride, 0) reads from in + in_off and
+ * writes out + out_off; taps outside [bands[2*band], bands[2*band+1]] (device
+ * int32 array) are treated as zero.  K = input axis length, nout = output axis
  * length, inner = contiguous extent for mode 1, rows_or_outer = rows (mode 0)
  * or outer batch count (mode 1) of the job, max_rows_or_outer its maximum.
  * fp64 != 0: float64 MFMA, in/out/tap are double* (tap in the flpr_taps
- * layout, row length nmax).  fp64 == 0: float32 in/out, products of the
- * three-term bfloat16 splits on bf16 MFMA (float32 accuracy), tap = the
- * flpr_conv_strips table of the float32 taps (row tap_off / nmax).
+ * layout, row length nmax); one tap row per job (powers 0 or 1, max_powers
+ * 1).  fp64 == 0: float32 in/out, products of the three-term bfloat16 splits
+ * on bf16 MFMA (float32 accuracy), tap = the flpr_conv_strips table of the
+ * float32 taps: a job applies the powers (1 to max_powers <= 3; 0 counts as
+ * 1) consecutive rows tap_off / nmax + p of that table to the same input and
+ * writes row p's output at out + out_off + p * out_stride, loading and
+ * splitting the input once for all of them.
  * Tiles are read whole: in must stay readable for 256*K + 64*inner + 512
  * elements past the end of every job's input (those values are discarded or
  * zeroed, never used). */
 int flpr_conv(int32_t mode, int32_t fp64, const void* in, void* out, const void* tap,
               int32_t nmax, const int64_t* jobs, const int32_t* bands, int32_t nj,
               int32_t max_rows_or_outer, int32_t K, int32_t nout, int32_t inner, int32_t base,
-              const int32_t* blocks, int32_t nblocks, void* stream);
+              int32_t max_powers, const int32_t* blocks, int32_t nblocks, void* stream);
 
 /* Tiles of one flpr_conv job (rows_or_outer of the job, max_rows_or_outer of
  * the launch) and the stride of the tile numbering.  With a block table

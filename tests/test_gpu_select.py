@@ -125,5 +125,9 @@ def test_conv_entry_points_reject_bad_arguments():
     assert lib.flpr_conv_strips(fake, fake, 5, 3, 256, fake, None) == -1
     assert lib.flpr_conv_strips(fake, fake, 6, 3, 0, fake, None) == -1
     # nmax must be positive (the float32 form addresses its strip rows by it)
-    assert lib.flpr_conv(0, 0, fake, fake, fake, 0, fake, fake, 1, 1, 267, 256, 1, 0,
+    assert lib.flpr_conv(0, 0, fake, fake, fake, 0, fake, fake, 1, 1, 267, 256, 1, 0, 1,
                          None, 0, None) == -1
+    # max_powers: 1..3 for the float32 form, 1 for the float64 form
+    for fp64, mp in ((0, 0), (0, 4), (1, 2), (1, 0)):
+        assert lib.flpr_conv(0, fp64, fake, fake, fake, 256, fake, fake, 1, 1, 267, 256, 1, 0,
+                             mp, None, 0, None) == -1

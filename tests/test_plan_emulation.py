@@ -62,7 +62,7 @@ def _fine_grid(geo, xmod, vals, acc):
     return out.real
 
 
-def _emulate(x, y, h, order, N, acc=None):
+def _emulate(x, y, h, order, N, acc=None, fp64_plan=True):
     n, d = x.shape
     acc = R.accuracy_for(n) if acc is None else acc
     sd = x.std(0, ddof=1)
@@ -76,7 +76,7 @@ def _emulate(x, y, h, order, N, acc=None):
     h, _, _ = E.filter_bandwidths(geo, h, n)
     dh = h.shape[0]
     C = 2
-    plan = E.Plan(d, order, geo.N, geo.L, geo.margin, C, dh, True, "cpu")
+    plan = E.Plan(d, order, geo.N, geo.L, geo.margin, C, dh, fp64_plan, "cpu")
     xmod = np.zeros((n, d))
     cell = np.zeros((n, d), int)
     for k in range(d):
@@ -121,7 +121,7 @@ def _emulate(x, y, h, order, N, acc=None):
     bands = E.tap_bands(geo, h, True).reshape(-1)
     stage0 = np.zeros(dh * plan.stage_per_h[0])
     _run(plan.pass_tables, [plan.chunk * plan.stage_per_h[k] for k in range(d + 1)],
-         grid.reshape(-1), stage0, tap, bands, d)
+         grid.reshape(-1), stage0, tap, bands, d, plan.nmax)
     return plan, stage0, h, acc, dict(geo=geo, tap=tap, bands=bands, xmod=xmod, cell=cell)
 
 
@@ -145,30 +145,40 @@ def _periodize(grid, g, d):
     return out
 
 
-def _run(tables, inter_sizes, grid, stage0, tap, bands, d):
-    """flpr_conv semantics over a plan's pass tables; every job's input and
-    output range is checked against its buffer."""
+def _run(tables, inter_sizes, grid, stage0, tap, bands, d, nmax):
+    """flpr_conv semantics over a plan's pass tables (a job applies its
+    `powers` consecutive tap rows to the same input, row p's output at
+    out_off + p * out_stride); every output range is checked against its
+    buffer and written at most once per pass."""
     for ci in range(len(tables)):
         inter = {k: np.zeros(inter_sizes[k]) for k in range(1, d)}
         for ps in tables[ci]:
             src = grid if ps.in_stage == d else inter[ps.in_stage]
             dst = stage0 if ps.out_stage == 0 else inter[ps.out_stage]
+            written = np.zeros(len(dst), bool)
+            assert ps.max_np == max(max(1, int(r[5])) for r in ps.jobs_host)
             for rec in ps.jobs_host:
-                ioff, ooff, toff, band, ro = (int(v) for v in rec[:5])
+                ioff, ooff, toff, band, ro, npw, ostride = (int(v) for v in rec[:7])
                 lo, hi = bands[2 * band], bands[2 * band + 1]
-                t = np.arange(ps.nout)[:, None] + ps.base - np.arange(ps.K)[None, :]
-                A = np.where((t >= lo) & (t <= hi), tap[np.clip(toff + t, 0, len(tap) - 1)], 0.0)
                 span_in = ro * ps.K * (ps.inner if ps.mode == 1 else 1)
                 span_out = ro * ps.nout * (ps.inner if ps.mode == 1 else 1)
                 assert 0 <= ioff and ioff + span_in <= len(src)
-                assert 0 <= ooff and ooff + span_out <= len(dst)
-                if ps.mode == 0:
-                    In = src[ioff:ioff + ro * ps.K].reshape(ro, ps.K)
-                    dst[ooff:ooff + ro * ps.nout] = (In @ A.T).reshape(-1)
-                else:
-                    In = src[ioff:ioff + ro * ps.K * ps.inner].reshape(ro, ps.K, ps.inner)
-                    dst[ooff:ooff + ro * ps.nout * ps.inner] = \
-                        np.einsum("is,osc->oic", A, In).reshape(-1)
+                for p in range(max(npw, 1)):
+                    t = np.arange(ps.nout)[:, None] + ps.base - np.arange(ps.K)[None, :]
+                    tp = toff + p * nmax
+                    A = np.where((t >= lo) & (t <= hi), tap[np.clip(tp + t, 0, len(tap) - 1)],
+                                 0.0)
+                    op = ooff + p * ostride
+                    assert 0 <= op and op + span_out <= len(dst)
+                    assert not written[op:op + span_out].any()
+                    written[op:op + span_out] = True
+                    if ps.mode == 0:
+                        In = src[ioff:ioff + ro * ps.K].reshape(ro, ps.K)
+                        dst[op:op + ro * ps.nout] = (In @ A.T).reshape(-1)
+                    else:
+                        In = src[ioff:ioff + ro * ps.K * ps.inner].reshape(ro, ps.K, ps.inner)
+                        dst[op:op + ro * ps.nout * ps.inner] = \
+                            np.einsum("is,osc->oic", A, In).reshape(-1)
 
 
 CASES = [(1, o, [40]) for o in (0, 1, 2)] + [(1, 1, [33])] + \
@@ -179,14 +189,19 @@ CASES = [(1, o, [40]) for o in (0, 1, 2)] + [(1, 1, [33])] + \
 CASES_HI = [(1, 1, [40], 7), (1, 2, [33], 12), (2, 1, [12, 10], 7), (2, 2, [11, 9], 9)]
 
 
+@pytest.mark.parametrize("fp64_plan", [True, False])
 @pytest.mark.parametrize("d,order,N,acc", [c + (None,) for c in CASES] + CASES_HI)
-def test_banded_plan_reproduces_fft_moments(d, order, N, acc):
+def test_banded_plan_reproduces_fft_moments(d, order, N, acc, fp64_plan):
+    """Both job tables (the float64 form's one power per job, the float32
+    form's multi-power jobs), executed in float64, give the FFT moments."""
     rng = np.random.default_rng(3 + d + order)
     n = 300
     x = rng.uniform(0, 1, (n, d))
     y = np.sin(6 * x[:, 0]) + 0.1 * rng.standard_normal(n)
     h = np.array([[0.05] * d, [0.2] * d, [0.6] * d]) if d == 1 else np.array([[0.1, 0.1], [0.3, 0.5]])
-    plan, st0, h, acc, _ = _emulate(x, y, h, order, N, acc)
+    plan, st0, h, acc, _ = _emulate(x, y, h, order, N, acc, fp64_plan)
+    if not fp64_plan and order > 0:
+        assert any(ps.max_np > 1 for ps in plan.pass_tables[0])
     P = R.Problem(x, h, {"order": order, "accuracy": acc, "N": N}, "f64")
     S = P.design(add_eps=False)
     T = P.weighted(y, True)
@@ -201,9 +216,10 @@ def test_banded_plan_reproduces_fft_moments(d, order, N, acc):
         assert np.max(np.abs(em - ref)) <= 1e-12 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("fp64_plan", [True, False])
 @pytest.mark.parametrize("d,order,N", [(1, 1, [40]), (1, 2, [33]), (2, 2, [12, 10]),
                                        (2, 1, [11, 9])])
-def test_y_only_passes_write_the_new_channel_moments(d, order, N):
+def test_y_only_passes_write_the_new_channel_moments(d, order, N, fp64_plan):
     """Plan.y_pass_tables (a second fit of the same x): the passes convolve
     only the new channel and write its T moments into channel 1 of the first
     fit's stage-0 slabs, leaving every S moment and the ones channel intact."""
@@ -213,7 +229,7 @@ def test_y_only_passes_write_the_new_channel_moments(d, order, N):
     y = np.sin(6 * x[:, 0]) + 0.1 * rng.standard_normal(n)
     y2 = np.cos(4 * x[:, -1]) + 0.2 * rng.standard_normal(n)
     h = np.array([[0.05] * d, [0.2] * d, [0.6] * d]) if d == 1 else np.array([[0.1, 0.1], [0.3, 0.5]])
-    plan, st0, h, acc, aux = _emulate(x, y, h, order, N)
+    plan, st0, h, acc, aux = _emulate(x, y, h, order, N, None, fp64_plan)
     before = st0.copy()
     geo, xmod, cell = aux["geo"], aux["xmod"], aux["cell"]
     g = geo.geom
@@ -232,7 +248,7 @@ def test_y_only_passes_write_the_new_channel_moments(d, order, N):
         grid2[sl] += y2[i] * w
     yper, tables = plan.y_pass_tables(1)
     _run(tables, [plan.chunk * yper[k] for k in range(d + 1)], grid2.reshape(-1), st0,
-         aux["tap"], aux["bands"], d)
+         aux["tap"], aux["bands"], d, plan.nmax)
     P = R.Problem(x, h, {"order": order, "accuracy": acc, "N": N}, "f64")
     T2 = P.weighted(y2, True)
     per_h, nodes, dh = plan.stage_per_h[0], plan.nodes, h.shape[0]
@@ -256,7 +272,13 @@ def test_plan_stage_layout_counts():
     assert sum(1 for _, cb in s0.values() if cb == 12) == 6
     assert p.stage_per_h[0] == (6 * 12 + 9) * 256 * 256
     assert len(p.s_off) == 21 and len(p.t_off) == 6
-    assert [len(t.jobs_host) for t in p.pass_tables[0]] == [5 * 30, 15 * 30]
+    # float32 form: ROW jobs = powers 0-2 of all channels + powers 3-4 of the
+    # ones channel; COL jobs per stage-1 slab b = 0..4: 2, 2, 2, 1, 1
+    assert [len(t.jobs_host) for t in p.pass_tables[0]] == [2 * 30, 8 * 30]
+    assert [t.max_np for t in p.pass_tables[0]] == [3, 3]
+    p64 = E.Plan(2, 2, (256, 256), (512, 512), 4, 12, 30, True, "cpu")
+    assert [len(t.jobs_host) for t in p64.pass_tables[0]] == [5 * 30, 15 * 30]
+    assert [t.max_np for t in p64.pass_tables[0]] == [1, 1]
 
 
 @pytest.mark.parametrize("fp64", [False, True])
