@@ -133,11 +133,11 @@ _SIGS = {
                   _I32, _P, _I32, _P],
     "flpr_max_moment": [_I32, _P, _I64, _I32, C.POINTER(_I64), _I32, _I64, _I32, _P, _P],
     "flpr_solve": [_I32, _P, _I64, _I32, _I64, _I32, _I32, _I32, C.POINTER(_I64), C.POINTER(_I64), _P,
-                   _D, _D, _P, _I32, _I32, _P, _P, _P, _P],
+                   _D, _D, _P, _I32, _I32, _P, _P, _P, _P, _I64, _P],
     "flpr_solve_refined": [_P, _I64, _I32, _I64, _I32, _I32, C.POINTER(_I64), C.POINTER(_I64), _P,
-                           _D, _D, _P, _I32, _I32, _P, _P, _P, _P],
+                           _D, _D, _P, _I32, _I32, _P, _P, _P, _P, _I64, _P],
     "flpr_adjoint": [_GP, _P, _P, _P, _P, _I64, _I32, _P, _P, _P, _P],
-    "flpr_quadform": [_GP, _P, _I32, _I32, _P, _P, _P],
+    "flpr_quadform": [_GP, _P, _I32, _I32, _P, _P, _P, _I64, _P],
     "flpr_interp_eval": [_GP, _P, _I64, _P, _I32, _P, _P, _P, _P],
     "flpr_fit_output": [_GP, _P, _I64, _P, _I32, _P, _P, _I32, _P, _P, _P, _P, _P, _P],
     "flpr_spline_eval": [_GP, _P, _I64, _P, _I32, _P, _P],
@@ -154,9 +154,10 @@ _SIGS = {
     "flpr_batch_max_moment": [_I32, _I32, _P, _I64, _I64, _I32, C.POINTER(_I64), _I32, _I64, _I32,
                               _P, _P],
     "flpr_batch_solve": [_I32, _I32, _P, _I64, _I64, _I32, _I64, _I32, _I32, C.POINTER(_I64),
-                         C.POINTER(_I64), _P, _D, _D, _P, _I32, _I32, _P, _I64, _P, _P, _I64, _P],
+                         C.POINTER(_I64), _P, _D, _D, _P, _I32, _I32, _P, _I64, _P, _P, _I64, _P,
+                         _I64, _P],
     "flpr_batch_adjoint": [_GP, _P, _I32, _I64, _P, _P, _P, _P, _I32, _P, _P, _P, _P],
-    "flpr_batch_quadform": [_GP, _I32, _P, _I64, _I32, _I32, _P, _P, _I64, _P],
+    "flpr_batch_quadform": [_GP, _I32, _P, _I64, _I32, _I32, _P, _P, _I64, _P, _I64, _P],
     "flpr_batch_select": [_P, _I32, _P, _I64, _I32, _I32, _I32, _I32, _P, _P, _P, _I32, _I32, _D,
                           _P, _I64, _P, _P],
     "flpr_batch_fit_output": [_GP, _P, _I32, _I64, _P, _P, _I64, _I32, _P, _P, _I32, _P, _P, _P,

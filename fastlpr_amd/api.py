@@ -374,6 +374,39 @@ def _y_channels(y_arr, dev):
     return torch.as_tensor(np.asarray(y_arr, dtype=np.float64), device=dev).contiguous()
 
 
+def _is_complex(a) -> bool:
+    return bool(a.is_complex()) if isinstance(a, torch.Tensor) else bool(np.iscomplexobj(a))
+
+
+def _queue_inputs(x, y, x_arr, y_arr, n: int, variance: bool, prior):
+    """x and the response channels as float64 on the device, and the x
+    statistics pass queued behind them (not read back).  Returns (dev, xd,
+    yd, yd_in, y_raw_orig, pending): yd is log(y + 1/n) for a variance fit
+    (api.py:390-396), yd_in the response before it, y_raw_orig its first
+    column (variance fits).  pending is ("y", sums) when prior holds the x
+    side of a fit of this same x (then only y's sums of squares and
+    non-finite count are queued, engine.y_stats), else ("x", the queued x
+    statistics pass), and None when y has too many columns for the pass."""
+    dev = x.device if _is_dev(x) else (y.device if _is_dev(y) else _device())
+    if not torch.cuda.is_available():
+        _device()
+    xd = (x_arr.to(dev) if isinstance(x_arr, torch.Tensor)
+          else torch.as_tensor(x_arr, device=dev)).to(torch.float64).contiguous()
+    yd = _y_channels(y_arr, dev)
+    y_raw_orig = yd[:, 0].contiguous() if variance else None
+    yd_in = yd
+    if variance:
+        yd = torch.log(yd + 1.0 / n)  # api.py:390-396
+    xs = prior.get("xshare") if prior is not None else None
+    if yd.shape[1] > 32:
+        pending = None
+    elif xs and xs.get("x") is xd and prior["key"][:3] == (xd.data_ptr(), xd._version, xd.shape):
+        pending = ("y", engine.y_stats(yd))
+    else:
+        pending = ("x", engine.x_stats_launch(xd, yd))
+    return dev, xd, yd, yd_in, y_raw_orig, pending
+
+
 def cv_fastlpr(x, y, h: Optional[Sequence[Sequence[float]]] = None,
                options: Optional[dict] = None) -> RegressionOutput:
     """Local polynomial regression (order 0/1/2) with GCV bandwidth selection
@@ -422,10 +455,21 @@ def cv_fastlpr_pair(x, y, h: Optional[Sequence[Sequence[float]]] = None,
     vopts.update(variance_options or {})
     vopts["y_type_out"] = "variance"
     r2 = (yd - share["yhat"][:, 0]) ** 2
-    finish_sigma, _ = _fit(xd, r2, h, vopts, prior=share, as_numpy=False)
+    finish_sigma, share_v = _fit(xd, r2, h, vopts, prior=share, as_numpy=False)
+    sel = (share["sel_out"], share_v["sel_out"])
     share.clear()
-    mu = finish_mu()
-    sigma = finish_sigma()
+    if sel[0] is not None and sel[1] is not None:
+        # both passes' selection results in one read (one host synchronisation)
+        k = sel[0].numel()
+        host = torch.empty((k + sel[1].numel(),), dtype=torch.float64, pin_memory=True)
+        host[:k].copy_(sel[0], non_blocking=True)
+        host[k:].copy_(sel[1], non_blocking=True)
+        torch.cuda.current_stream(xd.device).synchronize()
+        hv = host.numpy()
+        mu, sigma = finish_mu(hv[:k]), finish_sigma(hv[k:])
+    else:
+        mu = finish_mu()
+        sigma = finish_sigma()
     if as_numpy:
         mu, sigma = _to_numpy(mu), _to_numpy(sigma)
     return mu, sigma
@@ -502,19 +546,18 @@ def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
         raise NotImplementedError("the GPU engine supports 1-3 dimensions")
     if str(opts.get("kernel_type", "gaussian")).lower() not in ("gauss", "gaussian", "normal"):
         raise NotImplementedError("the GPU engine implements the Gaussian kernel")
+    variance = str(opts.get("y_type_out", "mean")) == "variance"
     _check_accuracy(int(opts["accuracy"]), dims)
     # explicit bandwidths are validated before any device work; the default
     # range needs the x statistics
     bw_given = _prepare_bandwidths(h, dims, opts, None) if h is not None else None
-    variance = str(opts.get("y_type_out", "mean")) == "variance"
     prec = str(opts.get("precision", "fp32")).lower()
     if prec not in ("fp32", "fp64", "mixed"):
         raise ValueError("options['precision'] must be 'fp32', 'fp64' or 'mixed'")
     fp64 = prec == "fp64"
     mixed = prec == "mixed"
     N = _grid_shape(opts, n, dims)
-    y_is_complex = bool(y_arr.is_complex()) if isinstance(y_arr, torch.Tensor) \
-        else bool(np.iscomplexobj(y_arr))
+    y_is_complex = _is_complex(y_arr)
     # value channels of the solve, checked before any device work
     if bool(opts.get("calc_dof", True)):
         K_req = int(opts.get("num_dof_sample", 10))
@@ -524,33 +567,24 @@ def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
             if len(rshape) == 2:
                 K_req = int(rshape[1])
         _check_channels(dy * (2 if y_is_complex else 1), max(K_req, 0))
-    dev = x.device if _is_dev(x) else (y.device if _is_dev(y) else _device())
-    if not torch.cuda.is_available():
-        _device()
-
-    xd = (x_arr.to(dev) if isinstance(x_arr, torch.Tensor)
-          else torch.as_tensor(x_arr, device=dev)).to(torch.float64).contiguous()
-    yd = _y_channels(y_arr, dev)
-    y_raw_orig = yd[:, 0].contiguous() if variance else None
+    # every check above runs before any device work
+    dev, xd, yd, yd_in, y_raw_orig, pending = _queue_inputs(x, y, x_arr, y_arr, n, variance,
+                                                            prior)
     ny = yd.shape[1]
     if ny > 32:
         raise NotImplementedError("the GPU engine takes at most 32 real response columns")
-    yd_in = yd
-    if variance:
-        yd = torch.log(yd + 1.0 / n)  # api.py:390-396
-    xs = prior.get("xshare") if prior is not None else None
     bad_y_dev = None
-    if xs and xs.get("x") is xd and prior["key"][:3] == (xd.data_ptr(), xd._version, xd.shape):
+    if pending[0] == "y":
         # the same x as the fit that left its x side in prior: its statistics,
         # and y's sums of squares and non-finite check on the device (read
         # back with the results)
-        s1 = xs["st"]
-        ys = engine.y_stats(yd)
+        s1 = prior["xshare"]["st"]
+        ys = pending[1]
         stats = engine.Stats(s1.xmin, s1.xmax, s1.mean, s1.std, 0, 0, None, ys[2:])
         bad_y_dev = ys[1:2]
     else:
         # one device pass: x statistics, non-finite checks of device inputs, sum y^2
-        stats = engine.x_stats(xd, yd)
+        stats = engine.x_stats_read(pending[1])
         if stats.bad_x:
             _check_finite(x_arr, "Input x", "cv_fastlpr")
         if stats.bad_y and isinstance(y_arr, torch.Tensor):
@@ -687,10 +721,10 @@ def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
     def conv(t):
         return t.cpu().numpy() if as_numpy else t
 
-    def finish():
+    def finish(hv=None):
         r = _finish(fit, sel_out, fac if variance else None, bad_y_dev, yd_in, dh, opts,
                     order, geo, yhat, resid, grid_var, N, y_is_complex, dy, as_numpy,
-                    reuse, conv)
+                    reuse, conv, hv)
         if binfo is not None:
             # the minimum on an inner edge of the bracket may lie outside it:
             # then the full sweep decides
@@ -703,17 +737,20 @@ def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
                                                            "fallback", "hlist")}
         return r
 
+    share["sel_out"] = sel_out
     return finish, share
 
 
 def _finish(fit, sel_out, fac, bad_y_dev, y_in, dh, opts, order, geo, yhat, resid, grid_var, N,
-            y_is_complex, dy, as_numpy, reuse, conv):
+            y_is_complex, dy, as_numpy, reuse, conv, hv=None):
     """The one host read of a fit (selection results, the variance scale, the
     deferred non-finite count of y) and the RegressionOutput; the grid values
-    and S_00 of the selected bandwidth are views of the fit's arrays."""
+    and S_00 of the selected bandwidth are views of the fit's arrays.  hv: the
+    selection results already read back (sel_out's values on the host)."""
     d_norm = None
     if sel_out is not None:
-        hv = sel_out.cpu().numpy()
+        if hv is None:
+            hv = sel_out.cpu().numpy()
         if bad_y_dev is not None and hv[4 * dh + 4] > 0:
             _check_finite(y_in, "Response y", "cv_fastlpr")
         if fac is not None:

@@ -504,9 +504,11 @@ class BatchPlan:
         if c.variance:
             y_all = torch.log(y_all + c.bufs["inv_n"][:, None]).contiguous()
         # zeroed scratch: key counts, spread sums, max keys, reductions, adjoint sums
+        # and the workspace of the fixed-order dot / quadratic-form sums
+        nsum = engine.sum_part_len(P * dh * nval, -(-max(nodes, rcells) // 256))
         zspec = [(2 * nkeys, i32), (P * C, f64), (P, torch.int64), (P * red_stride, f64),
-                 (P * nval * nodes, f64), (P * nw * rcells, f64)]
-        cnt, sums, key_max, red, q, gram = _zeroed(dev, zspec, s)
+                 (P * nval * nodes, f64), (P * nw * rcells, f64), (nsum, f64)]
+        cnt, sums, key_max, red, q, gram, sump = _zeroed(dev, zspec, s)
         # y statistics: non-finite count and sum of squares per problem
         ystat = torch.empty((P * (2 + ny),), dtype=f64, device=dev)
         ypart = torch.empty((P * 512 * (2 + ny),), dtype=f64, device=dev)
@@ -595,9 +597,9 @@ class BatchPlan:
                      c.order, pl.nt, i64_array(pl.s_off), i64_array(pl.t_off), ptr(key_max),
                      lam_scale, eps_add, ptr(shift), nval, ny,
                      m_y.data_ptr() + a * ny * nodes * 8, m_stride, ptr(q),
-                     red.data_ptr() + a * nval * 8, red_stride, s)
+                     red.data_ptr() + a * nval * 8, red_stride, ptr(sump), nsum, s)
             call("flpr_batch_quadform", gp, P, ptr(m_y), m_stride, ny, dh, ptr(gram),
-                 red.data_ptr() + dh * nval * 8, red_stride, s)
+                 red.data_ptr() + dh * nval * 8, red_stride, ptr(sump), nsum, s)
         else:
             # every channel's grid solutions, then the sums at the points
             # (spline or multilinear), as cv_fastlpr's cubic route
@@ -607,7 +609,8 @@ class BatchPlan:
                 call("flpr_batch_solve", int(gf), P, ptr(stage0), st0, per_h0, b - a, nodes,
                      c.order, pl.nt, i64_array(pl.s_off), i64_array(pl.t_off), ptr(key_max),
                      lam_scale, eps_add, ptr(shift), nval, nval,
-                     m_all.data_ptr() + a * nval * nodes * 8, a_stride, None, None, 0, s)
+                     m_all.data_ptr() + a * nval * nodes * 8, a_stride, None, None, 0, None, 0,
+                     s)
             coef = torch.empty_like(m_all)
             work = torch.empty_like(m_all) if d > 1 else coef
             call("flpr_batch_spline_coef", gp, P, ptr(c.bufs["inv"]), c.tabs["inv_stride"],

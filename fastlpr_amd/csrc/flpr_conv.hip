@@ -806,6 +806,7 @@ struct SolveTab {
 };
 
 constexpr int SOLVE_THREADS = 256;
+static_assert(SOLVE_THREADS == 256, "flpr_group_sum takes blocks of 256 threads");
 constexpr int MAXV = 32;  // value channels reduced per solve launch
 // value channels whose moments are loaded together (a divisor of MAXV)
 template <typename T>
@@ -846,7 +847,8 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(
     const T* __restrict__ mom, int64_t per_h, int64_t nodes, SolveTab tab,
     const uint64_t* __restrict__ key, double lam_scale, double eps_add,
     const double* __restrict__ shift, int nval, int nout, double* __restrict__ m_out,
-    const double* __restrict__ q, double* __restrict__ dot, FlprBz bz) {
+    const double* __restrict__ q, double* __restrict__ dot, double* __restrict__ sum_part,
+    FlprBz bz) {
   __shared__ double red[SOLVE_THREADS / 64][MAXV];
   {
     const int64_t z = blockIdx.z;
@@ -1131,7 +1133,7 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(
       }
     }
   }
-  if (q) {
+  if (q && nval > 0) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int cc = 0; cc < MAXV; ++cc) {
@@ -1141,11 +1143,12 @@ __global__ __launch_bounds__(SOLVE_THREADS) void k_solve(
       if (lane == 0) red[wid][cc] = v;
     }
     __syncthreads();
-    if (threadIdx.x < (unsigned)nval) {
-      double s = 0.0;
+    double s = 0.0;
+    if (threadIdx.x < (unsigned)nval)
       for (int w = 0; w < SOLVE_THREADS / 64; ++w) s += red[w][threadIdx.x];
-      atomicAdd(&dot[(int64_t)h * nval + threadIdx.x], s);
-    }
+    // the node blocks of (problem, bandwidth) add their dots in block order
+    flpr_group_sum(s, nval, blockIdx.x, gridDim.x, (int64_t)blockIdx.z * gridDim.y + h,
+                   (int64_t)gridDim.z * gridDim.y, sum_part, dot + (int64_t)h * nval);
   }
 }
 
@@ -1153,29 +1156,30 @@ template <typename T, int NT>
 void launch_solve(hipStream_t st, dim3 grd, bool piv, const T* mom, int64_t per_h,
                   int64_t nodes, const SolveTab& tab, const uint64_t* key, double lam_scale,
                   double eps_add, const double* shift, int nval, int nout, double* m_out,
-                  const double* q, double* dot, const FlprBz& bz, bool ref = false) {
+                  const double* q, double* dot, double* sum_part, const FlprBz& bz,
+                  bool ref = false) {
   const bool buf = sizeof(T) == 4 && per_h * (int64_t)sizeof(T) <= (int64_t)INT32_MAX;
   if (ref && sizeof(T) == 4 && NT > 1) {
     if (buf)
       hipLaunchKernelGGL((k_solve<T, NT, false, sizeof(T) == 4, true>), grd, dim3(SOLVE_THREADS),
                          0, st, mom, per_h, nodes, tab, key, lam_scale, eps_add, shift, nval,
-                         nout, m_out, q, dot, bz);
+                         nout, m_out, q, dot, sum_part, bz);
     else
       hipLaunchKernelGGL((k_solve<T, NT, false, false, true>), grd, dim3(SOLVE_THREADS), 0, st,
                          mom, per_h, nodes, tab, key, lam_scale, eps_add, shift, nval, nout,
-                         m_out, q, dot, bz);
+                         m_out, q, dot, sum_part, bz);
     return;
   }
   if (piv && NT > 1)
     hipLaunchKernelGGL((k_solve<T, NT, true, false>), grd, dim3(SOLVE_THREADS), 0, st, mom,
-                       per_h, nodes, tab, key, lam_scale, eps_add, shift, nval, nout, m_out, q, dot, bz);
+                       per_h, nodes, tab, key, lam_scale, eps_add, shift, nval, nout, m_out, q, dot, sum_part, bz);
   else if (buf)
     hipLaunchKernelGGL((k_solve<T, NT, false, sizeof(T) == 4>), grd, dim3(SOLVE_THREADS), 0, st,
                        mom, per_h, nodes, tab, key, lam_scale, eps_add, shift, nval, nout, m_out,
-                       q, dot, bz);
+                       q, dot, sum_part, bz);
   else
     hipLaunchKernelGGL((k_solve<T, NT, false, false>), grd, dim3(SOLVE_THREADS), 0, st, mom,
-                       per_h, nodes, tab, key, lam_scale, eps_add, shift, nval, nout, m_out, q, dot, bz);
+                       per_h, nodes, tab, key, lam_scale, eps_add, shift, nval, nout, m_out, q, dot, sum_part, bz);
 }
 
 template <typename T>
@@ -1183,11 +1187,11 @@ int dispatch_solve(int nt, hipStream_t st, dim3 grd, bool piv, const T* mom, int
                    int64_t nodes,
                    const SolveTab& tab, const uint64_t* key, double lam_scale, double eps_add,
                    const double* shift, int nval, int nout, double* m_out, const double* q,
-                   double* dot, const FlprBz& bz, bool ref = false) {
+                   double* dot, double* sum_part, const FlprBz& bz, bool ref = false) {
 #define FLPR_SOLVE_CASE(N)                                                                   \
   case N:                                                                                    \
     launch_solve<T, N>(st, grd, piv, mom, per_h, nodes, tab, key, lam_scale, eps_add, shift, \
-                       nval, nout, m_out, q, dot, bz, ref);                                  \
+                       nval, nout, m_out, q, dot, sum_part, bz, ref);                        \
     return 0;
   switch (nt) {
     FLPR_SOLVE_CASE(1)
@@ -1200,6 +1204,18 @@ int dispatch_solve(int nt, hipStream_t st, dim3 grd, bool piv, const T* mom, int
       return -1;
   }
 #undef FLPR_SOLVE_CASE
+}
+
+// Node blocks per bandwidth of a solve launch: enough to fill the device,
+// and, with probe dots (nval > 0), no more than the dot workspace holds
+// (groups = P * dh of nval sums each); 0 when the workspace is missing or
+// too small for one block per group.
+int solve_blocks(int64_t nodes, int dh, int P, int nval, const double* sum_part,
+                 int64_t sum_part_len) {
+  const int bx = flpr_grid_for(nodes, SOLVE_THREADS, 1024 / (dh > 64 ? 64 : dh) + 1);
+  if (nval < 1) return bx;
+  if (!sum_part) return 0;
+  return flpr_sum_blocks((int64_t)P * dh, nval, bx, sum_part_len);
 }
 
 }  // namespace
@@ -1323,7 +1339,7 @@ int flpr_solve(int32_t fp64, const void* mom, int64_t per_h, int32_t dh, int64_t
                int32_t order, int32_t nt, int32_t pivot, const int64_t* s_off,
                const int64_t* t_off, const uint64_t* key, double lam_scale, double eps_add,
                const double* shift, int32_t nval, int32_t nout, double* m_out, const double* q,
-               double* dot, void* stream) {
+               double* dot, double* sum_part, int64_t sum_part_len, void* stream) {
   FLPR_CLEAR_ERROR();
   if (!mom || !s_off || !t_off || !key || dh < 1 || dh > 65535 || nodes < 1 || nt < 1 ||
       nt > FLPR_MAX_TERMS || nval < 0 || nval > MAXV || nout < 0 || nout > nval)
@@ -1335,16 +1351,19 @@ int flpr_solve(int32_t fp64, const void* mom, int64_t per_h, int32_t dh, int64_t
   const int ns = nt * (nt + 1) / 2;
   for (int i = 0; i < ns; ++i) tab.s_off[i] = s_off[i];
   for (int i = 0; i < nt; ++i) tab.t_off[i] = t_off[i];
-  const int bx = flpr_grid_for(nodes, SOLVE_THREADS, 1024 / (dh > 64 ? 64 : dh) + 1);
+  const int bx = solve_blocks(nodes, dh, 1, q ? nval : 0, sum_part, sum_part_len);
+  if (bx < 1) return -3;
   dim3 grd((unsigned)bx, (unsigned)dh);
   hipStream_t st = as_stream(stream);
   int rc;
   if (fp64)
     rc = dispatch_solve<double>(nt, st, grd, pivot != 0, (const double*)mom, per_h, nodes, tab, key,
-                                lam_scale, eps_add, shift, nval, nout, m_out, q, dot, FlprBz{});
+                                lam_scale, eps_add, shift, nval, nout, m_out, q, dot, sum_part,
+                                FlprBz{});
   else
     rc = dispatch_solve<float>(nt, st, grd, pivot != 0, (const float*)mom, per_h, nodes, tab, key,
-                               lam_scale, eps_add, shift, nval, nout, m_out, q, dot, FlprBz{});
+                               lam_scale, eps_add, shift, nval, nout, m_out, q, dot, sum_part,
+                               FlprBz{});
   if (rc) return rc;
   FLPR_CHECK_LAUNCH();
   return 0;
@@ -1354,7 +1373,8 @@ int flpr_solve_refined(const void* mom, int64_t per_h, int32_t dh, int64_t nodes
                        int32_t order, int32_t nt, const int64_t* s_off, const int64_t* t_off,
                        const uint64_t* key, double lam_scale, double eps_add,
                        const double* shift, int32_t nval, int32_t nout, double* m_out,
-                       const double* q, double* dot, void* stream) {
+                       const double* q, double* dot, double* sum_part, int64_t sum_part_len,
+                       void* stream) {
   FLPR_CLEAR_ERROR();
   if (!mom || !s_off || !t_off || !key || dh < 1 || dh > 65535 || nodes < 1 || nt < 1 ||
       nt > FLPR_MAX_TERMS || nval < 0 || nval > MAXV || nout < 0 || nout > nval)
@@ -1366,11 +1386,12 @@ int flpr_solve_refined(const void* mom, int64_t per_h, int32_t dh, int64_t nodes
   const int ns = nt * (nt + 1) / 2;
   for (int i = 0; i < ns; ++i) tab.s_off[i] = s_off[i];
   for (int i = 0; i < nt; ++i) tab.t_off[i] = t_off[i];
-  const int bx = flpr_grid_for(nodes, SOLVE_THREADS, 1024 / (dh > 64 ? 64 : dh) + 1);
+  const int bx = solve_blocks(nodes, dh, 1, q ? nval : 0, sum_part, sum_part_len);
+  if (bx < 1) return -3;
   dim3 grd((unsigned)bx, (unsigned)dh);
   const int rc = dispatch_solve<float>(nt, as_stream(stream), grd, false, (const float*)mom, per_h,
                                        nodes, tab, key, lam_scale, eps_add, shift, nval, nout,
-                                       m_out, q, dot, FlprBz{}, true);
+                                       m_out, q, dot, sum_part, FlprBz{}, true);
   if (rc) return rc;
   FLPR_CHECK_LAUNCH();
   return 0;
@@ -1424,7 +1445,7 @@ int flpr_batch_solve(int32_t fp64, int32_t P, const void* mom, int64_t mom_strid
                      const int64_t* t_off, const uint64_t* key, double lam_scale, double eps_add,
                      const double* shift, int32_t nval, int32_t nout, double* m_out,
                      int64_t m_stride, const double* q, double* dot, int64_t dot_stride,
-                     void* stream) {
+                     double* sum_part, int64_t sum_part_len, void* stream) {
   FLPR_CLEAR_ERROR();
   if (P < 1 || P > 65535 || !mom || !s_off || !t_off || !key || dh < 1 || dh > 65535 ||
       nodes < 1 || nt < 1 || nt > FLPR_MAX_TERMS || nval < 0 || nval > MAXV || nout < 0 ||
@@ -1437,7 +1458,8 @@ int flpr_batch_solve(int32_t fp64, int32_t P, const void* mom, int64_t mom_strid
   const int ns = nt * (nt + 1) / 2;
   for (int i = 0; i < ns; ++i) tab.s_off[i] = s_off[i];
   for (int i = 0; i < nt; ++i) tab.t_off[i] = t_off[i];
-  const int bx = flpr_grid_for(nodes, SOLVE_THREADS, 1024 / (dh > 64 ? 64 : dh) + 1);
+  const int bx = solve_blocks(nodes, dh, P, q ? nval : 0, sum_part, sum_part_len);
+  if (bx < 1) return -3;
   dim3 grd((unsigned)bx, (unsigned)dh, (unsigned)P);
   FlprBz bz{};
   bz.s[0] = mom_stride;
@@ -1449,10 +1471,10 @@ int flpr_batch_solve(int32_t fp64, int32_t P, const void* mom, int64_t mom_strid
   int rc;
   if (fp64)
     rc = dispatch_solve<double>(nt, st, grd, false, (const double*)mom, per_h, nodes, tab, key,
-                                lam_scale, eps_add, shift, nval, nout, m_out, q, dot, bz);
+                                lam_scale, eps_add, shift, nval, nout, m_out, q, dot, sum_part, bz);
   else
     rc = dispatch_solve<float>(nt, st, grd, false, (const float*)mom, per_h, nodes, tab, key,
-                               lam_scale, eps_add, shift, nval, nout, m_out, q, dot, bz);
+                               lam_scale, eps_add, shift, nval, nout, m_out, q, dot, sum_part, bz);
   if (rc) return rc;
   FLPR_CHECK_LAUNCH();
   return 0;

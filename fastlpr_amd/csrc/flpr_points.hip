@@ -818,34 +818,47 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_wide3(
   }
 }
 
-// Per-channel totals of a grid (channel = blockIdx.y) into sums (zeroed by
-// the caller).
-__global__ __launch_bounds__(256) void k_grid_sums(const double* __restrict__ grid, int64_t cells,
-                                                   double* __restrict__ sums) {
-  __shared__ double red[4];
-  const int c = blockIdx.y;
-  double s = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cells;
-       i += (int64_t)gridDim.x * blockDim.x)
-    s += grid[(int64_t)c * cells + i];
-  s = block_sum(s, red);
-  if (threadIdx.x == 0) atomicAdd(&sums[c], s);
+// Per-channel totals of a grid: one block per channel (blockIdx.x) and
+// problem (blockIdx.y; grid at grid_stride, sums at C), which adds the
+// channel's cells in a fixed order, so the totals (and the centring shifts
+// made from them) are the same in every run: sums[c] += total (sums zeroed
+// by the caller).
+constexpr int CSUM_THREADS = 1024;
+__global__ __launch_bounds__(CSUM_THREADS) void k_grid_sums(const double* __restrict__ grid,
+                                                            int64_t cells, int64_t grid_stride,
+                                                            int C, double* __restrict__ sums) {
+  __shared__ double red[CSUM_THREADS / 64];
+  const int c = blockIdx.x;
+  const double* gp = grid + (int64_t)blockIdx.y * grid_stride + (int64_t)c * cells;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int64_t i = threadIdx.x;
+  for (; i + 3 * CSUM_THREADS < cells; i += 4 * CSUM_THREADS) {
+    s0 += gp[i];
+    s1 += gp[i + CSUM_THREADS];
+    s2 += gp[i + 2 * CSUM_THREADS];
+    s3 += gp[i + 3 * CSUM_THREADS];
+  }
+  for (; i < cells; i += CSUM_THREADS) s0 += gp[i];
+  const double s = block_sum((s0 + s1) + (s2 + s3), red);
+  if (threadIdx.x == 0) sums[(int64_t)blockIdx.y * C + c] += s;
+}
+
+void launch_grid_sums(hipStream_t st, const double* grid, int64_t cells, int64_t grid_stride,
+                      int C, int P, double* sums) {
+  hipLaunchKernelGGL(k_grid_sums, dim3((unsigned)C, (unsigned)P), dim3(CSUM_THREADS), 0, st,
+                     grid, cells, grid_stride, C, sums);
 }
 
 __device__ inline int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
 
 // Sum of the overlapping tile windows for every spread-grid cell (channel =
-// blockIdx.y); also the per-channel grid sums used by the centring step.
-// Batch: s[0] = part, s[1] = grid, s[2] = sums.
+// blockIdx.y).  Batch: s[0] = part, s[1] = grid.
 template <int D, typename A>
 __global__ __launch_bounds__(256) void k_spread_gather(FlprGeom g, const A* __restrict__ part,
                                                        int C, int64_t cells,
-                                                       double* __restrict__ grid,
-                                                       double* __restrict__ sums, FlprBz bz) {
-  __shared__ double red[4];
+                                                       double* __restrict__ grid, FlprBz bz) {
   part += blockIdx.z * bz.s[0];
   grid += blockIdx.z * bz.s[1];
-  sums += blockIdx.z * bz.s[2];
   const int c = blockIdx.y;
   const int msp = g.msp;
   int fp[3] = {1, 1, 1};
@@ -855,7 +868,6 @@ __global__ __launch_bounds__(256) void k_spread_gather(FlprGeom g, const A* __re
     fp[k] = g.tile[k] + 2 * msp;
     fcells *= fp[k];
   }
-  double csum = 0.0;
   for (int64_t lin = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; lin < cells;
        lin += (int64_t)gridDim.x * blockDim.x) {
     int sc[3] = {0, 0, 0}, tlo[3] = {0, 0, 0}, thi[3] = {0, 0, 0};
@@ -890,10 +902,7 @@ __global__ __launch_bounds__(256) void k_spread_gather(FlprGeom g, const A* __re
           }
     }
     grid[(int64_t)c * cells + lin] = v;
-    csum += v;
   }
-  csum = block_sum(csum, red);
-  if (threadIdx.x == 0) atomicAdd(&sums[c], csum);
 }
 
 // Centre every channel c >= 1 on the grid: F_c -= shift_c F_0 with
@@ -1475,6 +1484,10 @@ __global__ __launch_bounds__(256) void k_adjoint_gather(FlprGeom g, const double
 }
 
 constexpr int QF_THREADS = 256;
+static_assert(QF_THREADS == 256, "flpr_group_sum takes blocks of 256 threads");
+// Cell blocks per (problem, h, c): each block's last step waits for its
+// ticket, so a few long blocks per sum cost less than many short ones.
+constexpr int QF_BLOCKS = 16;
 
 // quad[h][c] = sum_cell sum_{a,b} W_ab(cell) m(h,c,node_a) m(h,c,node_b)
 // Batch: s[0] = m, s[1] = gram, s[2] = quad.
@@ -1483,7 +1496,8 @@ __global__ __launch_bounds__(QF_THREADS) void k_quadform(FlprGeom g, const doubl
                                                          int64_t nodes,
                                                          const double* __restrict__ gram,
                                                          int64_t rcells,
-                                                         double* __restrict__ quad, FlprBz bz) {
+                                                         double* __restrict__ quad,
+                                                         double* __restrict__ sum_part, FlprBz bz) {
   __shared__ double red[QF_THREADS / 64];
   m += blockIdx.z * bz.s[0];
   gram += blockIdx.z * bz.s[1];
@@ -1520,7 +1534,9 @@ __global__ __launch_bounds__(QF_THREADS) void k_quadform(FlprGeom g, const doubl
       }
   }
   s = block_sum(s, red);
-  if (threadIdx.x == 0) atomicAdd(&quad[hc], s);
+  // the cell blocks of (problem, h, c) add their sums in block order
+  flpr_group_sum(s, 1, blockIdx.x, gridDim.x, (int64_t)blockIdx.z * gridDim.y + hc,
+                 (int64_t)gridDim.z * gridDim.y, sum_part, quad + hc);
 }
 
 template <int D>
@@ -2170,13 +2186,13 @@ void launch_spread(int msp, dim3 grd, size_t lds, hipStream_t st, const FlprGeom
 
 template <int D>
 void launch_gather(dim3 gg, hipStream_t st, const FlprGeom& g, const double* part, int C,
-                   int64_t cells, double* grid, double* sums, const FlprBz& bz) {
+                   int64_t cells, double* grid, const FlprBz& bz) {
   if (g.spread_f32)
     hipLaunchKernelGGL((k_spread_gather<D, float>), gg, dim3(256), 0, st, g,
-                       reinterpret_cast<const float*>(part), C, cells, grid, sums, bz);
+                       reinterpret_cast<const float*>(part), C, cells, grid, bz);
   else
     hipLaunchKernelGGL((k_spread_gather<D, double>), gg, dim3(256), 0, st, g, part, C, cells,
-                       grid, sums, bz);
+                       grid, bz);
 }
 
 // Channels per block: all of them when the tiles alone fill the device, else
@@ -2403,8 +2419,7 @@ int flpr_spread_wide(const FlprGeom* g, const int32_t* key_start, const double* 
   }
 #undef FLPR_WIDE_CASE
   FLPR_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_grid_sums, dim3(flpr_grid_for(cells, 256, 256), (unsigned)C), dim3(256), 0,
-                     st, grid, cells, sums);
+  launch_grid_sums(st, grid, cells, (int64_t)C * cells, C, 1, sums);
   FLPR_CHECK_LAUNCH();
   return 0;
 }
@@ -2450,10 +2465,12 @@ int flpr_spread(const FlprGeom* g, const int32_t* key_start, const double* wab,
   FLPR_CHECK_LAUNCH();
   dim3 gg(flpr_grid_for(cells, 256, 256), (unsigned)C);
   switch (g->d) {
-    case 1: launch_gather<1>(gg, st, *g, part, C, cells, grid, sums, FlprBz{}); break;
-    case 2: launch_gather<2>(gg, st, *g, part, C, cells, grid, sums, FlprBz{}); break;
-    default: launch_gather<3>(gg, st, *g, part, C, cells, grid, sums, FlprBz{}); break;
+    case 1: launch_gather<1>(gg, st, *g, part, C, cells, grid, FlprBz{}); break;
+    case 2: launch_gather<2>(gg, st, *g, part, C, cells, grid, FlprBz{}); break;
+    default: launch_gather<3>(gg, st, *g, part, C, cells, grid, FlprBz{}); break;
   }
+  FLPR_CHECK_LAUNCH();
+  launch_grid_sums(st, grid, cells, (int64_t)C * cells, C, 1, sums);
   FLPR_CHECK_LAUNCH();
   return 0;
 }
@@ -2581,18 +2598,23 @@ int flpr_adjoint(const FlprGeom* g, const int32_t* key_start, const int32_t* raw
 }
 
 int flpr_quadform(const FlprGeom* g, const double* m, int32_t ncol, int32_t dh,
-                  const double* gram, double* quad, void* stream) {
+                  const double* gram, double* quad, double* sum_part, int64_t sum_part_len,
+                  void* stream) {
   FLPR_CLEAR_ERROR();
   if (!flpr_geom_ok(g) || !m || !gram || !quad || ncol < 1 || dh < 1) return -1;
   const int64_t nodes = flpr_nodes(g);
   int64_t rcells = 1;
   for (int k = 0; k < g->d; ++k) rcells *= g->n_grid[k] - 1;
-  dim3 grd(flpr_grid_for(rcells, QF_THREADS, 64), (unsigned)(dh * ncol));
+  const int nb = sum_part ? flpr_sum_blocks((int64_t)dh * ncol, 1,
+                                            flpr_grid_for(rcells, QF_THREADS, QF_BLOCKS), sum_part_len)
+                          : 0;
+  if (nb < 1) return -3;
+  dim3 grd((unsigned)nb, (unsigned)(dh * ncol));
   hipStream_t st = as_stream(stream);
   switch (g->d) {
-    case 1: hipLaunchKernelGGL(k_quadform<1>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad, FlprBz{}); break;
-    case 2: hipLaunchKernelGGL(k_quadform<2>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad, FlprBz{}); break;
-    default: hipLaunchKernelGGL(k_quadform<3>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad, FlprBz{}); break;
+    case 1: hipLaunchKernelGGL(k_quadform<1>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad, sum_part, FlprBz{}); break;
+    case 2: hipLaunchKernelGGL(k_quadform<2>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad, sum_part, FlprBz{}); break;
+    default: hipLaunchKernelGGL(k_quadform<3>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad, sum_part, FlprBz{}); break;
   }
   FLPR_CHECK_LAUNCH();
   return 0;
@@ -2822,13 +2844,14 @@ int flpr_batch_spread(const FlprGeom* g, const FlprBatchItem* items, int32_t P, 
   FlprBz bg{};
   bg.s[0] = ntiles * C * fcells;
   bg.s[1] = grid_stride;
-  bg.s[2] = C;
   dim3 gg(flpr_grid_for(cells, 256, 256), (unsigned)C, (unsigned)P);
   switch (g->d) {
-    case 1: launch_gather<1>(gg, st, *g, part, C, cells, grid, sums, bg); break;
-    case 2: launch_gather<2>(gg, st, *g, part, C, cells, grid, sums, bg); break;
-    default: launch_gather<3>(gg, st, *g, part, C, cells, grid, sums, bg); break;
+    case 1: launch_gather<1>(gg, st, *g, part, C, cells, grid, bg); break;
+    case 2: launch_gather<2>(gg, st, *g, part, C, cells, grid, bg); break;
+    default: launch_gather<3>(gg, st, *g, part, C, cells, grid, bg); break;
   }
+  FLPR_CHECK_LAUNCH();
+  launch_grid_sums(st, grid, cells, grid_stride, C, P, sums);
   FLPR_CHECK_LAUNCH();
   return 0;
 }
@@ -2903,7 +2926,8 @@ int flpr_batch_adjoint(const FlprGeom* g, const FlprBatchItem* items, int32_t P,
 
 int flpr_batch_quadform(const FlprGeom* g, int32_t P, const double* m, int64_t m_stride,
                         int32_t ncol, int32_t dh, const double* gram, double* quad,
-                        int64_t quad_stride, void* stream) {
+                        int64_t quad_stride, double* sum_part, int64_t sum_part_len,
+                        void* stream) {
   FLPR_CLEAR_ERROR();
   if (!flpr_geom_ok(g) || P < 1 || P > 65535 || !m || !gram || !quad || ncol < 1 || dh < 1)
     return -1;
@@ -2915,12 +2939,16 @@ int flpr_batch_quadform(const FlprGeom* g, int32_t P, const double* m, int64_t m
   bz.s[0] = m_stride;
   bz.s[1] = (int64_t)(nq * (nq + 1) / 2) * rcells;
   bz.s[2] = quad_stride;
-  dim3 grd(flpr_grid_for(rcells, QF_THREADS, 64), (unsigned)(dh * ncol), (unsigned)P);
+  const int nb = sum_part ? flpr_sum_blocks((int64_t)P * dh * ncol, 1,
+                                            flpr_grid_for(rcells, QF_THREADS, QF_BLOCKS), sum_part_len)
+                          : 0;
+  if (nb < 1) return -3;
+  dim3 grd((unsigned)nb, (unsigned)(dh * ncol), (unsigned)P);
   hipStream_t st = as_stream(stream);
   switch (g->d) {
-    case 1: hipLaunchKernelGGL(k_quadform<1>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad, bz); break;
-    case 2: hipLaunchKernelGGL(k_quadform<2>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad, bz); break;
-    default: hipLaunchKernelGGL(k_quadform<3>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad, bz); break;
+    case 1: hipLaunchKernelGGL(k_quadform<1>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad, sum_part, bz); break;
+    case 2: hipLaunchKernelGGL(k_quadform<2>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad, sum_part, bz); break;
+    default: hipLaunchKernelGGL(k_quadform<3>, grd, dim3(QF_THREADS), 0, st, *g, m, nodes, gram, rcells, quad, sum_part, bz); break;
   }
   FLPR_CHECK_LAUNCH();
   return 0;

@@ -60,7 +60,10 @@ FP64_BELOW_CELLS = {0: 0.0, 1: 0.25, 2: 0.4}
 
 
 def stream_handle() -> int:
-    return torch.cuda.current_stream().cuda_stream
+    """The current HIP stream of the current device, as an integer handle
+    (torch's raw-stream query: torch.cuda.current_stream() builds a Python
+    Stream object, several microseconds per call)."""
+    return torch._C._cuda_getCurrentRawStream(torch._C._cuda_getDevice())
 
 
 class KernelTimer:
@@ -127,6 +130,22 @@ TIMER = KernelTimer()
 
 def ptr(t: Optional[torch.Tensor]):
     return None if t is None else t.data_ptr()
+
+
+# Block partials per reduced value of the fixed-order cross-block sums (probe
+# dots of flpr_solve, quadratic forms of flpr_quadform): their workspace
+# holds one ticket per group and this many partials per value; a launch with
+# more blocks than that uses fewer (more nodes per thread).  The workspace
+# must be zero before its first launch; every launch leaves it zero.
+SUM_BLOCKS = 256
+
+
+def sum_part_len(values: int, blocks: int = SUM_BLOCKS) -> int:
+    """Doubles of a fixed-order sum workspace for up to `values` reduced
+    values (groups x values per group) of up to `blocks` partials each,
+    tickets included."""
+    values = max(int(values), 1)
+    return (values + 1) // 2 + values * max(1, min(int(blocks), SUM_BLOCKS))
 
 
 def zeroed(dev, *specs):
@@ -1148,10 +1167,11 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     # --- points: knots, cell keys, counting sort, sorted SoA data
     ntiles = int(np.prod([g.ntile[k] for k in range(d)]))
     nkeys = ntiles * int(np.prod([g.tile[k] for k in range(d)]))
-    cnt, sums, key_max, red, q, gram = zeroed(
+    nsum = sum_part_len(dh * nval) if need_q else 0
+    cnt, sums, key_max, red, q, gram, sump = zeroed(
         dev, (2 * nkeys, i32), (C, f64), (1, torch.int64),
         (dh * nval + nq_out if need_q else 0, f64), (nval * nodes_all if need_q else 0, f64),
-        (nw * rcells if (need_q and want_rss) else 0, f64))
+        (nw * rcells if (need_q and want_rss) else 0, f64), (nsum, f64))
     # cell keys, their starts, the sorted order and the ranks: one allocation
     ints = torch.empty((3 * n + nkeys + 1,), dtype=i32, device=dev)
     key, order_idx, rank = ints[:n], ints[n:2 * n], ints[2 * n:3 * n]
@@ -1297,15 +1317,16 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
         if refine and not gf:
             call("flpr_solve_refined", ptr(stage0), per_h0, b - a, nodes, order, plan.nt,
                  i64_array(plan.s_off), i64_array(plan.t_off), ptr(key_max), lam_scale, eps_add,
-                 ptr(shift), nval, nout, m_ptr, ptr(q), d_ptr, s)
+                 ptr(shift), nval, nout, m_ptr, ptr(q), d_ptr, ptr(sump), nsum, s)
         else:
             call("flpr_solve", int(gf), ptr(stage0), per_h0, b - a, nodes, order, plan.nt,
                  int(fine_s), i64_array(plan.s_off), i64_array(plan.t_off), ptr(key_max),
-                 lam_scale, eps_add, ptr(shift), nval, nout, m_ptr, ptr(q), d_ptr, s)
+                 lam_scale, eps_add, ptr(shift), nval, nout, m_ptr, ptr(q), d_ptr, ptr(sump),
+                 nsum, s)
     quad = None
     if nq_out:
         quad = red[dh * nval:]
-        call("flpr_quadform", gp, ptr(m_y), ny, dh, ptr(gram), ptr(quad), s)
+        call("flpr_quadform", gp, ptr(m_y), ny, dh, ptr(gram), ptr(quad), ptr(sump), nsum, s)
     s0_all = S00Rows(stage0s, per_h0, off0, nodes, eps_add if order == 0 else 0.0)
     if debug is not None:
         debug.update(grid64=grid64, stage0=stage0s[0][4], stage0s=stage0s, key=key_max, q=q,
@@ -1425,7 +1446,9 @@ def run_grid_y(sh: dict, x: torch.Tensor, vals_y: torch.Tensor, ny: int,
     grid64 = torch.empty((C2 * cells + pad,), dtype=f64, device=dev)
     grid64[:cells].copy_(sh["grid64"][:cells])
     nodes = sh["plan0"].nodes
-    sums, red, q = zeroed(dev, (C2, f64), (2 * dh * ny, f64), (ny * nodes, f64))
+    nsum = sum_part_len(dh * ny)
+    sums, red, q, sump = zeroed(dev, (C2, f64), (2 * dh * ny, f64), (ny * nodes, f64),
+                                (nsum, f64))
     sums[:1].copy_(sh["sums"][:1])
     if acc_y > 6:
         call("flpr_points_pack", gp, ptr(x), ptr(sh["order_idx"]), n, ptr(vals_y), ny, None, 0,
@@ -1490,13 +1513,16 @@ def run_grid_y(sh: dict, x: torch.Tensor, vals_y: torch.Tensor, ny: int,
             call("flpr_solve_refined", ptr(stage0), per_h0, b - a, nodes, order, plan.nt,
                  i64_array(plan.s_off), i64_array(plan.t_off), ptr(sh["key_max"]),
                  sh["lam_scale"], sh["eps_add"], ptr(shift), ny, ny,
-                 m_y.data_ptr() + a * ny * nodes * 8, ptr(q), red.data_ptr() + a * ny * 8, s)
+                 m_y.data_ptr() + a * ny * nodes * 8, ptr(q), red.data_ptr() + a * ny * 8,
+                 ptr(sump), nsum, s)
         else:
             call("flpr_solve", int(gf), ptr(stage0), per_h0, b - a, nodes, order, plan.nt,
                  int(sh["fine_s"]), i64_array(plan.s_off), i64_array(plan.t_off),
                  ptr(sh["key_max"]), sh["lam_scale"], sh["eps_add"], ptr(shift), ny, ny,
-                 m_y.data_ptr() + a * ny * nodes * 8, ptr(q), red.data_ptr() + a * ny * 8, s)
-    call("flpr_quadform", gp, ptr(m_y), ny, dh, ptr(sh["gram"]), red.data_ptr() + dh * ny * 8, s)
+                 m_y.data_ptr() + a * ny * nodes * 8, ptr(q), red.data_ptr() + a * ny * 8,
+                 ptr(sump), nsum, s)
+    call("flpr_quadform", gp, ptr(m_y), ny, dh, ptr(sh["gram"]), red.data_ptr() + dh * ny * 8,
+         ptr(sump), nsum, s)
     return DeviceFit(h, m_y.view(dh, ny, nodes), None, None, sh["s0_all"], geo, plan0,
                      sh["warn"], red)
 

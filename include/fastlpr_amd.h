@@ -146,8 +146,9 @@ int flpr_points_pack(const FlprGeom* g, const double* x, const int32_t* order, i
  * Every tile accumulates its footprint window in float64 and writes it to the
  * workspace part (ntiles * C * prod(tile + 2 msp) doubles); a second pass
  * adds the overlapping windows per cell (no atomics) into grid
- * (C, G0[, G1[, G2]]) float64 and adds the per-channel totals into sums (C
- * doubles, zeroed by the caller). */
+ * (C, G0[, G1[, G2]]) float64, and a third adds each channel's cells in a
+ * fixed order (one block per channel) into sums (C doubles, zeroed by the
+ * caller), so the totals are the same in every run. */
 int flpr_spread(const FlprGeom* g, const int32_t* key_start, const double* wab,
                 const double* vals, int64_t n, int32_t with_ones, int32_t nval, double* part,
                 double* grid, double* sums, void* stream);
@@ -157,8 +158,8 @@ int flpr_spread(const FlprGeom* g, const int32_t* key_start, const double* wab,
  * kernel.py:97-156): d == 3 and 7 <= msp <= 12 only.  Same inputs; no
  * partial windows: every (cell, footprint row, channel) adds its row into
  * grid with float64 atomics, so grid (C, G0, G1, G2) and sums (C) must be
- * zeroed by the caller, and the sums of the atomics vary between runs in
- * their last bits. */
+ * zeroed by the caller, and the grid values vary between runs in their last
+ * bits (the channel totals are added in a fixed order, as in flpr_spread). */
 int flpr_spread_wide(const FlprGeom* g, const int32_t* key_start, const double* wab,
                      const double* vals, int64_t n, int32_t with_ones, int32_t nval,
                      double* grid, double* sums, void* stream);
@@ -283,12 +284,20 @@ double flpr_key_to_double(uint64_t key);
  *   nval      value channels 1..nval are solved (<= 32)
  *   nout      the first nout of them are written to m_out (dh, nout, nodes) float64
  *   q, dot    when q (nval, nodes) float64 is given, dot[h*nval + c] accumulates
- *             sum_node q[c][node] m_c(h, node) (dot zeroed by the caller). */
+ *             sum_node q[c][node] m_c(h, node) (dot zeroed by the caller)
+ *   sum_part  with q: workspace of sum_part_len doubles for the dots' block
+ *             partials, added in block order (no floating-point atomics, so
+ *             dot is the same in every run); it must be all zero on entry
+ *             (each call leaves it all zero).  dh * nval * 256 +
+ *             (dh + 1) / 2 doubles always suffice (at most 256 node blocks
+ *             per bandwidth); with less the launch uses fewer node blocks
+ *             (-3 when not even one per bandwidth fits).
+ *             May be NULL without q. */
 int flpr_solve(int32_t fp64, const void* mom, int64_t per_h, int32_t dh, int64_t nodes,
                int32_t order, int32_t nt, int32_t pivot, const int64_t* s_off,
                const int64_t* t_off, const uint64_t* key, double lam_scale, double eps_add,
                const double* shift, int32_t nval, int32_t nout, double* m_out, const double* q,
-               double* dot, void* stream);
+               double* dot, double* sum_part, int64_t sum_part_len, void* stream);
 
 /* flpr_solve for float32 moments in the precision form (no pivoting): for
  * order >= 1 each node's S + lam I is factored in float32 (Cholesky) and z =
@@ -301,7 +310,8 @@ int flpr_solve_refined(const void* mom, int64_t per_h, int32_t dh, int64_t nodes
                        int32_t order, int32_t nt, const int64_t* s_off, const int64_t* t_off,
                        const uint64_t* key, double lam_scale, double eps_add,
                        const double* shift, int32_t nval, int32_t nout, double* m_out,
-                       const double* q, double* dot, void* stream);
+                       const double* q, double* dot, double* sum_part, int64_t sum_part_len,
+                       void* stream);
 
 /* Adjoint interpolation sums on the raw grid, from the sorted point data:
  *   q (nval, nodes)        q[c][node] = sum_i vals[i][c] w_i(node)
@@ -318,9 +328,14 @@ int flpr_adjoint(const FlprGeom* g, const int32_t* key_start, const int32_t* raw
                  const double* rawf, const double* vals, int64_t n, int32_t nval, double* part,
                  double* q, double* gram, void* stream);
 
-/* quad[h*ncol + c] += sum_cell m' W m for grids m (dh, ncol, nodes) float64. */
+/* quad[h*ncol + c] += sum_cell m' W m for grids m (dh, ncol, nodes) float64.
+ * The cell blocks' partial sums are added in block order through the
+ * workspace sum_part (sum_part_len doubles, all zero on entry and left zero;
+ * dh * ncol * 16 + (dh*ncol + 1) / 2 suffice, less uses fewer blocks; as
+ * flpr_solve), so quad is the same in every run. */
 int flpr_quadform(const FlprGeom* g, const double* m, int32_t ncol, int32_t dh,
-                  const double* gram, double* quad, void* stream);
+                  const double* gram, double* quad, double* sum_part, int64_t sum_part_len,
+                  void* stream);
 
 /* Multilinear interpolation of one grid (ncol channels, (ncol, nodes)) at n
  * points (api.py:760-763, interp_optimized.py / scipy linear with linear
@@ -446,13 +461,14 @@ int flpr_batch_max_moment(int32_t fp64, int32_t P, const void* src, int64_t prob
                           int64_t nodes, int32_t absval, uint64_t* key, void* stream);
 
 /* flpr_solve per problem (no pivoting): moments at mom_stride, key[p], shift
- * at nval + 1, m_out at m_stride, q at nval * nodes, dot at dot_stride. */
+ * at nval + 1, m_out at m_stride, q at nval * nodes, dot at dot_stride; the
+ * dot workspace as flpr_solve's with P * dh groups. */
 int flpr_batch_solve(int32_t fp64, int32_t P, const void* mom, int64_t mom_stride, int64_t per_h,
                      int32_t dh, int64_t nodes, int32_t order, int32_t nt, const int64_t* s_off,
                      const int64_t* t_off, const uint64_t* key, double lam_scale, double eps_add,
                      const double* shift, int32_t nval, int32_t nout, double* m_out,
                      int64_t m_stride, const double* q, double* dot, int64_t dot_stride,
-                     void* stream);
+                     double* sum_part, int64_t sum_part_len, void* stream);
 
 /* flpr_adjoint per problem: part P * (ntiles * max(nval, nw) * window
  * cells), q at nval * nodes, gram at nw * rcells (both zeroed by the
@@ -463,10 +479,12 @@ int flpr_batch_adjoint(const FlprGeom* g, const FlprBatchItem* items, int32_t P,
                        void* stream);
 
 /* flpr_quadform per problem: m at m_stride, gram at nw * rcells, quad at
- * quad_stride (zeroed by the caller). */
+ * quad_stride (zeroed by the caller); the workspace as flpr_quadform's with
+ * P * dh * ncol groups. */
 int flpr_batch_quadform(const FlprGeom* g, int32_t P, const double* m, int64_t m_stride,
                         int32_t ncol, int32_t dh, const double* gram, double* quad,
-                        int64_t quad_stride, void* stream);
+                        int64_t quad_stride, double* sum_part, int64_t sum_part_len,
+                        void* stream);
 
 /* flpr_select per problem (n from items): red at red_stride, ysq at ny, pp at
  * nprobe, out at out_stride (>= 4 dh + 3), sel[p]. */
