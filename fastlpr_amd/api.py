@@ -378,7 +378,7 @@ def _is_complex(a) -> bool:
     return bool(a.is_complex()) if isinstance(a, torch.Tensor) else bool(np.iscomplexobj(a))
 
 
-def _queue_inputs(x, y, x_arr, y_arr, n: int, variance: bool, prior):
+def _queue_inputs(x, y, x_arr, y_arr, n: int, variance: bool, prior, stats_pending=None):
     """x and the response channels as float64 on the device, and the x
     statistics pass queued behind them (not read back).  Returns (dev, xd,
     yd, yd_in, y_raw_orig, pending): yd is log(y + 1/n) for a variance fit
@@ -386,7 +386,9 @@ def _queue_inputs(x, y, x_arr, y_arr, n: int, variance: bool, prior):
     column (variance fits).  pending is ("y", sums) when prior holds the x
     side of a fit of this same x (then only y's sums of squares and
     non-finite count are queued, engine.y_stats), else ("x", the queued x
-    statistics pass), and None when y has too many columns for the pass."""
+    statistics pass, or stats_pending when the caller queued it already for
+    these same device x and y), and None when y has too many columns for the
+    pass."""
     dev = x.device if _is_dev(x) else (y.device if _is_dev(y) else _device())
     if not torch.cuda.is_available():
         _device()
@@ -402,6 +404,8 @@ def _queue_inputs(x, y, x_arr, y_arr, n: int, variance: bool, prior):
         pending = None
     elif xs and xs.get("x") is xd and prior["key"][:3] == (xd.data_ptr(), xd._version, xd.shape):
         pending = ("y", engine.y_stats(yd))
+    elif stats_pending is not None and not variance:
+        pending = ("x", stats_pending)
     else:
         pending = ("x", engine.x_stats_launch(xd, yd))
     return dev, xd, yd, yd_in, y_raw_orig, pending
@@ -448,9 +452,13 @@ def cv_fastlpr_pair(x, y, h: Optional[Sequence[Sequence[float]]] = None,
     if yt.is_complex() or (yt.dim() == 2 and yt.shape[0] > 1 and yt.shape[1] > 1):
         raise NotImplementedError("cv_fastlpr_pair takes a real single-column response")
     yd = yt.to(dev).to(torch.float64).reshape(-1).contiguous()
-    # both passes are queued before either result is read back, so the device
-    # runs them back to back
-    finish_mu, share = _fit(xd, yd, h, options, as_numpy=False, keep_x=True)
+    # the statistics pass goes first (x and y are already on the device), so
+    # that the option handling of the first fit overlaps it; both passes are
+    # queued before either result is read back, so the device runs them back
+    # to back
+    pending = engine.x_stats_launch(xd, yd.view(-1, 1))
+    finish_mu, share = _fit(xd, yd, h, options, as_numpy=False, keep_x=True,
+                            stats_pending=pending)
     vopts = dict(options or {})
     vopts.update(variance_options or {})
     vopts["y_type_out"] = "variance"
@@ -489,7 +497,7 @@ def _to_numpy(r: RegressionOutput) -> RegressionOutput:
                             d=r.d, s0=cv(r.s0))
 
 
-def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
+def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False, stats_pending=None):
     """cv_fastlpr's body, split in two: the call queues all device work and
     returns (finish, share); finish() reads the results back (one host
     synchronisation) and builds the RegressionOutput.  share identifies the
@@ -498,7 +506,9 @@ def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
     rest of the x-side state (engine.run_grid share_out), so that a later fit
     of the same x with a matching key (prior) only spreads, convolves and
     solves its own response columns (engine.run_grid_y) and needs no x
-    statistics."""
+    statistics.  stats_pending: the statistics pass of exactly these device
+    x and y, already queued (engine.x_stats_launch), used instead of a new
+    one."""
     if as_numpy is None:
         as_numpy = not (_is_dev(x) or _is_dev(y))
     x_arr = _as_column_matrix(x)
@@ -569,7 +579,7 @@ def _fit(x, y, h, options, prior=None, as_numpy=None, keep_x=False):
         _check_channels(dy * (2 if y_is_complex else 1), max(K_req, 0))
     # every check above runs before any device work
     dev, xd, yd, yd_in, y_raw_orig, pending = _queue_inputs(x, y, x_arr, y_arr, n, variance,
-                                                            prior)
+                                                            prior, stats_pending)
     ny = yd.shape[1]
     if ny > 32:
         raise NotImplementedError("the GPU engine takes at most 32 real response columns")

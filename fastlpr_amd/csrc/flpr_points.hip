@@ -54,6 +54,25 @@ __device__ inline Welford wf_merge(Welford a, Welford b) {
   return r;
 }
 
+// Welford merge over a block (valid on thread 0): an xor-shuffle tree inside
+// each wave, then the waves' results in wave order (a fixed shape, no LDS
+// tree with a barrier per level).
+__device__ inline Welford wf_block(Welford w, Welford* sh) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const Welford o = {__shfl_xor(w.n, off), __shfl_xor(w.mean, off), __shfl_xor(w.m2, off),
+                       __shfl_xor(w.mn, off), __shfl_xor(w.mx, off)};
+    w = wf_merge(w, o);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = w;
+  __syncthreads();
+  Welford r = sh[0];
+  if (threadIdx.x == 0)
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) r = wf_merge(r, sh[i]);
+  return r;
+}
+
 __device__ inline double block_sum(double v, double* sh) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -75,7 +94,7 @@ __global__ __launch_bounds__(XS_THREADS) void k_xstats(const double* __restrict_
                                                        int d, const double* __restrict__ y,
                                                        int ny, double* __restrict__ partials,
                                                        FlprBz bz) {
-  __shared__ Welford sh[XS_THREADS];
+  __shared__ Welford sh[XS_THREADS / 64];
   if (bz.items) {
     const FlprBatchItem& it = bz.items[blockIdx.z];
     if (x) x += it.pt * d;
@@ -86,7 +105,10 @@ __global__ __launch_bounds__(XS_THREADS) void k_xstats(const double* __restrict_
   __shared__ double red[XS_THREADS / 64];
   double bad_x = 0.0, bad_y = 0.0;
   for (int k = 0; k < d; ++k) {
-    Welford w = {0.0, 0.0, 0.0, INFINITY, -INFINITY};
+    // a thread's values as sums of v - s1, s1 the first point's value (when
+    // finite): no division per value; Welford form once per thread
+    const double s1 = (n > 0 && isfinite(x[k])) ? x[k] : 0.0;
+    double cnt = 0.0, t1 = 0.0, t2 = 0.0, mn = INFINITY, mx = -INFINITY;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
       const double v = x[i * d + k];
@@ -94,24 +116,24 @@ __global__ __launch_bounds__(XS_THREADS) void k_xstats(const double* __restrict_
         bad_x += 1.0;
         continue;
       }
-      w.n += 1.0;
-      const double delta = v - w.mean;
-      w.mean += delta / w.n;
-      w.m2 += delta * (v - w.mean);
-      w.mn = fmin(w.mn, v);
-      w.mx = fmax(w.mx, v);
+      const double dv = v - s1;
+      cnt += 1.0;
+      t1 += dv;
+      t2 = fma(dv, dv, t2);
+      mn = fmin(mn, v);
+      mx = fmax(mx, v);
     }
-    sh[threadIdx.x] = w;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-      if ((int)threadIdx.x < s) sh[threadIdx.x] = wf_merge(sh[threadIdx.x], sh[threadIdx.x + s]);
-      __syncthreads();
+    Welford w = {cnt, 0.0, 0.0, mn, mx};
+    if (cnt > 0.0) {
+      const double dm = t1 / cnt;
+      w.mean = s1 + dm;
+      w.m2 = fmax(t2 - t1 * dm, 0.0);
     }
+    w = wf_block(w, sh);
     if (threadIdx.x == 0) {
       double* p = partials + ((size_t)k * gridDim.x + blockIdx.x) * 5;
-      p[0] = sh[0].n; p[1] = sh[0].mean; p[2] = sh[0].m2; p[3] = sh[0].mn; p[4] = sh[0].mx;
+      p[0] = w.n; p[1] = w.mean; p[2] = w.m2; p[3] = w.mn; p[4] = w.mx;
     }
-    __syncthreads();
   }
   double* cnt = partials + (size_t)d * gridDim.x * 5;
   double* sq = cnt + (size_t)gridDim.x * 2;
@@ -139,7 +161,7 @@ __global__ __launch_bounds__(XS_THREADS) void k_xstats(const double* __restrict_
 __global__ __launch_bounds__(XS_THREADS) void k_xstats_final(const double* __restrict__ partials,
                                                              int nparts, int d, int ny,
                                                              double* __restrict__ out, FlprBz bz) {
-  __shared__ Welford sh[XS_THREADS];
+  __shared__ Welford sh[XS_THREADS / 64];
   partials += blockIdx.z * bz.s[0];
   out += blockIdx.z * bz.s[1];
   __shared__ double red[XS_THREADS / 64];
@@ -150,20 +172,14 @@ __global__ __launch_bounds__(XS_THREADS) void k_xstats_final(const double* __res
       const Welford o = {p[0], p[1], p[2], p[3], p[4]};
       w = wf_merge(w, o);
     }
-    sh[threadIdx.x] = w;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-      if ((int)threadIdx.x < s) sh[threadIdx.x] = wf_merge(sh[threadIdx.x], sh[threadIdx.x + s]);
-      __syncthreads();
-    }
+    w = wf_block(w, sh);
     if (threadIdx.x == 0) {
-      out[5 * k + 0] = sh[0].mn;
-      out[5 * k + 1] = sh[0].mx;
-      out[5 * k + 2] = sh[0].mean;
-      out[5 * k + 3] = sh[0].m2;
-      out[5 * k + 4] = sh[0].n;
+      out[5 * k + 0] = w.mn;
+      out[5 * k + 1] = w.mx;
+      out[5 * k + 2] = w.mean;
+      out[5 * k + 3] = w.m2;
+      out[5 * k + 4] = w.n;
     }
-    __syncthreads();
   }
   const double* cnt = partials + (size_t)d * nparts * 5;
   const double* sq = cnt + (size_t)nparts * 2;
