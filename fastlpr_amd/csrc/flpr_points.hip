@@ -834,35 +834,69 @@ __global__ __launch_bounds__(SPREAD_THREADS) void k_spread_wide3(
   }
 }
 
-// Per-channel totals of a grid: one block per channel (blockIdx.x) and
-// problem (blockIdx.y; grid at grid_stride, sums at C), which adds the
-// channel's cells in a fixed order, so the totals (and the centring shifts
-// made from them) are the same in every run: sums[c] += total (sums zeroed
-// by the caller).
-constexpr int CSUM_THREADS = 1024;
+// Per-channel totals of a grid, in a fixed order so that the totals (and the
+// centring shifts made from them) are the same in every run; sums[c] +=
+// total (sums zeroed by the caller; problem p's grid at p * grid_stride, its
+// sums at p * C).  With a workspace: block (b, c, p) adds the cells of chunk
+// b of channel c (thread t: cells t, t + 256, ... of the chunk, then a fixed
+// block tree) into part[(p * C + c) * nb + b], and k_grid_sums_final adds the
+// nb partials of each channel (lane l: partials l, l + 64, ...; then a fixed
+// xor tree).  Without one (the 3-D oversampled spread): one block per
+// channel adds all its cells.
+constexpr int CSUM_THREADS = 256;
+constexpr int CSUM_CHUNK = 8192;      // cells per block of the partial pass
+constexpr int CSUM_MAX_PARTS = 256;   // partials per channel at most
+
 __global__ __launch_bounds__(CSUM_THREADS) void k_grid_sums(const double* __restrict__ grid,
                                                             int64_t cells, int64_t grid_stride,
-                                                            int C, double* __restrict__ sums) {
+                                                            int C, int64_t chunk,
+                                                            double* __restrict__ dst) {
   __shared__ double red[CSUM_THREADS / 64];
-  const int c = blockIdx.x;
-  const double* gp = grid + (int64_t)blockIdx.y * grid_stride + (int64_t)c * cells;
+  const int b = blockIdx.x, c = blockIdx.y, p = blockIdx.z;
+  const double* gp = grid + (int64_t)p * grid_stride + (int64_t)c * cells;
+  const int64_t lo = (int64_t)b * chunk;
+  const int64_t hi = lo + chunk < cells ? lo + chunk : cells;
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int64_t i = threadIdx.x;
-  for (; i + 3 * CSUM_THREADS < cells; i += 4 * CSUM_THREADS) {
+  int64_t i = lo + threadIdx.x;
+  for (; i + 3 * CSUM_THREADS < hi; i += 4 * CSUM_THREADS) {
     s0 += gp[i];
     s1 += gp[i + CSUM_THREADS];
     s2 += gp[i + 2 * CSUM_THREADS];
     s3 += gp[i + 3 * CSUM_THREADS];
   }
-  for (; i < cells; i += CSUM_THREADS) s0 += gp[i];
+  for (; i < hi; i += CSUM_THREADS) s0 += gp[i];
   const double s = block_sum((s0 + s1) + (s2 + s3), red);
-  if (threadIdx.x == 0) sums[(int64_t)blockIdx.y * C + c] += s;
+  if (threadIdx.x == 0) {
+    if (gridDim.x == 1)
+      dst[(int64_t)p * C + c] += s;  // one block per channel: the total
+    else
+      dst[((int64_t)p * C + c) * gridDim.x + b] = s;
+  }
 }
 
+__global__ __launch_bounds__(64) void k_grid_sums_final(const double* __restrict__ part, int nb,
+                                                        int C, double* __restrict__ sums) {
+  const int c = blockIdx.x, p = blockIdx.y;
+  const double* pp = part + ((int64_t)p * C + c) * nb;
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 64) a += pp[i];
+  for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
+  if (threadIdx.x == 0) sums[(int64_t)p * C + c] += a;
+}
+
+// part: workspace of at least P * C * CSUM_MAX_PARTS doubles, or NULL.
 void launch_grid_sums(hipStream_t st, const double* grid, int64_t cells, int64_t grid_stride,
-                      int C, int P, double* sums) {
-  hipLaunchKernelGGL(k_grid_sums, dim3((unsigned)C, (unsigned)P), dim3(CSUM_THREADS), 0, st,
-                     grid, cells, grid_stride, C, sums);
+                      int C, int P, double* sums, double* part) {
+  int64_t nb = part ? (cells + CSUM_CHUNK - 1) / CSUM_CHUNK : 1;
+  if (nb > CSUM_MAX_PARTS) nb = CSUM_MAX_PARTS;
+  if (nb < 1) nb = 1;
+  const int64_t chunk = (cells + nb - 1) / nb;
+  hipLaunchKernelGGL(k_grid_sums, dim3((unsigned)nb, (unsigned)C, (unsigned)P),
+                     dim3(CSUM_THREADS), 0, st, grid, cells, grid_stride, C, chunk,
+                     nb == 1 ? sums : part);
+  if (nb > 1)
+    hipLaunchKernelGGL(k_grid_sums_final, dim3((unsigned)C, (unsigned)P), dim3(64), 0, st, part,
+                       (int)nb, C, sums);
 }
 
 __device__ inline int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
@@ -2435,7 +2469,7 @@ int flpr_spread_wide(const FlprGeom* g, const int32_t* key_start, const double* 
   }
 #undef FLPR_WIDE_CASE
   FLPR_CHECK_LAUNCH();
-  launch_grid_sums(st, grid, cells, (int64_t)C * cells, C, 1, sums);
+  launch_grid_sums(st, grid, cells, (int64_t)C * cells, C, 1, sums, nullptr);
   FLPR_CHECK_LAUNCH();
   return 0;
 }
@@ -2486,7 +2520,11 @@ int flpr_spread(const FlprGeom* g, const int32_t* key_start, const double* wab,
     default: launch_gather<3>(gg, st, *g, part, C, cells, grid, FlprBz{}); break;
   }
   FLPR_CHECK_LAUNCH();
-  launch_grid_sums(st, grid, cells, (int64_t)C * cells, C, 1, sums);
+  // the tile windows in part are dead once gathered: the channel totals'
+  // partials go there (ntiles * C * fcells >= C * CSUM_MAX_PARTS doubles, or
+  // one block per channel)
+  launch_grid_sums(st, grid, cells, (int64_t)C * cells, C, 1, sums,
+                   ntiles * fcells >= CSUM_MAX_PARTS ? part : nullptr);
   FLPR_CHECK_LAUNCH();
   return 0;
 }
@@ -2867,7 +2905,8 @@ int flpr_batch_spread(const FlprGeom* g, const FlprBatchItem* items, int32_t P, 
     default: launch_gather<3>(gg, st, *g, part, C, cells, grid, bg); break;
   }
   FLPR_CHECK_LAUNCH();
-  launch_grid_sums(st, grid, cells, grid_stride, C, P, sums);
+  launch_grid_sums(st, grid, cells, grid_stride, C, P, sums,
+                   ntiles * fcells >= CSUM_MAX_PARTS ? part : nullptr);
   FLPR_CHECK_LAUNCH();
   return 0;
 }
