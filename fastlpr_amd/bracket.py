@@ -157,11 +157,15 @@ def spread_grids(xs: Sequence[torch.Tensor], ys: Sequence[torch.Tensor], N=(256,
 
 
 def selector(mdl: "BracketModel", ysq_dev: torch.Tensor, info: dict):
-    """The select_h hook of engine.run_grid for one 2-D fit: features from the
-    fit's own spread grid (channels 0 and 1 before centring, ysq_dev = sum
-    y^2 on the device), the MLP's bracket centre and width (one host read),
-    and the BRACKET indices around the centre; info receives them.  Returns
-    None (full sweep) when the list is no longer than the bracket."""
+    """The select_h hook of engine.run_grid for one 2-D fit.  The hook queues
+    the features of the fit's own spread grid (channels 0 and 1 before
+    centring, ysq_dev = sum y^2 on the device) and the MLP's forward pass,
+    and returns a callable; the engine queues the list-wide regularisation
+    pass, then calls it.  The callable reads the bracket centre and width
+    (one host read, by which time the device is busy with that pass) and
+    returns the BRACKET indices around the centre; info receives them.  The
+    hook returns None (full sweep) when the list is no longer than the
+    bracket."""
     def select(geo, grid64, cells, sums, h):
         if len(geo.N) != 2 or h.shape[0] <= BRACKET:
             return None
@@ -172,11 +176,15 @@ def selector(mdl: "BracketModel", ysq_dev: torch.Tensor, info: dict):
               "npts": engine.device_table(np.asarray([float(info["n"])]), torch.float64, dev),
               "ysq": ysq_dev.reshape(-1)[:1].contiguous(),
               "dx": engine.device_table(np.asarray([grid_spacing(geo)]), torch.float64, dev)}
-        center, width = mdl.predict(grid_features(sg))[0]
-        idx = choose(float(center), h)
-        info.update(indices=idx, n_full=int(h.shape[0]), center=float(center),
-                    width=float(width), hlist=h.copy(), fallback=False)
-        return idx
+        out = mdl.queue(grid_features(sg))
+
+        def pick():
+            center, width = mdl.read(out)[0]
+            idx = choose(float(center), h)
+            info.update(indices=idx, n_full=int(h.shape[0]), center=float(center),
+                        width=float(width), hlist=h.copy(), fallback=False)
+            return idx
+        return pick
     return select
 
 
@@ -246,9 +254,18 @@ class BracketModel:
 
     def predict(self, feat: torch.Tensor) -> np.ndarray:
         """(P, 2) bracket centre and width in log10 h for raw features (P, 64)."""
+        return self.read(self.queue(feat))
+
+    def queue(self, feat: torch.Tensor) -> torch.Tensor:
+        """The forward pass for raw features (P, 64), queued on the device:
+        the standardised outputs (P, 2) float32, not read back."""
         p, fm, fs = self._on(feat.device)
         X = ((feat - fm) / fs).contiguous()
-        O = forward(p, X)[-1]
+        return forward(p, X)[-1]
+
+    def read(self, O: torch.Tensor) -> np.ndarray:
+        """Bracket centre and width in log10 h from queue()'s outputs (one
+        host read)."""
         return O.double().cpu().numpy() * self.tstd + self.tmean
 
 

@@ -1111,7 +1111,9 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
     complex responses, which that deconvolution must keep together.
     key_only: with moments_only, also reduce the global regularisation key
     over every bandwidth of h (DeviceFit.key, a device uint64) and stop;
-    select_h: a callable (geo, grid64, cells, sums, h) -> indices or None,
+    select_h: a callable (geo, grid64, cells, sums, h) -> None or a callable
+    returning indices (called after the list-wide regularisation pass is
+    queued),
     called once the spread grid is complete and before it is centred (so
     channels 0 and 1 are the raw sums of the weights and of y): with
     indices, only those bandwidths of the filtered list h are convolved
@@ -1226,9 +1228,12 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
         spread_fine(geo, acc_y, x, n, vals_y, ny, y_pairs, None, 0, False, grid64, sums, 1, s)
     del part
     if select_h is not None:
-        idx = select_h(geo, grid64, cells, sums, h)
-        if idx is not None:
+        pick = select_h(geo, grid64, cells, sums, h)
+        if pick is not None:
+            # lambda of the whole list first: the device computes it while
+            # the host reads the selector's result
             regularisation_key(geo, gkey, h, order, n, fp64, grid64, cells, key_max, s)
+            idx = pick()
             key_in = key_max
             h, _w, groups, band_np = bandwidth_tables(geo, gkey, h[np.asarray(idx)], n, order,
                                                       fp64)
