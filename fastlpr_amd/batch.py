@@ -427,7 +427,7 @@ class BatchPlan:
         graph = self._graph if (self._graph is not None and self._graph[0] == sig) else None
         if graph is not None:
             for c in classes:
-                c.bufs["y_in"].copy_(torch.cat([ydev[p.idx] for p in c.probs], dim=0))
+                torch.cat([ydev[p.idx] for p in c.probs], dim=0, out=c.bufs["y_in"])
             graph[1].replay()
             pending = graph[2]
             host = graph[3].cpu().numpy() if graph[3] is not None else None
@@ -446,8 +446,9 @@ class BatchPlan:
             nout = st["out"].numel()
             hv = host[off:off + nout].reshape(len(c.probs), -1)
             off += nout
+            arrays = self._class_arrays(c, st, hv, keep)
             for j, p in enumerate(c.probs):
-                results[p.idx] = self._result(c, st, j, p, hv[j], yl[p.idx], keep)
+                results[p.idx] = self._result(c, st, j, p, hv[j], yl[p.idx], arrays)
         # problems outside the batched path: one call each
         for p, y in zip(self.probs, yl):
             if p.idx not in results:
@@ -661,11 +662,51 @@ class BatchPlan:
             c.tabs["probe_base"] = base
         return c.tabs["probe_base"], c.tabs["probe_pp"]
 
+    def _class_arrays(self, c: _Class, st: dict, hv: np.ndarray, copy: bool) -> dict:
+        """The device arrays the results of class c slice, made with a few
+        launches for the whole class (not one per problem): the fitted values
+        and residuals (copied once when a later replay would overwrite them),
+        and, at each problem's selected bandwidth, its grid values and its
+        S_00 row (gathered per precision group).  hv: the class's host rows
+        of the selection results."""
+        dh = c.tabs["dh"]
+        P = len(c.probs)
+        nodes = int(np.prod(c.N))
+        # a problem without a finite score raises in _result; its index is
+        # clamped here only so that the gathers stay inside the buffers
+        sel = np.clip(np.nan_to_num(hv[:, 4 * dh]), 0, dh - 1).astype(np.int64)
+        dev = self.dev
+        yhat, resid = st["yhat"], st["resid"]
+        if copy:
+            yhat, resid = yhat.clone(), resid.clone()
+        jj = torch.arange(P, device=dev)
+        if c.variance:
+            grid = st["grid_v"].view(P, 1, nodes)
+            if copy:
+                grid = grid.clone()
+        else:
+            # advanced indexing: a gathered copy, independent of the buffer
+            grid = st["m_y"].view(P, dh, c.ny, nodes)[jj, torch.as_tensor(sel, device=dev)]
+        s0 = torch.empty((P, nodes), dtype=torch.float64, device=dev)
+        for (a, b, gf, pl, st0, stage0) in st["stage0s"]:
+            js = np.nonzero((sel >= a) & (sel < b))[0]
+            if js.size == 0:
+                continue
+            per_h = pl.stage_per_h[0]
+            off0 = pl.s_off[0]
+            blk = stage0[:P * st0].view(P, st0)[:, :(b - a) * per_h].view(P, b - a, per_h)
+            jt = torch.as_tensor(js, device=dev)
+            rows = blk[jt, torch.as_tensor(sel[js] - a, device=dev), off0:off0 + nodes]
+            s0[jt] = rows.to(torch.float64)
+        if c.order == 0:
+            s0 += float(np.finfo(float).eps)
+        return {"yhat": yhat, "resid": resid, "grid": grid, "s0": s0}
+
     def _result(self, c: _Class, st: dict, j: int, p: _Problem, hv: np.ndarray, y_in,
-                copy: bool = False):
+                arrays: dict):
         """RegressionOutput of problem j of class c from its host row of the
-        selection results (the layout of cv_fastlpr's readback); copy: take
-        the device arrays out of buffers that a later replay overwrites."""
+        selection results (the layout of cv_fastlpr's readback) and the class
+        arrays of _class_arrays."""
         dh = c.tabs["dh"]
         if hv[4 * dh + 4] > 0:
             api._check_finite(y_in, "Response y", "cv_fastlpr")
@@ -683,13 +724,9 @@ class BatchPlan:
         gcv_result = {"bandwidths": hf, "gcv_m": scores, "gcv_sd": errs, "idmin": idmin,
                       "id1se": sel, "h1se": hf[sel], "hmin": hf[idmin],
                       "dof": dof_all, "dof_stderr": dof_se}
-        nodes = int(np.prod(c.N))
-        ny = c.ny
         r0, r1 = int(c.tabs["pts"][j]), int(c.tabs["pts"][j + 1])
-        out = st["yhat"][r0:r1]
-        resid_r = st["resid"][r0:r1]
-        if copy:
-            out, resid_r = out.clone(), resid_r.clone()
+        out = arrays["yhat"][r0:r1]
+        resid_r = arrays["resid"][r0:r1]
         dy = c.dy
         if c.cplx:
             yhat = torch.view_as_complex(out.reshape(p.n, dy, 2))
@@ -699,25 +736,12 @@ class BatchPlan:
         if dy == 1:
             yhat = yhat[:, 0]
         resid = resid.reshape(-1)
-        m_y = st["m_y"].view(len(c.probs), dh, ny, nodes)[j]
         d_norm = None
         if c.variance:
-            grid_vals = st["grid_v"].view(len(c.probs), nodes)[j].view(1, -1)
             d_norm = 1.0 / float(hv[4 * dh + 3])
-        else:
-            grid_vals = m_y[sel]
-        if copy:
-            grid_vals = grid_vals.clone()
-        # S_00 at the selected bandwidth from the moment slabs of its group
-        s0 = None
-        for (a, b, gf, pl, st0, stage0) in st["stage0s"]:
-            if a <= sel < b:
-                off0 = pl.s_off[0]
-                blk = stage0[j * st0:(j + 1) * st0].view(b - a, pl.stage_per_h[0])
-                s0 = blk[sel - a, off0:off0 + nodes].to(torch.float64, copy=True)
-                if c.order == 0:
-                    s0 = s0 + float(np.finfo(float).eps)
-        s0 = s0.reshape(c.N)
+        grid_vals = arrays["grid"][j]
+        # S_00 at the selected bandwidth (gathered per class)
+        s0 = arrays["s0"][j].reshape(c.N)
         geo = p.geo
         interp = api.GridInterpolant(geo, geo.axes_raw, grid_vals, c.N, c.cplx, dy, p.as_numpy,
                                      dy == 1)
