@@ -587,8 +587,8 @@ class BatchPlan:
         per_h0 = plan0.stage_per_h[0]
         if not any(c.cubic):
             # adjoint sums, solves (+ dots), residual quadratic forms
-            wcells = int(np.prod([g0.tile[k] + 5 for k in range(d)]))
-            apart = torch.empty((P * ntiles * max(nval, nw) * wcells,), dtype=f64, device=dev)
+            apart = torch.empty((P * engine.adjoint_workspace(g0, d, ntiles, max(nval, nw)),),
+                                dtype=f64, device=dev)
             call("flpr_batch_adjoint", gp, items, P, nmax, ptr(key_start), ptr(rawi), ptr(rawf),
                  ptr(vals), nval, ptr(apart), ptr(q), ptr(gram), s)
             del apart

@@ -1330,7 +1330,12 @@ __global__ __launch_bounds__(256) void k_adjoint_walk(FlprGeom g,
                                                       const double* __restrict__ vals, int nch,
                                                       int64_t nkeys, double* __restrict__ out,
                                                       int64_t out_stride,
-                                                      double* __restrict__ part) {
+                                                      double* __restrict__ part, FlprBz bz) {
+  // batch: problem blockIdx.z (one geometry shape for all; the point data
+  // through its key range); s[0] = key_start, s[1] = out, s[2] = part
+  key_start += blockIdx.z * bz.s[0];
+  out += blockIdx.z * bz.s[1];
+  part += blockIdx.z * bz.s[2];
   constexpr int NQ2 = 1 << D;
   constexpr int NACC = (MODE == 0) ? NQ2 : NQ2 * (NQ2 + 1) / 2;
   int tc = 1;
@@ -1435,7 +1440,10 @@ template <int D>
 __global__ __launch_bounds__(256) void k_adjoint_walk_gather(FlprGeom g,
                                                              const double* __restrict__ part,
                                                              int nch, int64_t nodes,
-                                                             double* __restrict__ q) {
+                                                             double* __restrict__ q, FlprBz bz) {
+  // batch: s[0] = part, s[1] = q
+  part += blockIdx.z * bz.s[0];
+  q += blockIdx.z * bz.s[1];
   constexpr int NQ2 = 1 << D;
   const int c = blockIdx.y;
   int tc = 1, tstride[3] = {1, 1, 1}, lstride[3] = {1, 1, 1};
@@ -2569,6 +2577,67 @@ int flpr_grid_center(double* grid, int32_t C, int64_t cells, const double* sums,
   return 0;
 }
 
+// The walk form of the adjoint sums (every axis with an even node count) for
+// P problems of one geometry shape, problem p's key table at p * key_stride,
+// its q, gram and walk workspace at p * q_stride, gram_stride and
+// part_stride (P = 1: strides unused).  part: nkeys * nval * 2^d doubles per
+// problem.
+static int launch_adjoint_walk(const FlprGeom* g, int P, hipStream_t st, const int32_t* key_start,
+                        const int32_t* rawi, const double* rawf, const double* vals, int nval,
+                        double* part, double* q, double* gram, int64_t key_stride,
+                        int64_t q_stride, int64_t gram_stride, int64_t part_stride) {
+  int64_t tcells = 1, nodes = 1, rcells = 1;
+  for (int k = 0; k < g->d; ++k) {
+    tcells *= g->tile[k];
+    nodes *= g->n_grid[k];
+    rcells *= g->n_grid[k] - 1;
+  }
+  const int64_t nkeys = flpr_ntiles(g) * tcells;
+#define FLPR_WALK(M, out_, stride_, nch_, bz_)                                                   \
+    do {                                                                                        \
+      dim3 grd(flpr_grid_for(nkeys * (nch_) * WALK_LANES, 256, 16384), 1, (unsigned)P);        \
+      switch (g->d) {                                                                           \
+        case 1: hipLaunchKernelGGL((k_adjoint_walk<1, M>), grd, dim3(256), 0, st, *g, key_start, \
+                                   rawi, rawf, vals, nval, nkeys, out_, stride_, part, bz_);    \
+          break;                                                                                \
+        case 2: hipLaunchKernelGGL((k_adjoint_walk<2, M>), grd, dim3(256), 0, st, *g, key_start, \
+                                   rawi, rawf, vals, nval, nkeys, out_, stride_, part, bz_);    \
+          break;                                                                                \
+        default: hipLaunchKernelGGL((k_adjoint_walk<3, M>), grd, dim3(256), 0, st, *g,           \
+                                    key_start, rawi, rawf, vals, nval, nkeys, out_, stride_,    \
+                                    part, bz_);                                                 \
+          break;                                                                                \
+      }                                                                                         \
+      FLPR_CHECK_LAUNCH();                                                                      \
+    } while (0)
+  if (gram) {
+    FlprBz bw{};
+    bw.s[0] = key_stride;
+    bw.s[1] = gram_stride;
+    FLPR_WALK(1, gram, rcells, 1, bw);
+  }
+  if (nval > 0) {
+    // points outside their cell's main interval add into q directly
+    FlprBz bw{};
+    bw.s[0] = key_stride;
+    bw.s[1] = q_stride;
+    bw.s[2] = part_stride;
+    FLPR_WALK(0, q, nodes, nval, bw);
+    FlprBz bg{};
+    bg.s[0] = part_stride;
+    bg.s[1] = q_stride;
+    dim3 gg(flpr_grid_for(nodes, 256, 1024), (unsigned)nval, (unsigned)P);
+    switch (g->d) {
+      case 1: hipLaunchKernelGGL(k_adjoint_walk_gather<1>, gg, dim3(256), 0, st, *g, part, nval, nodes, q, bg); break;
+      case 2: hipLaunchKernelGGL(k_adjoint_walk_gather<2>, gg, dim3(256), 0, st, *g, part, nval, nodes, q, bg); break;
+      default: hipLaunchKernelGGL(k_adjoint_walk_gather<3>, gg, dim3(256), 0, st, *g, part, nval, nodes, q, bg); break;
+    }
+    FLPR_CHECK_LAUNCH();
+  }
+#undef FLPR_WALK
+  return 0;
+}
+
 int flpr_adjoint(const FlprGeom* g, const int32_t* key_start, const int32_t* rawi,
                  const double* rawf, const double* vals, int64_t n, int32_t nval, double* part,
                  double* q, double* gram, void* stream) {
@@ -2589,39 +2658,9 @@ int flpr_adjoint(const FlprGeom* g, const int32_t* key_start, const int32_t* raw
   // even node counts: both sums by k_adjoint_walk
   bool aligned = true;
   for (int k = 0; k < g->d; ++k) aligned = aligned && !(g->n_grid[k] & 1);
-  if (aligned) {
-    int64_t tcells = 1;
-    for (int k = 0; k < g->d; ++k) tcells *= g->tile[k];
-    const int64_t nkeys = flpr_ntiles(g) * tcells;
-#define FLPR_WALK(M, out_, stride_, nch_)                                                        \
-    do {                                                                                        \
-      dim3 grd(flpr_grid_for(nkeys * (nch_) * WALK_LANES, 256, 16384));                        \
-      switch (g->d) {                                                                           \
-        case 1: hipLaunchKernelGGL((k_adjoint_walk<1, M>), grd, dim3(256), 0, st, *g, key_start, \
-                                   rawi, rawf, vals, nval, nkeys, out_, stride_, part); break;  \
-        case 2: hipLaunchKernelGGL((k_adjoint_walk<2, M>), grd, dim3(256), 0, st, *g, key_start, \
-                                   rawi, rawf, vals, nval, nkeys, out_, stride_, part); break;  \
-        default: hipLaunchKernelGGL((k_adjoint_walk<3, M>), grd, dim3(256), 0, st, *g,           \
-                                    key_start, rawi, rawf, vals, nval, nkeys, out_, stride_,    \
-                                    part);                                                      \
-          break;                                                                                \
-      }                                                                                         \
-      FLPR_CHECK_LAUNCH();                                                                      \
-    } while (0)
-    if (gram) FLPR_WALK(1, gram, rcells, 1);
-    if (nval > 0) {
-      FLPR_WALK(0, q, nodes, nval);
-      dim3 gg(flpr_grid_for(nodes, 256, 1024), (unsigned)nval);
-      switch (g->d) {
-        case 1: hipLaunchKernelGGL(k_adjoint_walk_gather<1>, gg, dim3(256), 0, st, *g, part, nval, nodes, q); break;
-        case 2: hipLaunchKernelGGL(k_adjoint_walk_gather<2>, gg, dim3(256), 0, st, *g, part, nval, nodes, q); break;
-        default: hipLaunchKernelGGL(k_adjoint_walk_gather<3>, gg, dim3(256), 0, st, *g, part, nval, nodes, q); break;
-      }
-      FLPR_CHECK_LAUNCH();
-    }
-#undef FLPR_WALK
-    return 0;
-  }
+  if (aligned)
+    return launch_adjoint_walk(g, 1, st, key_start, rawi, rawf, vals, nval, part, q, gram, 0, 0,
+                               0, 0);
   for (int mode = 0; mode < 2; ++mode) {
     const int nch = (mode == 0) ? nval : (gram ? nw : 0);
     if (nch == 0) continue;
@@ -2943,8 +2982,18 @@ int flpr_batch_adjoint(const FlprGeom* g, const FlprBatchItem* items, int32_t P,
   const int nq = 1 << g->d;
   const int nw = nq * (nq + 1) / 2;
   const int64_t ntiles = flpr_ntiles(g);
-  const int64_t part_stride = ntiles * std::max(nval, nw) * wcells;
+  int64_t tcells = 1;
+  for (int k = 0; k < g->d; ++k) tcells *= g->tile[k];
+  // per problem: the LDS windows of k_adjoint or the corner sums of the walk
+  const int64_t part_stride = ntiles * std::max(nval, nw) * std::max(wcells, tcells << g->d);
   hipStream_t st = as_stream(stream);
+  // even node counts: both sums by the walk, as a single problem's
+  bool aligned = true;
+  for (int k = 0; k < g->d; ++k) aligned = aligned && !(g->n_grid[k] & 1);
+  if (aligned)
+    return launch_adjoint_walk(g, P, st, key_start, rawi, rawf, vals, nval, part, q, gram,
+                               flpr_nkeys(g), (int64_t)nval * nodes, (int64_t)nw * rcells,
+                               part_stride);
   for (int mode = 0; mode < 2; ++mode) {
     const int nch = (mode == 0) ? nval : (gram ? nw : 0);
     if (nch == 0) continue;

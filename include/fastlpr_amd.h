@@ -470,9 +470,10 @@ int flpr_batch_solve(int32_t fp64, int32_t P, const void* mom, int64_t mom_strid
                      int64_t m_stride, const double* q, double* dot, int64_t dot_stride,
                      double* sum_part, int64_t sum_part_len, void* stream);
 
-/* flpr_adjoint per problem: part P * (ntiles * max(nval, nw) * window
- * cells), q at nval * nodes, gram at nw * rcells (both zeroed by the
- * caller). */
+/* flpr_adjoint per problem: part P * (ntiles * max(nval, nw) *
+ * max(prod(tile + 5), 2^d prod(tile))) doubles, q at nval * nodes, gram at
+ * nw * rcells (both zeroed by the caller).  Even node counts on every axis
+ * use the cell walk, as flpr_adjoint does. */
 int flpr_batch_adjoint(const FlprGeom* g, const FlprBatchItem* items, int32_t P, int64_t nmax,
                        const int32_t* key_start, const int32_t* rawi, const double* rawf,
                        const double* vals, int32_t nval, double* part, double* q, double* gram,
