@@ -591,6 +591,13 @@ __host__ __device__ inline int fill_splits(int nsplit, int64_t items_per_split, 
 
 constexpr int SPREAD_THREADS = 256;
 constexpr int SPREAD_MAX_TAPS = 2 * FLPR_MAX_MSP + 1;
+// Minimum waves per SIMD.  The 3-D item's 9 x 9 float64 block left alone
+// takes just over 256 registers, one wave per SIMD, with nothing to hide the
+// point loads and the window adds behind; capped at 256 two waves fit (a few
+// registers spill).  The compensated float32 block (sum and compensation)
+// would spill far more and keeps one wave.
+template <int D, typename A>
+constexpr int SPREAD_WAVES = (D == 3 && sizeof(A) == 8) ? 2 : 1;
 
 // G[k][j] = exp(-hx_k^2 (j - MSP)^2 / (4 tau_k)); passed by value so that the
 // kernels read it through the scalar unit.
@@ -615,7 +622,7 @@ struct SpreadTab {
 // Batch: s[0] = key_start (the problem's key count), s[1] = part; a
 // problem's cell split follows its own point count (items).
 template <int D, int MSP, typename A>
-__global__ __launch_bounds__(SPREAD_THREADS) void k_spread_cells(
+__global__ __launch_bounds__(SPREAD_THREADS, (SPREAD_WAVES<D, A>)) void k_spread_cells(
     FlprGeom g, SpreadTab tab, const int32_t* __restrict__ key_start,
     const double* __restrict__ wab, const double* __restrict__ vals, int64_t n, int with_ones,
     int C, int cg, int nsplit, A* __restrict__ part, FlprBz bz) {

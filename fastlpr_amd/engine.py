@@ -44,7 +44,11 @@ import torch
 from . import _lib
 from ._lib import FlprGeom, call, i64_array
 
-TILE = {1: 16, 2: 8, 3: 8}
+# Spread tile edge per dimension.  In 3-D a tile of 4^3 cells keeps a dense
+# cluster's points on several blocks: on 1e7 Gaussian-mixture points (128^3)
+# edge 4 fits in 40 ms against 49 ms for edge 8 (3: 46 ms, 5: 41 ms); in 2-D
+# edges 4 and 16 are a few percent slower than 8 on 1e6 uniform points.
+TILE = {1: 16, 2: 8, 3: 4}
 # |u| beyond which the Gaussian taps are dropped: exp(-u^2/2) u^4 relative to the
 # peak is 1.7e-14 at u = 9 (float32 path) and 1e-27 at u = 12 (float64 path).
 BAND_CUT = {False: 9.0, True: 12.0}
