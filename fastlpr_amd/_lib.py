@@ -141,7 +141,7 @@ _SIGS = {
     "flpr_interp_eval": [_GP, _P, _I64, _P, _I32, _P, _P, _P, _P],
     "flpr_fit_output": [_GP, _P, _I64, _P, _I32, _P, _P, _I32, _P, _P, _P, _P, _P, _P],
     "flpr_spline_eval": [_GP, _P, _I64, _P, _I32, _P, _P],
-    "flpr_lcv": [_GP, _P, _I64, _P, _I32, _P, _P, _P, _P],
+    "flpr_lcv": [_GP, _P, _P, _I64, _P, _I32, _P, _P, _P, _P],
     "flpr_select": [_P, _I32, _I32, _I32, _I32, _P, _P, _P, _I64, _I32, _I32, _D, _P, _P, _P],
     "flpr_fill_zero": [_P, _I64, _P],
     "flpr_batch_xstats": [_P, _I32, _P, _I32, _P, _I32, _P, _P, _P],

@@ -874,7 +874,7 @@ def cv_fastkde(x, h: Optional[Sequence[Sequence[float]]] = None,
     sdx = s.sum(dim=1, keepdim=True) * dx_grid
     dens = torch.where(sdx > 0, s / sdx, s).clamp_min(0.0)
     k0h = K0_GAUSS / np.prod(hl, axis=1)
-    lv = engine.lcv(geo, xd, dens, k0h).cpu().numpy()
+    lv = engine.lcv(geo, xd, dens, k0h, fit.order_idx).cpu().numpy()
     scores, errs = lv[:dh].copy(), lv[dh:].copy()
     sel = _one_se_rule(scores, errs, hl, prefer_smaller=False)
     idmax = int(np.argmax(scores))

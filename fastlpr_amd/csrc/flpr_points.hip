@@ -1761,12 +1761,19 @@ __global__ __launch_bounds__(EVAL_THREADS) void k_fit_finish(
 // every bandwidth b and data point i
 //   f_i = linear interpolation of dens[b] at x_i
 //   l_i = log(max((n / (n - 1)) (f_i - k0h[b] / n), 1e-10))
-// and per block the count, sum and centred sum of squares of l_i (Welford in a
-// thread, Chan's merge across threads), written to part[(block * dh + b) * 3].
-// A block owns a contiguous run of points and loops over the bandwidths, so
-// the run's coordinates stay in cache across them.
+// and per block the count, sum and centred sum of squares of l_i, written to
+// part[(block * dh + b) * 3].  A block owns a contiguous run of points, taken
+// in the order of `order` when given (the points grouped by spread cell, so
+// that neighbouring lanes interpolate neighbouring cells and the density
+// reads stay in cache), and passes over it once per group of LCV_GROUP
+// bandwidths: a point's interpolation corners serve the whole group.  A
+// thread keeps sum l and the sums of (l - k) and (l - k)^2 with k its first
+// l, so no division per point; the centred sum of squares is
+// sum (l - k)^2 - (sum (l - k))^2 / count, merged across threads with Chan's
+// formula.
 constexpr int LCV_THREADS = 256;
 constexpr int LCV_BLOCKS = 1024;
+constexpr int LCV_GROUP = 8;
 
 struct Moments3 {
   double n, s, m2;  // count, sum, sum of squared deviations from the mean
@@ -1785,49 +1792,70 @@ __device__ inline Moments3 m3_merge(Moments3 a, Moments3 b) {
 
 template <int D>
 __global__ __launch_bounds__(LCV_THREADS) void k_lcv(FlprGeom g, const double* __restrict__ x,
+                                                     const int32_t* __restrict__ order,
                                                      int64_t n, const double* __restrict__ dens,
                                                      int dh, int64_t nodes,
                                                      const double* __restrict__ k0h,
                                                      double* __restrict__ part) {
-  __shared__ Moments3 sh[LCV_THREADS / 64];
+  __shared__ Moments3 sh[LCV_THREADS / 64][LCV_GROUP];
   const int64_t per = (n + gridDim.x - 1) / gridDim.x;
   const int64_t i0 = (int64_t)blockIdx.x * per;
   const int64_t i1 = min(n, i0 + per);
   const double nd = (double)n;
   const double ratio = nd / (nd - 1.0);
-  for (int b = 0; b < dh; ++b) {
-    const double* f = dens + (int64_t)b * nodes;
-    const double c = k0h[b] / nd;
-    Moments3 acc = {0.0, 0.0, 0.0};
-    double mean = 0.0;
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-      const CornersT<D> cr = corners<D>(g, x + i * D);
-      double fa = 0.0;
+  for (int b0 = 0; b0 < dh; b0 += LCV_GROUP) {
+    const int nb = min(LCV_GROUP, dh - b0);
+    double c[LCV_GROUP], k[LCV_GROUP], sl[LCV_GROUP], s1[LCV_GROUP], s2[LCV_GROUP];
 #pragma unroll
-      for (int q = 0; q < (1 << D); ++q) fa += cr.w[q] * f[cr.node[q]];
-      double lo = ratio * (fa - c);
-      lo = (lo != lo) ? lo : (lo > 1e-10 ? lo : 1e-10);
-      const double l = log(lo);
-      acc.n += 1.0;
-      acc.s += l;
-      const double dl = l - mean;
-      mean += dl / acc.n;
-      acc.m2 += dl * (l - mean);
+    for (int j = 0; j < LCV_GROUP; ++j) {
+      c[j] = j < nb ? k0h[b0 + j] / nd : 0.0;
+      k[j] = sl[j] = s1[j] = s2[j] = 0.0;
     }
-    // wave then block merge
-    for (int off = 32; off > 0; off >>= 1) {
-      Moments3 o;
-      o.n = __shfl_xor(acc.n, off);
-      o.s = __shfl_xor(acc.s, off);
-      o.m2 = __shfl_xor(acc.m2, off);
-      acc = m3_merge(acc, o);
+    double cnt = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+      const int64_t p = order ? (int64_t)order[i] : i;
+      const CornersT<D> cr = corners<D>(g, x + p * D);
+#pragma unroll
+      for (int j = 0; j < LCV_GROUP; ++j) {
+        if (j < nb) {
+          const double* f = dens + (int64_t)(b0 + j) * nodes;
+          double fa = 0.0;
+#pragma unroll
+          for (int q = 0; q < (1 << D); ++q) fa += cr.w[q] * f[cr.node[q]];
+          double lo = ratio * (fa - c[j]);
+          lo = (lo != lo) ? lo : (lo > 1e-10 ? lo : 1e-10);
+          const double l = log(lo);
+          if (cnt == 0.0) k[j] = l;
+          const double dl = l - k[j];
+          sl[j] += l;
+          s1[j] += dl;
+          s2[j] += dl * dl;
+        }
+      }
+      cnt += 1.0;
     }
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < LCV_GROUP; ++j) {
+      if (j >= nb) break;
+      Moments3 acc = {cnt, sl[j], cnt > 0.0 ? fmax(s2[j] - s1[j] * s1[j] / cnt, 0.0) : 0.0};
+      // s2 - s1^2 / cnt is NaN whenever an l is (as Welford's would be)
+      if (s2[j] != s2[j]) acc.m2 = s2[j];
+      for (int off = 32; off > 0; off >>= 1) {
+        Moments3 o;
+        o.n = __shfl_xor(acc.n, off);
+        o.s = __shfl_xor(acc.s, off);
+        o.m2 = __shfl_xor(acc.m2, off);
+        acc = m3_merge(acc, o);
+      }
+      if (lane == 0) sh[w][j] = acc;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      Moments3 r = sh[0];
-      for (int w = 1; w < LCV_THREADS / 64; ++w) r = m3_merge(r, sh[w]);
-      double* o = part + ((int64_t)blockIdx.x * dh + b) * 3;
+    if (threadIdx.x < nb) {
+      const int j = threadIdx.x;
+      Moments3 r = sh[0][j];
+      for (int v = 1; v < LCV_THREADS / 64; ++v) r = m3_merge(r, sh[v][j]);
+      double* o = part + ((int64_t)blockIdx.x * dh + b0 + j) * 3;
       o[0] = r.n;
       o[1] = r.s;
       o[2] = r.m2;
@@ -2763,17 +2791,18 @@ int flpr_fit_output(const FlprGeom* g, const double* x, int64_t n, const double*
   return 0;
 }
 
-int flpr_lcv(const FlprGeom* g, const double* x, int64_t n, const double* dens, int32_t dh,
-             const double* k0h, double* part, double* out, void* stream) {
+int flpr_lcv(const FlprGeom* g, const double* x, const int32_t* order, int64_t n,
+             const double* dens, int32_t dh, const double* k0h, double* part, double* out,
+             void* stream) {
   FLPR_CLEAR_ERROR();
   if (!flpr_geom_ok(g) || !x || !dens || !k0h || !part || !out || n < 2 || dh < 1) return -1;
   const int64_t nodes = flpr_nodes(g);
   hipStream_t st = as_stream(stream);
   dim3 grd(LCV_BLOCKS);
   switch (g->d) {
-    case 1: hipLaunchKernelGGL(k_lcv<1>, grd, dim3(LCV_THREADS), 0, st, *g, x, n, dens, dh, nodes, k0h, part); break;
-    case 2: hipLaunchKernelGGL(k_lcv<2>, grd, dim3(LCV_THREADS), 0, st, *g, x, n, dens, dh, nodes, k0h, part); break;
-    default: hipLaunchKernelGGL(k_lcv<3>, grd, dim3(LCV_THREADS), 0, st, *g, x, n, dens, dh, nodes, k0h, part); break;
+    case 1: hipLaunchKernelGGL(k_lcv<1>, grd, dim3(LCV_THREADS), 0, st, *g, x, order, n, dens, dh, nodes, k0h, part); break;
+    case 2: hipLaunchKernelGGL(k_lcv<2>, grd, dim3(LCV_THREADS), 0, st, *g, x, order, n, dens, dh, nodes, k0h, part); break;
+    default: hipLaunchKernelGGL(k_lcv<3>, grd, dim3(LCV_THREADS), 0, st, *g, x, order, n, dens, dh, nodes, k0h, part); break;
   }
   FLPR_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_lcv_final, dim3((dh + 255) / 256), dim3(256), 0, st, part, LCV_BLOCKS, dh,

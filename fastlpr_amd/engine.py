@@ -1043,6 +1043,7 @@ class DeviceFit:
     warn: Optional[str]
     red: Optional[torch.Tensor] = None   # device (dh, nval) dots then (dh, ny) quad forms
     key: Optional[torch.Tensor] = None   # global regularisation key (key_only)
+    order_idx: Optional[torch.Tensor] = None  # (n) int32 points grouped by spread cell
 
 
 class S00Rows:
@@ -1305,7 +1306,8 @@ def run_grid(x: torch.Tensor, vals_y: Optional[torch.Tensor], ny: int,
         return DeviceFit(h, None, None, None, None, geo, plan0, warn, None, key=key_max)
     if moments_only:
         s0_all = S00Rows(stage0s, per_h0, off0, nodes, eps_add if order == 0 else 0.0)
-        return DeviceFit(h, None, None, None, s0_all, geo, plan0, warn, None)
+        return DeviceFit(h, None, None, None, s0_all, geo, plan0, warn, None,
+                         order_idx=order_idx)
 
     # --- adjoint sums, solve (+ dots), residual quadratic form
     # red: one float64 readback buffer, probe / y dots (dh, nval) then
@@ -1536,16 +1538,20 @@ def run_grid_y(sh: dict, x: torch.Tensor, vals_y: torch.Tensor, ny: int,
                      sh["warn"], red)
 
 
-def lcv(geo: Geometry, x: torch.Tensor, dens: torch.Tensor, k0h: np.ndarray):
+def lcv(geo: Geometry, x: torch.Tensor, dens: torch.Tensor, k0h: np.ndarray,
+        order: Optional[torch.Tensor] = None):
     """LCV score and standard error per bandwidth (flpr_lcv): dens (dh, nodes)
-    float64 on the device.  Returns a (2 dh,) device tensor [scores, stderrs]."""
+    float64 on the device; order (n) int32, the points grouped by spread cell
+    (faster density reads), or None for their stored order.  Returns a (2 dh,)
+    device tensor [scores, stderrs]."""
     n = x.shape[0]
     dh = dens.shape[0]
     dev = x.device
     k0 = upload(np.asarray(k0h, dtype=float), torch.float64, dev)
     part = torch.empty((1024 * dh * 3,), dtype=torch.float64, device=dev)
     out = torch.empty((2 * dh,), dtype=torch.float64, device=dev)
-    call("flpr_lcv", _lib.C.byref(geo.geom), ptr(x), n, ptr(dens.contiguous()), dh, ptr(k0),
+    call("flpr_lcv", _lib.C.byref(geo.geom), ptr(x), ptr(order) if order is not None else None,
+         n, ptr(dens.contiguous()), dh, ptr(k0),
          ptr(part), ptr(out), stream_handle())
     return out
 

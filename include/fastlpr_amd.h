@@ -376,9 +376,13 @@ int flpr_spline_eval(const FlprGeom* g, const double* x, int64_t n, const double
  *   l_i = log(max((n/(n-1)) (f_b(x_i) - k0h[b]/n), 1e-10))
  * with f_b the linear interpolant (RegularGridInterpolator 'linear');
  * out[b] = sum_i l_i and out[dh + b] = std(l) sqrt(n) (ddof = 0).
+ * order (n) int32, may be NULL: the order in which the points are visited,
+ * e.g. flpr_points_bin's grouping by spread cell, which keeps the density
+ * reads of neighbouring threads in cache; NULL visits them as stored.
  * part is workspace of 1024 * dh * 3 doubles. */
-int flpr_lcv(const FlprGeom* g, const double* x, int64_t n, const double* dens, int32_t dh,
-             const double* k0h, double* part, double* out, void* stream);
+int flpr_lcv(const FlprGeom* g, const double* x, const int32_t* order, int64_t n,
+             const double* dens, int32_t dh, const double* k0h, double* part, double* out,
+             void* stream);
 
 /* Bandwidth selection from the per-bandwidth sums, on the device
  * (regression.py:1276-1354 Hutchinson DOF and penalties, regression.py:378-421
