@@ -681,6 +681,15 @@ __global__ __launch_bounds__(SPREAD_THREADS, (SPREAD_WAVES<D, A>)) void k_spread
       r[0] = row / ntap;
       r[1] = row % ntap;
     }
+    // G[r] of the fixed axes, chosen once per item
+    A gfix[FIX > 0 ? FIX : 1];
+#pragma unroll
+    for (int k = 0; k < FIX; ++k) {
+      A sel = A(0);
+#pragma unroll
+      for (int j = 0; j < ntap; ++j) sel = (j == r[k]) ? (A)tab.G[k][j] : sel;
+      gfix[k] = sel;
+    }
     A acc[NACC];
     A cmp[KAHAN ? NACC : 1];
 #pragma unroll
@@ -726,14 +735,19 @@ __global__ __launch_bounds__(SPREAD_THREADS, (SPREAD_WAVES<D, A>)) void k_spread
       }
       A w[D][ntap];
 #pragma unroll
-      for (int k = 0; k < D; ++k)
+      for (int k = FIX; k < D; ++k)
         tap_weights<MSP, A>((A)cur[2 * k], (A)cur[2 * k + 1], tab.G[k], w[k]);
+      // a fixed axis needs only the weight of offset r[k]: A B^r G[r], with
+      // B^r by squaring (r <= 2 MSP < 16)
 #pragma unroll
       for (int k = 0; k < FIX; ++k) {
-        A sel = A(0);
+        A bb = (A)cur[2 * k + 1], bp = A(1);
 #pragma unroll
-        for (int j = 0; j < ntap; ++j) sel = (j == r[k]) ? w[k][j] : sel;
-        s *= sel;
+        for (int bit = 0; (1 << bit) <= 2 * MSP; ++bit) {
+          bp *= ((r[k] >> bit) & 1) ? bb : A(1);
+          bb *= bb;
+        }
+        s *= (A)cur[2 * k] * bp * gfix[k];
       }
       if constexpr (F == 1) {
 #pragma unroll
