@@ -1879,16 +1879,31 @@ __global__ __launch_bounds__(LCV_THREADS) void k_lcv(FlprGeom g, const double* _
 }
 
 // out[b] = sum_i l_i (the LCV score), out[dh + b] = std(l) sqrt(n) (its
-// standard error, np.std with ddof = 0), merged over the blocks in order.
+// standard error, np.std with ddof = 0).  One block per bandwidth: thread t
+// merges block partials t, t + 256, ... in order, then a fixed xor-shuffle
+// tree and the four waves in order (Chan's merge is symmetric, so every lane
+// of the tree holds the same value).
 __global__ __launch_bounds__(256) void k_lcv_final(const double* __restrict__ part, int nblocks,
                                                    int dh, double* __restrict__ out) {
-  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < dh; b += gridDim.x * blockDim.x) {
-    Moments3 r = {0.0, 0.0, 0.0};
-    for (int k = 0; k < nblocks; ++k) {
-      const double* o = part + ((int64_t)k * dh + b) * 3;
-      Moments3 m = {o[0], o[1], o[2]};
-      r = m3_merge(r, m);
-    }
+  __shared__ Moments3 sh[4];
+  const int b = blockIdx.x;
+  Moments3 r = {0.0, 0.0, 0.0};
+  for (int k = threadIdx.x; k < nblocks; k += blockDim.x) {
+    const double* o = part + ((int64_t)k * dh + b) * 3;
+    r = m3_merge(r, Moments3{o[0], o[1], o[2]});
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    Moments3 o;
+    o.n = __shfl_xor(r.n, off);
+    o.s = __shfl_xor(r.s, off);
+    o.m2 = __shfl_xor(r.m2, off);
+    r = m3_merge(r, o);
+  }
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = r;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    r = sh[0];
+    for (int w = 1; w < 4; ++w) r = m3_merge(r, sh[w]);
     out[b] = r.s;
     out[dh + b] = sqrt(r.m2 / r.n) * sqrt(r.n);
   }
@@ -2819,8 +2834,7 @@ int flpr_lcv(const FlprGeom* g, const double* x, const int32_t* order, int64_t n
     default: hipLaunchKernelGGL(k_lcv<3>, grd, dim3(LCV_THREADS), 0, st, *g, x, order, n, dens, dh, nodes, k0h, part); break;
   }
   FLPR_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_lcv_final, dim3((dh + 255) / 256), dim3(256), 0, st, part, LCV_BLOCKS, dh,
-                     out);
+  hipLaunchKernelGGL(k_lcv_final, dim3(dh), dim3(256), 0, st, part, LCV_BLOCKS, dh, out);
   FLPR_CHECK_LAUNCH();
   return 0;
 }
