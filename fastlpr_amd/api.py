@@ -871,8 +871,13 @@ def cv_fastkde(x, h: Optional[Sequence[Sequence[float]]] = None,
     # normalise so that sum(f) dx = 1 on the raw grid, clip at zero (api.py:1062-1096)
     dx_grid = float(np.prod([ax[1] - ax[0] for ax in geo.axes_raw]))
     s = fit.s0_all.all()
-    sdx = s.sum(dim=1, keepdim=True) * dx_grid
-    dens = torch.where(sdx > 0, s / sdx, s).clamp_min(0.0)
+    # row sums in chunks of the row: a plain sum over dim 1 reduces each long
+    # row on few blocks
+    chunks = math.gcd(s.shape[1], 4096)
+    sdx = (s.reshape(dh, chunks, -1).sum(dim=2).sum(dim=1, keepdim=True) if chunks >= 64
+           else s.sum(dim=1, keepdim=True)) * dx_grid
+    # s / sdx where sdx > 0, else s; then clipped at zero, in place
+    dens = torch.div(s, torch.where(sdx > 0, sdx, torch.ones_like(sdx))).clamp_min_(0.0)
     k0h = K0_GAUSS / np.prod(hl, axis=1)
     lv = engine.lcv(geo, xd, dens, k0h, fit.order_idx).cpu().numpy()
     scores, errs = lv[:dh].copy(), lv[dh:].copy()
